@@ -1,0 +1,170 @@
+/*
+ * pthip.h - C ABI of the MI355X path-tracing back end (libpthip.so).
+ *
+ * This boundary replaces the reference's Vulkan ray-tracing back end: the C++ class
+ * PathTracer (src/pathtrace/pathtrace.h:43-103) as driven by PathTraceDispatcher
+ * (src/pathtrace/ptdispatch.h:24-55), its GPU contract (descriptor bindings
+ * src/pathtrace/pathtrace.cpp:36-69, uniform block src/pathtrace/pathtrace.h:21-41) and the
+ * driver-built acceleration structures (src/pathtrace/accstruct.cpp:69-286).
+ *
+ * Plain C types only. Every call returns PT_OK (0) or a negative PT_E* status; no C++
+ * exception crosses this boundary. The message of the last failure on a context is
+ * available from pt_last_error(ctx); failures of pt_create are reported by
+ * pt_last_error(NULL) (thread-local).
+ *
+ * Threading: one context per host thread. A context owns one HIP stream and all device
+ * memory it uses (scene copy, BLAS/TLAS, wavefront queues, accumulation image).
+ */
+#ifndef PTHIP_H_INCLUDED
+#define PTHIP_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_EINVAL (-1)  /* bad argument (e.g. spp == 0, mirrors pathtrace.cpp:400-401) */
+#define PT_ENOMEM (-2)  /* device allocation failed */
+#define PT_EDEVICE (-3) /* a HIP runtime call or kernel failed */
+#define PT_ESTATE (-4)  /* call not valid in the current state (e.g. no camera set) */
+#define PT_ENODEV (-5)  /* no HIP device present */
+
+/* Byte sizes of the packed scene records; identical to the reference's GPU layout. */
+#define PT_VERTEX_BYTES 32u    /* src/scene/mesh.h:10-14: pos f32x3 | normal f32x3 | uv f32x2 */
+#define PT_INSTANCE_BYTES 128u /* src/scene/scenebuffer.h:13-20 (InstanceData)               */
+#define PT_MATERIAL_BYTES 96u  /* src/scene/material.h:24-29 (PbrMaterial)                   */
+#define PT_LIGHT_BIN_WORDS 10u /* share/constants.h:27 */
+#define PT_EMITTER_BIN_WORDS 6u /* share/constants.h:28 */
+#define PT_NO_EMITTER 0xFFFFFFFFu
+
+/* One mesh of the packed scene: the region of the vertex/index arrays it owns
+ * (SceneBuffer::get_start_indices, src/scene/scenebuffer.h:53-54). One BLAS is built per mesh,
+ * as AccelerationStructure::create_blas does (src/pathtrace/accstruct.cpp:69-164). */
+typedef struct pt_mesh_desc {
+    uint32_t vertex_start; /* in vertices */
+    uint32_t index_start;  /* in u32 indices (3 per triangle) */
+    uint32_t tri_count;
+    uint32_t vertex_count;
+} pt_mesh_desc;
+
+/* The packed scene, exactly the regions of the reference's SceneBuffer
+ * (src/scene/scenebuffer.cpp:65-168) plus the instance->mesh map the TLAS needs
+ * (src/pathtrace/accstruct.cpp:197-216). Host memory; only read during pt_create. */
+typedef struct pt_scene_blob {
+    const void* vertices; /* vertex_count x 32 B */
+    uint64_t vertex_count;
+    const uint32_t* indices; /* index_count u32, mesh-relative vertex numbers */
+    uint64_t index_count;
+    const void* instances; /* instance_count x 128 B InstanceData */
+    uint32_t instance_count;
+    const void* materials; /* material_count x 96 B PbrMaterial */
+    uint32_t material_count;
+    const uint32_t* light_words; /* TableLightSampler::copy output (lightsample.cpp:209-221) */
+    uint64_t light_word_count;
+    uint32_t light_count; /* bins of the light alias table (SceneBuffer::get_light_count) */
+    uint32_t mesh_count;
+    const pt_mesh_desc* meshes;
+    const uint32_t* instance_mesh; /* instance_count entries: mesh (BLAS) of each instance */
+    float envcol[4];               /* linear environment colour (UBO environment_colour) */
+} pt_scene_blob;
+
+/* Camera block: the same content PathTracer::set_camera writes into the uniform block
+ * (src/pathtrace/pathtrace.cpp:381-396): inverse view and inverse projection, row-major. */
+typedef struct pt_camera {
+    float inv_view[16];
+    float inv_proj[16];
+    float near_clip;
+    float far_clip;
+    float focus_dist;
+    float lens_radius;
+    float exposure;
+} pt_camera;
+
+/* A ray for pt_trace_rays (the TraceRay call sites: pathtrace.rg.hlsl:35,51,
+ * pathtrace.ch.hlsl:199). */
+typedef struct pt_ray {
+    float origin[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} pt_ray;
+
+/* A hit: InstanceID(), PrimitiveIndex(), RayTCurrent() and the two barycentrics the
+ * closest-hit shader receives (pathtrace.ch.hlsl:228-240). instance == PT_NO_HIT on a miss. */
+#define PT_NO_HIT 0xFFFFFFFFu
+typedef struct pt_hit {
+    float t;
+    float u;
+    float v;
+    uint32_t instance;
+    uint32_t primitive;
+} pt_hit;
+
+typedef struct pt_stats {
+    uint64_t extension_rays; /* rays traced by closest-hit traversal, last pt_render */
+    uint64_t shadow_rays;    /* rays traced by any-hit traversal, last pt_render */
+    uint64_t paths;          /* pixel samples, last pt_render */
+    double render_ms;        /* device time of the last pt_render (HIP events) */
+    double bvh_build_ms;     /* device time of the BLAS+TLAS build in pt_create */
+    uint64_t bvh_bytes;      /* bytes of nodes + triangle records */
+    uint32_t sample_index;   /* next sample index (PathTracer::get_sample_index) */
+    uint32_t max_bounces;
+    uint32_t width;
+    uint32_t height;
+    uint32_t node_count;
+    uint32_t tri_record_count;
+} pt_stats;
+
+typedef struct pt_ctx pt_ctx;
+
+/* PathTracer::PathTracer + SceneBuffer upload + AccelerationStructure build
+ * (src/pathtrace/pathtrace.cpp:81-116, accstruct.cpp:288-296). Builds one LBVH per mesh and
+ * a TLAS over the instances on the device. width/height size the accumulation image
+ * (PathTraceDispatcher ctor, src/pathtrace/ptdispatch.cpp:26-27). */
+int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out);
+
+/* PathTracer::set_camera (src/pathtrace/pathtrace.cpp:381-396). Resets sample_index to 0. */
+int pt_set_camera(pt_ctx* ctx, const pt_camera* camera);
+
+/* PathTracer::set_max_bounces (src/pathtrace/pathtrace.cpp:403-407). Resets sample_index. */
+int pt_set_max_bounces(pt_ctx* ctx, uint32_t max_bounces);
+
+/* PathTraceDispatcher::render (src/pathtrace/ptdispatch.cpp:97-134): traces spp further
+ * samples per pixel into the accumulation image, one sample index after another exactly as
+ * the reference's 1-spp submits do (running mean of correct_radiance(L*exposure)), and
+ * advances sample_index by spp (PathTracer::update_uniforms, pathtrace.cpp:511-516).
+ * spp == 0 -> PT_EINVAL (pathtrace.cpp:400-401). Asynchronous on the context's stream. */
+int pt_render(pt_ctx* ctx, uint32_t spp);
+
+/* Readback of the accumulation image (ptdispatch.cpp:77-94 copy_image + vkMapMemory):
+ * width*height RGBA32F rows of row_pitch bytes into caller memory. Synchronises. */
+int pt_read_accumulation(pt_ctx* ctx, float* rgba, size_t row_pitch);
+
+/* Primary-ray hits of one sample index: per pixel, the first TraceRay of the raygen shader
+ * (pathtrace.rg.hlsl:16-35) with its camera ray and [near, far] interval. hits has
+ * width*height entries, row-major. Synchronises. */
+int pt_trace_primary(pt_ctx* ctx, uint32_t sample_index, pt_hit* hits);
+
+/* Arbitrary rays through the same traversal: closest hit (any_hit == 0, the TraceRay of
+ * pathtrace.rg.hlsl:51) or first hit (any_hit != 0, the shadow TraceRay of
+ * pathtrace.ch.hlsl:199 with ACCEPT_FIRST_HIT_AND_END_SEARCH). Synchronises. */
+int pt_trace_rays(pt_ctx* ctx, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit);
+
+/* Counters and timings (the reference prints its timing in ptdispatch.cpp:65-70). */
+int pt_get_stats(pt_ctx* ctx, pt_stats* out);
+
+/* Waits for all work queued on the context's stream. */
+int pt_synchronize(pt_ctx* ctx);
+
+const char* pt_last_error(const pt_ctx* ctx);
+
+void pt_destroy(pt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

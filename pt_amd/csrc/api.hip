@@ -1,0 +1,500 @@
+// C ABI of the device back end (include/pthip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "pthip.h"
+
+using namespace ptk;
+
+struct pt_ctx {
+    Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(Ctx* c, int code, const std::string& msg)
+{
+    if (c)
+        c->error = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PT_ENOMEM : PT_EDEVICE;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(c, expr, what)                \
+    do {                                      \
+        hipError_t e_ = (expr);               \
+        if (e_ != hipSuccess)                 \
+            return hip_fail((c), e_, (what)); \
+    } while (0)
+
+template <typename T>
+hipError_t upload(T*& dst, const void* src, size_t bytes, hipStream_t st)
+{
+    hipError_t e = hipMalloc((void**)&dst, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess)
+        return e;
+    if (bytes)
+        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e;
+}
+
+void release(Ctx& c)
+{
+    void* ptrs[] = {c.d_vertices, c.d_indices, c.d_instances, c.d_materials, c.d_light,
+                    c.d_meshes, c.d_instance_mesh, c.d_inst_opacity, c.accum};
+    for (void* p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    free_bvh(c.bvh);
+    wavefront_free(c.wf);
+    if (c.ev_start)
+        (void)hipEventDestroy(c.ev_start);
+    if (c.ev_stop)
+        (void)hipEventDestroy(c.ev_stop);
+    if (c.stream)
+        (void)hipStreamDestroy(c.stream);
+}
+
+int validate_blob(const pt_scene_blob* b, std::string& why)
+{
+    if (!b)
+        return (why = "scene is null", PT_EINVAL);
+    if ((b->vertex_count && !b->vertices) || (b->index_count && !b->indices) || (b->instance_count && !b->instances) ||
+        (b->material_count && !b->materials) || (b->light_word_count && !b->light_words) ||
+        (b->mesh_count && !b->meshes) || (b->instance_count && !b->instance_mesh))
+        return (why = "scene array pointer is null", PT_EINVAL);
+    if (b->vertex_count > 0x7FFFFFFFull || b->index_count > 0xFFFFFFFFull)
+        return (why = "scene exceeds 32-bit addressing", PT_EINVAL);
+    for (uint32_t m = 0; m < b->mesh_count; m++) {
+        const pt_mesh_desc& d = b->meshes[m];
+        if (uint64_t(d.index_start) + uint64_t(d.tri_count) * 3 > b->index_count ||
+            uint64_t(d.vertex_start) + d.vertex_count > b->vertex_count)
+            return (why = "mesh " + std::to_string(m) + " exceeds the vertex/index arrays", PT_EINVAL);
+        for (uint64_t k = 0; k < uint64_t(d.tri_count) * 3; k++)
+            if (b->indices[d.index_start + k] >= d.vertex_count)
+                return (why = "mesh " + std::to_string(m) + " has an index out of range", PT_EINVAL);
+    }
+    const auto* inst = static_cast<const InstanceData*>(b->instances);
+    for (uint32_t i = 0; i < b->instance_count; i++) {
+        if (b->instance_mesh[i] >= b->mesh_count)
+            return (why = "instance " + std::to_string(i) + " names a missing mesh", PT_EINVAL);
+        if (inst[i].material_index >= b->material_count)
+            return (why = "instance " + std::to_string(i) + " names a missing material", PT_EINVAL);
+        const pt_mesh_desc& d = b->meshes[b->instance_mesh[i]];
+        if (inst[i].vertex_index != d.vertex_start || inst[i].index_index != d.index_start)
+            return (why = "instance " + std::to_string(i) + " does not point at its mesh", PT_EINVAL);
+    }
+    if (b->light_count) {
+        if (uint64_t(b->light_count) * PT_LIGHT_BIN_WORDS > b->light_word_count)
+            return (why = "light table exceeds the light words", PT_EINVAL);
+        for (uint32_t l = 0; l < b->light_count; l++) {
+            const uint32_t* bin = b->light_words + size_t(l) * PT_LIGHT_BIN_WORDS;
+            if (bin[2] >= b->light_count)
+                return (why = "light alias out of range", PT_EINVAL);
+            if (bin[3] == 3u) {
+                const uint64_t off = bin[4], tris = bin[5], i0 = bin[6], ni = bin[7];
+                if (tris == 0 || off + tris * PT_EMITTER_BIN_WORDS > b->light_word_count || ni == 0 ||
+                    i0 + ni > b->instance_count)
+                    return (why = "emitter light bin out of range", PT_EINVAL);
+                uint32_t min_verts = 0xFFFFFFFFu;
+                for (uint64_t i = i0; i < i0 + ni; i++)
+                    min_verts = std::min(min_verts, b->meshes[b->instance_mesh[i]].vertex_count);
+                for (uint64_t t = 0; t < tris; t++) {
+                    const uint32_t* eb = b->light_words + off + t * PT_EMITTER_BIN_WORDS;
+                    if (eb[2] >= tris || eb[3] >= min_verts || eb[4] >= min_verts || eb[5] >= min_verts)
+                        return (why = "emitter triangle bin out of range", PT_EINVAL);
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < b->instance_count; i++)
+        if (inst[i].emitter_index != PT_NO_EMITTER && uint64_t(inst[i].emitter_index) >= b->light_count)
+            return (why = "instance " + std::to_string(i) + " has an emitter index beyond the light table", PT_EINVAL);
+    return PT_OK;
+}
+
+// Host check of the built hierarchy before any traversal kernel runs: every child index in
+// range, every triangle record referenced exactly once, every instance reachable once.
+int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& why)
+{
+    const BvhBuffers& bv = c.bvh;
+    std::vector<Node4> nodes(bv.node_count);
+    if (bv.node_count &&
+        hipMemcpy(nodes.data(), bv.nodes, size_t(bv.node_count) * sizeof(Node4), hipMemcpyDeviceToHost) != hipSuccess)
+        return (why = "BVH readback failed", PT_EDEVICE);
+    std::vector<TriRec> tris(bv.tri_count);
+    if (bv.tri_count &&
+        hipMemcpy(tris.data(), bv.tris, size_t(bv.tri_count) * sizeof(TriRec), hipMemcpyDeviceToHost) != hipSuccess)
+        return (why = "BVH readback failed", PT_EDEVICE);
+    std::vector<uint8_t> tri_seen(bv.tri_count, 0), inst_seen(c.instance_count, 0);
+    std::vector<uint32_t> stack;
+    uint32_t tri_base = 0;
+    for (uint32_t m = 0; m < meshes.size(); m++) {
+        const uint32_t nt = meshes[m].tri_count;
+        if (nt) {
+            stack.assign(1, m);
+            while (!stack.empty()) {
+                const uint32_t n = stack.back();
+                stack.pop_back();
+                if (n >= bv.node_count)
+                    return (why = "BLAS child index out of range", PT_EDEVICE);
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t ch = nodes[n].child[k];
+                    if (ch == CHILD_EMPTY)
+                        continue;
+                    if (ch & CHILD_LEAF) {
+                        const uint32_t f = ch & LEAF_FIRST_MASK, cnt = ((ch >> 27) & 0xFu) + 1u;
+                        if (f < tri_base || f + cnt > tri_base + nt)
+                            return (why = "BLAS leaf outside its mesh", PT_EDEVICE);
+                        for (uint32_t q = f; q < f + cnt; q++) {
+                            if (tri_seen[q]++)
+                                return (why = "triangle referenced twice", PT_EDEVICE);
+                            if (float_as_u32(tris[q].v0.w) >= nt)
+                                return (why = "triangle record with a bad primitive index", PT_EDEVICE);
+                        }
+                    } else {
+                        if (ch >= bv.node_count || ch < meshes.size())
+                            return (why = "BLAS child index out of range", PT_EDEVICE);
+                        stack.push_back(ch);
+                    }
+                }
+                if (stack.size() > bv.node_count)
+                    return (why = "BLAS has a cycle", PT_EDEVICE);
+            }
+        }
+        tri_base += nt;
+    }
+    for (uint32_t q = 0; q < bv.tri_count; q++)
+        if (!tri_seen[q])
+            return (why = "triangle record not reachable", PT_EDEVICE);
+    if (c.instance_count) {
+        if (bv.tlas_root >= bv.node_count)
+            return (why = "TLAS root out of range", PT_EDEVICE);
+        stack.assign(1, bv.tlas_root);
+        size_t visited = 0;
+        while (!stack.empty()) {
+            const uint32_t n = stack.back();
+            stack.pop_back();
+            if (++visited > bv.node_count)
+                return (why = "TLAS has a cycle", PT_EDEVICE);
+            for (int k = 0; k < 4; k++) {
+                const uint32_t ch = nodes[n].child[k];
+                if (ch == CHILD_EMPTY)
+                    continue;
+                if (ch & CHILD_LEAF) {
+                    const uint32_t id = ch & ~CHILD_LEAF;
+                    if (id >= c.instance_count || inst_seen[id]++)
+                        return (why = "TLAS leaf out of range or repeated", PT_EDEVICE);
+                } else {
+                    if (ch >= bv.node_count || ch < bv.tlas_root)
+                        return (why = "TLAS child index out of range", PT_EDEVICE);
+                    stack.push_back(ch);
+                }
+            }
+        }
+        for (uint32_t i = 0; i < c.instance_count; i++)
+            if (!inst_seen[i])
+                return (why = "instance not reachable from the TLAS", PT_EDEVICE);
+    }
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out)
+{
+    if (!out)
+        return fail(nullptr, PT_EINVAL, "out is null");
+    *out = nullptr;
+    if (width == 0 || height == 0 || uint64_t(width) * height > 0x7FFFFFFFull)
+        return fail(nullptr, PT_EINVAL, "invalid image size");
+    std::string why;
+    if (int v = validate_blob(scene, why))
+        return fail(nullptr, v, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, PT_ENODEV, "no HIP device");
+
+    pt_ctx* h = new (std::nothrow) pt_ctx;
+    if (!h)
+        return fail(nullptr, PT_ENOMEM, "out of host memory");
+    Ctx& c = h->c;
+    auto bail = [&](int code) {
+        g_create_error = c.error;
+        release(c);
+        delete h;
+        return code;
+    };
+    int code;
+    if ((code = [&]() -> int {
+             HIP_TRY(&c, hipGetDevice(&c.device), "hipGetDevice");
+             HIP_TRY(&c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+             HIP_TRY(&c, hipEventCreate(&c.ev_start), "hipEventCreate");
+             HIP_TRY(&c, hipEventCreate(&c.ev_stop), "hipEventCreate");
+             HIP_TRY(&c, upload_sobol_table(), "sobol table");
+             hipStream_t st = c.stream;
+             HIP_TRY(&c, upload(c.d_vertices, scene->vertices, scene->vertex_count * 32, st), "vertices");
+             HIP_TRY(&c, upload(c.d_indices, scene->indices, scene->index_count * 4, st), "indices");
+             HIP_TRY(&c, upload(c.d_instances, scene->instances, size_t(scene->instance_count) * 128, st), "instances");
+             HIP_TRY(&c, upload(c.d_materials, scene->materials, size_t(scene->material_count) * 96, st), "materials");
+             HIP_TRY(&c, upload(c.d_light, scene->light_words, scene->light_word_count * 4, st), "light sampler");
+             HIP_TRY(&c, upload(c.d_meshes, scene->meshes, size_t(scene->mesh_count) * sizeof(pt_mesh_desc), st),
+                     "meshes");
+             HIP_TRY(&c, upload(c.d_instance_mesh, scene->instance_mesh, size_t(scene->instance_count) * 4, st),
+                     "instance meshes");
+             // Opacity per instance: transparent materials (constant base-colour alpha) let
+             // shadow rays through with transmittance 1 - alpha (shadowray.ah.hlsl:12-40).
+             std::vector<float> opacity(std::max<uint32_t>(scene->instance_count, 1), 1.0f);
+             const auto* inst = static_cast<const InstanceData*>(scene->instances);
+             const auto* mats = static_cast<const MaterialRec*>(scene->materials);
+             for (uint32_t i = 0; i < scene->instance_count; i++) {
+                 const MaterialRec& m = mats[inst[i].material_index];
+                 const uint32_t bits = float_as_u32(m.rmnb.w);
+                 if (bits & (1u << 13))
+                     opacity[i] = m.base_colour.w;
+             }
+             HIP_TRY(&c, upload(c.d_inst_opacity, opacity.data(), opacity.size() * 4, st), "opacity");
+             c.instance_count = scene->instance_count;
+
+             std::vector<pt_mesh_desc> meshes(scene->meshes, scene->meshes + scene->mesh_count);
+             HIP_TRY(&c, hipEventRecord(c.ev_start, st), "event");
+             BvhBuildParams bp;
+             if (const char* env = getenv("PT_BVH_LEAF"))
+                 bp.blas_leaf_size = uint32_t(atoi(env));
+             HIP_TRY(&c,
+                     build_bvh(st, c.d_vertices, c.d_indices, c.d_meshes, meshes, c.d_instances, c.d_instance_mesh,
+                               scene->instance_count, c.d_materials, bp, c.bvh),
+                     "BVH build");
+             HIP_TRY(&c, hipEventRecord(c.ev_stop, st), "event");
+             HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
+             float ms = 0.0f;
+             HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
+             c.bvh_build_ms = ms;
+             {
+                 std::string why;
+                 if (int v = validate_bvh(c, meshes, why))
+                     return fail(&c, v, "BVH validation: " + why);
+             }
+
+             DeviceScene& s = c.scene;
+             s.vertices = c.d_vertices;
+             s.indices = c.d_indices;
+             s.instances = c.d_instances;
+             s.materials = c.d_materials;
+             s.light = c.d_light;
+             s.light_count = scene->light_count;
+             s.instance_count = scene->instance_count;
+             s.nodes = c.bvh.nodes;
+             s.tris = c.bvh.tris;
+             s.inst_inv = c.bvh.inst_inv;
+             s.inst_blas_root = c.bvh.inst_blas_root;
+             s.inst_opacity = c.d_inst_opacity;
+             s.tlas_root = c.bvh.tlas_root;
+             s.envcol = make_float3(scene->envcol[0], scene->envcol[1], scene->envcol[2]);
+
+             c.width = width;
+             c.height = height;
+             HIP_TRY(&c, hipMalloc(&c.accum, size_t(width) * height * 16), "accumulation image");
+             HIP_TRY(&c, hipMemsetAsync(c.accum, 0, size_t(width) * height * 16, st), "accumulation image");
+
+             hipDeviceProp_t prop;
+             HIP_TRY(&c, hipGetDeviceProperties(&prop, c.device), "device properties");
+             const uint32_t cus = std::max(1, prop.multiProcessorCount);
+             c.grid_blocks = ((cus * 4 + 7) / 8) * 8;
+             const uint32_t npix = width * height;
+             uint32_t per_batch = std::max<uint32_t>(1, std::min<uint32_t>(64, (4u << 20) / npix));
+             if (const char* env = getenv("PT_BATCH_SPP"))
+                 per_batch = std::max(1, atoi(env));
+             HIP_TRY(&c, wavefront_alloc(c.wf, npix * per_batch, c.grid_blocks * TRACE_BLOCK), "wavefront buffers");
+             HIP_TRY(&c, hipStreamSynchronize(st), "upload");
+             return PT_OK;
+         }()) != PT_OK)
+        return bail(code);
+    *out = h;
+    return PT_OK;
+}
+
+int pt_set_camera(pt_ctx* h, const pt_camera* cam)
+{
+    if (!h)
+        return PT_EINVAL;
+    if (!cam)
+        return fail(&h->c, PT_EINVAL, "camera is null");
+    static_assert(sizeof(CameraBlock) == sizeof(pt_camera), "camera block layout");
+    memcpy(&h->c.cam, cam, sizeof(CameraBlock));
+    h->c.have_camera = true;
+    h->c.sample_index = 0;
+    return PT_OK;
+}
+
+int pt_set_max_bounces(pt_ctx* h, uint32_t max_bounces)
+{
+    if (!h)
+        return PT_EINVAL;
+    if (max_bounces > 1024)
+        return fail(&h->c, PT_EINVAL, "max_bounces too large");
+    h->c.max_bounces = max_bounces;
+    h->c.sample_index = 0;
+    return PT_OK;
+}
+
+int pt_render(pt_ctx* h, uint32_t spp)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (spp == 0)
+        return fail(&c, PT_EINVAL, "Sample count was zero.");
+    if (!c.have_camera)
+        return fail(&c, PT_ESTATE, "no camera set");
+    if (uint64_t(c.sample_index) + spp > 0xFFFFFFFFull)
+        return fail(&c, PT_EINVAL, "sample index overflow");
+    HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
+    HIP_TRY(&c, render_samples(c, spp), "render");
+    HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
+    c.sample_index += spp;
+    c.last_paths = uint64_t(spp) * c.width * c.height;
+    return PT_OK;
+}
+
+int pt_read_accumulation(pt_ctx* h, float* rgba, size_t row_pitch)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!rgba || row_pitch < size_t(c.width) * 16)
+        return fail(&c, PT_EINVAL, "bad output buffer");
+    HIP_TRY(&c,
+            hipMemcpy2DAsync(rgba, row_pitch, c.accum, size_t(c.width) * 16, size_t(c.width) * 16, c.height,
+                             hipMemcpyDeviceToHost, c.stream),
+            "readback");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "readback");
+    return PT_OK;
+}
+
+int pt_trace_primary(pt_ctx* h, uint32_t sample_index, pt_hit* hits)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!hits)
+        return fail(&c, PT_EINVAL, "hits is null");
+    if (!c.have_camera)
+        return fail(&c, PT_ESTATE, "no camera set");
+    const size_t n = size_t(c.width) * c.height;
+    pt_hit* d = nullptr;
+    HIP_TRY(&c, hipMalloc(&d, n * sizeof(pt_hit)), "hits");
+    hipError_t e = primary_hits(c, sample_index, d);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hits, d, n * sizeof(pt_hit), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c.stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return hip_fail(&c, e, "primary hits");
+    return PT_OK;
+}
+
+int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (count == 0)
+        return PT_OK;
+    if (!rays || !hits)
+        return fail(&c, PT_EINVAL, "null ray or hit array");
+    pt_ray* dr = nullptr;
+    pt_hit* dh = nullptr;
+    hipError_t e = hipMalloc(&dr, size_t(count) * sizeof(pt_ray));
+    if (e == hipSuccess)
+        e = hipMalloc(&dh, size_t(count) * sizeof(pt_hit));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dr, rays, size_t(count) * sizeof(pt_ray), hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess)
+        e = trace_rays(c, dr, count, dh, any_hit != 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hits, dh, size_t(count) * sizeof(pt_hit), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c.stream);
+    if (dr)
+        (void)hipFree(dr);
+    if (dh)
+        (void)hipFree(dh);
+    if (e != hipSuccess)
+        return hip_fail(&c, e, "trace rays");
+    return PT_OK;
+}
+
+int pt_get_stats(pt_ctx* h, pt_stats* out)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!out)
+        return fail(&c, PT_EINVAL, "out is null");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    unsigned long long totals[2] = {0, 0};
+    uint32_t counters[4] = {0, 0, 0, 0};
+    HIP_TRY(&c, hipMemcpy(totals, c.wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
+    HIP_TRY(&c, hipMemcpy(counters, c.wf.counters, sizeof(counters), hipMemcpyDeviceToHost), "stats");
+    float ms = 0.0f;
+    if (c.last_paths)
+        HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
+    pt_stats s{};
+    s.extension_rays = totals[0];
+    s.shadow_rays = totals[1];
+    s.paths = c.last_paths;
+    s.render_ms = ms;
+    s.bvh_build_ms = c.bvh_build_ms;
+    s.bvh_bytes = uint64_t(c.bvh.node_count) * sizeof(Node4) + uint64_t(c.bvh.tri_count) * sizeof(TriRec);
+    s.sample_index = c.sample_index;
+    s.max_bounces = c.max_bounces;
+    s.width = c.width;
+    s.height = c.height;
+    s.node_count = c.bvh.node_count;
+    s.tri_record_count = c.bvh.tri_count;
+    *out = s;
+    if (counters[3])
+        return fail(&c, PT_EDEVICE, "traversal stack overflow");
+    return PT_OK;
+}
+
+int pt_synchronize(pt_ctx* h)
+{
+    if (!h)
+        return PT_EINVAL;
+    HIP_TRY(&h->c, hipStreamSynchronize(h->c.stream), "sync");
+    return PT_OK;
+}
+
+const char* pt_last_error(const pt_ctx* h) { return h ? h->c.error.c_str() : g_create_error.c_str(); }
+
+void pt_destroy(pt_ctx* h)
+{
+    if (!h)
+        return;
+    (void)hipStreamSynchronize(h->c.stream);
+    release(h->c);
+    delete h;
+}
+
+}  // extern "C"

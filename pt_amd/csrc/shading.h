@@ -1,0 +1,388 @@
+// Device-side sampler, camera, BSDF and light sampling of the path tracer.
+//
+// Semantics follow the reference shaders line by line (cited per function) so that, with
+// the same seeds, a path makes the same decisions as the reference's megakernel. The
+// arithmetic is fp32 as in the shaders.
+#pragma once
+
+#include "common.h"
+
+namespace ptk {
+
+constexpr float PI_F = 3.141592654f;  // util.hlsli:4
+constexpr float ORIGIN_OFFSET = 1e-5f; // pathtrace.hlsli:6
+__device__ __forceinline__ float just_below_one() { return u32_as_float(0x3F7FFFFFu); }
+
+// --- hashing (util.hlsli:8-27) ------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash_combine(uint32_t seed, uint32_t x)
+{
+    return seed ^ (x + (seed << 6) + (seed >> 2));
+}
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+// --- Sobol + hash-based Owen scrambling (sample.hlsli:94-166) -----------------------------
+
+// Direction numbers of Sobol dimensions 0-3 (primitive polynomials 1, x+1, x^2+x+1,
+// x^3+x+1 with initial numbers 1 | 1,3 | 1,3,1), filled in by the host at library load.
+extern __constant__ uint32_t c_sobol[4][32];
+
+__device__ __forceinline__ uint32_t lk_permutation(uint32_t x, uint32_t seed)
+{
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint32_t seed)
+{
+    return __builtin_bitreverse32(lk_permutation(__builtin_bitreverse32(x), seed));
+}
+
+__device__ __forceinline__ uint32_t sobol(uint32_t index, uint32_t dim)
+{
+    if (dim == 0)
+        return __builtin_bitreverse32(index);
+    uint32_t x = 0;
+#pragma unroll 8
+    for (int b = 0; b < 32; b++)
+        x ^= ((index >> b) & 1u) ? c_sobol[dim][b] : 0u;
+    return x;
+}
+
+struct Sampler {
+    uint32_t index, seed, dimension, shuffled;
+
+    __device__ __forceinline__ void start(uint32_t px, uint32_t py, uint32_t sample_index, uint32_t s)
+    {
+        seed = hash_combine(s, hash_combine(px, py));
+        index = sample_index;
+        dimension = 4;
+        shuffled = 0;
+    }
+    __device__ __forceinline__ void pad()
+    {
+        dimension = 0;
+        seed = hash_u32(seed);
+        shuffled = owen_scramble(index, seed);
+    }
+    __device__ __forceinline__ float value(uint32_t d) const
+    {
+        return float(owen_scramble(sobol(shuffled, d), hash_combine(seed, d))) * 0x1p-32f;
+    }
+    __device__ __forceinline__ float s1()
+    {
+        if (dimension >= 4)
+            pad();
+        const float r = value(dimension);
+        dimension += 1;
+        return fminf(r, just_below_one());
+    }
+    __device__ __forceinline__ float2 s2()
+    {
+        if (dimension + 1 >= 4)
+            pad();
+        const float a = value(dimension), b = value(dimension + 1);
+        dimension += 2;
+        return make_float2(fminf(a, just_below_one()), fminf(b, just_below_one()));
+    }
+};
+
+// --- sampling helpers (sample.hlsli:193-246) ----------------------------------------------
+
+__device__ __forceinline__ float3 cosine_weighted_rand_dir(float2 u)
+{
+    const float r = sqrtf(u.x);
+    return f3(r * cosf(2.0f * PI_F * u.y), r * sinf(2.0f * PI_F * u.y), sqrtf(1.0f - u.x));
+}
+
+__device__ __forceinline__ float3 random_barycentric(float2 u)
+{
+    const float beta = 1.0f - sqrtf(u.x);
+    const float gamma = (1.0f - beta) * u.y;
+    return f3(1.0f - beta - gamma, beta, gamma);
+}
+
+__device__ __forceinline__ float2 sample_concentric_disk(float2 u)
+{
+    const float a = 2.0f * u.x - 1.0f;
+    const float b = 2.0f * u.y - 1.0f;
+    float r, phi;
+    if (a * a > b * b) {
+        r = a;
+        phi = (PI_F / 4.0f) * (b / a);
+    } else {
+        r = b;
+        phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (a / b);
+    }
+    return make_float2(r * cosf(phi), r * sinf(phi));
+}
+
+__device__ __forceinline__ float3 sample_visible_micronormal(float3 v, float2 alpha, float2 u)
+{
+    v = normalize3(f3(alpha.x * v.x, alpha.y * v.y, v.z));
+    const float3 x = (v.z < 0.99999f) ? normalize3(cross3(f3(0, 0, 1), v)) : f3(1, 0, 0);
+    const float3 y = cross3(v, x);
+    const float a = 1.0f / (1.0f + v.z);
+    const float r = sqrtf(u.x);
+    const float phi = (u.y < a) ? u.y / a * PI_F : PI_F + (u.y - a) / (1.0f - a) * PI_F;
+    const float p1 = r * cosf(phi);
+    const float p2 = r * sinf(phi) * ((u.y < a) ? 1.0f : v.z);
+    const float3 m = p1 * x + p2 * y + sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2)) * v;
+    return normalize3(f3(alpha.x * m.x, alpha.y * m.y, fmaxf(1e-6f, m.z)));
+}
+
+// --- camera (sample.hlsli:258-280) --------------------------------------------------------
+
+struct CameraBlock {
+    float inv_view[16];
+    float inv_proj[16];
+    float near_clip, far_clip, focus_dist, lens_radius, exposure;
+};
+
+__device__ __forceinline__ void camera_ray(const CameraBlock& c, uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                                           float2 u_pixel, float2 u_lens, float3& origin, float3& dir)
+{
+    const float nx = ((float(px) + u_pixel.x) / float(w)) * 2.0f - 1.0f;
+    const float ny = ((float(py) + u_pixel.y) / float(h)) * 2.0f - 1.0f;
+    const float* P = c.inv_proj;
+    float3 rd = f3(P[0] * nx + P[1] * ny + P[2] * 1.0f + P[3] * 1.0f, P[4] * nx + P[5] * ny + P[6] * 1.0f + P[7] * 1.0f,
+                   P[8] * nx + P[9] * ny + P[10] * 1.0f + P[11] * 1.0f);
+    float3 ro = f3(0, 0, 0);
+    if (c.lens_radius > 0.0f) {
+        const float2 disk = sample_concentric_disk(u_lens);
+        const float lx = c.lens_radius * disk.x, ly = c.lens_radius * disk.y;
+        const float t_focus = c.focus_dist / -rd.z;
+        const float3 fp = ro + t_focus * rd;
+        ro = f3(lx, ly, 0.0f);
+        rd = fp - ro;
+    }
+    const float* V = c.inv_view;
+    origin = f3(V[0] * ro.x + V[1] * ro.y + V[2] * ro.z + V[3], V[4] * ro.x + V[5] * ro.y + V[6] * ro.z + V[7],
+                V[8] * ro.x + V[9] * ro.y + V[10] * ro.z + V[11]);
+    dir = normalize3(f3(V[0] * rd.x + V[1] * rd.y + V[2] * rd.z, V[4] * rd.x + V[5] * rd.y + V[6] * rd.z,
+                        V[8] * rd.x + V[9] * rd.y + V[10] * rd.z));
+}
+
+// --- BSDF (pathtrace.ch.hlsl:22-157) ------------------------------------------------------
+
+struct BrdfInput {
+    float3 base_colour;
+    float2 alpha;
+    float metalness;
+    float3 specular;
+    float3 sheen;
+};
+
+__device__ __forceinline__ float sq(float x) { return x * x; }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ float sin2_theta(float3 v) { return fmaxf(0.0f, 1.0f - sq(v.z)); }
+
+__device__ __forceinline__ float lambda_fn(float3 m, float2 alpha)
+{
+    const float s2 = sin2_theta(m);
+    const float tan2 = s2 / sq(m.z);
+    if (isinf(tan2))
+        return 0.0f;
+    const float s = sqrtf(s2);
+    const float cp = s == 0.0f ? 1.0f : fminf(fmaxf(m.x / s, -1.0f), 1.0f);
+    const float sp = s == 0.0f ? 0.0f : fminf(fmaxf(m.y / s, -1.0f), 1.0f);
+    const float a2 = sq(cp * alpha.x) + sq(sp * alpha.y);
+    return (sqrtf(1.0f + a2 * tan2) - 1.0f) / 2.0f;
+}
+
+__device__ __forceinline__ float distribution(float3 m, float2 alpha)
+{
+    const float s2 = sin2_theta(m);
+    const float c2 = sq(m.z);
+    const float tan2 = s2 / c2;
+    if (isinf(tan2))
+        return 0.0f;
+    const float cos4 = sq(c2);
+    const float s = sqrtf(s2);
+    const float cp = s == 0.0f ? 1.0f : fminf(fmaxf(m.x / s, -1.0f), 1.0f);
+    const float sp = s == 0.0f ? 0.0f : fminf(fmaxf(m.y / s, -1.0f), 1.0f);
+    const float e = tan2 * (sq(cp / alpha.x) + sq(sp / alpha.y));
+    return 1.0f / (PI_F * alpha.x * alpha.y * cos4 * sq(1.0f + e));
+}
+
+__device__ __forceinline__ float diffuse_sample_probability(float metalness) { return lerpf(0.5f, 0.0f, metalness); }
+
+// Clearcoat is computed by the shader but never added (ch.hlsl:93-114), so it is omitted.
+__device__ __forceinline__ float3 evaluate_brdf(const BrdfInput& in, float3 i, float3 o, float3 m, float& pdf)
+{
+    const float3 f0 = f3(lerpf(in.specular.x * 0.08f, in.base_colour.x, in.metalness),
+                         lerpf(in.specular.y * 0.08f, in.base_colour.y, in.metalness),
+                         lerpf(in.specular.z * 0.08f, in.base_colour.z, in.metalness));
+    const float cos_d = dot3(m, o);
+    const float base = 1.0f - cos_d;
+    const float f = base * base * base * base * base;
+    const float lo = lambda_fn(o, in.alpha);
+    const float g = 1.0f / (1.0f + lo + lambda_fn(i, in.alpha));
+    const float d = distribution(m, in.alpha);
+    const float3 fr = f3(lerpf(f0.x, 1.0f, f), lerpf(f0.y, 1.0f, f), lerpf(f0.z, 1.0f, f));
+    const float denom = 4.0f * i.z * o.z;
+    const float3 spec = f3(d * fr.x * g, d * fr.y * g, d * fr.z * g) / denom;
+    const float3 sheen = in.sheen * f;
+    const float3 diffuse = (in.base_colour + sheen) * (1.0f - in.metalness) / PI_F * (1.0f - f);
+    const float diffuse_prob = diffuse_sample_probability(in.metalness);
+    const float pdf_diffuse = i.z / PI_F;
+    const float g1o = 1.0f / (1.0f + lo);
+    const float pdf_m = (g1o / fabsf(o.z)) * d * fabsf(cos_d);
+    const float pdf_gtr = pdf_m / (4.0f * fabsf(cos_d));
+    pdf = pdf_diffuse * diffuse_prob + pdf_gtr * (1.0f - diffuse_prob);
+    return spec + diffuse;
+}
+
+__device__ __forceinline__ float3 sample_direction(float3 o, float2 alpha, float metalness, Sampler& smp, float3& m,
+                                                   float& w, float& pdf)
+{
+    const float diffuse_prob = diffuse_sample_probability(metalness);
+    const bool sample_diffuse = smp.s1() < diffuse_prob;
+    const float2 u = smp.s2();
+    float3 i;
+    if (sample_diffuse) {
+        i = cosine_weighted_rand_dir(u);
+        m = normalize3(o + i);
+    } else {
+        m = sample_visible_micronormal(o, alpha, u);
+        i = 2.0f * dot3(o, m) * m - o;
+    }
+    float pdf_diffuse = i.z / PI_F;
+    const float cos_d = dot3(m, o);
+    const float g1o = 1.0f / (1.0f + lambda_fn(o, alpha));
+    const float pdf_m = (g1o / fabsf(o.z)) * distribution(m, alpha) * fabsf(cos_d);
+    float pdf_gtr = pdf_m / (4.0f * fabsf(cos_d));
+    pdf_diffuse *= diffuse_prob;
+    pdf_gtr *= (1.0f - diffuse_prob);
+    pdf = sample_diffuse ? pdf_diffuse : pdf_gtr;
+    w = pdf / (pdf_diffuse + pdf_gtr);
+    return i;
+}
+
+// --- light sampling (lightsample.hlsli:7-143) ---------------------------------------------
+
+__device__ __forceinline__ uint32_t sample_alias_table(const uint32_t* buf, uint32_t bin_count, uint32_t bin_size,
+                                                       uint32_t table_offset, float& u, float& p)
+{
+    uint32_t i = min(bin_count - 1, uint32_t(u * float(bin_count)));
+    uint32_t bo = table_offset + i * bin_size;
+    u = fminf((u * float(bin_count)) - float(i), just_below_one());
+    const float q = u32_as_float(buf[bo + 1]);
+    if (u < q) {
+        u = fminf(u / q, just_below_one());
+    } else {
+        i = buf[bo + 2];
+        bo = table_offset + i * bin_size;
+        u = fminf((u - q) / (1.0f - q), just_below_one());
+    }
+    p = u32_as_float(buf[bo + 0]);
+    return bo + 3;
+}
+
+__device__ __forceinline__ float light_pdf(const uint32_t* light, uint32_t emitter_index, uint32_t prim)
+{
+    if (emitter_index == PT_NO_EMITTER)
+        return 0.0f;
+    uint32_t bo = emitter_index * PT_LIGHT_BIN_WORDS;
+    float p = u32_as_float(light[bo + 0]);
+    if (light[bo + 3] == 3u) {
+        const uint32_t table_offset = light[bo + 4];
+        bo = table_offset + prim * PT_EMITTER_BIN_WORDS;
+        p *= u32_as_float(light[bo + 0]);
+    }
+    return p;
+}
+
+__device__ __forceinline__ float3 xform_point(const InstanceData& in, float3 p)
+{
+    return f3(in.m[0][0] * p.x + in.m[0][1] * p.y + in.m[0][2] * p.z + in.m[0][3],
+              in.m[1][0] * p.x + in.m[1][1] * p.y + in.m[1][2] * p.z + in.m[1][3],
+              in.m[2][0] * p.x + in.m[2][1] * p.y + in.m[2][2] * p.z + in.m[2][3]);
+}
+
+__device__ __forceinline__ float3 xform_vec(const InstanceData& in, float3 v)
+{
+    return f3(in.m[0][0] * v.x + in.m[0][1] * v.y + in.m[0][2] * v.z, in.m[1][0] * v.x + in.m[1][1] * v.y + in.m[1][2] * v.z,
+              in.m[2][0] * v.x + in.m[2][1] * v.y + in.m[2][2] * v.z);
+}
+
+__device__ __forceinline__ float3 xform_normal(const InstanceData& in, float3 v)
+{
+    return f3(in.n[0][0] * v.x + in.n[0][1] * v.y + in.n[0][2] * v.z, in.n[1][0] * v.x + in.n[1][1] * v.y + in.n[1][2] * v.z,
+              in.n[2][0] * v.x + in.n[2][1] * v.y + in.n[2][2] * v.z);
+}
+
+__device__ __forceinline__ float3 vertex_pos(const DeviceScene& s, uint32_t v)
+{
+    return xyz(s.vertices[size_t(v) * 2]);
+}
+
+__device__ __forceinline__ float3 emission_of(const MaterialRec& m)
+{
+    return f3(m.emission.x, m.emission.y, m.emission.z) * m.emission.w;
+}
+
+__device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, float3 pt, float3& light_dir,
+                                               float& light_dist, float& pdf, bool& is_delta)
+{
+    uint32_t bo = sample_alias_table(s.light, s.light_count, PT_LIGHT_BIN_WORDS, 0, u.x, pdf);
+    const uint32_t type = s.light[bo + 0];
+    if (type == 3u) {
+        const uint32_t table_offset = s.light[bo + 1];
+        const uint32_t tri_count = s.light[bo + 2];
+        const uint32_t inst_off = s.light[bo + 3];
+        const uint32_t inst_count = s.light[bo + 4];
+        float p;
+        bo = sample_alias_table(s.light, tri_count, PT_EMITTER_BIN_WORDS, table_offset, u.x, p);
+        pdf *= p;
+        const uint32_t inst = inst_off + min(inst_count - 1, uint32_t(u.x * float(inst_count)));
+        const InstanceData& in = s.instances[inst];
+        const float3 pa = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 0]));
+        const float3 pb = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 1]));
+        const float3 pc = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 2]));
+        const float3 bary = random_barycentric(make_float2(u.y, u.z));
+        const float3 on_tri = bary.x * pa + bary.y * pb + bary.z * pc;
+        light_dir = on_tri - pt;
+        light_dist = 1.0f;
+        const float3 tn = cross3(pb - pa, pc - pa);
+        const float area = sqrtf(dot3(tn, tn)) * 0.5f;
+        const float cosine = fabsf(dot3(normalize3(light_dir), normalize3(tn)));
+        const float dist2 = dot3(light_dir, light_dir);
+        pdf *= dist2 / (area * cosine);
+        is_delta = false;
+        if (!isfinite(pdf))
+            return f3(0, 0, 0);
+        return emission_of(s.materials[in.material_index]);
+    } else if (type == 1u) {
+        const float3 lp = f3(u32_as_float(s.light[bo + 1]), u32_as_float(s.light[bo + 2]), u32_as_float(s.light[bo + 3]));
+        light_dir = lp - pt;
+        light_dist = 1.0f;
+        is_delta = true;
+        return (1.0f / dot3(light_dir, light_dir)) *
+               f3(u32_as_float(s.light[bo + 4]), u32_as_float(s.light[bo + 5]), u32_as_float(s.light[bo + 6]));
+    } else if (type == 2u) {
+        light_dir = f3(u32_as_float(s.light[bo + 1]), u32_as_float(s.light[bo + 2]), u32_as_float(s.light[bo + 3]));
+        light_dist = INFINITY;
+        is_delta = true;
+        return f3(u32_as_float(s.light[bo + 4]), u32_as_float(s.light[bo + 5]), u32_as_float(s.light[bo + 6]));
+    }
+    light_dist = 0.0f;
+    is_delta = true;
+    return f3(0, 0, 0);
+}
+
+__device__ __forceinline__ float correct_component(float x) { return (x < 0.0f || !isfinite(x)) ? 0.0f : x; }
+
+}  // namespace ptk

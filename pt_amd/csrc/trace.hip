@@ -1,0 +1,800 @@
+// Traversal and the wavefront stages of the path tracer.
+//
+// One batch of paths runs as: generate (camera rays, pathtrace.rg.hlsl:16-35) -> per
+// segment: extend (closest hit of every queued ray) -> shade (miss / closest-hit logic of
+// pathtrace.ms.hlsl and pathtrace.ch.hlsl, Russian roulette of pathtrace.rg.hlsl:53, and
+// compaction of continuing paths and shadow rays into queues) -> shadow (any hit, adds
+// the unoccluded next-event contribution) -> accumulate (running mean per pixel in sample
+// order, pathtrace.rg.hlsl:61-67).
+//
+// Traversal: two-level 4-wide BVH, one lane per ray, near-to-far child order, triangle
+// leaves tested on the spot, a short per-lane stack in LDS that spills its oldest entries to
+// global memory, and the watertight ray/triangle test of Woop, Benthin and Wald (2013).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "device.h"
+
+namespace ptk {
+
+__constant__ uint32_t c_sobol[4][32];
+
+hipError_t upload_sobol_table()
+{
+    // Joe-Kuo direction numbers for dimensions 0-3 (polynomials 1, x+1, x^2+x+1, x^3+x+1).
+    uint32_t v[4][32];
+    for (int b = 0; b < 32; b++)
+        v[0][b] = 1u << (31 - b);
+    const int deg[3] = {1, 2, 3};
+    const uint32_t poly[3] = {0, 1, 1};
+    const uint32_t minit[3][3] = {{1, 0, 0}, {1, 3, 0}, {1, 3, 1}};
+    for (int d = 1; d < 4; d++) {
+        const int s = deg[d - 1];
+        uint32_t m[32];
+        for (int k = 0; k < s; k++)
+            m[k] = minit[d - 1][k];
+        for (int k = s; k < 32; k++) {
+            uint32_t val = m[k - s] ^ (m[k - s] << s);
+            for (int j = 1; j < s; j++)
+                if ((poly[d - 1] >> (s - 1 - j)) & 1u)
+                    val ^= m[k - j] << j;
+            m[k] = val;
+        }
+        for (int k = 0; k < 32; k++)
+            v[d][k] = m[k] << (31 - k);
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), v, sizeof(v));
+}
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+// --- traversal -----------------------------------------------------------------------------
+
+struct RayPre {
+    float3 o, d, inv;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
+{
+    r.o = o;
+    r.d = d;
+    const float ex = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+    const float ey = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+    const float ez = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+    r.inv = f3(1.0f / ex, 1.0f / ey, 1.0f / ez);
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = 0;
+    if (ay > ax)
+        kz = 1;
+    if (az > fmaxf(ax, ay))
+        kz = 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (comp(d, kz) < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    const float dz = comp(d, kz);
+    r.sx = comp(d, kx) / dz;
+    r.sy = comp(d, ky) / dz;
+    r.sz = 1.0f / dz;
+}
+
+// Watertight test; on a hit in [tmin, tmax] returns t and the barycentric weights of v1, v2.
+__device__ __forceinline__ bool intersect_tri(const RayPre& r, float3 v0, float3 v1, float3 v2, float tmin, float tmax,
+                                              float& t, float& b1, float& b2)
+{
+    const float3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    const float Ax = comp(A, r.kx) - r.sx * Akz, Ay = comp(A, r.ky) - r.sy * Akz;
+    const float Bx = comp(B, r.kx) - r.sx * Bkz, By = comp(B, r.ky) - r.sy * Bkz;
+    const float Cx = comp(C, r.kx) - r.sx * Ckz, Cy = comp(C, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {  // edge case: recompute the edge functions in fp64
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+        return false;
+    const float det = U + V + W;
+    if (det == 0.0f)
+        return false;
+    const float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
+    const float rcp = 1.0f / det;
+    t = T * rcp;
+    if (!(t >= tmin && t < tmax))
+        return false;
+    b1 = V * rcp;
+    b2 = W * rcp;
+    return true;
+}
+
+struct Stack {
+    uint32_t* lds;    // this lane's column: entry k at lds[k * TRACE_BLOCK]
+    uint32_t* spill;  // this thread's spill area: entry k at spill[k * spill_stride]
+    uint32_t spill_stride;
+    uint32_t sp;
+    bool overflow;
+
+    __device__ __forceinline__ void push(uint32_t v)
+    {
+        if (sp >= LDS_STACK) {
+            const uint32_t deep = sp - LDS_STACK;
+            if (deep >= SPILL_STACK) {
+                overflow = true;
+                return;
+            }
+            spill[size_t(deep) * spill_stride] = lds[(sp & (LDS_STACK - 1)) * TRACE_BLOCK];
+        }
+        lds[(sp & (LDS_STACK - 1)) * TRACE_BLOCK] = v;
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        sp--;
+        const uint32_t slot = (sp & (LDS_STACK - 1)) * TRACE_BLOCK;
+        const uint32_t v = lds[slot];
+        if (sp >= LDS_STACK)
+            lds[slot] = spill[size_t(sp - LDS_STACK) * spill_stride];
+        return v;
+    }
+};
+
+struct HitOut {
+    float t, u, v;
+    uint32_t inst, prim;
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
+// transmittance through transparent instances is returned in trans (shadowray.ah.hlsl).
+template <bool ANY>
+__device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stack& stk, HitOut& hit,
+                         float& trans)
+{
+    hit.inst = NONE;
+    trans = 1.0f;
+    if (s.tlas_root == CHILD_EMPTY)
+        return false;
+    float best = tmax;
+    RayPre r;
+    precompute(r, wo, wd);
+    uint32_t inst = NONE;
+    uint32_t blas_sp = NONE;
+    float opacity = 1.0f;
+    uint32_t node = s.tlas_root;
+    stk.sp = 0;
+    const float cons = 1.0000004f;  // conservative slab far distance
+
+    while (true) {
+        // ---- visit an internal node
+        const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
+        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+        const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
+        const float4 nx = r.inv.x >= 0.0f ? lox : hix, fx = r.inv.x >= 0.0f ? hix : lox;
+        const float4 ny = r.inv.y >= 0.0f ? loy : hiy, fy = r.inv.y >= 0.0f ? hiy : loy;
+        const float4 nz = r.inv.z >= 0.0f ? loz : hiz, fz = r.inv.z >= 0.0f ? hiz : loz;
+        float tn[4];
+        uint32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+        const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+        const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+        const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float t0 = fmaxf(fmaxf((nxa[k] - r.o.x) * r.inv.x, (nya[k] - r.o.y) * r.inv.y),
+                                   fmaxf((nza[k] - r.o.z) * r.inv.z, tmin));
+            const float t1 = fminf(fminf((fxa[k] - r.o.x) * r.inv.x, (fya[k] - r.o.y) * r.inv.y),
+                                   fminf((fza[k] - r.o.z) * r.inv.z, best)) * cons;
+            const bool h = cc[k] != CHILD_EMPTY && t0 <= t1;
+            tn[k] = h ? t0 : INFINITY;
+            if (!h)
+                cc[k] = NONE;
+        }
+        // sort the (up to) four hits near to far
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3 - a; b++)
+                if (tn[b + 1] < tn[b]) {
+                    const float tt = tn[b];
+                    tn[b] = tn[b + 1];
+                    tn[b + 1] = tt;
+                    const uint32_t cu = cc[b];
+                    cc[b] = cc[b + 1];
+                    cc[b + 1] = cu;
+                }
+        uint32_t next = NONE;
+        uint32_t deferred[3];
+        int nd = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t c = cc[k];
+            if (c == NONE || tn[k] > best)
+                continue;
+            if ((c & CHILD_LEAF) && inst != NONE) {
+                // triangle leaf of the current BLAS
+                const uint32_t first = c & LEAF_FIRST_MASK;
+                const uint32_t count = ((c >> 27) & 0xFu) + 1u;
+                for (uint32_t q = 0; q < count; q++) {
+                    const TriRec tr = s.tris[first + q];
+                    float t, b1, b2;
+                    if (!intersect_tri(r, xyz(tr.v0), xyz(tr.v1), xyz(tr.v2), tmin, best, t, b1, b2))
+                        continue;
+                    if (ANY) {
+                        if (opacity < 1.0f) {
+                            trans *= (1.0f - opacity);
+                            if (trans < 1e-5f) {
+                                hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
+                                return true;
+                            }
+                            continue;
+                        }
+                        hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
+                        return true;
+                    }
+                    best = t;
+                    hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
+                }
+            } else if (next == NONE) {
+                next = c;
+            } else {
+                deferred[nd++] = c;
+            }
+        }
+        for (int k = nd - 1; k >= 0; k--)
+            stk.push(deferred[k]);
+
+        // ---- choose the next internal node (entering instances as needed)
+        while (true) {
+            if (next == NONE) {
+                if (stk.sp == blas_sp) {  // the current BLAS is finished: back to world space
+                    inst = NONE;
+                    blas_sp = NONE;
+                    precompute(r, wo, wd);
+                }
+                if (stk.sp == 0)
+                    return hit.inst != NONE;
+                next = stk.pop();
+            }
+            if ((next & CHILD_LEAF) && inst == NONE) {  // TLAS leaf: enter the instance
+                const uint32_t id = next & ~CHILD_LEAF;
+                const uint32_t root = s.inst_blas_root[id];
+                next = NONE;
+                if (root == CHILD_EMPTY)
+                    continue;
+                const InstXform x = s.inst_inv[id];
+                const float3 oo = f3(x.r0.x * wo.x + x.r0.y * wo.y + x.r0.z * wo.z + x.r0.w,
+                                     x.r1.x * wo.x + x.r1.y * wo.y + x.r1.z * wo.z + x.r1.w,
+                                     x.r2.x * wo.x + x.r2.y * wo.y + x.r2.z * wo.z + x.r2.w);
+                const float3 od = f3(x.r0.x * wd.x + x.r0.y * wd.y + x.r0.z * wd.z,
+                                     x.r1.x * wd.x + x.r1.y * wd.y + x.r1.z * wd.z,
+                                     x.r2.x * wd.x + x.r2.y * wd.y + x.r2.z * wd.z);
+                precompute(r, oo, od);
+                inst = id;
+                blas_sp = stk.sp;
+                if (ANY)
+                    opacity = s.inst_opacity[id];
+                // Roots are single nodes; continue with the BLAS root as an internal node.
+                next = root;
+            }
+            break;
+        }
+        node = next;
+    }
+}
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid)
+{
+    // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous share of the
+    // logical blocks so neighbouring rays share an L2.
+    const uint32_t per = grid / 8u;
+    if (per == 0 || grid % 8u != 0)
+        return b;
+    return (b % 8u) * per + b / 8u;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends `want` lanes of the wave to a queue with one atomic per wave; returns the slot.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
+{
+    const unsigned long long mask = __ballot(want);
+    const uint32_t lane = lane_id();
+    const uint32_t n = __popcll(mask);
+    uint32_t base = 0;
+    if (n) {
+        const uint32_t leader = __ffsll((long long)mask) - 1;
+        if (lane == leader)
+            base = atomicAdd(counter, n);
+        base = __shfl(base, leader);
+    }
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// --- kernels -------------------------------------------------------------------------------
+
+struct BatchParams {
+    uint32_t width, height, npix;
+    uint32_t sample_base;   // sample index of batch-local sample 0
+    uint32_t samples;       // samples in this batch
+    uint32_t max_bounces;
+    uint32_t segment;
+    uint32_t final_segment; // segment >= max_bounces
+};
+
+__global__ __launch_bounds__(256) void k_generate(CameraBlock cam, BatchParams bp, Wavefront wf)
+{
+    const uint32_t n = bp.npix * bp.samples;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t pix = p % bp.npix, sl = p / bp.npix;
+        const uint32_t px = pix % bp.width, py = pix / bp.width;
+        Sampler smp;
+        smp.start(px, py, bp.sample_base + sl, 0);
+        const float2 up = smp.s2();
+        const float2 ul = smp.s2();
+        float3 o, d;
+        camera_ray(cam, px, py, bp.width, bp.height, up, ul, o, d);
+        wf.thr_pdf[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+        wf.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wf.q_path[0][p] = p;
+        wf.q_o[0][p] = make_float4(o.x, o.y, o.z, cam.near_clip);
+        wf.q_d[0][p] = make_float4(d.x, d.y, d.z, cam.far_clip);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        wf.counters[0] = n;
+        wf.counters[1] = 0;
+        wf.counters[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront wf, int q)
+{
+    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    const uint32_t count = wf.counters[q];
+    const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
+        const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
+        HitOut h;
+        float trans;
+        traverse<false>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans);
+        wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
+        wf.hit_prim[i] = h.prim;
+        if (stk.overflow)
+            atomicOr(&wf.counters[3], 1u);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&wf.totals[0], (unsigned long long)count);
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront wf)
+{
+    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    const uint32_t count = wf.counters[2];
+    const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
+        const float4 o = wf.s_o[i], d = wf.s_d[i];
+        HitOut h;
+        float trans;
+        const bool occluded = traverse<true>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans);
+        if (!occluded && trans >= 1e-5f) {
+            const uint32_t p = wf.s_path[i];
+            const float4 c = wf.s_c[i];
+            float4 r = wf.rad[p];
+            r.x += c.x * trans;
+            r.y += c.y * trans;
+            r.z += c.z * trans;
+            wf.rad[p] = r;
+        }
+        if (stk.overflow)
+            atomicOr(&wf.counters[3], 1u);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&wf.totals[1], (unsigned long long)count);
+}
+
+// Miss / closest-hit / Russian roulette of one segment for every ray of queue q.
+__global__ __launch_bounds__(256) void k_shade(DeviceScene s, BatchParams bp, Wavefront wf, int q)
+{
+    const uint32_t count = wf.counters[q];
+    const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
+        const bool valid = i < count;
+        bool cont = false, shadow = false;
+        uint32_t p = 0;
+        float3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
+        float3 sh_o = f3(0, 0, 0), sh_d = f3(0, 0, 0), sh_c = f3(0, 0, 0);
+        float sh_tmax = 0.0f;
+        if (valid) {
+            p = wf.q_path[q][i];
+            const float4 ro4 = wf.q_o[q][i], rd4 = wf.q_d[q][i];
+            const float3 ro = xyz(ro4), rd = xyz(rd4);
+            const float4 hh = wf.hit[i];
+            const uint32_t hinst = float_as_u32(hh.w);
+            float4 tp = wf.thr_pdf[p];
+            float3 T = xyz(tp);
+            float brdf_pdf = tp.w;
+            float4 rr = wf.rad[p];
+            float3 L = xyz(rr);
+            bool final_seg = bp.final_segment != 0;
+            const uint32_t pix = p % bp.npix, sl = p / bp.npix;
+            const uint32_t px = pix % bp.width, py = pix / bp.width;
+            Sampler smp;
+            smp.start(px, py, bp.sample_base + sl, bp.segment + 1);
+            bool terminated = false;
+
+            if (hinst == NONE) {
+                L = L + T * s.envcol;  // pathtrace.ms.hlsl:7-11
+                terminated = true;
+            } else {
+                const uint32_t prim = wf.hit_prim[i];
+                const float t = hh.x, bu = hh.y, bv = hh.z;
+                const InstanceData& in = s.instances[hinst];
+                const MaterialRec mat = s.materials[in.material_index];
+                const uint32_t ib = in.index_index + prim * 3;
+                const uint32_t ia = s.indices[ib], ibb = s.indices[ib + 1], ic = s.indices[ib + 2];
+                const float4 a0 = s.vertices[size_t(in.vertex_index + ia) * 2], a1 = s.vertices[size_t(in.vertex_index + ia) * 2 + 1];
+                const float4 b0 = s.vertices[size_t(in.vertex_index + ibb) * 2], b1 = s.vertices[size_t(in.vertex_index + ibb) * 2 + 1];
+                const float4 c0 = s.vertices[size_t(in.vertex_index + ic) * 2], c1 = s.vertices[size_t(in.vertex_index + ic) * 2 + 1];
+                const float3 pa = xyz(a0), pb = xyz(b0), pc = xyz(c0);
+                const float3 na = f3(a0.w, a1.x, a1.y), nb = f3(b0.w, b1.x, b1.y), nc = f3(c0.w, c1.x, c1.y);
+                const float2 uva = make_float2(a1.z, a1.w), uvb = make_float2(b1.z, b1.w), uvc = make_float2(c1.z, c1.w);
+                // get_vertex_attributes (geometry.hlsli:39-48)
+                const float wa = 1.0f - bu - bv;
+                const float3 obj_n = normalize3(wa * na + bu * nb + bv * nc);
+                const float3 emission = emission_of(mat);
+                // ObjectRayDirection(): the world->object transform applied to the direction
+                const InstXform x = s.inst_inv[hinst];
+                const float3 odir = f3(x.r0.x * rd.x + x.r0.y * rd.y + x.r0.z * rd.z, x.r1.x * rd.x + x.r1.y * rd.y + x.r1.z * rd.z,
+                                       x.r2.x * rd.x + x.r2.y * rd.y + x.r2.z * rd.z);
+                if (maxc3(emission) > 0.0f && dot3(odir, obj_n) < 0.0f) {
+                    if (brdf_pdf < 0.0f) {
+                        L = L + T * emission;
+                    } else {
+                        const float3 wab = xform_vec(in, pb - pa);
+                        const float3 wac = xform_vec(in, pc - pa);
+                        const float3 sn = cross3(wab, wac);
+                        const float area = sqrtf(dot3(sn, sn)) * 0.5f;
+                        const float dist2 = t * t;
+                        const float cosine = fabsf(dot3(rd, normalize3(sn)));
+                        float pdf_l = light_pdf(s.light, in.emitter_index, prim);
+                        pdf_l *= dist2 / (area * cosine);
+                        if (pdf_l > 0.0f && dist2 > 0.0f) {
+                            const float w = brdf_pdf / (brdf_pdf + pdf_l);
+                            L = L + T * (w * emission);
+                        }
+                    }
+                }
+                if (final_seg) {
+                    terminated = true;
+                } else if (mat.base_colour.w < smp.s1()) {
+                    // stochastic transparency: continue straight through (ch.hlsl:279-283)
+                    next_o = ro + rd * (t + ORIGIN_OFFSET);
+                    next_d = rd;
+                } else {
+                    BrdfInput bi;
+                    bi.base_colour = xyz(mat.base_colour);
+                    float roughness = mat.rmnb.x;
+                    bi.metalness = mat.rmnb.y;
+                    bi.specular = xyz(mat.specular);
+                    bi.sheen = xyz(mat.sheen);
+                    const float anisotropy = mat.cc.z;
+                    roughness = fmaxf(0.00001f, roughness * roughness);
+                    const float aspect = sqrtf(1.0f - 0.9f * fabsf(anisotropy));
+                    bi.alpha = make_float2(roughness / aspect, roughness * aspect);
+                    if (anisotropy < 0.0f)
+                        bi.alpha = make_float2(bi.alpha.y, bi.alpha.x);
+                    // get_vertex_vectors (geometry.hlsli:50-68)
+                    const float2 duv_ab = make_float2(uvb.x - uva.x, uvb.y - uva.y);
+                    const float2 duv_ac = make_float2(uvc.x - uva.x, uvc.y - uva.y);
+                    const float3 ab = pb - pa, ac = pc - pa;
+                    const float3 true_n = cross3(ab, ac);
+                    const float denom = duv_ab.x * duv_ac.y - duv_ac.x * duv_ab.y;
+                    float3 tangent;
+                    if (fabsf(denom) < 1e-5f) {
+                        tangent = cross3(obj_n, fabsf(obj_n.y) > 1e-5f ? f3(1, 0, 0) : f3(0, 1, 0));
+                    } else {
+                        tangent = (duv_ac.y * ab - duv_ab.y * ac) / denom;
+                        tangent = tangent - dot3(tangent, obj_n) * obj_n;
+                    }
+                    const float3 bitangent = cross3(obj_n, tangent);
+                    float3 wx = normalize3(xform_normal(in, tangent));
+                    float3 wy = normalize3(xform_normal(in, bitangent));
+                    float3 wn = normalize3(xform_normal(in, obj_n));
+                    if (dot3(wn, rd) > 0.0f) {
+                        wx = -wx;
+                        wy = -wy;
+                        wn = -wn;
+                    }
+                    float3 twn = normalize3(xform_normal(in, true_n));
+                    if (dot3(rd, twn) > 0.0f)
+                        twn = -twn;
+                    const float3 nrd = -rd;
+                    const float3 o = normalize3(f3(dot3(wx, nrd), dot3(wy, nrd), dot3(wn, nrd)));
+                    if (o.z <= 0.0f) {
+                        terminated = true;
+                    } else {
+                        const float3 xp = ro + rd * t + twn * ORIGIN_OFFSET;
+                        // next-event estimation (ch.hlsl:159-225): the shadow ray is queued
+                        if (s.light_count > 0) {
+                            const float u0 = smp.s1();
+                            const float2 u12 = smp.s2();
+                            float3 light_dir;
+                            float light_dist, lpdf;
+                            bool is_delta;
+                            float3 le = sample_light(s, f3(u0, u12.x, u12.y), xp, light_dir, light_dist, lpdf, is_delta);
+                            if (maxc3(le) > 0.0f) {
+                                float3 li = f3(dot3(wx, light_dir), dot3(wy, light_dir), dot3(wn, light_dir));
+                                if (li.z > 0.0f) {
+                                    li = normalize3(li);
+                                    const float3 m = normalize3(li + o);
+                                    float pdf_b;
+                                    const float3 brdf = evaluate_brdf(bi, li, o, m, pdf_b);
+                                    float3 contrib;
+                                    if (is_delta) {
+                                        contrib = le * brdf * li.z / lpdf;
+                                    } else {
+                                        const float w = lpdf / (lpdf + pdf_b);
+                                        contrib = w * le * brdf * li.z / lpdf;
+                                    }
+                                    shadow = true;
+                                    sh_o = xp;
+                                    sh_d = light_dir;
+                                    sh_tmax = light_dist - 1e-5f;
+                                    sh_c = T * contrib;
+                                }
+                            }
+                        }
+                        float3 m;
+                        float w, pdf;
+                        const float3 dir_i = sample_direction(o, bi.alpha, bi.metalness, smp, m, w, pdf);
+                        if (dir_i.z * o.z <= 0.0f) {
+                            terminated = true;
+                        } else {
+                            float pdf_brdf;
+                            const float3 brdf = evaluate_brdf(bi, dir_i, o, m, pdf_brdf);
+                            const float3 thr = w * brdf * dir_i.z / pdf;
+                            brdf_pdf = pdf_brdf;
+                            T = T * thr;
+                            next_o = xp;
+                            next_d = normalize3(wx * dir_i.x + wy * dir_i.y + wn * dir_i.z);
+                        }
+                    }
+                }
+            }
+            // Russian roulette after segments >= 1 (pathtrace.rg.hlsl:53, pathtrace.hlsli:118-129)
+            if (!terminated && bp.segment >= 1) {
+                const float mt = maxc3(T);
+                const float u = smp.s1();
+                if (mt < 1.0f) {
+                    const float qq = fmaxf(0.0f, 1.0f - mt);
+                    if (u < qq)
+                        terminated = true;
+                    else
+                        T = T * (1.0f / (1.0f - qq));
+                }
+            }
+            cont = !terminated;
+            wf.thr_pdf[p] = make_float4(T.x, T.y, T.z, brdf_pdf);
+            wf.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
+        const uint32_t slot = wave_append(cont, &wf.counters[1 - q]);
+        if (cont) {
+            wf.q_path[1 - q][slot] = p;
+            wf.q_o[1 - q][slot] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
+            wf.q_d[1 - q][slot] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
+        }
+        const uint32_t sslot = wave_append(shadow, &wf.counters[2]);
+        if (shadow) {
+            wf.s_path[sslot] = p;
+            wf.s_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
+            wf.s_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
+            wf.s_c[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
+        }
+    }
+}
+
+// Clears the queue counters a segment is about to fill.
+__global__ void k_prepare_segment(Wavefront wf, int q)
+{
+    if (threadIdx.x == 0) {
+        wf.counters[1 - q] = 0;
+        wf.counters[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(BatchParams bp, const float4* rad, float4* accum, float exposure)
+{
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < bp.npix; pix += gridDim.x * blockDim.x) {
+        float4 acc = accum[pix];
+        for (uint32_t sl = 0; sl < bp.samples; sl++) {
+            const float4 r4 = rad[size_t(sl) * bp.npix + pix];
+            const float rx = correct_component(r4.x * exposure);
+            const float ry = correct_component(r4.y * exposure);
+            const float rz = correct_component(r4.z * exposure);
+            const uint32_t si = bp.sample_base + sl;
+            if (si == 0) {
+                acc = make_float4(rx, ry, rz, 1.0f);
+            } else {
+                const float n = float(si + 1u);
+                acc.x = acc.x + (rx - 1.0f * acc.x) / n;
+                acc.y = acc.y + (ry - 1.0f * acc.y) / n;
+                acc.z = acc.z + (rz - 1.0f * acc.z) / n;
+            }
+        }
+        accum[pix] = acc;
+    }
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, CameraBlock cam, uint32_t width,
+                                                              uint32_t height, uint32_t sample_index, pt_hit* hits,
+                                                              Wavefront wf)
+{
+    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    const uint32_t n = width * height;
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    for (uint32_t p = gtid; p < n; p += gridDim.x * TRACE_BLOCK) {
+        const uint32_t px = p % width, py = p / width;
+        Sampler smp;
+        smp.start(px, py, sample_index, 0);
+        const float2 up = smp.s2();
+        const float2 ul = smp.s2();
+        float3 o, d;
+        camera_ray(cam, px, py, width, height, up, ul, o, d);
+        HitOut h;
+        float trans;
+        traverse<false>(s, o, d, cam.near_clip, cam.far_clip, stk, h, trans);
+        hits[p] = h.inst == NONE ? pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT} : pt_hit{h.t, h.u, h.v, h.inst, h.prim};
+        if (stk.overflow)
+            atomicOr(&wf.counters[3], 1u);
+    }
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene s, const pt_ray* rays, uint32_t n, pt_hit* hits,
+                                                            int any_hit, Wavefront wf)
+{
+    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    for (uint32_t p = gtid; p < n; p += gridDim.x * TRACE_BLOCK) {
+        const pt_ray r = rays[p];
+        HitOut h;
+        float trans;
+        bool found;
+        const float3 o = f3(r.origin[0], r.origin[1], r.origin[2]), d = f3(r.dir[0], r.dir[1], r.dir[2]);
+        if (any_hit)
+            found = traverse<true>(s, o, d, r.tmin, r.tmax, stk, h, trans);
+        else
+            found = traverse<false>(s, o, d, r.tmin, r.tmax, stk, h, trans);
+        hits[p] = found ? pt_hit{h.t, h.u, h.v, h.inst, h.prim} : pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT};
+        if (stk.overflow)
+            atomicOr(&wf.counters[3], 1u);
+    }
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned block, unsigned cap)
+{
+    const uint64_t g = (n + block - 1) / block;
+    return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+}
+
+}  // namespace
+
+hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_threads)
+{
+    wavefront_free(wf);
+    wf.capacity = capacity;
+    wf.spill_threads = spill_threads;
+    const size_t c = std::max<uint32_t>(capacity, 1);
+    hipError_t e = hipSuccess;
+    auto A = [&](auto*& p, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMalloc((void**)&p, bytes);
+    };
+    A(wf.thr_pdf, c * 16);
+    A(wf.rad, c * 16);
+    for (int k = 0; k < 2; k++) {
+        A(wf.q_path[k], c * 4);
+        A(wf.q_o[k], c * 16);
+        A(wf.q_d[k], c * 16);
+    }
+    A(wf.hit, c * 16);
+    A(wf.hit_prim, c * 4);
+    A(wf.s_path, c * 4);
+    A(wf.s_o, c * 16);
+    A(wf.s_d, c * 16);
+    A(wf.s_c, c * 16);
+    A(wf.counters, 16 * 4);
+    A(wf.totals, 4 * 8);
+    A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
+    if (e == hipSuccess)
+        e = hipMemset(wf.counters, 0, 16 * 4);
+    if (e == hipSuccess)
+        e = hipMemset(wf.totals, 0, 4 * 8);
+    return e;
+}
+
+void wavefront_free(Wavefront& wf)
+{
+    void* ptrs[] = {wf.thr_pdf, wf.rad, wf.q_path[0], wf.q_path[1], wf.q_o[0], wf.q_o[1], wf.q_d[0], wf.q_d[1],
+                    wf.hit, wf.hit_prim, wf.s_path, wf.s_o, wf.s_d, wf.s_c, wf.counters, wf.totals, wf.spill};
+    for (void* p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    wf = Wavefront{};
+}
+
+hipError_t render_samples(Ctx& ctx, uint32_t spp)
+{
+    Wavefront& wf = ctx.wf;
+    const uint32_t npix = ctx.width * ctx.height;
+    const uint32_t per_batch = std::max<uint32_t>(1, wf.capacity / npix);
+    hipStream_t st = ctx.stream;
+    hipError_t e;
+    if ((e = hipMemsetAsync(wf.totals, 0, 4 * 8, st)) != hipSuccess)
+        return e;
+    uint32_t done = 0;
+    while (done < spp) {
+        BatchParams bp{};
+        bp.width = ctx.width;
+        bp.height = ctx.height;
+        bp.npix = npix;
+        bp.sample_base = ctx.sample_index + done;
+        bp.samples = std::min(per_batch, spp - done);
+        bp.max_bounces = ctx.max_bounces;
+        const uint64_t n = uint64_t(npix) * bp.samples;
+        hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256, 65535u * 4)), dim3(256), 0, st, ctx.cam, bp, wf);
+        int q = 0;
+        for (uint32_t seg = 0; seg <= ctx.max_bounces; seg++) {
+            bp.segment = seg;
+            bp.final_segment = seg >= ctx.max_bounces ? 1u : 0u;
+            hipLaunchKernelGGL(k_prepare_segment, dim3(1), dim3(64), 0, st, wf, q);
+            hipLaunchKernelGGL(k_extend, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
+            hipLaunchKernelGGL(k_shade, dim3(blocks_for(n, 256, 65535u * 4)), dim3(256), 0, st, ctx.scene, bp, wf, q);
+            hipLaunchKernelGGL(k_shadow, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
+            q = 1 - q;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256, 65535u * 4)), dim3(256), 0, st, bp, wf.rad,
+                           ctx.accum, ctx.cam.exposure);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        done += bp.samples;
+    }
+    return hipSuccess;
+}
+
+hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits)
+{
+    hipLaunchKernelGGL(k_primary_hits, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, ctx.stream, ctx.scene, ctx.cam,
+                       ctx.width, ctx.height, sample_index, d_hits, ctx.wf);
+    return hipGetLastError();
+}
+
+hipError_t trace_rays(Ctx& ctx, const pt_ray* d_rays, uint32_t count, pt_hit* d_hits, bool any_hit)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_trace_rays, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, ctx.stream, ctx.scene, d_rays, count,
+                       d_hits, any_hit ? 1 : 0, ctx.wf);
+    return hipGetLastError();
+}
+
+}  // namespace ptk

@@ -1,0 +1,168 @@
+// C ABI of the host scene surface (include/pthost.h).
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "hdrio.h"
+#include "pthost.h"
+#include "scene.h"
+
+struct pth_scene {
+    pth::Scene scene;
+    pth::PackedScene packed;
+};
+
+static thread_local std::string g_error;
+
+template <typename F>
+static int guarded(F&& f)
+{
+    try {
+        f();
+        return PT_OK;
+    } catch (const std::bad_alloc&) {
+        g_error = "out of host memory";
+        return PT_ENOMEM;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return PT_EINVAL;
+    }
+}
+
+extern "C" {
+
+int pth_scene_load(const char* scn_path, pth_scene** out)
+{
+    if (!scn_path || !out) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        pth_scene* s = new pth_scene;
+        try {
+            s->scene = pth::load_scn(scn_path);
+            s->packed = pth::pack_scene(s->scene);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+const pt_scene_blob* pth_scene_blob(const pth_scene* s) { return s ? &s->packed.blob : nullptr; }
+
+int pth_scene_info_get(const pth_scene* s, pth_scene_info* out)
+{
+    if (!s || !out) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    pth_scene_info info{};
+    const pth::Scene& sc = s->scene;
+    info.object_variants = uint32_t(sc.variants.size());
+    info.materials = uint32_t(sc.materials.size());
+    info.point_lights = uint32_t(sc.point_lights.size());
+    info.directional_lights = uint32_t(sc.dir_lights.size());
+    for (const pth::Mesh& m : sc.meshes) {
+        info.unique_vertices += m.vertices.size();
+        info.unique_triangles += m.tris.size();
+    }
+    for (const pth::Variant& v : sc.variants) {
+        info.instances += uint32_t(v.instances.size());
+        info.total_triangles += uint64_t(v.instances.size()) * sc.meshes[v.mesh].tris.size();
+        if (sc.materials[v.material].is_emitter())
+            info.emissive_instances += uint32_t(v.instances.size());
+    }
+    info.light_count = s->packed.light_count;
+    *out = info;
+    return PT_OK;
+}
+
+int pth_scene_camera_params(const pth_scene* s, uint32_t width, uint32_t height, pth_camera_params* out)
+{
+    if (!s || !out || width == 0 || height == 0) {
+        g_error = "invalid argument";
+        return PT_EINVAL;
+    }
+    const pth::CameraSection& c = s->scene.camera;
+    pth_camera_params p{};
+    for (int i = 0; i < 3; i++) {
+        p.position[i] = c.position[i];
+        p.lookdir[i] = c.lookdir[i];
+        p.up[i] = c.up[i];
+    }
+    p.fov_deg = c.fov;
+    p.near_clip = c.near_clip;
+    p.far_clip = c.far_clip;
+    p.focus_dist = c.focus_dist;
+    p.lens_radius = c.lens_radius;
+    p.exposure = c.exposure;
+    p.aspect = float(width) / float(height);  // main.cpp:98
+    *out = p;
+    return PT_OK;
+}
+
+void pth_scene_free(pth_scene* s) { delete s; }
+
+int pth_camera_block(const pth_camera_params* p, pt_camera* out)
+{
+    if (!p || !out) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] {
+        pth::CameraSection c;
+        c.position = pth::V3(p->position[0], p->position[1], p->position[2]);
+        c.lookdir = pth::V3(p->lookdir[0], p->lookdir[1], p->lookdir[2]);
+        c.up = pth::V3(p->up[0], p->up[1], p->up[2]);
+        c.fov = p->fov_deg;
+        c.near_clip = p->near_clip;
+        c.far_clip = p->far_clip;
+        c.focus_dist = p->focus_dist;
+        c.lens_radius = p->lens_radius;
+        c.exposure = p->exposure;
+        pth::camera_block(c, p->aspect, out);
+    });
+}
+
+int pth_write_hdr(const char* path, const float* rgba, uint32_t width, uint32_t height, size_t row_pitch)
+{
+    if (!path || !rgba) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] { pth::write_hdr_file(path, rgba, width, height, row_pitch); });
+}
+
+int pth_read_hdr(const char* path, float** out, uint32_t* width, uint32_t* height)
+{
+    if (!path || !out || !width || !height) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] {
+        std::vector<float> img = pth::read_hdr_file(path, *width, *height);
+        float* buf = (float*)std::malloc(img.size() * sizeof(float));
+        if (!buf)
+            throw std::bad_alloc();
+        memcpy(buf, img.data(), img.size() * sizeof(float));
+        *out = buf;
+    });
+}
+
+double pth_meansq(const float* img, const float* ref, uint32_t width, uint32_t height)
+{
+    return pth::meansq(img, ref, width, height);
+}
+
+float pth_from_srgb(float x) { return pth::from_srgb(x); }
+float pth_to_srgb(float x) { return pth::to_srgb(x); }
+
+void pth_free(void* p) { std::free(p); }
+
+const char* pth_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

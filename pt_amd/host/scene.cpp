@@ -1,0 +1,931 @@
+#include "scene.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <numeric>
+#include <optional>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "json.h"
+
+namespace pth {
+
+// ---------------------------------------------------------------------------------------
+// Materials and colour (src/scene/material.cpp, src/colour.cpp)
+
+uint32_t Material::map_bits() const
+{
+    uint32_t b;
+    memcpy(&b, &rough_metal_normal_bits[3], 4);
+    return b;
+}
+
+void Material::set_map_bits(uint32_t bits) { memcpy(&rough_metal_normal_bits[3], &bits, 4); }
+
+bool Material::is_emitter() const
+{
+    return ((map_bits() & EMISSION_MAP_BIT) ||
+            std::max(std::max(emission[0], emission[1]), emission[2]) > 0.0f) &&
+           emission[3] > 0.0f;
+}
+
+// The thresholds are the reference's own (colour.cpp:4 uses 0.00031308 for the encode).
+float from_srgb(float x) { return (x <= 0.04045) ? (x / 12.92) : std::pow((x + 0.055) / 1.055, 2.4); }
+float to_srgb(float x) { return x <= 0.00031308 ? 12.92 * x : 1.055 * std::pow(x, (1.0 / 2.4)) - 0.055; }
+
+static V3 from_srgb(V3 v) { return V3(from_srgb(v.x), from_srgb(v.y), from_srgb(v.z)); }
+
+// ---------------------------------------------------------------------------------------
+// Files
+
+static std::string read_file(const std::string& path, bool binary)
+{
+    std::ifstream f(path, binary ? std::ios::binary : std::ios::in);
+    if (f.fail())
+        throw std::runtime_error("Could not open file " + path);
+    return std::string((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+static std::string directory_of(const std::string& file)
+{
+    size_t i = file.find_last_of("\\/");
+    return i == std::string::npos ? "" : file.substr(0, i + 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// glTF 2.0 subset
+
+namespace {
+
+std::vector<uint8_t> decode_base64(const std::string& s)
+{
+    auto val = [](char c) -> int {
+        if (c >= 'A' && c <= 'Z') return c - 'A';
+        if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+        if (c >= '0' && c <= '9') return c - '0' + 52;
+        if (c == '+' || c == '-') return 62;
+        if (c == '/' || c == '_') return 63;
+        return -1;
+    };
+    std::vector<uint8_t> out;
+    out.reserve(s.size() * 3 / 4);
+    uint32_t acc = 0;
+    int bits = 0;
+    for (char c : s) {
+        int v = val(c);
+        if (v < 0)
+            continue;  // padding / whitespace
+        acc = (acc << 6) | uint32_t(v);
+        bits += 6;
+        if (bits >= 8) {
+            bits -= 8;
+            out.push_back(uint8_t((acc >> bits) & 0xFF));
+        }
+    }
+    return out;
+}
+
+struct Gltf {
+    Json doc;
+    std::vector<std::vector<uint8_t>> buffers;
+    std::string dir;
+
+    // Reads accessor `index` as `comps` floats per element (float components only).
+    std::vector<float> floats(int index, int comps) const
+    {
+        const Json& acc = doc["accessors"][size_t(index)];
+        if (acc.kind != Json::OBJ)
+            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
+        if (acc.has("sparse"))
+            throw std::runtime_error("glTF: sparse accessors are not supported");
+        if (int(acc["componentType"].number(0)) != 5126)
+            throw std::runtime_error("glTF: expected a float accessor");
+        const size_t count = size_t(acc["count"].number(0));
+        std::vector<float> out(count * comps, 0.0f);
+        if (!acc.has("bufferView"))
+            return out;
+        const uint8_t* base;
+        size_t stride;
+        view(acc, comps * 4, base, stride, count);
+        for (size_t i = 0; i < count; i++)
+            memcpy(&out[i * comps], base + i * stride, size_t(comps) * 4);
+        return out;
+    }
+
+    std::vector<uint32_t> uints(int index) const
+    {
+        const Json& acc = doc["accessors"][size_t(index)];
+        if (acc.kind != Json::OBJ)
+            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
+        const int ct = int(acc["componentType"].number(0));
+        const size_t count = size_t(acc["count"].number(0));
+        const int csize = ct == 5125 ? 4 : ct == 5123 ? 2 : ct == 5121 ? 1 : 0;
+        if (!csize)
+            throw std::runtime_error("glTF: unsupported index component type");
+        std::vector<uint32_t> out(count, 0);
+        if (!acc.has("bufferView"))
+            return out;
+        const uint8_t* base;
+        size_t stride;
+        view(acc, csize, base, stride, count);
+        for (size_t i = 0; i < count; i++) {
+            const uint8_t* p = base + i * stride;
+            if (csize == 4) {
+                uint32_t v;
+                memcpy(&v, p, 4);
+                out[i] = v;
+            } else if (csize == 2) {
+                uint16_t v;
+                memcpy(&v, p, 2);
+                out[i] = v;
+            } else {
+                out[i] = *p;
+            }
+        }
+        return out;
+    }
+
+    void view(const Json& acc, size_t elem_bytes, const uint8_t*& base, size_t& stride, size_t count) const
+    {
+        const Json& bv = doc["bufferViews"][size_t(acc["bufferView"].number(-1))];
+        if (bv.kind != Json::OBJ)
+            throw std::runtime_error("glTF: bad bufferView");
+        const size_t b = size_t(bv["buffer"].number(-1));
+        if (b >= buffers.size())
+            throw std::runtime_error("glTF: bad buffer index");
+        const size_t off = size_t(bv["byteOffset"].number(0)) + size_t(acc["byteOffset"].number(0));
+        stride = bv.has("byteStride") ? size_t(bv["byteStride"].number(0)) : elem_bytes;
+        if (count > 0 && off + (count - 1) * stride + elem_bytes > buffers[b].size())
+            throw std::runtime_error("glTF: accessor exceeds its buffer");
+        base = buffers[b].data() + off;
+    }
+};
+
+Gltf open_gltf(const std::string& path)
+{
+    Gltf g;
+    g.dir = directory_of(path);
+    std::string raw = read_file(path, true);
+    std::vector<uint8_t> glb_bin;
+    bool have_glb_bin = false;
+    if (raw.size() >= 12 && raw.compare(0, 4, "glTF") == 0) {
+        uint32_t total;
+        memcpy(&total, raw.data() + 8, 4);
+        size_t p = 12;
+        std::string json_text;
+        while (p + 8 <= raw.size()) {
+            uint32_t len, type;
+            memcpy(&len, raw.data() + p, 4);
+            memcpy(&type, raw.data() + p + 4, 4);
+            p += 8;
+            if (p + len > raw.size())
+                throw std::runtime_error("GLB chunk exceeds the file: " + path);
+            if (type == 0x4E4F534Au)
+                json_text.assign(raw.data() + p, len);
+            else if (type == 0x004E4942u) {
+                glb_bin.assign(raw.begin() + long(p), raw.begin() + long(p + len));
+                have_glb_bin = true;
+            }
+            p += (len + 3u) & ~3u;
+        }
+        g.doc = parse_json(json_text);
+    } else {
+        g.doc = parse_json(raw);
+    }
+    const Json& bufs = g.doc["buffers"];
+    for (size_t i = 0; i < bufs.size(); i++) {
+        const Json& b = bufs[i];
+        if (!b.has("uri")) {
+            if (!have_glb_bin)
+                throw std::runtime_error("glTF buffer without uri: " + path);
+            g.buffers.push_back(glb_bin);
+            continue;
+        }
+        const std::string& uri = b["uri"].str;
+        if (uri.compare(0, 5, "data:") == 0) {
+            size_t comma = uri.find(',');
+            if (comma == std::string::npos || uri.find(";base64") == std::string::npos)
+                throw std::runtime_error("glTF: unsupported data URI");
+            g.buffers.push_back(decode_base64(uri.substr(comma + 1)));
+        } else {
+            std::string data = read_file(g.dir + uri, true);
+            g.buffers.emplace_back(data.begin(), data.end());
+        }
+    }
+    return g;
+}
+
+M4 node_matrix(const Json& node)
+{
+    if (node.has("matrix")) {
+        M4 m;
+        for (int i = 0; i < 16; i++)
+            m.c[i / 4][i % 4] = float(node["matrix"][size_t(i)].number(0));
+        return m;
+    }
+    M4 t = M4::identity(), r = M4::identity(), s = M4::identity();
+    if (node.has("translation"))
+        t = translation(V3(float(node["translation"][0].number(0)), float(node["translation"][1].number(0)),
+                           float(node["translation"][2].number(0))));
+    if (node.has("rotation")) {
+        Quat q;
+        q.x = float(node["rotation"][0].number(0));
+        q.y = float(node["rotation"][1].number(0));
+        q.z = float(node["rotation"][2].number(0));
+        q.w = float(node["rotation"][3].number(1));
+        r = mat4_from_quat(q);
+    }
+    if (node.has("scale"))
+        s = scaling(V3(float(node["scale"][0].number(1)), float(node["scale"][1].number(1)),
+                       float(node["scale"][2].number(1))));
+    return t * r * s;
+}
+
+bool has_texture(const Json& j) { return j.kind == Json::OBJ && j.has("index"); }
+
+Material convert_material(const Json& m)
+{
+    // Field defaults of src/scene/scene.cpp:74-295 where the importer provides no value; the
+    // glTF core factors are always present after import (spec defaults 1/1/1 and 0 emission).
+    const Json& pbr = m["pbrMetallicRoughness"];
+    if (has_texture(pbr["baseColorTexture"]) || has_texture(pbr["metallicRoughnessTexture"]) ||
+        has_texture(m["emissiveTexture"]) || has_texture(m["normalTexture"]))
+        throw std::runtime_error("glTF materials with textures are not supported by this loader");
+    Material out;
+    uint32_t bits = 0;
+    float bc[4] = {1, 1, 1, 1};
+    if (pbr.has("baseColorFactor"))
+        for (int i = 0; i < 4; i++)
+            bc[i] = float(pbr["baseColorFactor"][size_t(i)].number(1));
+    V3 lin = from_srgb(V3(bc[0], bc[1], bc[2]));
+    out.base_colour[0] = lin.x;
+    out.base_colour[1] = lin.y;
+    out.base_colour[2] = lin.z;
+    out.base_colour[3] = bc[3];
+
+    V3 e(0, 0, 0);
+    if (m.has("emissiveFactor"))
+        e = V3(float(m["emissiveFactor"][0].number(0)), float(m["emissiveFactor"][1].number(0)),
+               float(m["emissiveFactor"][2].number(0)));
+    V3 le = from_srgb(e);
+    out.emission[0] = le.x;
+    out.emission[1] = le.y;
+    out.emission[2] = le.z;
+    const Json& ext = m["extensions"];
+    out.emission[3] = float(ext["KHR_materials_emissive_strength"]["emissiveStrength"].number(1.0));
+
+    out.rough_metal_normal_bits[0] = float(pbr["roughnessFactor"].number(1.0));
+    out.rough_metal_normal_bits[1] = float(pbr["metallicFactor"].number(1.0));
+    out.rough_metal_normal_bits[2] = 0.0f;
+
+    const Json& spec = ext["KHR_materials_specular"];
+    if (spec.kind == Json::OBJ && spec.has("specularColorFactor")) {
+        for (int i = 0; i < 3; i++)
+            out.specular[i] = float(spec["specularColorFactor"][size_t(i)].number(1));
+    } else {
+        out.specular[0] = out.specular[1] = out.specular[2] = 0.5f;
+    }
+    const Json& sheen = ext["KHR_materials_sheen"];
+    for (int i = 0; i < 3; i++)
+        out.sheen[i] = float(sheen["sheenColorFactor"][size_t(i)].number(0));
+    const Json& cc = ext["KHR_materials_clearcoat"];
+    out.cc_ccrgh_aniso[0] = float(cc["clearcoatFactor"].number(0));
+    out.cc_ccrgh_aniso[1] = float(cc["clearcoatRoughnessFactor"].number(0.5));
+    out.cc_ccrgh_aniso[2] = float(ext["KHR_materials_anisotropy"]["anisotropyStrength"].number(0));
+
+    // has_transparency (scene.cpp:344-354): constant base-colour alpha below one.
+    if (out.base_colour[3] < 1.0f)
+        bits |= MATERIAL_IS_TRANSPARENT_BIT;
+    out.set_map_bits(bits);
+    return out;
+}
+
+void walk_nodes(const Json& doc, size_t node_index, const M4& parent, std::vector<std::vector<M4>>& mesh_instances,
+                const std::vector<std::vector<uint32_t>>& mesh_prims, int depth)
+{
+    if (depth > 256)
+        throw std::runtime_error("glTF node hierarchy too deep (cycle?)");
+    const Json& node = doc["nodes"][node_index];
+    if (node.kind != Json::OBJ)
+        throw std::runtime_error("glTF: bad node index");
+    const M4 m = parent * node_matrix(node);
+    if (node.has("mesh")) {
+        size_t mi = size_t(node["mesh"].number(-1));
+        if (mi >= mesh_prims.size())
+            throw std::runtime_error("glTF: bad mesh index");
+        for (uint32_t prim_mesh : mesh_prims[mi])
+            mesh_instances[prim_mesh].push_back(m);
+    }
+    const Json& ch = node["children"];
+    for (size_t i = 0; i < ch.size(); i++)
+        walk_nodes(doc, size_t(ch[i].number(0)), m, mesh_instances, mesh_prims, depth + 1);
+}
+
+}  // namespace
+
+Scene load_gltf(const std::string& path)
+{
+    Gltf g = open_gltf(path);
+    Scene scene;
+    scene.resource_directory = g.dir;
+
+    const Json& mats = g.doc["materials"];
+    for (size_t i = 0; i < mats.size(); i++)
+        scene.materials.push_back(convert_material(mats[i]));
+
+    const Json& meshes = g.doc["meshes"];
+    std::vector<std::vector<uint32_t>> mesh_prims(meshes.size());
+    for (size_t mi = 0; mi < meshes.size(); mi++) {
+        const Json& prims = meshes[mi]["primitives"];
+        for (size_t pi = 0; pi < prims.size(); pi++) {
+            const Json& prim = prims[pi];
+            if (int(prim["mode"].number(4)) != 4)
+                throw std::runtime_error("glTF: only triangle primitives are supported");
+            const Json& attr = prim["attributes"];
+            if (!attr.has("POSITION") || !attr.has("NORMAL"))
+                throw std::runtime_error("glTF: primitives need POSITION and NORMAL");
+            if (!prim.has("material"))
+                throw std::runtime_error("glTF: primitives need a material");
+            std::vector<float> pos = g.floats(int(attr["POSITION"].number(-1)), 3);
+            std::vector<float> nrm = g.floats(int(attr["NORMAL"].number(-1)), 3);
+            std::vector<float> uv;
+            if (attr.has("TEXCOORD_0"))
+                uv = g.floats(int(attr["TEXCOORD_0"].number(-1)), 2);
+            else
+                std::cerr << "Warning: mesh \"" << meshes[mi]["name"].str << "\" has no UVs." << std::endl;
+            const size_t nv = pos.size() / 3;
+            if (nrm.size() / 3 != nv || (!uv.empty() && uv.size() / 2 != nv))
+                throw std::runtime_error("glTF: attribute counts differ");
+            Mesh mesh;
+            mesh.vertices.resize(nv);
+            for (size_t v = 0; v < nv; v++) {
+                Vertex& vx = mesh.vertices[v];
+                for (int k = 0; k < 3; k++) {
+                    vx.pos[k] = pos[v * 3 + k];
+                    vx.normal[k] = nrm[v * 3 + k];
+                }
+                // Assimp's glTF importer flips v and aiProcess_FlipUVs flips it back.
+                vx.uv[0] = uv.empty() ? 0.0f : uv[v * 2 + 0];
+                vx.uv[1] = uv.empty() ? 0.0f : uv[v * 2 + 1];
+            }
+            std::vector<uint32_t> idx;
+            if (prim.has("indices")) {
+                idx = g.uints(int(prim["indices"].number(-1)));
+            } else {
+                idx.resize(nv);
+                std::iota(idx.begin(), idx.end(), 0u);
+            }
+            if (idx.size() % 3 != 0)
+                throw std::runtime_error("glTF: index count is not a multiple of 3");
+            mesh.tris.resize(idx.size() / 3);
+            for (size_t t = 0; t < mesh.tris.size(); t++) {
+                uint32_t a = idx[t * 3], b = idx[t * 3 + 1], c = idx[t * 3 + 2];
+                if (a >= nv || b >= nv || c >= nv)
+                    throw std::runtime_error("glTF: index out of range");
+                // FindDegenerates without removal turns a face with coincident corners into a
+                // line/point, which process_meshes leaves as an all-zero triangle
+                // (scene.cpp:433-437).
+                auto same = [&](uint32_t p, uint32_t q) {
+                    return memcmp(mesh.vertices[p].pos, mesh.vertices[q].pos, 12) == 0;
+                };
+                if (same(a, b) || same(b, c) || same(a, c))
+                    mesh.tris[t] = Tri{{0, 0, 0}};
+                else
+                    mesh.tris[t] = Tri{{a, b, c}};
+            }
+            const uint32_t mat = uint32_t(prim["material"].number(-1));
+            if (mat >= scene.materials.size())
+                throw std::runtime_error("glTF: bad material index");
+            mesh_prims[mi].push_back(uint32_t(scene.meshes.size()));
+            Variant var;
+            var.mesh = uint32_t(scene.meshes.size());
+            var.material = mat;
+            scene.meshes.push_back(std::move(mesh));
+            scene.variants.push_back(var);
+        }
+    }
+
+    std::vector<std::vector<M4>> mesh_instances(scene.meshes.size());
+    std::vector<size_t> roots;
+    if (g.doc.has("scenes") && g.doc["scenes"].size() > 0) {
+        const size_t si = size_t(g.doc["scene"].number(0));
+        const Json& sc = g.doc["scenes"][si];
+        for (size_t i = 0; i < sc["nodes"].size(); i++)
+            roots.push_back(size_t(sc["nodes"][i].number(0)));
+    } else {
+        std::vector<bool> is_child(g.doc["nodes"].size(), false);
+        for (size_t n = 0; n < g.doc["nodes"].size(); n++)
+            for (size_t c = 0; c < g.doc["nodes"][n]["children"].size(); c++) {
+                size_t ci = size_t(g.doc["nodes"][n]["children"][c].number(0));
+                if (ci < is_child.size())
+                    is_child[ci] = true;
+            }
+        for (size_t n = 0; n < is_child.size(); n++)
+            if (!is_child[n])
+                roots.push_back(n);
+    }
+    for (size_t r : roots)
+        walk_nodes(g.doc, r, M4::identity(), mesh_instances, mesh_prims, 0);
+    for (size_t i = 0; i < scene.variants.size(); i++)
+        scene.variants[i].instances = std::move(mesh_instances[i]);
+    return scene;
+}
+
+// ---------------------------------------------------------------------------------------
+// .scn parsing (src/scene/scenebuild.cpp, src/io/tokenstream.cpp, src/io/parseutil.cpp)
+
+namespace {
+
+struct Tokens {
+    std::vector<std::string> t;
+    size_t i = 0;
+    Tokens(const std::string& raw, char delim)
+    {
+        std::stringstream ss(raw);
+        std::string tok;
+        while (std::getline(ss, tok, delim))
+            if (!tok.empty())
+                t.push_back(tok);
+    }
+    bool done() const { return i >= t.size(); }
+    size_t remaining() const { return t.size() - i; }
+    const std::string& next()
+    {
+        if (done())
+            throw std::runtime_error("Token stream has no next token");
+        return t[i++];
+    }
+    void rollback(size_t n)
+    {
+        if (i < n)
+            throw std::runtime_error("Cannot roll further back than the start");
+        i -= n;
+    }
+};
+
+float stof_checked(const std::string& s)
+{
+    // std::stof: leading whitespace skipped, trailing garbage ignored, no digits -> throw.
+    const char* b = s.c_str();
+    char* e = nullptr;
+    errno = 0;
+    float v = std::strtof(b, &e);
+    if (e == b)
+        throw std::invalid_argument("stof");
+    if (errno == ERANGE)
+        throw std::out_of_range("stof");
+    return v;
+}
+
+V3 parse_vector(Tokens& tk)
+{
+    if (tk.remaining() < 3)
+        throw std::runtime_error("Vectors need three components");
+    V3 v;
+    v.x = stof_checked(tk.next());
+    v.y = stof_checked(tk.next());
+    v.z = stof_checked(tk.next());
+    return v;
+}
+
+V3 parse_exact3(Tokens& tk, const char* what)
+{
+    if (tk.remaining() != 3)
+        throw std::runtime_error(what);
+    V3 v;
+    v.x = stof_checked(tk.next());
+    v.y = stof_checked(tk.next());
+    v.z = stof_checked(tk.next());
+    return v;
+}
+
+struct SceneInstance {
+    M4 transform;
+};
+struct Subscene {
+    std::string identifier, path;
+    std::vector<SceneInstance> instances;
+};
+
+struct ParserState {
+    std::optional<std::string> current_object;
+    M4 current_transform = M4::identity();
+    std::vector<Subscene> subscenes;
+    std::unordered_map<std::string, uint32_t> subscene_indices;
+    std::vector<PointLight> point_lights;
+    std::vector<DirLight> dir_lights;
+    std::optional<CameraSection> camera;
+    V3 envcol{0, 0, 0};
+    std::string envmap;
+};
+
+bool is_section_label(const std::string& tok) { return tok[tok.size() - 1] == ':'; }
+
+void parse_camera(Tokens& lines, ParserState& st)
+{
+    if (st.camera)
+        throw std::runtime_error("Camera already set.");
+    CameraSection cam;
+    while (!lines.done()) {
+        Tokens tk(lines.next(), ' ');
+        if (tk.done())
+            continue;
+        const std::string tok = tk.next();
+        if (tok == "position") cam.position = parse_vector(tk);
+        else if (tok == "lookdir") cam.lookdir = parse_vector(tk);
+        else if (tok == "up") cam.up = parse_vector(tk);
+        else if (tok == "fov") cam.fov = stof_checked(tk.next());
+        else if (tok == "near") cam.near_clip = stof_checked(tk.next());
+        else if (tok == "far") cam.far_clip = stof_checked(tk.next());
+        else if (tok == "focusdist") cam.focus_dist = stof_checked(tk.next());
+        else if (tok == "lensradius") cam.lens_radius = stof_checked(tk.next());
+        else if (tok == "exposure") cam.exposure = stof_checked(tk.next());
+        else if (tok == "envcol") st.envcol = from_srgb(parse_vector(tk));
+        else if (tok == "envmap") st.envmap = tk.next();
+        else if (is_section_label(tok)) {
+            lines.rollback(1);
+            break;
+        } else
+            throw std::runtime_error("Unrecognised token in camera section " + tok);
+    }
+    st.camera = cam;
+}
+
+void finish_instance(ParserState& st)
+{
+    if (st.current_object.has_value())
+        st.subscenes[st.subscene_indices[st.current_object.value()]].instances.push_back({st.current_transform});
+}
+
+void parse_light(Tokens& tk, ParserState& st, bool point)
+{
+    if (tk.remaining() != 7)
+        throw std::runtime_error("Invalid light syntax.");
+    V3 a = parse_vector(tk);
+    if (tk.next() != ",")
+        throw std::runtime_error("Invalid light syntax.");
+    V3 col = from_srgb(parse_vector(tk));
+    if (point)
+        st.point_lights.push_back({a, col});
+    else
+        st.dir_lights.push_back({a, col});
+}
+
+void parse_objects(Tokens& lines, ParserState& st)
+{
+    while (!lines.done()) {
+        Tokens tk(lines.next(), ' ');
+        if (tk.done())
+            continue;
+        const std::string tok = tk.next();
+        if (tok == "t") {
+            if (!st.current_object)
+                throw std::runtime_error("No instance to translate.");
+            st.current_transform = translation(parse_exact3(tk, "Translation did not have three coordinates")) *
+                                   st.current_transform;
+        } else if (tok == "r") {
+            if (!st.current_object)
+                throw std::runtime_error("No instance to rotate.");
+            if (tk.remaining() != 4)
+                throw std::runtime_error("Rotation did not have three coordinates and angle");
+            V3 u;
+            u.x = stof_checked(tk.next());
+            u.y = stof_checked(tk.next());
+            u.z = stof_checked(tk.next());
+            float angle = stof_checked(tk.next());
+            st.current_transform = rotation(u, angle) * st.current_transform;
+        } else if (tok == "s") {
+            if (!st.current_object)
+                throw std::runtime_error("No instance to scale.");
+            st.current_transform = scaling(parse_exact3(tk, "Scaling did not have three coordinates")) *
+                                   st.current_transform;
+        } else if (tok == "basecol" || tok == "roughness" || tok == "metalness" || tok == "specular") {
+            // Accepted and ignored, as in the reference (scenebuild.cpp:222-237).
+            if (!st.current_object)
+                throw std::runtime_error("No instance to apply " + tok + " parameter to.");
+        } else if (tok == "plight") {
+            parse_light(tk, st, true);
+        } else if (tok == "dlight") {
+            parse_light(tk, st, false);
+        } else if (is_section_label(tok)) {
+            lines.rollback(1);
+            break;
+        } else if (tk.done()) {
+            if (!st.subscene_indices.count(tok))
+                throw std::runtime_error("Object " + tok + " not recognised.");
+            finish_instance(st);
+            st.current_object = tok;
+            st.current_transform = M4::identity();
+        } else {
+            st.subscene_indices[tok] = uint32_t(st.subscenes.size());
+            st.subscenes.push_back({tok, tk.next(), {}});
+        }
+    }
+    finish_instance(st);
+}
+
+}  // namespace
+
+Scene load_scn(const std::string& path)
+{
+    Scene scene;
+    scene.resource_directory = directory_of(path);
+    std::string text = read_file(path, false);
+    text.erase(std::remove_if(text.begin(), text.end(), [](char c) { return c == '\r' || c == '\t'; }),
+               text.end());
+    Tokens lines(text, '\n');
+    ParserState st;
+    while (!lines.done()) {
+        Tokens tk(lines.next(), ' ');
+        if (tk.done())
+            continue;
+        const std::string tok = tk.next();
+        if (tok == "camera:")
+            parse_camera(lines, st);
+        else if (tok == "objects:")
+            parse_objects(lines, st);
+        else
+            throw std::runtime_error("Unrecognised token " + tok);
+    }
+    if (!st.camera)
+        throw std::runtime_error("No camera section in scene description.");
+    scene.camera = *st.camera;
+    scene.envcol = st.envcol;
+    scene.point_lights = st.point_lights;
+    scene.dir_lights = st.dir_lights;
+
+    for (const Subscene& sub : st.subscenes) {
+        if (sub.instances.empty())
+            continue;
+        const uint32_t mesh_offset = uint32_t(scene.meshes.size());
+        const uint32_t material_offset = uint32_t(scene.materials.size());
+        Scene loaded = load_gltf(scene.resource_directory + sub.path);
+        for (Mesh& m : loaded.meshes)
+            scene.meshes.push_back(std::move(m));
+        for (const Variant& lv : loaded.variants) {
+            Variant v;
+            v.mesh = lv.mesh + mesh_offset;
+            v.material = lv.material + material_offset;
+            v.instances.reserve(lv.instances.size() * sub.instances.size());
+            for (const SceneInstance& si : sub.instances)
+                for (const M4& oi : lv.instances)
+                    v.instances.push_back(si.transform * oi);
+            scene.variants.push_back(std::move(v));
+        }
+        scene.materials.insert(scene.materials.end(), loaded.materials.begin(), loaded.materials.end());
+    }
+    return scene;
+}
+
+// ---------------------------------------------------------------------------------------
+// Alias tables and the light sampler (src/aliastable.h, src/scene/lightsample.cpp)
+
+std::vector<uint32_t> build_alias_table(const std::vector<float>& weights, const std::vector<uint32_t>& values,
+                                        uint32_t value_words)
+{
+    const size_t n = weights.size();
+    if (n == 0)
+        throw std::runtime_error("alias table needs at least one bin");
+    struct Bin {
+        float p, q;
+        uint32_t alias;
+    };
+    std::vector<Bin> bins(n);
+    const double inv_sum = 1.0 / std::accumulate(weights.begin(), weights.end(), 0.0);
+    for (size_t i = 0; i < n; i++) {
+        bins[i].p = float(weights[i] * inv_sum);
+        bins[i].alias = uint32_t(i);
+    }
+    // Vose's method with LIFO under/overfull lists; leftovers get q = 1.
+    std::vector<size_t> under, over;
+    for (size_t i = 0; i < n; i++) {
+        float u = bins[i].p * float(n);
+        bins[i].q = u;
+        (u < 1.0f ? under : over).push_back(i);
+    }
+    while (!under.empty() && !over.empty()) {
+        const size_t s = under.back(), l = over.back();
+        bins[s].alias = uint32_t(l);
+        bins[l].q += bins[s].q - 1.0f;
+        under.pop_back();
+        if (bins[l].q < 1.0f) {
+            under.push_back(l);
+            over.pop_back();
+        }
+    }
+    for (size_t i : under)
+        bins[i].q = 1.0f;
+    for (size_t i : over)
+        bins[i].q = 1.0f;
+
+    const size_t stride = 3 + value_words;
+    std::vector<uint32_t> words(n * stride, 0u);
+    for (size_t i = 0; i < n; i++) {
+        memcpy(&words[i * stride + 0], &bins[i].p, 4);
+        memcpy(&words[i * stride + 1], &bins[i].q, 4);
+        words[i * stride + 2] = bins[i].alias;
+        for (uint32_t k = 0; k < value_words; k++)
+            words[i * stride + 3 + k] = values[i * value_words + k];
+    }
+    return words;
+}
+
+std::vector<uint32_t> build_light_words(const Scene& scene, const std::vector<uint32_t>& instance_offsets,
+                                        uint32_t* light_count)
+{
+    const uint32_t LIGHT_VALUE_WORDS = PT_LIGHT_BIN_WORDS - 3;     // type + 6 payload words
+    const uint32_t EMITTER_VALUE_WORDS = PT_EMITTER_BIN_WORDS - 3; // 3 vertex indices
+    std::vector<float> weights;
+    std::vector<uint32_t> values;  // LIGHT_VALUE_WORDS per light
+    std::vector<std::vector<uint32_t>> emitter_tables;
+    std::map<uint32_t, size_t> mesh_table;  // mesh -> emitter table index
+
+    for (size_t i = 0; i < scene.variants.size(); i++) {
+        const Variant& var = scene.variants[i];
+        const Material& mat = scene.materials[var.material];
+        if (!mat.is_emitter())
+            continue;
+        if (mat.map_bits() & EMISSION_MAP_BIT)
+            throw std::runtime_error("textured emitters are not supported by this loader");
+        size_t table_index;
+        auto it = mesh_table.find(var.mesh);
+        if (it == mesh_table.end()) {
+            const Mesh& mesh = scene.meshes[var.mesh];
+            if (mesh.tris.empty())
+                throw std::runtime_error("emissive mesh without triangles");
+            std::vector<float> w(mesh.tris.size());
+            std::vector<uint32_t> v(mesh.tris.size() * EMITTER_VALUE_WORDS);
+            for (size_t t = 0; t < mesh.tris.size(); t++) {
+                const Tri& tri = mesh.tris[t];
+                const Vertex& a = mesh.vertices[tri.i[0]];
+                const Vertex& b = mesh.vertices[tri.i[1]];
+                const Vertex& c = mesh.vertices[tri.i[2]];
+                V3 ab(b.pos[0] - a.pos[0], b.pos[1] - a.pos[1], b.pos[2] - a.pos[2]);
+                V3 ac(c.pos[0] - a.pos[0], c.pos[1] - a.pos[1], c.pos[2] - a.pos[2]);
+                w[t] = length(cross(ab, ac));
+                for (int k = 0; k < 3; k++)
+                    v[t * 3 + k] = tri.i[k];
+            }
+            table_index = emitter_tables.size();
+            emitter_tables.push_back(build_alias_table(w, v, EMITTER_VALUE_WORDS));
+            mesh_table[var.mesh] = table_index;
+        } else {
+            table_index = it->second;
+        }
+        // calculate_object_intensity for an untextured emitter (lightsample.cpp:85-86)
+        const float intensity = (mat.emission[0] + mat.emission[1] + mat.emission[2]) * mat.emission[3];
+        weights.push_back(intensity);
+        uint32_t payload[7] = {LIGHT_TYPE_EMITTER,
+                               0u,  // table offset, fixed below
+                               uint32_t(emitter_tables[table_index].size() / PT_EMITTER_BIN_WORDS),
+                               instance_offsets[i],
+                               uint32_t(var.instances.size()),
+                               0u,
+                               0u};
+        values.insert(values.end(), payload, payload + 7);
+    }
+    // Every emitter table offset ends up at the end of the light table: the reference's
+    // running offset adds the size of the still-empty light table (lightsample.cpp:126), so
+    // all emitters address the first emitter table. Kept as is; scenes built for parity
+    // give every emissive object the same geometry.
+    const size_t bin_count = scene.point_lights.size() + scene.dir_lights.size() + emitter_tables.size();
+    for (size_t k = 0; k < weights.size(); k++)
+        values[k * LIGHT_VALUE_WORDS + 1] = uint32_t(bin_count * PT_LIGHT_BIN_WORDS);
+    auto push_infinite = [&](uint32_t type, V3 a, V3 col) {
+        weights.push_back(col.x + col.y + col.z);
+        uint32_t payload[7] = {type, 0, 0, 0, 0, 0, 0};
+        memcpy(&payload[1], &a.x, 4);
+        memcpy(&payload[2], &a.y, 4);
+        memcpy(&payload[3], &a.z, 4);
+        memcpy(&payload[4], &col.x, 4);
+        memcpy(&payload[5], &col.y, 4);
+        memcpy(&payload[6], &col.z, 4);
+        values.insert(values.end(), payload, payload + 7);
+    };
+    for (const PointLight& pl : scene.point_lights)
+        push_infinite(LIGHT_TYPE_POINT, pl.pos, pl.col);
+    for (const DirLight& dl : scene.dir_lights)
+        push_infinite(LIGHT_TYPE_DIRECTIONAL, dl.dir, dl.col);
+
+    std::vector<uint32_t> words = build_alias_table(weights, values, LIGHT_VALUE_WORDS);
+    *light_count = uint32_t(weights.size());
+    for (const auto& t : emitter_tables)
+        words.insert(words.end(), t.begin(), t.end());
+    return words;
+}
+
+// ---------------------------------------------------------------------------------------
+// Packing (src/scene/scenebuffer.cpp:65-168)
+
+void PackedScene::bind_blob()
+{
+    blob = pt_scene_blob{};
+    blob.vertices = vertices.data();
+    blob.vertex_count = vertices.size();
+    blob.indices = indices.data();
+    blob.index_count = indices.size();
+    blob.instances = instances.data();
+    blob.instance_count = uint32_t(instances.size() / PT_INSTANCE_BYTES);
+    blob.materials = materials.data();
+    blob.material_count = uint32_t(materials.size());
+    blob.light_words = light_words.data();
+    blob.light_word_count = light_words.size();
+    blob.light_count = light_count;
+    blob.mesh_count = uint32_t(meshes.size());
+    blob.meshes = meshes.data();
+    blob.instance_mesh = instance_mesh.data();
+    for (int i = 0; i < 4; i++)
+        blob.envcol[i] = envcol[i];
+}
+
+PackedScene pack_scene(const Scene& scene)
+{
+    PackedScene ps;
+    if (scene.variants.size() != scene.meshes.size())
+        throw std::runtime_error("scene has a different number of object variants and meshes");
+    std::vector<pt_mesh_desc> starts(scene.meshes.size() + 1);
+    starts[0] = pt_mesh_desc{0, 0, 0, 0};
+    for (size_t i = 0; i < scene.meshes.size(); i++) {
+        const Mesh& m = scene.meshes[i];
+        ps.vertices.insert(ps.vertices.end(), m.vertices.begin(), m.vertices.end());
+        for (const Tri& t : m.tris)
+            ps.indices.insert(ps.indices.end(), t.i, t.i + 3);
+        pt_mesh_desc d{starts[i].vertex_start, starts[i].index_start, uint32_t(m.tris.size()),
+                       uint32_t(m.vertices.size())};
+        ps.meshes.push_back(d);
+        starts[i + 1].vertex_start = d.vertex_start + d.vertex_count;
+        starts[i + 1].index_start = d.index_start + d.tri_count * 3;
+        if (ps.vertices.size() > 0xFFFFFFFFull || ps.indices.size() > 0xFFFFFFFFull)
+            throw std::runtime_error("scene exceeds 32-bit vertex/index addressing");
+    }
+    std::vector<uint32_t> instance_offsets(scene.variants.size() + 1, 0);
+    uint32_t emitter_index = 0;
+    for (size_t i = 0; i < scene.variants.size(); i++) {
+        const Variant& v = scene.variants[i];
+        const bool emitter = scene.materials[v.material].is_emitter();
+        for (const M4& m : v.instances) {
+            uint8_t rec[PT_INSTANCE_BYTES] = {};
+            uint32_t head[4] = {starts[i].vertex_start, starts[i].index_start,
+                                emitter ? emitter_index++ : PT_NO_EMITTER, v.material};
+            memcpy(rec, head, 16);
+            // transform = transpose(M) in glm's column-major storage == M row-major.
+            const M4 t = transpose(m);
+            memcpy(rec + 16, &t, 64);
+            // normal = Mat3x4(transpose(inverse(transpose(M)))): columns 0-2 of that matrix.
+            const M4 n = transpose(inverse(t));
+            memcpy(rec + 80, &n.c[0], 16);
+            memcpy(rec + 96, &n.c[1], 16);
+            memcpy(rec + 112, &n.c[2], 16);
+            ps.instances.insert(ps.instances.end(), rec, rec + PT_INSTANCE_BYTES);
+            ps.instance_mesh.push_back(v.mesh);
+        }
+        instance_offsets[i + 1] = instance_offsets[i] + uint32_t(v.instances.size());
+    }
+    ps.materials = scene.materials;
+    size_t lights = scene.point_lights.size() + scene.dir_lights.size();
+    for (const Variant& v : scene.variants)
+        if (scene.materials[v.material].is_emitter())
+            lights += v.instances.size();
+    if (lights > 0)
+        ps.light_words = build_light_words(scene, instance_offsets, &ps.light_count);
+    ps.envcol[0] = scene.envcol.x;
+    ps.envcol[1] = scene.envcol.y;
+    ps.envcol[2] = scene.envcol.z;
+    ps.envcol[3] = 0.0f;
+    ps.bind_blob();
+    return ps;
+}
+
+// ---------------------------------------------------------------------------------------
+// Camera
+
+void camera_block(const CameraSection& cam, float aspect, pt_camera* out)
+{
+    const V3 x = normalize(cam.lookdir);
+    const V3 y = normalize(cross(cam.up, x));
+    const V3 z = cross(x, y);
+    const M4 rot = mat4_from_quat(quat_from_mat3(y, z, x));
+    const M4 view = inverse(translation(V3(-cam.position.x, -cam.position.y, -cam.position.z)) * rot);
+    M4 proj = perspective_rh_zo(radians(cam.fov), aspect, cam.near_clip, cam.far_clip);
+    proj.at(1, 1) *= -1.0f;
+    const M4 inv_view = inverse(view);
+    const M4 inv_proj = inverse(proj);
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            out->inv_view[r * 4 + c] = inv_view.at(c, r);
+            out->inv_proj[r * 4 + c] = inv_proj.at(c, r);
+        }
+    out->near_clip = cam.near_clip;
+    out->far_clip = cam.far_clip;
+    out->focus_dist = cam.focus_dist;
+    out->lens_radius = cam.lens_radius;
+    out->exposure = cam.exposure;
+}
+
+}  // namespace pth

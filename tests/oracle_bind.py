@@ -1,0 +1,121 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so). Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+import pt_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_lib = None
+
+
+class Counts(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise ImportError("oracle/liboracle.so is not built: run make -C oracle")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.oracle_scene_create.argtypes = [C.POINTER(pt_amd.SceneBlob), C.POINTER(P)]
+        L.oracle_scene_free.argtypes = [P]
+        L.oracle_render.argtypes = [P, C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_void_p, C.c_uint32, C.c_int, C.POINTER(Counts)]
+        L.oracle_primary.argtypes = [P, C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p]
+        L.oracle_trace_rays.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
+        L.oracle_sampler_draw.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                          C.c_void_p]
+        L.oracle_eval_brdf.argtypes = [C.c_void_p] * 6
+        L.oracle_sample_direction.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]
+        L.oracle_camera_ray.argtypes = [C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_sample_light.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Oracle:
+    def __init__(self, scene):
+        h = C.c_void_p()
+        rc = lib().oracle_scene_create(scene.blob, C.byref(h))
+        if rc != 0:
+            raise RuntimeError("oracle_scene_create failed: %d" % rc)
+        self._h = h
+        self.scene = scene
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_scene_free(self._h)
+            self._h = None
+
+    def render(self, cam, width, height, spp, max_bounces, sample_index=0, accum=None, rows_step=1, threads=0):
+        if accum is None:
+            accum = np.zeros((height, width, 4), np.float32)
+        cnt = Counts()
+        rc = lib().oracle_render(self._h, C.byref(cam), width, height, sample_index, spp, max_bounces,
+                                 accum.ctypes.data, rows_step, threads, C.byref(cnt))
+        assert rc == 0, rc
+        return accum, {k: getattr(cnt, k) for k, _ in Counts._fields_}
+
+    def primary(self, cam, width, height, sample_index, want_rays=False):
+        hits = np.zeros(width * height, pt_amd.HIT_DTYPE)
+        rays = np.zeros(width * height, pt_amd.RAY_DTYPE) if want_rays else None
+        rc = lib().oracle_primary(self._h, C.byref(cam), width, height, sample_index, hits.ctypes.data,
+                                  rays.ctypes.data if want_rays else None)
+        assert rc == 0
+        hits = hits.reshape(height, width)
+        return (hits, rays.reshape(height, width)) if want_rays else hits
+
+    def trace_rays(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, dtype=pt_amd.RAY_DTYPE)
+        out = np.zeros(len(rays), pt_amd.HIT_DTYPE)
+        assert lib().oracle_trace_rays(self._h, rays.ctypes.data, len(rays), out.ctypes.data, int(any_hit)) == 0
+        return out
+
+    def sample_light(self, point, u3):
+        e, d, dist, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), C.c_float(), C.c_float()
+        rc = lib().oracle_sample_light(self._h, _f32(point).ctypes.data, _f32(u3).ctypes.data, e.ctypes.data,
+                                       d.ctypes.data, C.byref(dist), C.byref(pdf))
+        assert rc == 0
+        return e, d, dist.value, pdf.value
+
+
+def sampler_draw(px, py, sample_index, seed, pattern):
+    pat = np.asarray(pattern, np.uint8)
+    out = np.zeros(int(sum(pattern)), np.float32)
+    lib().oracle_sampler_draw(px, py, sample_index, seed, pat.ctypes.data, len(pat), out.ctypes.data)
+    return out
+
+
+def eval_brdf(params, i, o, m):
+    rgb = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().oracle_eval_brdf(_f32(params).ctypes.data, _f32(i).ctypes.data, _f32(o).ctypes.data, _f32(m).ctypes.data,
+                           rgb.ctypes.data, pdf.ctypes.data)
+    return rgb, float(pdf[0])
+
+
+def sample_direction(o, alpha, metalness, u3):
+    i, m = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    w, pdf = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib().oracle_sample_direction(_f32(o).ctypes.data, _f32(alpha).ctypes.data, metalness, _f32(u3).ctypes.data,
+                                  i.ctypes.data, m.ctypes.data, w.ctypes.data, pdf.ctypes.data)
+    return i, m, float(w[0]), float(pdf[0])
+
+
+def camera_ray(cam, px, py, w, h, u_pixel, u_lens):
+    o, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().oracle_camera_ray(C.byref(cam), px, py, w, h, _f32(u_pixel).ctypes.data, _f32(u_lens).ctypes.data,
+                            o.ctypes.data, d.ctypes.data)
+    return o, d
