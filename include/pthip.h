@@ -103,11 +103,13 @@ typedef struct pt_hit {
     uint32_t primitive;
 } pt_hit;
 
+/* Counters and times accumulate over the pt_render calls since the last pt_set_camera,
+ * pt_set_max_bounces, pt_set_option or pt_reset_stats. */
 typedef struct pt_stats {
-    uint64_t extension_rays; /* rays traced by closest-hit traversal, last pt_render */
-    uint64_t shadow_rays;    /* rays traced by any-hit traversal, last pt_render */
-    uint64_t paths;          /* pixel samples, last pt_render */
-    double render_ms;        /* device time of the last pt_render (HIP events) */
+    uint64_t extension_rays; /* rays traced by closest-hit traversal */
+    uint64_t shadow_rays;    /* rays traced by any-hit traversal */
+    uint64_t paths;          /* pixel samples */
+    double render_ms;        /* device time from the first to the last pt_render (HIP events) */
     double bvh_build_ms;     /* device time of the BLAS+TLAS build in pt_create */
     uint64_t bvh_bytes;      /* bytes of nodes + triangle records */
     uint32_t sample_index;   /* next sample index (PathTracer::get_sample_index) */
@@ -116,7 +118,26 @@ typedef struct pt_stats {
     uint32_t height;
     uint32_t node_count;
     uint32_t tri_record_count;
+    /* With PT_OPT_COUNT_FETCHES: 128-byte node and 48-byte triangle-record fetches. */
+    uint64_t ext_node_fetches;
+    uint64_t ext_tri_fetches;
+    uint64_t shadow_node_fetches;
+    uint64_t shadow_tri_fetches;
+    /* With PT_OPT_PROFILE_KERNELS: summed HIP-event durations and launch counts of the
+     * extend (closest hit), shade and shadow (any hit) stages of the last pt_render. */
+    double extend_ms;
+    double shade_ms;
+    double shadow_ms;
+    uint32_t extend_launches;
+    uint32_t shade_launches;
+    uint32_t shadow_launches;
+    uint32_t reserved;
 } pt_stats;
+
+/* Options for pt_set_option (instrumentation; no counterpart in the reference, whose only
+ * instrumentation is the wall-clock print of src/pathtrace/ptdispatch.cpp:65-70). */
+#define PT_OPT_PROFILE_KERNELS 1u /* value != 0: time every stage launch with HIP events */
+#define PT_OPT_COUNT_FETCHES 2u   /* value != 0: count BVH node / triangle fetches per ray */
 
 typedef struct pt_ctx pt_ctx;
 
@@ -155,6 +176,12 @@ int pt_trace_rays(pt_ctx* ctx, const pt_ray* rays, uint32_t count, pt_hit* hits,
 
 /* Counters and timings (the reference prints its timing in ptdispatch.cpp:65-70). */
 int pt_get_stats(pt_ctx* ctx, pt_stats* out);
+
+/* Restarts the pt_stats counters at the next pt_render (no counterpart in the reference). */
+int pt_reset_stats(pt_ctx* ctx);
+
+/* Sets an instrumentation option (PT_OPT_*). */
+int pt_set_option(pt_ctx* ctx, uint32_t option, int64_t value);
 
 /* Waits for all work queued on the context's stream. */
 int pt_synchronize(pt_ctx* ctx);

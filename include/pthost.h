@@ -82,6 +82,12 @@ int pth_read_hdr(const char* path, float** out, uint32_t* width, uint32_t* heigh
 /* util/meansq/meansq.cpp:30-41: sum over RGB of (img - ref)^2 / (3*W*H - 1). */
 double pth_meansq(const float* img_rgba, const float* ref_rgba, uint32_t width, uint32_t height);
 
+/* AliasTable<T>::build (src/aliastable.h:21-67) + copy (aliastable.h:73-77): bins of
+ * {f32 p, f32 q, u32 alias, value_words value words} for n weights, written to out_words
+ * (n * (3 + value_words) words). */
+int pth_alias_table(const float* weights, uint32_t n, const uint32_t* values, uint32_t value_words,
+                    uint32_t* out_words);
+
 /* from_srgb / to_srgb (src/colour.cpp:3-15). */
 float pth_from_srgb(float x);
 float pth_to_srgb(float x);

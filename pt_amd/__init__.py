@@ -76,7 +76,15 @@ class Stats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
                 ("render_ms", C.c_double), ("bvh_build_ms", C.c_double), ("bvh_bytes", C.c_uint64),
                 ("sample_index", C.c_uint32), ("max_bounces", C.c_uint32), ("width", C.c_uint32),
-                ("height", C.c_uint32), ("node_count", C.c_uint32), ("tri_record_count", C.c_uint32)]
+                ("height", C.c_uint32), ("node_count", C.c_uint32), ("tri_record_count", C.c_uint32),
+                ("ext_node_fetches", C.c_uint64), ("ext_tri_fetches", C.c_uint64),
+                ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64),
+                ("extend_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_ms", C.c_double),
+                ("extend_launches", C.c_uint32), ("shade_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+PT_OPT_PROFILE_KERNELS = 1
+PT_OPT_COUNT_FETCHES = 2
 
 
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance", "<u4"), ("primitive", "<u4")])
@@ -110,6 +118,7 @@ def host_lib():
         lib.pth_to_srgb.argtypes = [C.c_float]
         lib.pth_to_srgb.restype = C.c_float
         lib.pth_free.argtypes = [C.c_void_p]
+        lib.pth_alias_table.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.pth_last_error.restype = C.c_char_p
         _host = lib
     return _host
@@ -133,6 +142,8 @@ def hip_lib():
         lib.pt_trace_rays.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
         lib.pt_get_stats.argtypes = [P, C.POINTER(Stats)]
         lib.pt_synchronize.argtypes = [P]
+        lib.pt_set_option.argtypes = [P, C.c_uint32, C.c_int64]
+        lib.pt_reset_stats.argtypes = [P]
         lib.pt_last_error.argtypes = [P]
         lib.pt_last_error.restype = C.c_char_p
         lib.pt_destroy.argtypes = [P]
@@ -143,6 +154,15 @@ def hip_lib():
 def _check_host(code):
     if code != PT_OK:
         raise PtError(code, host_lib().pth_last_error().decode())
+
+
+def alias_table(weights, values, value_words=1):
+    """AliasTable::build + copy: returns the packed words (uint32)."""
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    out = np.zeros(len(w) * (3 + value_words), np.uint32)
+    _check_host(host_lib().pth_alias_table(w.ctypes.data, len(w), v.ctypes.data, value_words, out.ctypes.data))
+    return out
 
 
 def from_srgb(x):
@@ -296,6 +316,12 @@ class Tracer:
 
     def render(self, spp):
         self._check(self._lib.pt_render(self._h, spp))
+
+    def reset_stats(self):
+        self._check(self._lib.pt_reset_stats(self._h))
+
+    def set_option(self, option, value):
+        self._check(self._lib.pt_set_option(self._h, option, int(value)))
 
     def synchronize(self):
         self._check(self._lib.pt_synchronize(self._h))

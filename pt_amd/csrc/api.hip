@@ -63,6 +63,7 @@ void release(Ctx& c)
             (void)hipFree(p);
     free_bvh(c.bvh);
     wavefront_free(c.wf);
+    c.timer.destroy();
     if (c.ev_start)
         (void)hipEventDestroy(c.ev_start);
     if (c.ev_stop)
@@ -342,6 +343,7 @@ int pt_set_camera(pt_ctx* h, const pt_camera* cam)
     memcpy(&h->c.cam, cam, sizeof(CameraBlock));
     h->c.have_camera = true;
     h->c.sample_index = 0;
+    h->c.stats_fresh = true;
     return PT_OK;
 }
 
@@ -353,6 +355,7 @@ int pt_set_max_bounces(pt_ctx* h, uint32_t max_bounces)
         return fail(&h->c, PT_EINVAL, "max_bounces too large");
     h->c.max_bounces = max_bounces;
     h->c.sample_index = 0;
+    h->c.stats_fresh = true;
     return PT_OK;
 }
 
@@ -367,11 +370,17 @@ int pt_render(pt_ctx* h, uint32_t spp)
         return fail(&c, PT_ESTATE, "no camera set");
     if (uint64_t(c.sample_index) + spp > 0xFFFFFFFFull)
         return fail(&c, PT_EINVAL, "sample index overflow");
-    HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
+    if (c.stats_fresh) {
+        c.timer.reset();
+        HIP_TRY(&c, hipMemsetAsync(c.wf.totals, 0, 8 * 8, c.stream), "stats");
+        HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
+        c.last_paths = 0;
+        c.stats_fresh = false;
+    }
     HIP_TRY(&c, render_samples(c, spp), "render");
     HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
     c.sample_index += spp;
-    c.last_paths = uint64_t(spp) * c.width * c.height;
+    c.last_paths += uint64_t(spp) * c.width * c.height;
     return PT_OK;
 }
 
@@ -452,7 +461,7 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
     if (!out)
         return fail(&c, PT_EINVAL, "out is null");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
-    unsigned long long totals[2] = {0, 0};
+    unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
     uint32_t counters[4] = {0, 0, 0, 0};
     HIP_TRY(&c, hipMemcpy(totals, c.wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
     HIP_TRY(&c, hipMemcpy(counters, c.wf.counters, sizeof(counters), hipMemcpyDeviceToHost), "stats");
@@ -472,10 +481,52 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
     s.height = c.height;
     s.node_count = c.bvh.node_count;
     s.tri_record_count = c.bvh.tri_count;
+    s.ext_node_fetches = totals[2];
+    s.ext_tri_fetches = totals[3];
+    s.shadow_node_fetches = totals[4];
+    s.shadow_tri_fetches = totals[5];
+    if (c.timer.enabled) {
+        double kms[KERNEL_COUNT];
+        uint32_t kl[KERNEL_COUNT];
+        HIP_TRY(&c, c.timer.collect(kms, kl), "kernel timing");
+        s.extend_ms = kms[KERNEL_EXTEND];
+        s.shade_ms = kms[KERNEL_SHADE];
+        s.shadow_ms = kms[KERNEL_SHADOW];
+        s.extend_launches = kl[KERNEL_EXTEND];
+        s.shade_launches = kl[KERNEL_SHADE];
+        s.shadow_launches = kl[KERNEL_SHADOW];
+    }
     *out = s;
     if (counters[3])
         return fail(&c, PT_EDEVICE, "traversal stack overflow");
     return PT_OK;
+}
+
+int pt_reset_stats(pt_ctx* h)
+{
+    if (!h)
+        return PT_EINVAL;
+    h->c.stats_fresh = true;
+    return PT_OK;
+}
+
+int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    switch (option) {
+    case PT_OPT_PROFILE_KERNELS:
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.timer.enabled = value != 0;
+        c.stats_fresh = true;
+        return PT_OK;
+    case PT_OPT_COUNT_FETCHES:
+        c.count_fetches = value != 0;
+        return PT_OK;
+    default:
+        return fail(&c, PT_EINVAL, "unknown option " + std::to_string(option));
+    }
 }
 
 int pt_synchronize(pt_ctx* h)

@@ -70,13 +70,38 @@ struct BuildSet {
 __global__ void k_centroid_bounds(uint32_t n, const float4* lo, const float4* hi, const uint32_t* seg,
                                   uint32_t* cmin, uint32_t* cmax)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float4 a = lo[i], b = hi[i];
-        const uint32_t s = seg[i];
-        const float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
-        for (int k = 0; k < 3; k++) {
-            atomicMin(&cmin[s * 3 + k], ordered(c[k]));
-            atomicMax(&cmax[s * 3 + k], ordered(c[k]));
+    // Whole waves iterate together so that a wave whose 64 primitives share one segment
+    // reduces in registers and issues 6 atomics instead of 384.
+    const uint32_t rounded = (n + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
+        const bool valid = i < n;
+        uint32_t s = 0xFFFFFFFFu;
+        uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+        if (valid) {
+            const float4 a = lo[i], b = hi[i];
+            s = seg[i];
+            const float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+            for (int k = 0; k < 3; k++)
+                mn[k] = mx[k] = ordered(c[k]);
+        }
+        const uint32_t s0 = __shfl(s, 0);
+        const bool uniform = __all(!valid || s == s0) && s0 != 0xFFFFFFFFu;
+        if (uniform) {
+            for (int off = 32; off >= 1; off >>= 1)
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = min(mn[k], (uint32_t)__shfl_xor(mn[k], off));
+                    mx[k] = max(mx[k], (uint32_t)__shfl_xor(mx[k], off));
+                }
+            if (__lane_id() == 0)
+                for (int k = 0; k < 3; k++) {
+                    atomicMin(&cmin[s0 * 3 + k], mn[k]);
+                    atomicMax(&cmax[s0 * 3 + k], mx[k]);
+                }
+        } else if (valid) {
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&cmin[s * 3 + k], mn[k]);
+                atomicMax(&cmax[s * 3 + k], mx[k]);
+            }
         }
     }
 }

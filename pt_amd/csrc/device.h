@@ -31,9 +31,66 @@ struct Wavefront {
     float4* s_d = nullptr;         // direction
     float4* s_c = nullptr;         // unoccluded contribution
     uint32_t* counters = nullptr;  // [0],[1] extension queues, [2] shadow queue, [3] error flags
-    unsigned long long* totals = nullptr;  // [0] extension rays, [1] shadow rays
+    // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
+    // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
+    unsigned long long* totals = nullptr;
     uint32_t* spill = nullptr;     // traversal stack spill, SPILL_STACK words per grid thread
     uint32_t spill_threads = 0;
+};
+
+enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_COUNT = 3 };
+
+// Optional per-launch timing of the main stages with events on the context's stream.
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> marks;  // (kernel, event index of the start mark)
+    size_t used = 0;
+
+    hipError_t mark(hipStream_t st, int kernel, bool start)
+    {
+        if (!enabled)
+            return hipSuccess;
+        if (used == pool.size()) {
+            hipEvent_t ev;
+            hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess)
+                return e;
+            pool.push_back(ev);
+        }
+        if (start)
+            marks.push_back({kernel, used});
+        return hipEventRecord(pool[used++], st);
+    }
+    void reset()
+    {
+        used = 0;
+        marks.clear();
+    }
+    // Sums the durations per kernel (the stream must be idle).
+    hipError_t collect(double* ms, uint32_t* launches)
+    {
+        for (int k = 0; k < KERNEL_COUNT; k++) {
+            ms[k] = 0.0;
+            launches[k] = 0;
+        }
+        for (const auto& m : marks) {
+            float t = 0.0f;
+            hipError_t e = hipEventElapsedTime(&t, pool[m.second], pool[m.second + 1]);
+            if (e != hipSuccess)
+                return e;
+            ms[m.first] += t;
+            launches[m.first]++;
+        }
+        return hipSuccess;
+    }
+    void destroy()
+    {
+        for (hipEvent_t ev : pool)
+            (void)hipEventDestroy(ev);
+        pool.clear();
+        reset();
+    }
 };
 
 struct Ctx {
@@ -66,6 +123,9 @@ struct Ctx {
 
     Wavefront wf;
     uint32_t grid_blocks = 0;  // persistent grid of the traversal kernels
+    bool count_fetches = false;
+    KernelTimer timer;
+    bool stats_fresh = true;  // counters restart at the next pt_render
 
     // stats of the last render
     unsigned long long last_ext = 0, last_shadow = 0, last_paths = 0;

@@ -162,9 +162,9 @@ struct HitOut {
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
 // transmittance through transparent instances is returned in trans (shadowray.ah.hlsl).
-template <bool ANY>
+template <bool ANY, bool COUNT = false>
 __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stack& stk, HitOut& hit,
-                         float& trans)
+                         float& trans, uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr)
 {
     hit.inst = NONE;
     trans = 1.0f;
@@ -182,6 +182,8 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
 
     while (true) {
         // ---- visit an internal node
+        if (COUNT)
+            ++*n_nodes;
         const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
         const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
         const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
@@ -230,6 +232,8 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 const uint32_t first = c & LEAF_FIRST_MASK;
                 const uint32_t count = ((c >> 27) & 0xFu) + 1u;
                 for (uint32_t q = 0; q < count; q++) {
+                    if (COUNT)
+                        ++*n_tris;
                     const TriRec tr = s.tris[first + q];
                     float t, b1, b2;
                     if (!intersect_tri(r, xyz(tr.v0), xyz(tr.v1), xyz(tr.v2), tmin, best, t, b1, b2))
@@ -361,6 +365,7 @@ __global__ __launch_bounds__(256) void k_generate(CameraBlock cam, BatchParams b
     }
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
@@ -368,20 +373,26 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
         HitOut h;
         float trans;
-        traverse<false>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans);
+        traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &n_nodes, &n_tris);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
         if (stk.overflow)
             atomicOr(&wf.counters[3], 1u);
     }
+    if (COUNT && (n_nodes | n_tris)) {
+        atomicAdd(&wf.totals[2], (unsigned long long)n_nodes);
+        atomicAdd(&wf.totals[3], (unsigned long long)n_tris);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&wf.totals[0], (unsigned long long)count);
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront wf)
 {
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
@@ -389,11 +400,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
         HitOut h;
         float trans;
-        const bool occluded = traverse<true>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans);
+        const bool occluded = traverse<true, COUNT>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
         if (!occluded && trans >= 1e-5f) {
             const uint32_t p = wf.s_path[i];
             const float4 c = wf.s_c[i];
@@ -405,6 +417,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
         }
         if (stk.overflow)
             atomicOr(&wf.counters[3], 1u);
+    }
+    if (COUNT && (n_nodes | n_tris)) {
+        atomicAdd(&wf.totals[4], (unsigned long long)n_nodes);
+        atomicAdd(&wf.totals[5], (unsigned long long)n_tris);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&wf.totals[1], (unsigned long long)count);
@@ -723,12 +739,12 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_thre
     A(wf.s_d, c * 16);
     A(wf.s_c, c * 16);
     A(wf.counters, 16 * 4);
-    A(wf.totals, 4 * 8);
+    A(wf.totals, 8 * 8);
     A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
     if (e == hipSuccess)
         e = hipMemset(wf.counters, 0, 16 * 4);
     if (e == hipSuccess)
-        e = hipMemset(wf.totals, 0, 4 * 8);
+        e = hipMemset(wf.totals, 0, 8 * 8);
     return e;
 }
 
@@ -749,8 +765,6 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
     const uint32_t per_batch = std::max<uint32_t>(1, wf.capacity / npix);
     hipStream_t st = ctx.stream;
     hipError_t e;
-    if ((e = hipMemsetAsync(wf.totals, 0, 4 * 8, st)) != hipSuccess)
-        return e;
     uint32_t done = 0;
     while (done < spp) {
         BatchParams bp{};
@@ -767,9 +781,27 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
             bp.segment = seg;
             bp.final_segment = seg >= ctx.max_bounces ? 1u : 0u;
             hipLaunchKernelGGL(k_prepare_segment, dim3(1), dim3(64), 0, st, wf, q);
-            hipLaunchKernelGGL(k_extend, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
+            if ((e = ctx.timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
+                return e;
+            if (ctx.count_fetches)
+                hipLaunchKernelGGL(k_extend<true>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
+            else
+                hipLaunchKernelGGL(k_extend<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
+            if ((e = ctx.timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
+                return e;
+            if ((e = ctx.timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
+                return e;
             hipLaunchKernelGGL(k_shade, dim3(blocks_for(n, 256, 65535u * 4)), dim3(256), 0, st, ctx.scene, bp, wf, q);
-            hipLaunchKernelGGL(k_shadow, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
+            if ((e = ctx.timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
+                return e;
+            if ((e = ctx.timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
+                return e;
+            if (ctx.count_fetches)
+                hipLaunchKernelGGL(k_shadow<true>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
+            else
+                hipLaunchKernelGGL(k_shadow<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
+            if ((e = ctx.timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
+                return e;
             q = 1 - q;
         }
         hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256, 65535u * 4)), dim3(256), 0, st, bp, wf.rad,

@@ -158,6 +158,21 @@ double pth_meansq(const float* img, const float* ref, uint32_t width, uint32_t h
     return pth::meansq(img, ref, width, height);
 }
 
+int pth_alias_table(const float* weights, uint32_t n, const uint32_t* values, uint32_t value_words,
+                    uint32_t* out_words)
+{
+    if (!weights || n == 0 || (value_words && !values) || !out_words) {
+        g_error = "invalid argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] {
+        std::vector<float> w(weights, weights + n);
+        std::vector<uint32_t> v(values, values + size_t(n) * value_words);
+        std::vector<uint32_t> words = pth::build_alias_table(w, v, value_words);
+        memcpy(out_words, words.data(), words.size() * 4);
+    });
+}
+
 float pth_from_srgb(float x) { return pth::from_srgb(x); }
 float pth_to_srgb(float x) { return pth::to_srgb(x); }
 

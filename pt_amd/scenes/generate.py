@@ -103,15 +103,15 @@ def _value_noise(x, z, lattice):
     return (a * (1 - sx) + b * sx) * (1 - sz) + (c * (1 - sx) + d * sx) * sz
 
 
-def terrain(res, seed, amplitude=0.25, octaves=4):
-    """res x res quads over [-1,1]^2, height = 4-octave value noise (seeded lattices)."""
-    g = np.linspace(-1.0, 1.0, res + 1)
+def terrain(res, seed, amplitude=0.25, octaves=4, half_size=1.0):
+    """res x res quads over [-s,s]^2, height = 4-octave value noise (seeded lattices)."""
+    g = np.linspace(-half_size, half_size, res + 1)
     x, z = np.meshgrid(g, g, indexing="ij")
     h = np.zeros_like(x)
     freq, amp = 4.0, 1.0
     for o in range(octaves):
         lattice = splitmix32_floats(seed + 7919 * o, 64 * 64).reshape(64, 64).astype(np.float64)
-        h += amp * (_value_noise((x + 1) * freq, (z + 1) * freq, lattice) - 0.5)
+        h += amp * (_value_noise((x / half_size + 1) * freq, (z / half_size + 1) * freq, lattice) - 0.5)
         freq *= 2.0
         amp *= 0.5
     y = amplitude * h
@@ -120,7 +120,7 @@ def terrain(res, seed, amplitude=0.25, octaves=4):
     nrm = np.stack([-dx, np.ones_like(y), -dz], axis=-1)
     nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
     pos = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float32)
-    uv = np.stack([(x + 1) / 2, (z + 1) / 2], axis=-1).reshape(-1, 2).astype(np.float32)
+    uv = np.stack([(x / half_size + 1) / 2, (z / half_size + 1) / 2], axis=-1).reshape(-1, 2).astype(np.float32)
     r = res + 1
     i = np.arange(res)[:, None]
     j = np.arange(res)[None, :]
@@ -274,7 +274,7 @@ INSTANCED_EMITTERS = 20
 
 
 def instanced_scene(out_dir, seed=2, counts=None, emitters=INSTANCED_EMITTERS, grid_res=512,
-                    soup_tris=200000, camera_distance=120.0):
+                    soup_tris=200000, camera_z=0.0):
     """300 instances of 5 seeded meshes with per-instance materials. Returns the .scn path.
 
     Each instance is its own .scn object (its own glTF file, one material) because a .scn
@@ -288,7 +288,7 @@ def instanced_scene(out_dir, seed=2, counts=None, emitters=INSTANCED_EMITTERS, g
         "ico2": icosphere(2),
         "ico4": icosphere(4),
         "ico6": icosphere(6),
-        "grid": terrain(grid_res, seed * 1000 + 17),
+        "grid": terrain(grid_res, seed * 1000 + 17, amplitude=2.0, half_size=8.0),
         "soup": triangle_soup(soup_tris, seed * 1000 + 29),
     }
     layouts = {}
@@ -335,8 +335,10 @@ def instanced_scene(out_dir, seed=2, counts=None, emitters=INSTANCED_EMITTERS, g
         k += 1
     scn = os.path.join(out_dir, "instanced.scn")
     with open(scn, "w") as f:
-        f.write("camera:\n position 0 0 %.9g\n lookdir 0 0 -1\n fov 60\n focusdist %.9g\n lensradius 0.03\n"
-                " exposure 1\n envcol 0.5 0.55 0.6\n\nobjects:\n" % (camera_distance, camera_distance))
+        # The eye sits inside the instance cloud (the .scn position is negated by the
+        # camera's view matrix, src/scene/camera.h:16) and looks along +z.
+        f.write("camera:\n position 0 0 %.9g\n lookdir 0 0 -1\n fov 60\n focusdist 30\n lensradius 0.03\n"
+                " exposure 1\n envcol 0.5 0.55 0.6\n\nobjects:\n" % (-camera_z))
         f.write("\n".join(lines_decl) + "\n\n" + "\n".join(lines_inst) + "\n")
     return scn
 

@@ -85,7 +85,8 @@ class Oracle:
 
     def sample_light(self, point, u3):
         e, d, dist, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), C.c_float(), C.c_float()
-        rc = lib().oracle_sample_light(self._h, _f32(point).ctypes.data, _f32(u3).ctypes.data, e.ctypes.data,
+        pt, u = _f32(point), _f32(u3)  # held for the duration of the call
+        rc = lib().oracle_sample_light(self._h, pt.ctypes.data, u.ctypes.data, e.ctypes.data,
                                        d.ctypes.data, C.byref(dist), C.byref(pdf))
         assert rc == 0
         return e, d, dist.value, pdf.value
@@ -101,7 +102,8 @@ def sampler_draw(px, py, sample_index, seed, pattern):
 def eval_brdf(params, i, o, m):
     rgb = np.zeros(3, np.float32)
     pdf = np.zeros(1, np.float32)
-    lib().oracle_eval_brdf(_f32(params).ctypes.data, _f32(i).ctypes.data, _f32(o).ctypes.data, _f32(m).ctypes.data,
+    a = [_f32(params), _f32(i), _f32(o), _f32(m)]  # held for the duration of the call
+    lib().oracle_eval_brdf(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, a[3].ctypes.data,
                            rgb.ctypes.data, pdf.ctypes.data)
     return rgb, float(pdf[0])
 
@@ -109,13 +111,14 @@ def eval_brdf(params, i, o, m):
 def sample_direction(o, alpha, metalness, u3):
     i, m = np.zeros(3, np.float32), np.zeros(3, np.float32)
     w, pdf = np.zeros(1, np.float32), np.zeros(1, np.float32)
-    lib().oracle_sample_direction(_f32(o).ctypes.data, _f32(alpha).ctypes.data, metalness, _f32(u3).ctypes.data,
+    a = [_f32(o), _f32(alpha), _f32(u3)]  # held for the duration of the call
+    lib().oracle_sample_direction(a[0].ctypes.data, a[1].ctypes.data, metalness, a[2].ctypes.data,
                                   i.ctypes.data, m.ctypes.data, w.ctypes.data, pdf.ctypes.data)
     return i, m, float(w[0]), float(pdf[0])
 
 
 def camera_ray(cam, px, py, w, h, u_pixel, u_lens):
     o, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
-    lib().oracle_camera_ray(C.byref(cam), px, py, w, h, _f32(u_pixel).ctypes.data, _f32(u_lens).ctypes.data,
-                            o.ctypes.data, d.ctypes.data)
+    up, ul = _f32(u_pixel), _f32(u_lens)  # held for the duration of the call
+    lib().oracle_camera_ray(C.byref(cam), px, py, w, h, up.ctypes.data, ul.ctypes.data, o.ctypes.data, d.ctypes.data)
     return o, d

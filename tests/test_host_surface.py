@@ -1,0 +1,190 @@
+"""Host-side surface: .scn parsing semantics and errors, glTF loading, the packed scene and
+light tables, the CLI's argument handling, and the C ABI's exported symbols and argument
+checks (no GPU needed; nothing here launches a kernel)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pt_amd
+from pt_amd.scenes import generate
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _scn(tmp_path, text, gltf=True):
+    if gltf:
+        from test_golden_host import _write_one_tri_gltf
+        _write_one_tri_gltf(str(tmp_path / "tri.gltf"))
+    p = tmp_path / "s.scn"
+    p.write_text(text)
+    return str(p)
+
+
+# --- .scn semantics (src/scene/scenebuild.cpp) ----------------------------------------------
+
+def test_scn_defaults_and_envcol_srgb(tmp_path):
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\n envcol 0.5 0.25 1\nobjects:\n tri tri.gltf\n tri\n"))
+    p = s.camera_params(400, 200)
+    assert list(p.lookdir) == [0, 0, -1] and list(p.up) == [0, 1, 0]
+    assert p.fov_deg == 45 and abs(p.near_clip - 0.1) < 1e-7 and p.far_clip == 1000
+    assert p.focus_dist == 0 and p.lens_radius == 0 and p.exposure == 1 and p.aspect == 2.0
+    env = s.arrays()["envcol"]
+    np.testing.assert_allclose(env[:3], [pt_amd.from_srgb(0.5), pt_amd.from_srgb(0.25), 1.0], rtol=1e-6)
+
+
+def test_scn_tabs_and_carriage_returns_are_removed(tmp_path):
+    # "\t" and "\r" are deleted, not replaced (scenebuild.cpp:284-286)
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\r\n\tfov\t 30\r\nobjects:\r\n tri tri.gltf\r\n tri\r\n"))
+    assert s.camera_params(1, 1).fov_deg == 30
+
+
+def test_scn_transform_order_and_material_keywords(tmp_path):
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n tri\n  basecol 1 0 0\n  s 2 2 2\n"
+                                    "  t 1 0 0\n tri\n  t 1 0 0\n  s 2 2 2\n"))
+    inst = s.arrays()["instances"].view(np.float32).reshape(-1, 32)
+    m0 = inst[0, 4:20].reshape(4, 4)  # row-major M
+    m1 = inst[1, 4:20].reshape(4, 4)
+    assert m0[0, 3] == 1.0 and m0[0, 0] == 2.0   # T * S: translation unscaled
+    assert m1[0, 3] == 2.0 and m1[0, 0] == 2.0   # S * T: translation scaled
+
+
+def test_scn_rotation_below_threshold_is_identity(tmp_path):
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n tri\n  r 0 0 0 0.000001\n"))
+    inst = s.arrays()["instances"].view(np.float32).reshape(-1, 32)
+    np.testing.assert_array_equal(inst[0, 4:20].reshape(4, 4), np.eye(4, dtype=np.float32))
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("objects:\n tri tri.gltf\n tri\n", "No camera section"),
+    ("camera:\ncamera:\n", "Camera already set"),
+    ("camera:\n bogus 1\n", "Unrecognised token in camera section"),
+    ("stuff:\n", "Unrecognised token"),
+    ("camera:\nobjects:\n tri tri.gltf\n other\n", "Object other not recognised"),
+    ("camera:\nobjects:\n t 1 2 3\n", "No instance to translate"),
+    ("camera:\nobjects:\n tri tri.gltf\n tri\n r 1 0 0\n", "Rotation did not have three coordinates and angle"),
+    ("camera:\nobjects:\n tri tri.gltf\n tri\n s 1 1\n", "Scaling did not have three coordinates"),
+    ("camera:\nobjects:\n tri tri.gltf\n tri\n r 0 0 0 1\n", "Rotation axis was (close to) zero"),
+    ("camera:\nobjects:\n plight 1 2 3 4 5 6\n", "Invalid light syntax"),
+    ("camera:\n position 1 2\n", "Vectors need three components"),
+    ("camera:\nobjects:\n tri missing.gltf\n tri\n", "Could not open file"),
+])
+def test_scn_errors(tmp_path, text, msg):
+    with pytest.raises(pt_amd.PtError, match=re.escape(msg)):
+        pt_amd.Scene(_scn(tmp_path, text))
+
+
+def test_point_and_directional_lights_in_light_table(tmp_path):
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n plight 0 2 0 , 1 1 1\n dlight 0 -1 0 , 0.5 0.5 0.5\n"
+                                    " tri tri.gltf\n tri\n"))
+    a = s.arrays()
+    assert a["light_count"] == 2
+    words = a["light_words"].reshape(2, 10)
+    assert words[0, 3] == 1 and words[1, 3] == 2  # LIGHT_TYPE_POINT, LIGHT_TYPE_DIRECTIONAL
+    np.testing.assert_allclose(words[0, 4:7].view(np.float32), [0, 2, 0])
+    np.testing.assert_allclose(words[1, 7:10].view(np.float32), [pt_amd.from_srgb(0.5)] * 3, rtol=1e-6)
+    p = words[:, 0].view(np.float32)
+    f = 3 * pt_amd.from_srgb(0.5)  # light colours are sRGB-decoded; weight = r+g+b power
+    np.testing.assert_allclose(p, [3 / (3 + f), f / (3 + f)], rtol=1e-6)
+
+
+# --- the generated correctness scene ------------------------------------------------------
+
+def test_box_scene_packing(box_scn):
+    s = pt_amd.Scene(box_scn)
+    info = s.info()
+    assert info["unique_triangles"] == 1292 and info["instances"] == 5 and info["emissive_instances"] == 1
+    a = s.arrays()
+    assert a["light_count"] == 1
+    words = a["light_words"]
+    # light bin: p = 1, q = 1, alias 0, emitter type, table at word 10, 2 triangles, instance 4
+    assert list(words[:10]) == [0x3F800000, 0x3F800000, 0, 3, 10, 2, 4, 1, 0, 0]
+    tri_bins = words[10:].reshape(2, 6)
+    np.testing.assert_allclose(tri_bins[:, 0].view(np.float32), [0.5, 0.5])  # equal-area halves
+    inst = a["instances"].view(np.uint32).reshape(-1, 32)
+    assert inst[4, 2] == 0 and all(inst[k, 2] == 0xFFFFFFFF for k in range(4))
+    mats = a["materials"]
+    np.testing.assert_allclose(mats[4, 4:8], [1, 1, 1, 15])  # emission rgb, strength
+    np.testing.assert_allclose(mats[0, 0:3], [0.7, 0.7, 0.7], rtol=1e-5)  # linear albedo 0.7
+
+
+def test_identical_emitters_share_the_first_table(tmp_path):
+    # lightsample.cpp:126: every emitter's table offset points at the first emitter table.
+    scn = generate.instanced_scene(str(tmp_path), seed=9, counts={"ico2": 2}, emitters=3, grid_res=4,
+                                   soup_tris=10)
+    s = pt_amd.Scene(scn)
+    a = s.arrays()
+    assert a["light_count"] == 3
+    bins = a["light_words"][:30].reshape(3, 10)
+    assert set(bins[:, 4]) == {30}
+    assert list(bins[:, 6]) == [2, 3, 4] and list(bins[:, 7]) == [1, 1, 1]
+
+
+# --- CLI (src/main.cpp, src/argparse.cpp) -----------------------------------------------------
+
+PT = os.path.join(ROOT, "pt_amd", "bin", "pt")
+
+
+@pytest.mark.parametrize("args,code,msg", [
+    ([], 1, "Expected path to scene"),
+    (["a.scn", "b.scn"], 1, "Unexpected number of arguments"),
+    (["-s0", "a.scn"], 1, "Invalid sample count: 0"),
+    (["-b", "-1", "a.scn"], 1, "Invalid number of ray bounces: -1"),
+    (["--headless", "a.scn"], 1, "No output file was given"),
+    (["-w0", "a.scn"], 1, "Invalid dimensions: (0, 400)"),
+    (["-f", "exr", "a.scn"], 1, "Output format is invalid: exr"),
+    (["--nope", "a.scn"], 1, "Unknown option --nope."),
+    (["-s", "x", "a.scn"], 1, "was not an integer: x"),
+    (["--help"], 0, "Usage:"),
+])
+def test_cli_argument_errors(args, code, msg):
+    if not os.path.exists(PT):
+        pytest.skip("pt CLI not built")
+    r = subprocess.run([PT] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == code
+    assert msg in r.stdout + r.stderr
+
+
+# --- C ABI ---------------------------------------------------------------------------------
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(pth?_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+@pytest.mark.parametrize("header,lib", [("pthip.h", "libpthip.so"), ("pthost.h", "libpthost.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    path = os.path.join(ROOT, "pt_amd", "lib", lib)
+    names = _declared(header)
+    assert len(names) >= 8
+    dl = C.CDLL(path)
+    for n in names:
+        assert hasattr(dl, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    for n in names:
+        assert re.search(r"\bT %s$" % n, out, re.M), n
+
+
+def test_pt_create_argument_checks():
+    lib = pt_amd.hip_lib()
+    h = C.c_void_p()
+    assert lib.pt_create(None, 16, 16, C.byref(h)) == -1
+    assert b"scene is null" in lib.pt_last_error(None)
+    blob = pt_amd.SceneBlob()
+    assert lib.pt_create(C.byref(blob), 0, 16, C.byref(h)) == -1
+    # a mesh whose indices run past its vertices is rejected before any device work
+    verts = np.zeros((3, 8), np.float32)
+    idx = np.array([0, 1, 5], np.uint32)
+    mesh = pt_amd.MeshDesc(0, 0, 1, 3)
+    blob.vertices = verts.ctypes.data
+    blob.vertex_count = 3
+    blob.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+    blob.index_count = 3
+    blob.mesh_count = 1
+    blob.meshes = C.pointer(mesh)
+    assert lib.pt_create(C.byref(blob), 16, 16, C.byref(h)) == -1
+    assert b"index out of range" in lib.pt_last_error(None)
+    assert lib.pt_render(None, 1) == -1
