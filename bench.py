@@ -5,10 +5,11 @@ Workload (the real-shape configuration): the instanced scene - 300 instances of 
 meshes (icospheres L2/L4/L6, a 512x512 noise terrain, a 200k-triangle soup; ~2.95M
 triangles, per-instance materials, 20 emissive spheres) - at 1920x1080, max depth 8, fp32.
 
-A step is one sample per pixel of the whole 1920x1080 frame (the reference dispatches one
-spp per submit, src/pathtrace/ptdispatch.cpp:101); the default 64 steps are the 64-spp
-frame. Inputs are resident in HBM before the timed region (scene, BVH, camera); the timed
-region is K back-to-back pt_render(1) calls bracketed by stream synchronisation.
+A step is one batch of 8 samples per pixel over the whole 1920x1080 frame (pt_render(8):
+8 sample passes, each traced exactly as the reference's 1-spp submits,
+src/pathtrace/ptdispatch.cpp:101); the default 8 steps are the 64-spp frame. Inputs are
+resident in HBM before the timed region (scene, BVH, camera); the timed region is K
+back-to-back pt_render(8) calls bracketed by stream synchronisation.
 
 The JSON line also carries:
   roofline      the dominant stage's algorithmic bytes per launch / its average HIP-event
@@ -81,7 +82,8 @@ def measured_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--spp-per-step", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -104,13 +106,14 @@ def main():
     log("BVH: %.1f ms build, %d nodes, %.1f MB" % (st0["bvh_build_ms"], st0["node_count"], st0["bvh_bytes"] / 1e6))
 
     # warm-up
-    tr.render(max(1, a.warmup))
+    for _ in range(max(1, a.warmup)):
+        tr.render(a.spp_per_step)
     tr.synchronize()
 
     # instrumented replay of the first steps' sample indices: fetch counts per ray
     tr.set_option(pt_amd.PT_OPT_COUNT_FETCHES, 1)
     tr.set_camera(cam)
-    tr.render(min(4, a.steps))
+    tr.render(a.spp_per_step)
     cs = tr.stats()
     tr.set_option(pt_amd.PT_OPT_COUNT_FETCHES, 0)
     n_node_ext = cs["ext_node_fetches"] / max(1, cs["extension_rays"])
@@ -128,7 +131,7 @@ def main():
     tr.synchronize()
     t_start = time.perf_counter()
     for _ in range(a.steps):
-        tr.render(1)
+        tr.render(a.spp_per_step)
     tr.synchronize()
     elapsed = time.perf_counter() - t_start
     st = tr.stats()
@@ -166,11 +169,11 @@ def main():
         "metric": "Mrays/s", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": "instanced scene, 300 instances / %d triangles, 1920x1080, 1 spp per step, depth %d"
-                               % (info["total_triangles"], DEPTH),
-                   "spp_total": a.steps, "width": W, "height": H, "max_depth": DEPTH},
-        "samples_per_s": round(a.steps / elapsed, 3),
-        "pixel_samples_per_s": round(a.steps * W * H / elapsed, 1),
+        "config": {"workload": "instanced scene, 300 instances / %d triangles, 1920x1080, %d spp per step, depth %d"
+                               % (info["total_triangles"], a.spp_per_step, DEPTH),
+                   "spp_total": a.steps * a.spp_per_step, "width": W, "height": H, "max_depth": DEPTH},
+        "samples_per_s": round(a.steps * a.spp_per_step / elapsed, 3),
+        "pixel_samples_per_s": round(a.steps * a.spp_per_step * W * H / elapsed, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "fetches_per_ray": {"ext_nodes": round(n_node_ext, 3), "ext_tris": round(n_tri_ext, 3),
                             "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)},

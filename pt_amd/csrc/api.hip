@@ -319,12 +319,19 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              hipDeviceProp_t prop;
              HIP_TRY(&c, hipGetDeviceProperties(&prop, c.device), "device properties");
              const uint32_t cus = std::max(1, prop.multiProcessorCount);
-             c.grid_blocks = ((cus * 4 + 7) / 8) * 8;
+             uint32_t blocks_per_cu = 4;
+             if (const char* env = getenv("PT_BLOCKS_PER_CU"))
+                 blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
+             c.grid_blocks = ((cus * blocks_per_cu + 7) / 8) * 8;
              const uint32_t npix = width * height;
-             uint32_t per_batch = std::max<uint32_t>(1, std::min<uint32_t>(64, (4u << 20) / npix));
-             if (const char* env = getenv("PT_BATCH_SPP"))
-                 per_batch = std::max(1, atoi(env));
-             HIP_TRY(&c, wavefront_alloc(c.wf, npix * per_batch, c.grid_blocks * TRACE_BLOCK), "wavefront buffers");
+             // Path pool: 4M slots (fewer for small images); per-job radiance buffer for up to
+             // 2^27 paths (64 spp at 1920x1080).
+             uint32_t pool = std::min<uint32_t>(4u << 20, std::max<uint32_t>(npix * 4u, 65536u));
+             if (const char* env = getenv("PT_POOL_SLOTS"))
+                 pool = uint32_t(std::max(1024, atoi(env)));
+             const uint32_t job_spp = std::max<uint32_t>(1, std::min<uint32_t>(1024, (1u << 27) / npix));
+             HIP_TRY(&c, wavefront_alloc(c.wf, pool, npix * job_spp, c.grid_blocks * TRACE_BLOCK),
+                     "wavefront buffers");
              HIP_TRY(&c, hipStreamSynchronize(st), "upload");
              return PT_OK;
          }()) != PT_OK)

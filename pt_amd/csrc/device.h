@@ -16,26 +16,38 @@ constexpr uint32_t TRACE_BLOCK = 256;     // threads per traversal block (4 wave
 constexpr uint32_t LDS_STACK = 16;        // stack entries per lane held in LDS
 constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to global memory
 
-// Path state and ray queues of one batch of paths (struct-of-arrays, one slot per path).
+// A pool of path slots with their ray queues (struct-of-arrays). Paths of a job (every
+// pixel x every sample of a pt_render chunk) stream through the pool: a slot whose path
+// ends is refilled with the next path of the job, so every extend/shadow launch stays full
+// until the job's last paths drain.
 struct Wavefront {
-    uint32_t capacity = 0;         // paths
+    uint32_t capacity = 0;         // slots
+    uint32_t* path_id = nullptr;   // path of each slot: sample * npix + pixel (job-relative)
+    uint32_t* seg = nullptr;       // current segment of each slot
     float4* thr_pdf = nullptr;     // throughput rgb, brdf pdf
     float4* rad = nullptr;         // radiance rgb
-    uint32_t* q_path[2] = {nullptr, nullptr};
-    float4* q_o[2] = {nullptr, nullptr};  // origin, tmin
-    float4* q_d[2] = {nullptr, nullptr};  // direction, tmax
-    float4* hit = nullptr;         // t, u, v, instance bits
+    uint32_t* q_slot[2] = {nullptr, nullptr};  // extension queues (ping-pong)
+    float4* q_o[2] = {nullptr, nullptr};       // origin, tmin
+    float4* q_d[2] = {nullptr, nullptr};       // direction, tmax
+    float4* hit = nullptr;         // t, u, v, instance bits (by queue position)
     uint32_t* hit_prim = nullptr;
-    uint32_t* s_path = nullptr;    // shadow rays
+    uint32_t* s_slot = nullptr;    // shadow rays
     float4* s_o = nullptr;         // origin, tmax
     float4* s_d = nullptr;         // direction
     float4* s_c = nullptr;         // unoccluded contribution
-    uint32_t* counters = nullptr;  // [0],[1] extension queues, [2] shadow queue, [3] error flags
+    uint32_t* retire = nullptr;    // slots whose path ended this iteration
+    float4* rad_out = nullptr;     // radiance of every path of the job, by path id
+    uint32_t rad_out_capacity = 0; // paths
+    // [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] retire queue,
+    // [5] next path to start
+    uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
     unsigned long long* totals = nullptr;
     uint32_t* spill = nullptr;     // traversal stack spill, SPILL_STACK words per grid thread
     uint32_t spill_threads = 0;
+    uint32_t* host_counts = nullptr;  // pinned: queue counts read back with a one-iteration lag
+    hipEvent_t count_ready[2] = {nullptr, nullptr};
 };
 
 enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_COUNT = 3 };
@@ -132,7 +144,7 @@ struct Ctx {
     double last_ms = 0.0;
 };
 
-hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_threads);
+hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_paths, uint32_t spill_threads);
 void wavefront_free(Wavefront& wf);
 
 // Renders samples [ctx.sample_index, ctx.sample_index + spp) into ctx.accum.

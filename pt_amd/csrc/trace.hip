@@ -331,37 +331,47 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
 
 // --- kernels -------------------------------------------------------------------------------
 
-struct BatchParams {
+struct JobParams {
     uint32_t width, height, npix;
-    uint32_t sample_base;   // sample index of batch-local sample 0
-    uint32_t samples;       // samples in this batch
+    uint32_t sample_base;   // sample index of job-local sample 0
+    uint32_t paths;         // npix * samples of the job
     uint32_t max_bounces;
-    uint32_t segment;
-    uint32_t final_segment; // segment >= max_bounces
 };
 
-__global__ __launch_bounds__(256) void k_generate(CameraBlock cam, BatchParams bp, Wavefront wf)
+// Starts path p in slot s: sampler, camera ray (pathtrace.rg.hlsl:18-33), fresh payload.
+__device__ __forceinline__ void start_path(const CameraBlock& cam, const JobParams& jp, const Wavefront& wf,
+                                           uint32_t s, uint32_t p, float3& o, float3& d)
 {
-    const uint32_t n = bp.npix * bp.samples;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t pix = p % bp.npix, sl = p / bp.npix;
-        const uint32_t px = pix % bp.width, py = pix / bp.width;
-        Sampler smp;
-        smp.start(px, py, bp.sample_base + sl, 0);
-        const float2 up = smp.s2();
-        const float2 ul = smp.s2();
+    const uint32_t pix = p % jp.npix, sl = p / jp.npix;
+    const uint32_t px = pix % jp.width, py = pix / jp.width;
+    Sampler smp;
+    smp.start(px, py, jp.sample_base + sl, 0);
+    const float2 up = smp.s2();
+    const float2 ul = smp.s2();
+    camera_ray(cam, px, py, jp.width, jp.height, up, ul, o, d);
+    wf.path_id[s] = p;
+    wf.seg[s] = 0;
+    wf.thr_pdf[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+    wf.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Fills the pool with the job's first paths; their camera rays form queue 0.
+__global__ __launch_bounds__(256) void k_start_job(CameraBlock cam, JobParams jp, Wavefront wf)
+{
+    const uint32_t n = min(jp.paths, wf.capacity);
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         float3 o, d;
-        camera_ray(cam, px, py, bp.width, bp.height, up, ul, o, d);
-        wf.thr_pdf[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-        wf.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        wf.q_path[0][p] = p;
-        wf.q_o[0][p] = make_float4(o.x, o.y, o.z, cam.near_clip);
-        wf.q_d[0][p] = make_float4(d.x, d.y, d.z, cam.far_clip);
+        start_path(cam, jp, wf, s, s, o, d);
+        wf.q_slot[0][s] = s;
+        wf.q_o[0][s] = make_float4(o.x, o.y, o.z, cam.near_clip);
+        wf.q_d[0][s] = make_float4(d.x, d.y, d.z, cam.far_clip);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         wf.counters[0] = n;
         wf.counters[1] = 0;
         wf.counters[2] = 0;
+        wf.counters[4] = 0;
+        wf.counters[5] = n;
     }
 }
 
@@ -407,13 +417,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
         float trans;
         const bool occluded = traverse<true, COUNT>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
         if (!occluded && trans >= 1e-5f) {
-            const uint32_t p = wf.s_path[i];
+            const uint32_t slot = wf.s_slot[i];
             const float4 c = wf.s_c[i];
-            float4 r = wf.rad[p];
+            float4 r = wf.rad[slot];
             r.x += c.x * trans;
             r.y += c.y * trans;
             r.z += c.z * trans;
-            wf.rad[p] = r;
+            wf.rad[slot] = r;
         }
         if (stk.overflow)
             atomicOr(&wf.counters[3], 1u);
@@ -426,34 +436,38 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
         atomicAdd(&wf.totals[1], (unsigned long long)count);
 }
 
-// Miss / closest-hit / Russian roulette of one segment for every ray of queue q.
-__global__ __launch_bounds__(256) void k_shade(DeviceScene s, BatchParams bp, Wavefront wf, int q)
+// Miss / closest-hit / Russian roulette of one segment for every ray of queue q. Continuing
+// paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
+__global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wavefront wf, int q)
 {
     const uint32_t count = wf.counters[q];
     const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
         const bool valid = i < count;
         bool cont = false, shadow = false;
-        uint32_t p = 0;
+        uint32_t slot = 0;
         float3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
         float3 sh_o = f3(0, 0, 0), sh_d = f3(0, 0, 0), sh_c = f3(0, 0, 0);
         float sh_tmax = 0.0f;
+        uint32_t segment = 0;
         if (valid) {
-            p = wf.q_path[q][i];
+            slot = wf.q_slot[q][i];
+            const uint32_t p = wf.path_id[slot];
+            segment = wf.seg[slot];
             const float4 ro4 = wf.q_o[q][i], rd4 = wf.q_d[q][i];
             const float3 ro = xyz(ro4), rd = xyz(rd4);
             const float4 hh = wf.hit[i];
             const uint32_t hinst = float_as_u32(hh.w);
-            float4 tp = wf.thr_pdf[p];
+            float4 tp = wf.thr_pdf[slot];
             float3 T = xyz(tp);
             float brdf_pdf = tp.w;
-            float4 rr = wf.rad[p];
+            float4 rr = wf.rad[slot];
             float3 L = xyz(rr);
-            bool final_seg = bp.final_segment != 0;
-            const uint32_t pix = p % bp.npix, sl = p / bp.npix;
-            const uint32_t px = pix % bp.width, py = pix / bp.width;
+            const bool final_seg = segment >= jp.max_bounces;
+            const uint32_t pix = p % jp.npix, sl = p / jp.npix;
+            const uint32_t px = pix % jp.width, py = pix / jp.width;
             Sampler smp;
-            smp.start(px, py, bp.sample_base + sl, bp.segment + 1);
+            smp.start(px, py, jp.sample_base + sl, segment + 1);
             bool terminated = false;
 
             if (hinst == NONE) {
@@ -596,7 +610,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, BatchParams bp, Wa
                 }
             }
             // Russian roulette after segments >= 1 (pathtrace.rg.hlsl:53, pathtrace.hlsli:118-129)
-            if (!terminated && bp.segment >= 1) {
+            if (!terminated && segment >= 1) {
                 const float mt = maxc3(T);
                 const float u = smp.s1();
                 if (mt < 1.0f) {
@@ -608,44 +622,80 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, BatchParams bp, Wa
                 }
             }
             cont = !terminated;
-            wf.thr_pdf[p] = make_float4(T.x, T.y, T.z, brdf_pdf);
-            wf.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            wf.thr_pdf[slot] = make_float4(T.x, T.y, T.z, brdf_pdf);
+            wf.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            if (cont)
+                wf.seg[slot] = segment + 1;
         }
-        const uint32_t slot = wave_append(cont, &wf.counters[1 - q]);
+        const uint32_t pos = wave_append(cont, &wf.counters[1 - q]);
         if (cont) {
-            wf.q_path[1 - q][slot] = p;
-            wf.q_o[1 - q][slot] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
-            wf.q_d[1 - q][slot] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
+            wf.q_slot[1 - q][pos] = slot;
+            wf.q_o[1 - q][pos] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
+            wf.q_d[1 - q][pos] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
         }
-        const uint32_t sslot = wave_append(shadow, &wf.counters[2]);
+        const uint32_t rpos = wave_append(valid && !cont, &wf.counters[4]);
+        if (valid && !cont)
+            wf.retire[rpos] = slot;
+        const uint32_t spos = wave_append(shadow, &wf.counters[2]);
         if (shadow) {
-            wf.s_path[sslot] = p;
-            wf.s_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
-            wf.s_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
-            wf.s_c[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
+            wf.s_slot[spos] = slot;
+            wf.s_o[spos] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
+            wf.s_d[spos] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
+            wf.s_c[spos] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
         }
     }
 }
 
-// Clears the queue counters a segment is about to fill.
-__global__ void k_prepare_segment(Wavefront wf, int q)
+// Stores the radiance of every ended path and refills its slot with the job's next path,
+// whose camera ray joins queue 1-q.
+__global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams jp, Wavefront wf, int q)
+{
+    const uint32_t count = wf.counters[4];
+    const uint32_t rounded = (count + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
+        const bool valid = i < count;
+        uint32_t slot = 0;
+        if (valid) {
+            slot = wf.retire[i];
+            wf.rad_out[wf.path_id[slot]] = wf.rad[slot];
+        }
+        // one atomic per wave hands out the next path ids
+        const uint32_t first = wave_append(valid, &wf.counters[5]);
+        const bool start = valid && first < jp.paths;
+        float3 o = f3(0, 0, 0), d = f3(0, 0, 0);
+        if (start)
+            start_path(cam, jp, wf, slot, first, o, d);
+        const uint32_t pos = wave_append(start, &wf.counters[1 - q]);
+        if (start) {
+            wf.q_slot[1 - q][pos] = slot;
+            wf.q_o[1 - q][pos] = make_float4(o.x, o.y, o.z, cam.near_clip);
+            wf.q_d[1 - q][pos] = make_float4(d.x, d.y, d.z, cam.far_clip);
+        }
+    }
+}
+
+// Clears the queue counters an iteration is about to fill.
+__global__ void k_prepare(Wavefront wf, int q)
 {
     if (threadIdx.x == 0) {
         wf.counters[1 - q] = 0;
         wf.counters[2] = 0;
+        wf.counters[4] = 0;
     }
 }
 
-__global__ __launch_bounds__(256) void k_accumulate(BatchParams bp, const float4* rad, float4* accum, float exposure)
+// Running mean per pixel in sample order (pathtrace.rg.hlsl:61-67, one sample per dispatch).
+__global__ __launch_bounds__(256) void k_accumulate(JobParams jp, const float4* rad, float4* accum, float exposure)
 {
-    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < bp.npix; pix += gridDim.x * blockDim.x) {
+    const uint32_t samples = jp.paths / jp.npix;
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < jp.npix; pix += gridDim.x * blockDim.x) {
         float4 acc = accum[pix];
-        for (uint32_t sl = 0; sl < bp.samples; sl++) {
-            const float4 r4 = rad[size_t(sl) * bp.npix + pix];
+        for (uint32_t sl = 0; sl < samples; sl++) {
+            const float4 r4 = rad[size_t(sl) * jp.npix + pix];
             const float rx = correct_component(r4.x * exposure);
             const float ry = correct_component(r4.y * exposure);
             const float rz = correct_component(r4.z * exposure);
-            const uint32_t si = bp.sample_base + sl;
+            const uint32_t si = jp.sample_base + sl;
             if (si == 0) {
                 acc = make_float4(rx, ry, rz, 1.0f);
             } else {
@@ -714,10 +764,11 @@ inline unsigned blocks_for(uint64_t n, unsigned block, unsigned cap)
 
 }  // namespace
 
-hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_threads)
+hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_paths, uint32_t spill_threads)
 {
     wavefront_free(wf);
     wf.capacity = capacity;
+    wf.rad_out_capacity = rad_out_paths;
     wf.spill_threads = spill_threads;
     const size_t c = std::max<uint32_t>(capacity, 1);
     hipError_t e = hipSuccess;
@@ -725,22 +776,30 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_thre
         if (e == hipSuccess)
             e = hipMalloc((void**)&p, bytes);
     };
+    A(wf.path_id, c * 4);
+    A(wf.seg, c * 4);
     A(wf.thr_pdf, c * 16);
     A(wf.rad, c * 16);
     for (int k = 0; k < 2; k++) {
-        A(wf.q_path[k], c * 4);
+        A(wf.q_slot[k], c * 4);
         A(wf.q_o[k], c * 16);
         A(wf.q_d[k], c * 16);
     }
     A(wf.hit, c * 16);
     A(wf.hit_prim, c * 4);
-    A(wf.s_path, c * 4);
+    A(wf.s_slot, c * 4);
     A(wf.s_o, c * 16);
     A(wf.s_d, c * 16);
     A(wf.s_c, c * 16);
+    A(wf.retire, c * 4);
+    A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 16);
     A(wf.counters, 16 * 4);
     A(wf.totals, 8 * 8);
     A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&wf.host_counts, 4 * sizeof(uint32_t), hipHostMallocDefault);
+    for (int k = 0; k < 2 && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&wf.count_ready[k], hipEventDisableTiming);
     if (e == hipSuccess)
         e = hipMemset(wf.counters, 0, 16 * 4);
     if (e == hipSuccess)
@@ -750,11 +809,17 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t spill_thre
 
 void wavefront_free(Wavefront& wf)
 {
-    void* ptrs[] = {wf.thr_pdf, wf.rad, wf.q_path[0], wf.q_path[1], wf.q_o[0], wf.q_o[1], wf.q_d[0], wf.q_d[1],
-                    wf.hit, wf.hit_prim, wf.s_path, wf.s_o, wf.s_d, wf.s_c, wf.counters, wf.totals, wf.spill};
+    void* ptrs[] = {wf.path_id, wf.seg,   wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
+                    wf.q_o[1],  wf.q_d[0], wf.q_d[1], wf.hit,      wf.hit_prim,  wf.s_slot,    wf.s_o,
+                    wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
+    if (wf.host_counts)
+        (void)hipHostFree(wf.host_counts);
+    for (int k = 0; k < 2; k++)
+        if (wf.count_ready[k])
+            (void)hipEventDestroy(wf.count_ready[k]);
     wf = Wavefront{};
 }
 
@@ -762,25 +827,27 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
 {
     Wavefront& wf = ctx.wf;
     const uint32_t npix = ctx.width * ctx.height;
-    const uint32_t per_batch = std::max<uint32_t>(1, wf.capacity / npix);
+    const uint32_t per_job = std::max<uint32_t>(1, wf.rad_out_capacity / npix);
     hipStream_t st = ctx.stream;
     hipError_t e;
     uint32_t done = 0;
     while (done < spp) {
-        BatchParams bp{};
-        bp.width = ctx.width;
-        bp.height = ctx.height;
-        bp.npix = npix;
-        bp.sample_base = ctx.sample_index + done;
-        bp.samples = std::min(per_batch, spp - done);
-        bp.max_bounces = ctx.max_bounces;
-        const uint64_t n = uint64_t(npix) * bp.samples;
-        hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256, 65535u * 4)), dim3(256), 0, st, ctx.cam, bp, wf);
+        JobParams jp{};
+        jp.width = ctx.width;
+        jp.height = ctx.height;
+        jp.npix = npix;
+        jp.sample_base = ctx.sample_index + done;
+        const uint32_t samples = std::min(per_job, spp - done);
+        jp.paths = npix * samples;
+        jp.max_bounces = ctx.max_bounces;
+        hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min(jp.paths, wf.capacity), 256, 65535u * 4)), dim3(256),
+                           0, st, ctx.cam, jp, wf);
+        const unsigned shade_blocks = blocks_for(wf.capacity, 256, 65535u * 4);
         int q = 0;
-        for (uint32_t seg = 0; seg <= ctx.max_bounces; seg++) {
-            bp.segment = seg;
-            bp.final_segment = seg >= ctx.max_bounces ? 1u : 0u;
-            hipLaunchKernelGGL(k_prepare_segment, dim3(1), dim3(64), 0, st, wf, q);
+        // Iterate until the queue an iteration leaves behind is empty. The count of iteration
+        // i is read back (pinned memory, event) while iteration i+1 is already queued.
+        for (uint32_t it = 0; it < 1000000; it++) {
+            hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q);
             if ((e = ctx.timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
                 return e;
             if (ctx.count_fetches)
@@ -791,7 +858,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                 return e;
             if ((e = ctx.timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(k_shade, dim3(blocks_for(n, 256, 65535u * 4)), dim3(256), 0, st, ctx.scene, bp, wf, q);
+            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, st, ctx.scene, jp, wf, q);
             if ((e = ctx.timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
                 return e;
             if ((e = ctx.timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
@@ -802,13 +869,27 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                 hipLaunchKernelGGL(k_shadow<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
             if ((e = ctx.timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
                 return e;
+            hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, ctx.cam, jp, wf, q);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            if ((e = hipMemcpyAsync(&wf.host_counts[it & 1], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st)) !=
+                hipSuccess)
+                return e;
+            if ((e = hipEventRecord(wf.count_ready[it & 1], st)) != hipSuccess)
+                return e;
             q = 1 - q;
+            if (it >= 1) {
+                if ((e = hipEventSynchronize(wf.count_ready[(it - 1) & 1])) != hipSuccess)
+                    return e;
+                if (wf.host_counts[(it - 1) & 1] == 0)
+                    break;  // iteration it-1 left nothing to trace; iteration it was empty
+            }
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256, 65535u * 4)), dim3(256), 0, st, bp, wf.rad,
+        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256, 65535u * 4)), dim3(256), 0, st, jp, wf.rad_out,
                            ctx.accum, ctx.cam.exposure);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        done += bp.samples;
+        done += samples;
     }
     return hipSuccess;
 }
