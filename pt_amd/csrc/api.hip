@@ -324,9 +324,9 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
                  blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
              c.grid_blocks = ((cus * blocks_per_cu + 7) / 8) * 8;
              const uint32_t npix = width * height;
-             // Path pool: 4M slots (fewer for small images); per-job radiance buffer for up to
+             // Path pool: 8M slots (fewer for small images); per-job radiance buffer for up to
              // 2^27 paths (64 spp at 1920x1080).
-             uint32_t pool = std::min<uint32_t>(4u << 20, std::max<uint32_t>(npix * 4u, 65536u));
+             uint32_t pool = std::min<uint32_t>(8u << 20, std::max<uint32_t>(npix * 4u, 65536u));
              if (const char* env = getenv("PT_POOL_SLOTS"))
                  pool = uint32_t(std::max(1024, atoi(env)));
              const uint32_t job_spp = std::max<uint32_t>(1, std::min<uint32_t>(1024, (1u << 27) / npix));
