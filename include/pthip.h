@@ -183,6 +183,66 @@ int pt_reset_stats(pt_ctx* ctx);
 /* Sets an instrumentation option (PT_OPT_*). */
 int pt_set_option(pt_ctx* ctx, uint32_t option, int64_t value);
 
+/* ---- Learned radiance cache ---------------------------------------------------------------
+ * No counterpart in the reference: an optional estimator on top of its path tracer. A small
+ * MLP (96 encoded inputs, five hidden layers of 64 ReLU units, RGB out; BF16 matrix cores,
+ * FP32 accumulation and FP32 Adam) learns, online, the radiance leaving the second path
+ * vertex from the tracer's own training paths. With query != 0 every other path ends at
+ * that vertex with the cache's prediction (a biased, lower-variance estimate); with
+ * query == 0 the renderer only trains, and its images equal the plain tracer's. */
+#define PT_CACHE_FEATURES 96 /* encoded input width of pt_cache_forward / pt_cache_train_batch */
+
+typedef struct pt_cache_config {
+    uint32_t enable;       /* 0: free the cache, plain path tracing */
+    uint32_t query;        /* != 0: end non-training paths at the 2nd vertex with the prediction */
+    uint32_t train_stride; /* one path in train_stride is a training path (>= 1) */
+    uint32_t train_steps;  /* optimiser steps after each render job */
+    uint32_t batch;        /* samples per optimiser step */
+    float learning_rate;   /* Adam step size */
+    uint32_t seed;         /* weight initialisation (He-uniform from a Splitmix32 stream) */
+    uint32_t reserved;
+} pt_cache_config;
+
+typedef struct pt_cache_stats {
+    uint32_t enabled;
+    uint32_t query;
+    uint32_t steps;         /* optimiser steps since pt_cache_configure */
+    uint32_t last_records;  /* training records of the last job */
+    uint64_t records_total;
+    float last_loss;        /* relative L2 loss of the last optimiser step's batch */
+    uint32_t param_count;
+    float bounds_lo[3];     /* world box the positions are normalised to */
+    float bounds_hi[3];
+} pt_cache_stats;
+
+/* (Re)initialises the cache (fresh weights, Adam state and counters) and resets
+ * sample_index. Synchronises. */
+int pt_cache_configure(pt_ctx* ctx, const pt_cache_config* config);
+int pt_cache_get_stats(pt_ctx* ctx, pt_cache_stats* out);
+
+/* Parameter vector: per layer W[out][in] (row-major) then b[out]; layers 96->64, 4 x 64->64,
+ * 64->3. */
+uint32_t pt_cache_param_count(void);
+int pt_cache_get_params(pt_ctx* ctx, float* params, size_t count);
+int pt_cache_set_params(pt_ctx* ctx, const float* params, size_t count);
+
+/* Predictions of n explicit feature rows (n x PT_CACHE_FEATURES floats, host memory) into
+ * out (n x 3). Inputs and weights are rounded to BF16 as in the renderer. */
+int pt_cache_forward(pt_ctx* ctx, const float* features, uint32_t n, float* out);
+
+/* Input encoding of n records (13 floats each: hit position, direction towards the previous
+ * vertex, shading normal, roughness, base colour) into n x PT_CACHE_FEATURES floats: the
+ * position normalised to the scene's world box, its sines and cosines at 12 octaves
+ * (sin, cos of pi 2^b p_c for b = 0..11, c = x, y, z), then p, direction, normal, roughness,
+ * base colour and zero padding, each rounded to BF16. Synchronises. */
+int pt_cache_encode(pt_ctx* ctx, const float* records, uint32_t n, float* features);
+
+/* Loss mean_{i,c} (y - t)^2 / (y^2 + 0.01) (the denominator held constant) of one batch and
+ * its gradient with respect to the parameter vector, into grads / loss when non-null; with
+ * apply != 0 one Adam step follows. Synchronises. */
+int pt_cache_train_batch(pt_ctx* ctx, const float* features, const float* targets, uint32_t n, int apply,
+                         float* grads, float* loss);
+
 /* Waits for all work queued on the context's stream. */
 int pt_synchronize(pt_ctx* ctx);
 

@@ -85,6 +85,19 @@ class Stats(C.Structure):
 
 PT_OPT_PROFILE_KERNELS = 1
 PT_OPT_COUNT_FETCHES = 2
+PT_CACHE_FEATURES = 96
+
+
+class CacheConfig(C.Structure):
+    _fields_ = [("enable", C.c_uint32), ("query", C.c_uint32), ("train_stride", C.c_uint32),
+                ("train_steps", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
+                ("seed", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("enabled", C.c_uint32), ("query", C.c_uint32), ("steps", C.c_uint32),
+                ("last_records", C.c_uint32), ("records_total", C.c_uint64), ("last_loss", C.c_float),
+                ("param_count", C.c_uint32), ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3)]
 
 
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance", "<u4"), ("primitive", "<u4")])
@@ -147,6 +160,16 @@ def hip_lib():
         lib.pt_last_error.argtypes = [P]
         lib.pt_last_error.restype = C.c_char_p
         lib.pt_destroy.argtypes = [P]
+        lib.pt_cache_configure.argtypes = [P, C.POINTER(CacheConfig)]
+        lib.pt_cache_get_stats.argtypes = [P, C.POINTER(CacheStats)]
+        lib.pt_cache_param_count.argtypes = []
+        lib.pt_cache_param_count.restype = C.c_uint32
+        lib.pt_cache_get_params.argtypes = [P, C.c_void_p, C.c_size_t]
+        lib.pt_cache_set_params.argtypes = [P, C.c_void_p, C.c_size_t]
+        lib.pt_cache_forward.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.pt_cache_encode.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.pt_cache_train_batch.argtypes = [P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                             C.c_void_p]
         _hip = lib
     return _hip
 
@@ -346,3 +369,50 @@ class Tracer:
         s = Stats()
         self._check(self._lib.pt_get_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    # ---- learned radiance cache (no counterpart in the reference)
+
+    def cache_configure(self, enable=True, query=True, train_stride=32, train_steps=4, batch=16384,
+                        learning_rate=1e-3, seed=1):
+        cfg = CacheConfig(int(enable), int(query), train_stride, train_steps, batch, learning_rate, seed, 0)
+        self._check(self._lib.pt_cache_configure(self._h, C.byref(cfg)))
+
+    def cache_stats(self):
+        s = CacheStats()
+        self._check(self._lib.pt_cache_get_stats(self._h, C.byref(s)))
+        out = {k: getattr(s, k) for k, _ in CacheStats._fields_}
+        out["bounds_lo"] = list(s.bounds_lo)
+        out["bounds_hi"] = list(s.bounds_hi)
+        return out
+
+    def cache_params(self):
+        n = self._lib.pt_cache_param_count()
+        out = np.zeros(n, np.float32)
+        self._check(self._lib.pt_cache_get_params(self._h, out.ctypes.data, n))
+        return out
+
+    def cache_set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        self._check(self._lib.pt_cache_set_params(self._h, params.ctypes.data, params.size))
+
+    def cache_forward(self, features):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, PT_CACHE_FEATURES)
+        out = np.zeros((len(f), 3), np.float32)
+        self._check(self._lib.pt_cache_forward(self._h, f.ctypes.data, len(f), out.ctypes.data))
+        return out
+
+    def cache_encode(self, records):
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 13)
+        out = np.zeros((len(r), PT_CACHE_FEATURES), np.float32)
+        self._check(self._lib.pt_cache_encode(self._h, r.ctypes.data, len(r), out.ctypes.data))
+        return out
+
+    def cache_train_batch(self, features, targets, apply=False):
+        """Loss and parameter gradient of one batch; apply=True also takes an Adam step."""
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, PT_CACHE_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        grads = np.zeros(self._lib.pt_cache_param_count(), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self._lib.pt_cache_train_batch(self._h, f.ctypes.data, t.ctypes.data, len(f), int(apply),
+                                                   grads.ctypes.data, loss.ctypes.data))
+        return float(loss[0]), grads

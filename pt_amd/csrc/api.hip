@@ -63,6 +63,7 @@ void release(Ctx& c)
             (void)hipFree(p);
     free_bvh(c.bvh);
     wavefront_free(c.wf);
+    cache_free(c.cache);
     c.timer.destroy();
     if (c.ev_start)
         (void)hipEventDestroy(c.ev_start);
@@ -212,9 +213,50 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
         for (uint32_t i = 0; i < c.instance_count; i++)
             if (!inst_seen[i])
                 return (why = "instance not reachable from the TLAS", PT_EDEVICE);
+        // world bounds (union of the root's children) normalise the radiance cache's positions
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const Node4& root = nodes[bv.tlas_root];
+        for (int k = 0; k < 4; k++) {
+            if (root.child[k] == CHILD_EMPTY)
+                continue;
+            lo[0] = std::min(lo[0], root.lo_x[k]), hi[0] = std::max(hi[0], root.hi_x[k]);
+            lo[1] = std::min(lo[1], root.lo_y[k]), hi[1] = std::max(hi[1], root.hi_y[k]);
+            lo[2] = std::min(lo[2], root.lo_z[k]), hi[2] = std::max(hi[2], root.hi_z[k]);
+        }
+        for (int k = 0; k < 3; k++)
+            if (std::isfinite(lo[k]) && std::isfinite(hi[k]) && hi[k] >= lo[k]) {
+                c.cache.bounds_lo[k] = lo[k];
+                c.cache.bounds_hi[k] = hi[k];
+            }
     }
     return PT_OK;
 }
+
+int need_cache(Ctx& c)
+{
+    if (!c.cache.configured)
+        return fail(&c, PT_ESTATE, "radiance cache not configured");
+    return PT_OK;
+}
+
+// Device copies of test inputs; freed by the destructor.
+struct DevBufs {
+    std::vector<void*> p;
+    ~DevBufs()
+    {
+        for (void* q : p)
+            (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t alloc(T*& dst, size_t bytes)
+    {
+        hipError_t e = hipMalloc((void**)&dst, std::max<size_t>(bytes, 16));
+        if (e == hipSuccess)
+            p.push_back(dst);
+        return e;
+    }
+};
+
 
 }  // namespace
 
@@ -534,6 +576,177 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
     default:
         return fail(&c, PT_EINVAL, "unknown option " + std::to_string(option));
     }
+}
+
+int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!cfg)
+        return fail(&c, PT_EINVAL, "config is null");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    if (!cfg->enable) {
+        cache_free(c.cache);
+        c.sample_index = 0;
+        c.stats_fresh = true;
+        return PT_OK;
+    }
+    if (cfg->train_stride == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f) || cfg->batch > (1u << 24))
+        return fail(&c, PT_EINVAL, "invalid cache configuration");
+    CacheState& cs = c.cache;
+    cs.query = cfg->query != 0;
+    cs.train_stride = cfg->train_stride;
+    cs.train_steps = cfg->train_steps;
+    cs.batch = cfg->batch;
+    cs.lr = cfg->learning_rate;
+    cs.seed = cfg->seed;
+    cs.records_total = 0;
+    cs.last_records = 0;
+    // one training record per training path of a job
+    const uint64_t job_paths = uint64_t(c.wf.rad_out_capacity);
+    const uint32_t rec_cap = uint32_t(std::min<uint64_t>(job_paths / cfg->train_stride + 1024, 1u << 24));
+    HIP_TRY(&c, cache_init(cs, c.wf.capacity, rec_cap, c.stream), "radiance cache");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "radiance cache");
+    c.sample_index = 0;
+    c.stats_fresh = true;
+    return PT_OK;
+}
+
+int pt_cache_get_stats(pt_ctx* h, pt_cache_stats* out)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!out)
+        return fail(&c, PT_EINVAL, "out is null");
+    pt_cache_stats s{};
+    const CacheState& cs = c.cache;
+    s.enabled = cs.configured ? 1u : 0u;
+    s.query = cs.query ? 1u : 0u;
+    s.steps = cs.steps_done;
+    s.last_records = cs.last_records;
+    s.records_total = cs.records_total;
+    s.param_count = uint32_t(mlp_param_count());
+    if (cs.configured) {
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        HIP_TRY(&c, hipMemcpy(&s.last_loss, cs.loss_acc, 4, hipMemcpyDeviceToHost), "cache loss");
+    }
+    for (int k = 0; k < 3; k++) {
+        s.bounds_lo[k] = cs.bounds_lo[k];
+        s.bounds_hi[k] = cs.bounds_hi[k];
+    }
+    *out = s;
+    return PT_OK;
+}
+
+uint32_t pt_cache_param_count(void) { return uint32_t(mlp_param_count()); }
+
+
+int pt_cache_get_params(pt_ctx* h, float* params, size_t count)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (!params || count != size_t(mlp_param_count()))
+        return fail(&c, PT_EINVAL, "parameter buffer must hold pt_cache_param_count() floats");
+    HIP_TRY(&c, cache_get_params(c.cache, params, c.stream), "cache parameters");
+    return PT_OK;
+}
+
+int pt_cache_set_params(pt_ctx* h, const float* params, size_t count)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (!params || count != size_t(mlp_param_count()))
+        return fail(&c, PT_EINVAL, "parameter buffer must hold pt_cache_param_count() floats");
+    HIP_TRY(&c, cache_set_params(c.cache, params, c.stream), "cache parameters");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache parameters");
+    return PT_OK;
+}
+
+int pt_cache_forward(pt_ctx* h, const float* features, uint32_t n, float* out)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (n == 0)
+        return PT_OK;
+    if (!features || !out)
+        return fail(&c, PT_EINVAL, "null buffer");
+    DevBufs d;
+    float *df = nullptr, *dout = nullptr;
+    HIP_TRY(&c, d.alloc(df, size_t(n) * PT_CACHE_FEATURES * 4), "cache buffers");
+    HIP_TRY(&c, d.alloc(dout, size_t(n) * 3 * 4), "cache buffers");
+    HIP_TRY(&c, hipMemcpyAsync(df, features, size_t(n) * PT_CACHE_FEATURES * 4, hipMemcpyHostToDevice, c.stream),
+            "upload");
+    HIP_TRY(&c, cache_forward(c.cache, df, n, dout, c.stream), "cache forward");
+    HIP_TRY(&c, hipMemcpyAsync(out, dout, size_t(n) * 3 * 4, hipMemcpyDeviceToHost, c.stream), "readback");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache forward");
+    return PT_OK;
+}
+
+int pt_cache_encode(pt_ctx* h, const float* records, uint32_t n, float* features)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (n == 0)
+        return PT_OK;
+    if (!records || !features)
+        return fail(&c, PT_EINVAL, "null buffer");
+    DevBufs d;
+    float *dr = nullptr, *df = nullptr;
+    HIP_TRY(&c, d.alloc(dr, size_t(n) * 13 * 4), "cache buffers");
+    HIP_TRY(&c, d.alloc(df, size_t(n) * PT_CACHE_FEATURES * 4), "cache buffers");
+    HIP_TRY(&c, hipMemcpyAsync(dr, records, size_t(n) * 13 * 4, hipMemcpyHostToDevice, c.stream), "upload");
+    HIP_TRY(&c, cache_encode(c.cache, dr, n, df, c.stream), "cache encode");
+    HIP_TRY(&c, hipMemcpyAsync(features, df, size_t(n) * PT_CACHE_FEATURES * 4, hipMemcpyDeviceToHost, c.stream),
+            "readback");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache encode");
+    return PT_OK;
+}
+
+int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets, uint32_t n, int apply,
+                         float* grads, float* loss)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (n == 0 || !features || !targets)
+        return fail(&c, PT_EINVAL, "empty or null batch");
+    DevBufs d;
+    float *df = nullptr, *dt = nullptr;
+    HIP_TRY(&c, d.alloc(df, size_t(n) * PT_CACHE_FEATURES * 4), "cache buffers");
+    HIP_TRY(&c, d.alloc(dt, size_t(n) * 3 * 4), "cache buffers");
+    HIP_TRY(&c, hipMemcpyAsync(df, features, size_t(n) * PT_CACHE_FEATURES * 4, hipMemcpyHostToDevice, c.stream),
+            "upload");
+    HIP_TRY(&c, hipMemcpyAsync(dt, targets, size_t(n) * 3 * 4, hipMemcpyHostToDevice, c.stream), "upload");
+    HIP_TRY(&c, hipMemsetAsync(c.cache.grads, 0, size_t(mlp_param_count()) * 4, c.stream), "cache gradients");
+    HIP_TRY(&c, cache_backward(c.cache, df, dt, n, 0, 0, 1, c.stream), "cache backward");
+    if (grads)
+        HIP_TRY(&c,
+                hipMemcpyAsync(grads, c.cache.grads, size_t(mlp_param_count()) * 4, hipMemcpyDeviceToHost, c.stream),
+                "readback");
+    if (loss)
+        HIP_TRY(&c, hipMemcpyAsync(loss, c.cache.loss_acc, 4, hipMemcpyDeviceToHost, c.stream), "readback");
+    if (apply)
+        HIP_TRY(&c, cache_adam(c.cache, c.stream), "cache update");
+    else
+        HIP_TRY(&c, hipMemsetAsync(c.cache.grads, 0, size_t(mlp_param_count()) * 4, c.stream), "cache gradients");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache train");
+    return PT_OK;
 }
 
 int pt_synchronize(pt_ctx* h)

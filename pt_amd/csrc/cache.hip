@@ -1,0 +1,567 @@
+// Learned radiance cache: fused MLP inference and training on BF16 MFMA (gfx950
+// v_mfma_f32_16x16x32_bf16, FP32 accumulation) and an FP32 Adam update.
+//
+// One wave evaluates 32 samples through all six layers; activations stay in LDS as bf16
+// (sample-major rows). Every product is a 16x16x32 MFMA tile:
+//   forward       Y[s][o]  = sum_k X[s][k] W[o][k]      A = X rows, B = W rows
+//   input grad    G'[s][k] = sum_o G[s][o] W[o][k]      A = G rows, B = W^T rows
+//   weight grad   dW[o][k] = sum_s G[s][o] X[s][k]      A = G columns, B = X columns
+// Operand lane layout of the 16x16x32 bf16 MFMA (lane l, r = l % 16, g = l / 16): A supplies
+// row r, k = 8g..8g+7; B supplies column r, k = 8g..8g+7; D holds column r, rows 4g..4g+3.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cache.h"
+#include "common.h"
+
+namespace ptk {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int WS = 32;       // samples per wave
+constexpr int ST = 104;      // LDS row stride (bf16 elements) of every activation buffer
+constexpr MlpLayout LAYOUT{};
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u)
+        return uint16_t(u >> 16);  // inf / nan pass through
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ bf16x8_t load8(const uint16_t* p)  // 16-byte aligned
+{
+    return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(p));
+}
+
+__device__ __forceinline__ bf16x8_t gather8(const uint16_t* p, int stride)
+{
+    uint16_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        v[j] = p[j * stride];
+    uint4 u;
+    u.x = uint32_t(v[0]) | (uint32_t(v[1]) << 16);
+    u.y = uint32_t(v[2]) | (uint32_t(v[3]) << 16);
+    u.z = uint32_t(v[4]) | (uint32_t(v[5]) << 16);
+    u.w = uint32_t(v[6]) | (uint32_t(v[7]) << 16);
+    return __builtin_bit_cast(bf16x8_t, u);
+}
+
+__device__ __forceinline__ f32x4_t mfma(bf16x8_t a, bf16x8_t b, f32x4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Input encoding (96 features): position normalised to the scene bounds and its sines and
+// cosines at 12 octaves (72), the position itself (3), direction (3), normal (3),
+// roughness (1), albedo (3); the last 11 are zero.
+__device__ __forceinline__ void encode_row(const float* rec, const float* lo, const float* inv_ext, uint16_t* row)
+{
+    float p[3];
+    for (int c = 0; c < 3; c++)
+        p[c] = fminf(fmaxf((rec[c] - lo[c]) * inv_ext[c], 0.0f), 1.0f);
+    int f = 0;
+    for (int b = 0; b < 12; b++) {
+        const float s = float(1 << b) * 3.14159265f;
+        for (int c = 0; c < 3; c++) {
+            float sn, cs;
+            sincosf(s * p[c], &sn, &cs);
+            row[f++] = f2bf(sn);
+            row[f++] = f2bf(cs);
+        }
+    }
+    for (int c = 0; c < 3; c++)
+        row[f++] = f2bf(p[c]);
+    for (int c = 3; c < 13; c++)
+        row[f++] = f2bf(rec[c]);
+    while (f < MLP_IN)
+        row[f++] = 0;
+}
+
+// LDS buffer holding the input of layer l: one buffer per layer (training keeps them all for
+// the backward pass) or two alternating buffers (inference).
+template <bool PING>
+__device__ __forceinline__ uint16_t* act_buf(uint16_t* base, int l)
+{
+    return base + (PING ? (l & 1) : l) * WS * ST;
+}
+
+// Forward of one 32-sample tile whose encoded inputs are in act_buf(base, 0); the output of
+// the last layer is returned in y (lane holds rows 4g..4g+3 of tile mt for output r < 3).
+template <bool PING>
+__device__ __forceinline__ void forward_tile(uint16_t* base, const uint16_t* wb, const float* params, f32x4_t y[2])
+{
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        const int K = mlp_in(l), NO = mlp_outp(l);
+        const uint16_t* W = wb + LAYOUT.wb[l];
+        const uint16_t* X = act_buf<PING>(base, l);
+        uint16_t* Y = act_buf<PING>(base, l + 1);
+        for (int nt = 0; nt < NO / 16; nt++) {
+            const int o = nt * 16 + r;
+            const float bias = o < mlp_out(l) ? params[LAYOUT.b[l] + o] : 0.0f;
+            for (int mt = 0; mt < 2; mt++) {
+                f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int kc = 0; kc < K / 32; kc++) {
+                    const bf16x8_t a = load8(X + (mt * 16 + r) * ST + kc * 32 + 8 * g);
+                    const bf16x8_t b = load8(W + (nt * 16 + r) * K + kc * 32 + 8 * g);
+                    acc = mfma(a, b, acc);
+                }
+                if (l < MLP_LAYERS - 1) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        Y[(mt * 16 + 4 * g + i) * ST + o] = f2bf(fmaxf(acc[i] + bias, 0.0f));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[i] += bias;
+                    y[mt] = acc;
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+struct CacheArgs {
+    const float* params;
+    const uint16_t* wb;
+    const uint16_t* wt;
+    float* grads;
+    float* loss_acc;
+    float lo[3], inv_ext[3];
+};
+
+// Training: forward + backward over n samples; gradients (summed over the batch, already
+// scaled by the loss normalisation) are added into grads. Samples come from records
+// (perm index start + i * stride mod n_rec) or, when features != nullptr, from explicit
+// feature rows and targets (n x 96 and n x 3 floats).
+__global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, const float* records, uint32_t n_rec, uint32_t start,
+                                                   uint32_t stride, const float* features, const float* targets,
+                                                   uint32_t n)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t lds[(MLP_LAYERS + 2) * WS * ST];
+    uint16_t* const gbase = lds + MLP_LAYERS * WS * ST;  // two gradient buffers
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    const float inv_n3 = 1.0f / (3.0f * float(n));
+    float loss = 0.0f;
+
+    for (uint32_t base = blockIdx.x * WS; base < n; base += gridDim.x * WS) {
+        float tgt[3] = {0, 0, 0};
+        const uint32_t s = base + uint32_t(lane);
+        if (lane < WS) {
+            uint16_t* row = lds + lane * ST;
+            if (s < n) {
+                if (features) {
+                    for (int f = 0; f < MLP_IN; f++)
+                        row[f] = f2bf(features[size_t(s) * MLP_IN + f]);
+                    for (int c = 0; c < 3; c++)
+                        tgt[c] = targets[size_t(s) * 3 + c];
+                } else {
+                    const uint32_t j = uint32_t((uint64_t(start) + uint64_t(s) * stride) % n_rec);
+                    const float* rec = records + size_t(j) * MLP_REC_FLOATS;
+                    encode_row(rec, a.lo, a.inv_ext, row);
+                    for (int c = 0; c < 3; c++)
+                        tgt[c] = rec[13 + c];
+                }
+            } else {
+                for (int f = 0; f < MLP_IN; f++)
+                    row[f] = 0;
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        f32x4_t y[2];
+        forward_tile<false>(lds, a.wb, a.params, y);
+
+        // dL/dy of the relative L2 loss mean((y - t)^2 / (sg(y)^2 + 0.01)), into G5 (32 wide).
+        // Targets live with lane s; the D layout holds sample 4g+i of tile mt in lane (r, g).
+        uint16_t* G = gbase + ((MLP_LAYERS - 1) & 1) * WS * ST;
+        for (int mt = 0; mt < 2; mt++)
+            for (int i = 0; i < 4; i++) {
+                const int srow = mt * 16 + 4 * g + i;
+                float t_c = 0.0f;
+                // fetch target component r of sample srow from the lane that loaded it
+                const float t0 = __shfl(tgt[0], srow), t1 = __shfl(tgt[1], srow), t2 = __shfl(tgt[2], srow);
+                t_c = r == 0 ? t0 : (r == 1 ? t1 : t2);
+                float gr = 0.0f;
+                if (r < 3 && base + uint32_t(srow) < n) {
+                    const float yv = y[mt][i];
+                    const float den = yv * yv + 0.01f;
+                    const float d = yv - t_c;
+                    loss += d * d / den * inv_n3;
+                    gr = 2.0f * d / den * inv_n3;
+                }
+                G[srow * ST + r] = f2bf(gr);
+                G[srow * ST + 16 + r] = 0;
+            }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+
+#pragma unroll 1
+        for (int l = MLP_LAYERS - 1; l >= 0; l--) {
+            const int K = mlp_in(l), NO = mlp_outp(l), NT = mlp_outt(l), NV = mlp_out(l);
+            uint16_t* Gl = gbase + (l & 1) * WS * ST;
+            const uint16_t* Xl = act_buf<false>(lds, l);
+            float* gW = a.grads + LAYOUT.w[l];
+            // weight gradient tiles: rows o (outputs), columns k (inputs), reduction over samples
+            for (int ot = 0; ot < NO / 16; ot++)
+                for (int kt = 0; kt < K / 16; kt++) {
+                    const bf16x8_t A = gather8(Gl + (8 * g) * ST + ot * 16 + r, ST);
+                    const bf16x8_t B = gather8(Xl + (8 * g) * ST + kt * 16 + r, ST);
+                    const f32x4_t d = mfma(A, B, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int o = ot * 16 + 4 * g + i;
+                        if (o < NV)
+                            atomicAdd(&gW[o * K + kt * 16 + r], d[i]);
+                    }
+                }
+            // bias gradient: one output per lane
+            if (lane < NV) {
+                float sb = 0.0f;
+                for (int sr = 0; sr < WS; sr++)
+                    sb += bf2f(Gl[sr * ST + lane]);
+                atomicAdd(&a.grads[LAYOUT.b[l] + lane], sb);
+            }
+            if (l == 0)
+                break;
+            // input gradient through W, masked by the ReLU of layer l-1
+            uint16_t* Gp = gbase + ((l - 1) & 1) * WS * ST;
+            const uint16_t* WT = a.wt + LAYOUT.wt[l];
+            for (int kt = 0; kt < K / 16; kt++)
+                for (int mt = 0; mt < 2; mt++) {
+                    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    for (int oc = 0; oc < NT / 32; oc++) {
+                        const bf16x8_t A = load8(Gl + (mt * 16 + r) * ST + oc * 32 + 8 * g);
+                        const bf16x8_t B = load8(WT + (kt * 16 + r) * NT + oc * 32 + 8 * g);
+                        acc = mfma(A, B, acc);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int srow = mt * 16 + 4 * g + i, k = kt * 16 + r;
+                        const bool on = Xl[srow * ST + k] != 0;  // post-ReLU activation > 0
+                        Gp[srow * ST + k] = on ? f2bf(acc[i]) : uint16_t(0);
+                    }
+                }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // wave-reduce the loss
+    for (int off = 32; off >= 1; off >>= 1)
+        loss += __shfl_xor(loss, off);
+    if (lane == 0)
+        atomicAdd(a.loss_acc, loss);
+}
+
+// Inference. Query mode (features == nullptr): each query record's prediction, clamped at
+// zero and weighted by the record's throughput, is added to the radiance of its slot.
+// Test mode: predictions of explicit feature rows are written to out (n x 3).
+__global__ __launch_bounds__(256) void k_mlp_infer(CacheArgs a, const float* queries, const uint32_t* query_slot,
+                                                   const uint32_t* count_ptr, float4* rad, const float* features,
+                                                   uint32_t n_explicit, float* out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t lds[4 * 2 * WS * ST];
+    const int w = threadIdx.x >> 6;
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    uint16_t* const wbuf = lds + w * 2 * WS * ST;
+    const uint32_t n = features ? n_explicit : *count_ptr;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t base = (blockIdx.x * 4 + w) * WS; base < n; base += waves * WS) {
+        const uint32_t s = base + uint32_t(lane);
+        if (lane < WS) {
+            uint16_t* row = wbuf + lane * ST;
+            if (s < n) {
+                if (features)
+                    for (int f = 0; f < MLP_IN; f++)
+                        row[f] = f2bf(features[size_t(s) * MLP_IN + f]);
+                else
+                    encode_row(queries + size_t(s) * MLP_REC_FLOATS, a.lo, a.inv_ext, row);
+            } else {
+                for (int f = 0; f < MLP_IN; f++)
+                    row[f] = 0;
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        f32x4_t y[2];
+        forward_tile<true>(wbuf, a.wb, a.params, y);
+        for (int mt = 0; mt < 2; mt++)
+            for (int i = 0; i < 4; i++) {
+                const uint32_t q = base + uint32_t(mt * 16 + 4 * g + i);
+                if (r >= 3 || q >= n)
+                    continue;
+                if (features) {
+                    out[size_t(q) * 3 + r] = y[mt][i];
+                } else {
+                    const float T = queries[size_t(q) * MLP_REC_FLOATS + 13 + r];
+                    float* dst = reinterpret_cast<float*>(&rad[query_slot[q]]) + r;
+                    *dst += T * fmaxf(y[mt][i], 0.0f);
+                }
+            }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Encoded features of explicit records (13 floats each), as floats (bf16 values).
+__global__ void k_encode(CacheArgs a, const float* rec13, uint32_t n, float* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint16_t row[MLP_IN];
+        encode_row(rec13 + size_t(i) * 13, a.lo, a.inv_ext, row);
+        for (int f = 0; f < MLP_IN; f++)
+            out[size_t(i) * MLP_IN + f] = bf2f(row[f]);
+    }
+}
+
+__global__ void k_adam(float* params, float* grads, float* m, float* v, int count, float lr, float b1t, float b2t)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const float gr = grads[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gr;
+        const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+        m[i] = mi;
+        v[i] = vi;
+        const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
+        params[i] -= lr * mh / (sqrtf(vh) + eps);
+        grads[i] = 0.0f;
+    }
+}
+
+// bf16 copies of the weights in both layouts (padding rows/columns are zero).
+__global__ void k_sync_bf16(const float* params, uint16_t* wb, uint16_t* wt)
+{
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        const int K = mlp_in(l), NO = mlp_outp(l), NT = mlp_outt(l), NV = mlp_out(l);
+        const float* W = params + LAYOUT.w[l];
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NO * K; e += gridDim.x * blockDim.x) {
+            const int o = e / K, k = e % K;
+            wb[LAYOUT.wb[l] + e] = o < NV ? f2bf(W[o * K + k]) : uint16_t(0);
+        }
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * NT; e += gridDim.x * blockDim.x) {
+            const int k = e / NT, o = e % NT;
+            wt[LAYOUT.wt[l] + e] = o < NV ? f2bf(W[o * K + k]) : uint16_t(0);
+        }
+    }
+}
+
+CacheArgs args_of(const CacheState& c)
+{
+    CacheArgs a;
+    a.params = c.params;
+    a.wb = c.wb;
+    a.wt = c.wt;
+    a.grads = c.grads;
+    a.loss_acc = c.loss_acc;
+    for (int k = 0; k < 3; k++) {
+        a.lo[k] = c.bounds_lo[k];
+        const float ext = c.bounds_hi[k] - c.bounds_lo[k];
+        a.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    return a;
+}
+
+}  // namespace
+
+int mlp_param_count() { return LAYOUT.count; }
+
+void cache_free(CacheState& c)
+{
+    void* ptrs[] = {c.params, c.grads, c.adam_m, c.adam_v, c.wb, c.wt, c.records, c.queries,
+                    c.query_slot, c.counters, c.loss_acc, c.train_rec};
+    for (void* p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    const float lo[3] = {c.bounds_lo[0], c.bounds_lo[1], c.bounds_lo[2]};
+    const float hi[3] = {c.bounds_hi[0], c.bounds_hi[1], c.bounds_hi[2]};
+    c = CacheState{};
+    memcpy(c.bounds_lo, lo, sizeof(lo));
+    memcpy(c.bounds_hi, hi, sizeof(hi));
+}
+
+hipError_t cache_sync_weights(CacheState& c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.params, c.wb, c.wt);
+    return hipGetLastError();
+}
+
+// He-uniform initialisation from a Splitmix32 stream; biases zero.
+hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capacity, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    auto A = [&](auto*& p, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMalloc((void**)&p, std::max<size_t>(bytes, 16));
+    };
+    if (!c.params) {
+        A(c.params, size_t(LAYOUT.count) * 4);
+        A(c.grads, size_t(LAYOUT.count) * 4);
+        A(c.adam_m, size_t(LAYOUT.count) * 4);
+        A(c.adam_v, size_t(LAYOUT.count) * 4);
+        A(c.wb, size_t(LAYOUT.wb_count) * 2);
+        A(c.wt, size_t(LAYOUT.wt_count) * 2);
+        A(c.counters, 8 * 4);
+        A(c.loss_acc, 4);
+    }
+    if (e == hipSuccess && record_capacity > c.record_capacity) {
+        if (c.records)
+            (void)hipFree(c.records);
+        c.records = nullptr;
+        A(c.records, size_t(record_capacity) * MLP_REC_FLOATS * 4);
+        c.record_capacity = record_capacity;
+    }
+    if (e == hipSuccess && pool_slots > c.query_capacity) {
+        if (c.queries)
+            (void)hipFree(c.queries);
+        if (c.query_slot)
+            (void)hipFree(c.query_slot);
+        if (c.train_rec)
+            (void)hipFree(c.train_rec);
+        c.queries = nullptr;
+        c.query_slot = nullptr;
+        c.train_rec = nullptr;
+        A(c.queries, size_t(pool_slots) * MLP_REC_FLOATS * 4);
+        A(c.query_slot, size_t(pool_slots) * 4);
+        A(c.train_rec, size_t(pool_slots) * 4);
+        c.query_capacity = pool_slots;
+    }
+    if (e != hipSuccess)
+        return e;
+    std::vector<float> p(LAYOUT.count, 0.0f);
+    uint32_t state = c.seed;
+    auto next = [&]() {  // Splitmix32
+        uint32_t z = (state += 0x9e3779b9u);
+        z = (z ^ (z >> 16)) * 0x85ebca6bu;
+        z = (z ^ (z >> 13)) * 0xc2b2ae35u;
+        return z ^ (z >> 16);
+    };
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        const float lim = std::sqrt(6.0f / float(mlp_in(l)));
+        for (int i = 0; i < mlp_out(l) * mlp_in(l); i++)
+            p[LAYOUT.w[l] + i] = (float(next()) * 0x1p-32f * 2.0f - 1.0f) * lim;
+    }
+    if ((e = hipMemcpyAsync(c.params, p.data(), p.size() * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(c.grads, 0, p.size() * 4, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(c.adam_m, 0, p.size() * 4, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(c.adam_v, 0, p.size() * 4, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(c.counters, 0, 8 * 4, st)) != hipSuccess)
+        return e;
+    c.steps_done = 0;
+    c.configured = true;
+    return cache_sync_weights(c, st);
+}
+
+hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(c.params, host, size_t(LAYOUT.count) * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess)
+        return e;
+    return cache_sync_weights(c, st);
+}
+
+hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(host, c.params, size_t(LAYOUT.count) * 4, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess)
+        return e;
+    return hipStreamSynchronize(st);
+}
+
+hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, float* d_out, hipStream_t st)
+{
+    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + 4 * WS - 1) / (4 * WS) + 1, 4096));
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), nullptr, nullptr, nullptr, nullptr,
+                       d_features, n, d_out);
+    return hipGetLastError();
+}
+
+hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* d_out, hipStream_t st)
+{
+    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_encode, dim3(std::max(1u, blocks)), dim3(256), 0, st, args_of(c), d_rec13, n, d_out);
+    return hipGetLastError();
+}
+
+hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* rad, unsigned blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), c.queries, c.query_slot, d_count, rad,
+                       nullptr, 0u, nullptr);
+    return hipGetLastError();
+}
+
+// Gradients of the batch into c.grads (zeroed first) and the loss into *loss (device).
+hipError_t cache_backward(CacheState& c, const float* d_features, const float* d_targets, uint32_t n,
+                          uint32_t n_rec, uint32_t start, uint32_t stride, hipStream_t st)
+{
+    hipError_t e = hipMemsetAsync(c.loss_acc, 0, 4, st);
+    if (e != hipSuccess)
+        return e;
+    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + WS - 1) / WS, 2048));
+    hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), c.records, n_rec, start,
+                       stride, d_features, d_targets, n);
+    return hipGetLastError();
+}
+
+hipError_t cache_adam(CacheState& c, hipStream_t st)
+{
+    c.steps_done++;
+    const float b1t = std::pow(0.9f, float(c.steps_done)), b2t = std::pow(0.999f, float(c.steps_done));
+    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, st, c.params, c.grads, c.adam_m, c.adam_v, LAYOUT.count, c.lr,
+                       b1t, b2t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    return cache_sync_weights(c, st);
+}
+
+// Trains on the records collected by the last job: steps minibatches of the batch size,
+// each a pseudo-shuffled window (start + i * stride mod n_records, stride coprime with n).
+hipError_t cache_train_records(CacheState& c, uint32_t n_records, hipStream_t st)
+{
+    if (n_records == 0)
+        return hipSuccess;
+    const uint32_t batch = std::min(c.batch, n_records);
+    uint32_t stride = 7919u % n_records;
+    auto gcd = [](uint32_t x, uint32_t y) {
+        while (y) {
+            uint32_t t = x % y;
+            x = y;
+            y = t;
+        }
+        return x;
+    };
+    if (stride == 0)
+        stride = 1;
+    while (gcd(stride, n_records) != 1)
+        stride++;
+    hipError_t e;
+    for (uint32_t k = 0; k < c.train_steps; k++) {
+        const uint32_t start = uint32_t((uint64_t(c.steps_done + k) * 2654435761u) % n_records);
+        if ((e = cache_backward(c, nullptr, nullptr, batch, n_records, start, stride, st)) != hipSuccess)
+            return e;
+        if ((e = cache_adam(c, st)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace ptk

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "cache.h"
 #include "common.h"
 #include "shading.h"
 
@@ -50,7 +51,20 @@ struct Wavefront {
     hipEvent_t count_ready[2] = {nullptr, nullptr};
 };
 
-enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_COUNT = 3 };
+// What the path stages need of the radiance cache.
+struct CacheDev {
+    float* records;           // training records (MLP_REC_FLOATS each)
+    uint32_t* counters;       // [0] training records of the job, [1] queries of the iteration
+    uint32_t record_capacity;
+    float* queries;
+    uint32_t* query_slot;
+    uint32_t* train_rec;      // per slot
+    uint32_t train_stride;
+    uint32_t collect;         // record training paths at the 2nd bounce
+    uint32_t query;           // end other paths there with the cache's prediction
+};
+
+enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_CACHE = 3, KERNEL_COUNT = 4 };
 
 // Optional per-launch timing of the main stages with events on the context's stream.
 struct KernelTimer {
@@ -138,6 +152,7 @@ struct Ctx {
     bool count_fetches = false;
     KernelTimer timer;
     bool stats_fresh = true;  // counters restart at the next pt_render
+    CacheState cache;
 
     // stats of the last render
     unsigned long long last_ext = 0, last_shadow = 0, last_paths = 0;
@@ -157,5 +172,19 @@ hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits);
 hipError_t trace_rays(Ctx& ctx, const pt_ray* d_rays, uint32_t count, pt_hit* d_hits, bool any_hit);
 
 hipError_t upload_sobol_table();
+
+// Radiance cache (cache.hip)
+int mlp_param_count();
+void cache_free(CacheState& c);
+hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capacity, hipStream_t st);
+hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st);
+hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st);
+hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, float* d_out, hipStream_t st);
+hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* d_out, hipStream_t st);
+hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* rad, unsigned blocks, hipStream_t st);
+hipError_t cache_backward(CacheState& c, const float* d_features, const float* d_targets, uint32_t n, uint32_t n_rec,
+                          uint32_t start, uint32_t stride, hipStream_t st);
+hipError_t cache_adam(CacheState& c, hipStream_t st);
+hipError_t cache_train_records(CacheState& c, uint32_t n_records, hipStream_t st);
 
 }  // namespace ptk

@@ -340,8 +340,10 @@ struct JobParams {
 
 // Starts path p in slot s: sampler, camera ray (pathtrace.rg.hlsl:18-33), fresh payload.
 __device__ __forceinline__ void start_path(const CameraBlock& cam, const JobParams& jp, const Wavefront& wf,
-                                           uint32_t s, uint32_t p, float3& o, float3& d)
+                                           const CacheDev& cd, uint32_t s, uint32_t p, float3& o, float3& d)
 {
+    if (cd.collect)
+        cd.train_rec[s] = NONE;
     const uint32_t pix = p % jp.npix, sl = p / jp.npix;
     const uint32_t px = pix % jp.width, py = pix / jp.width;
     Sampler smp;
@@ -356,12 +358,16 @@ __device__ __forceinline__ void start_path(const CameraBlock& cam, const JobPara
 }
 
 // Fills the pool with the job's first paths; their camera rays form queue 0.
-__global__ __launch_bounds__(256) void k_start_job(CameraBlock cam, JobParams jp, Wavefront wf)
+__global__ __launch_bounds__(256) void k_start_job(CameraBlock cam, JobParams jp, Wavefront wf, CacheDev cd)
 {
     const uint32_t n = min(jp.paths, wf.capacity);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cd.collect) {
+        cd.counters[0] = 0;
+        cd.counters[1] = 0;
+    }
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         float3 o, d;
-        start_path(cam, jp, wf, s, s, o, d);
+        start_path(cam, jp, wf, cd, s, s, o, d);
         wf.q_slot[0][s] = s;
         wf.q_o[0][s] = make_float4(o.x, o.y, o.z, cam.near_clip);
         wf.q_d[0][s] = make_float4(d.x, d.y, d.z, cam.far_clip);
@@ -438,13 +444,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
 
 // Miss / closest-hit / Russian roulette of one segment for every ray of queue q. Continuing
 // paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
-__global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wavefront wf, int q)
+__global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wavefront wf, int q, CacheDev cd)
 {
     const uint32_t count = wf.counters[q];
     const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
         const bool valid = i < count;
         bool cont = false, shadow = false;
+        bool want_train = false, want_query = false;
+        float cr[13];
+        float3 cT = f3(0, 0, 0), cL = f3(0, 0, 0);
         uint32_t slot = 0;
         float3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
         float3 sh_o = f3(0, 0, 0), sh_d = f3(0, 0, 0), sh_c = f3(0, 0, 0);
@@ -558,9 +567,30 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
                         twn = -twn;
                     const float3 nrd = -rd;
                     const float3 o = normalize3(f3(dot3(wx, nrd), dot3(wy, nrd), dot3(wn, nrd)));
+                    bool cached = false;
+                    if (o.z > 0.0f && cd.collect && segment == 1) {
+                        // Second-bounce vertex: a training path records the cache input here
+                        // (its target is completed when the path ends); any other path ends
+                        // here with the cache's prediction of the radiance leaving it.
+                        const float3 hp = ro + rd * t;
+                        cr[0] = hp.x, cr[1] = hp.y, cr[2] = hp.z;
+                        cr[3] = -rd.x, cr[4] = -rd.y, cr[5] = -rd.z;
+                        cr[6] = wn.x, cr[7] = wn.y, cr[8] = wn.z;
+                        cr[9] = mat.rmnb.x;
+                        cr[10] = mat.base_colour.x, cr[11] = mat.base_colour.y, cr[12] = mat.base_colour.z;
+                        cT = T;
+                        cL = L;
+                        if (hash_u32(p ^ (jp.sample_base * 0x9E3779B9u)) % cd.train_stride == 0) {
+                            want_train = true;
+                        } else if (cd.query) {
+                            want_query = true;
+                            cached = true;
+                            terminated = true;
+                        }
+                    }
                     if (o.z <= 0.0f) {
                         terminated = true;
-                    } else {
+                    } else if (!cached) {
                         const float3 xp = ro + rd * t + twn * ORIGIN_OFFSET;
                         // next-event estimation (ch.hlsl:159-225): the shadow ray is queued
                         if (s.light_count > 0) {
@@ -643,12 +673,33 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
             wf.s_d[spos] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
             wf.s_c[spos] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
         }
+        if (cd.collect) {
+            // training record: inputs, throughput and radiance so far (target filled at the end)
+            const uint32_t tpos = wave_append(want_train, &cd.counters[0]);
+            if (want_train && tpos < cd.record_capacity) {
+                float* rec = cd.records + size_t(tpos) * MLP_REC_FLOATS;
+                for (int k = 0; k < 13; k++)
+                    rec[k] = cr[k];
+                rec[13] = cT.x, rec[14] = cT.y, rec[15] = cT.z;
+                rec[16] = cL.x, rec[17] = cL.y, rec[18] = cL.z;
+                rec[19] = 0.0f;
+                cd.train_rec[slot] = tpos;
+            }
+            const uint32_t qpos = wave_append(want_query, &cd.counters[1]);
+            if (want_query) {
+                float* rec = cd.queries + size_t(qpos) * MLP_REC_FLOATS;
+                for (int k = 0; k < 13; k++)
+                    rec[k] = cr[k];
+                rec[13] = cT.x, rec[14] = cT.y, rec[15] = cT.z;
+                cd.query_slot[qpos] = slot;
+            }
+        }
     }
 }
 
 // Stores the radiance of every ended path and refills its slot with the job's next path,
 // whose camera ray joins queue 1-q.
-__global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams jp, Wavefront wf, int q)
+__global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams jp, Wavefront wf, int q, CacheDev cd)
 {
     const uint32_t count = wf.counters[4];
     const uint32_t rounded = (count + 63u) & ~63u;
@@ -657,14 +708,28 @@ __global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams j
         uint32_t slot = 0;
         if (valid) {
             slot = wf.retire[i];
-            wf.rad_out[wf.path_id[slot]] = wf.rad[slot];
+            const float4 lf = wf.rad[slot];
+            wf.rad_out[wf.path_id[slot]] = lf;
+            if (cd.collect) {
+                // training target: radiance gathered after the 2nd-bounce vertex, divided by
+                // the throughput that reached it
+                const uint32_t tr = cd.train_rec[slot];
+                if (tr != NONE) {
+                    float* rec = cd.records + size_t(tr) * MLP_REC_FLOATS;
+                    const float lv[3] = {lf.x, lf.y, lf.z};
+                    for (int c = 0; c < 3; c++) {
+                        const float tc = rec[13 + c];
+                        rec[13 + c] = tc > 0.0f ? (lv[c] - rec[16 + c]) / tc : 0.0f;
+                    }
+                }
+            }
         }
         // one atomic per wave hands out the next path ids
         const uint32_t first = wave_append(valid, &wf.counters[5]);
         const bool start = valid && first < jp.paths;
         float3 o = f3(0, 0, 0), d = f3(0, 0, 0);
         if (start)
-            start_path(cam, jp, wf, slot, first, o, d);
+            start_path(cam, jp, wf, cd, slot, first, o, d);
         const uint32_t pos = wave_append(start, &wf.counters[1 - q]);
         if (start) {
             wf.q_slot[1 - q][pos] = slot;
@@ -675,12 +740,14 @@ __global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams j
 }
 
 // Clears the queue counters an iteration is about to fill.
-__global__ void k_prepare(Wavefront wf, int q)
+__global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
 {
     if (threadIdx.x == 0) {
         wf.counters[1 - q] = 0;
         wf.counters[2] = 0;
         wf.counters[4] = 0;
+        if (cd.collect)
+            cd.counters[1] = 0;
     }
 }
 
@@ -840,14 +907,27 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
         const uint32_t samples = std::min(per_job, spp - done);
         jp.paths = npix * samples;
         jp.max_bounces = ctx.max_bounces;
+        CacheState& cs = ctx.cache;
+        CacheDev cd{};
+        if (cs.configured) {
+            cd.records = cs.records;
+            cd.counters = cs.counters;
+            cd.record_capacity = cs.record_capacity;
+            cd.queries = cs.queries;
+            cd.query_slot = cs.query_slot;
+            cd.train_rec = cs.train_rec;
+            cd.train_stride = std::max<uint32_t>(1, cs.train_stride);
+            cd.collect = 1;
+            cd.query = (cs.query && cs.steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
+        }
         hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min(jp.paths, wf.capacity), 256, 65535u * 4)), dim3(256),
-                           0, st, ctx.cam, jp, wf);
+                           0, st, ctx.cam, jp, wf, cd);
         const unsigned shade_blocks = blocks_for(wf.capacity, 256, 65535u * 4);
         int q = 0;
         // Iterate until the queue an iteration leaves behind is empty. The count of iteration
         // i is read back (pinned memory, event) while iteration i+1 is already queued.
         for (uint32_t it = 0; it < 1000000; it++) {
-            hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q);
+            hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd);
             if ((e = ctx.timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
                 return e;
             if (ctx.count_fetches)
@@ -858,7 +938,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                 return e;
             if ((e = ctx.timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, st, ctx.scene, jp, wf, q);
+            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, st, ctx.scene, jp, wf, q, cd);
             if ((e = ctx.timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
                 return e;
             if ((e = ctx.timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
@@ -869,7 +949,15 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                 hipLaunchKernelGGL(k_shadow<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
             if ((e = ctx.timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, ctx.cam, jp, wf, q);
+            if (cd.query) {
+                if ((e = ctx.timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
+                    return e;
+                if ((e = cache_query(cs, &cs.counters[1], wf.rad, 4096, st)) != hipSuccess)
+                    return e;
+                if ((e = ctx.timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
+                    return e;
+            }
+            hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, ctx.cam, jp, wf, q, cd);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
             if ((e = hipMemcpyAsync(&wf.host_counts[it & 1], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st)) !=
@@ -889,6 +977,19 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                            ctx.accum, ctx.cam.exposure);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
+        if (cd.collect && cs.train_steps > 0) {
+            // online training on the path suffixes this job recorded
+            uint32_t n_rec = 0;
+            if ((e = hipMemcpyAsync(&n_rec, &cs.counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+                return e;
+            if ((e = hipStreamSynchronize(st)) != hipSuccess)
+                return e;
+            n_rec = std::min(n_rec, cs.record_capacity);
+            cs.records_total += n_rec;
+            cs.last_records = n_rec;
+            if ((e = cache_train_records(cs, n_rec, st)) != hipSuccess)
+                return e;
+        }
         done += samples;
     }
     return hipSuccess;

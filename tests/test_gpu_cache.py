@@ -1,0 +1,186 @@
+"""The learned radiance cache on the device: MLP kernels against the NumPy model, Adam, and
+the cache inside the renderer (unchanged images when only training, gain when querying)."""
+import numpy as np
+import pytest
+
+import mlp_ref
+import pt_amd
+
+pytestmark = pytest.mark.gpu
+
+W, H = 96, 96
+
+
+@pytest.fixture(scope="module")
+def box(box_scn):
+    scene = pt_amd.Scene(box_scn)
+    cam = scene.camera(W, H)
+    tr = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    tr.set_camera(cam)
+    tr.cache_configure(query=False, seed=7)
+    return scene, cam, tr
+
+
+def _features(n, seed):
+    r = np.random.default_rng(seed)
+    return r.uniform(-1, 1, (n, 96)).astype(np.float32)
+
+
+def _params(seed, scale=1.0):
+    r = np.random.default_rng(seed)
+    p = np.zeros(mlp_ref.layout()[1], np.float32)
+    for w, b, ni, no in mlp_ref.layout()[0]:
+        p[w:w + ni * no] = r.uniform(-1, 1, ni * no) * np.sqrt(6.0 / ni) * scale
+        p[b:b + no] = r.uniform(-0.2, 0.2, no)
+    return p
+
+
+def test_param_count_and_initialisation(box):
+    _, _, tr = box
+    assert pt_amd.hip_lib().pt_cache_param_count() == mlp_ref.layout()[1]
+    tr.cache_configure(query=False, seed=7)
+    np.testing.assert_array_equal(tr.cache_params(), mlp_ref.he_uniform(7))
+    st = tr.cache_stats()
+    assert st["enabled"] == 1 and st["steps"] == 0 and st["param_count"] == mlp_ref.layout()[1]
+    # world box of the scene: the box [-1,1] x [-2,0] x [-1,1] and the emitter inside it
+    np.testing.assert_allclose(st["bounds_lo"], [-1, -2, -1], atol=1e-3)
+    np.testing.assert_allclose(st["bounds_hi"], [1, 0, 1], atol=1e-3)
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 1000])
+def test_forward_matches_numpy(box, n):
+    _, _, tr = box
+    p = _params(11)
+    tr.cache_set_params(p)
+    X = _features(n, n)
+    y = tr.cache_forward(X)
+    yb, _ = mlp_ref.forward(p, X, "bf16")
+    ye, _ = mlp_ref.forward(p, X, "exact")
+    # same rounding points as the kernel: only accumulation order and rare BF16 ties differ
+    assert np.linalg.norm(y - yb) <= 5e-3 * np.linalg.norm(yb) + 1e-6, np.abs(y - yb).max()
+    assert np.linalg.norm(y - ye) <= 5e-2 * np.linalg.norm(ye)
+
+
+@pytest.mark.parametrize("n", [32, 77, 512])
+def test_gradients_match_numpy(box, n):
+    _, _, tr = box
+    p = _params(12)
+    tr.cache_set_params(p)
+    X = _features(n, 100 + n)
+    T = np.random.default_rng(n).uniform(0, 3, (n, 3)).astype(np.float32)
+    loss, g = tr.cache_train_batch(X, T, apply=False)
+    Lb, gb = mlp_ref.loss_and_grad(p, X, T, "bf16")
+    Le, ge = mlp_ref.loss_and_grad(p, X, T, "exact")
+    assert loss == pytest.approx(Lb, rel=5e-3)
+    assert loss == pytest.approx(Le, rel=5e-2)
+    for l, (w, b, ni, no) in enumerate(mlp_ref.layout()[0]):
+        for lo, hi in ((w, w + ni * no), (b, b + no)):
+            d = np.linalg.norm(g[lo:hi] - gb[lo:hi])
+            assert d <= 2e-2 * np.linalg.norm(gb[lo:hi]) + 1e-7, (l, lo, d, np.linalg.norm(gb[lo:hi]))
+            # no further from the fp64 gradient than BF16 arithmetic itself puts the model
+            # (deep layers amplify the rounding of the back-propagated gradients)
+            gap = np.linalg.norm(gb[lo:hi] - ge[lo:hi])
+            assert np.linalg.norm(g[lo:hi] - ge[lo:hi]) <= 1.5 * gap + 2e-2 * np.linalg.norm(ge[lo:hi]) + 1e-7
+    # parameters untouched without apply
+    np.testing.assert_array_equal(tr.cache_params(), p)
+
+
+def test_adam_step_matches_numpy(box):
+    _, _, tr = box
+    tr.cache_configure(query=False, seed=3)  # fresh Adam state
+    p = _params(13)
+    tr.cache_set_params(p)
+    X = _features(256, 5)
+    T = np.random.default_rng(5).uniform(0, 1, (256, 3)).astype(np.float32)
+    m = np.zeros_like(p, np.float64)
+    v = np.zeros_like(p, np.float64)
+    cur = p.astype(np.float64)
+    for step in (1, 2, 3):
+        _, g = tr.cache_train_batch(X, T, apply=True)
+        cur, m, v = mlp_ref.adam(cur, g.astype(np.float64), m, v, step, lr=1e-3)
+        got = tr.cache_params()
+        np.testing.assert_allclose(got, cur, rtol=1e-5, atol=1e-6)
+        cur = got.astype(np.float64)
+    assert tr.cache_stats()["steps"] == 3
+
+
+def test_encoding_matches_numpy(box):
+    _, _, tr = box
+    r = np.random.default_rng(8)
+    rec = np.zeros((500, 13), np.float32)
+    rec[:, :3] = r.uniform([-1.2, -2.2, -1.2], [1.2, 0.2, 1.2], (500, 3))  # some outside the box
+    rec[:, 3:13] = r.uniform(-1, 1, (500, 10))
+    st = tr.cache_stats()
+    f = tr.cache_encode(rec)
+    ref = mlp_ref.encode(rec, st["bounds_lo"], st["bounds_hi"])
+    # BF16 rounding (2^-8 relative) plus fp32 argument reduction at the highest octave
+    np.testing.assert_allclose(f, ref, rtol=2 ** -7, atol=2e-3)
+    np.testing.assert_array_equal(f, mlp_ref.bf16(f))
+
+
+def test_training_reduces_loss(box):
+    _, _, tr = box
+    tr.cache_configure(query=False, seed=4, learning_rate=3e-3)
+    r = np.random.default_rng(9)
+    X = r.uniform(-1, 1, (4096, 96)).astype(np.float32)
+    T = (1.0 + 0.5 * np.sin(3 * X[:, :3]) + 0.3 * X[:, 3:6] ** 2).astype(np.float32)
+    first = None
+    for it in range(300):
+        sl = slice((it % 8) * 512, (it % 8 + 1) * 512)
+        loss, _ = tr.cache_train_batch(X[sl], T[sl], apply=True)
+        first = loss if first is None else first
+    final, _ = tr.cache_train_batch(X[:512], T[:512], apply=False)
+    assert final < 0.1 * first, (first, final)
+
+
+def test_training_only_leaves_the_image_unchanged(box):
+    scene, cam, tr = box
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    plain.render(8)
+    tr.cache_configure(query=False, seed=1, train_stride=4, train_steps=2, batch=4096)
+    tr.set_camera(cam)
+    for _ in range(8):
+        tr.render(1)
+    np.testing.assert_array_equal(tr.read(), plain.read())
+    st = tr.cache_stats()
+    assert st["steps"] == 16
+    assert st["records_total"] > 0.1 * 8 * W * H / 4  # one path in 4 trains when it reaches vertex 2
+    assert np.isfinite(st["last_loss"]) and st["last_loss"] > 0
+    plain.close()
+
+
+def test_cached_estimate_gains_at_equal_samples(box):
+    """The cached estimator (paths end at their 2nd vertex with the prediction) has a lower
+    error against the converged plain image than the plain estimator at the same sample
+    count, and traces fewer rays."""
+    scene, cam, tr = box
+    ref_tr = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    ref_tr.set_camera(cam)
+    ref_tr.render(4096)
+    ref = ref_tr.read()
+    ref_tr.close()
+
+    spp = 16
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    plain.render(spp)
+    e_plain = pt_amd.rel_mse(plain.read(), ref)
+    rays_plain = plain.stats()["extension_rays"]
+    plain.close()
+
+    tr.cache_configure(query=True, seed=2, train_stride=16, train_steps=8, batch=8192)
+    tr.set_camera(cam)
+    for _ in range(64):  # warm-up frames train the network
+        tr.render(1)
+    tr.set_camera(cam)  # restart the accumulation with the trained network
+    tr.reset_stats()
+    for _ in range(spp):
+        tr.render(1)
+    img = tr.read()
+    e_cache = pt_amd.rel_mse(img, ref)
+    rays_cache = tr.stats()["extension_rays"]
+    print(f"relMSE plain {e_plain:.3e} cached {e_cache:.3e}; rays plain {rays_plain} cached {rays_cache}")
+    assert np.all(np.isfinite(img))
+    assert e_cache < e_plain
+    assert rays_cache < rays_plain
