@@ -17,8 +17,14 @@ The JSON line also carries:
                 + 128 x node fetches + 48 x triangle-record fetches (extension) and
                 32 + 4 + 128 x nodes + 48 x tris (shadow); the fetch counts come from an
                 instrumented replay of the same sample indices.
+  convergence   seconds of rendering until the relative MSE against a 4096-spp plain
+                reference is <= 1e-3, for the plain estimator (16-spp render calls) and for the
+                learned-cache estimator (16-spp render calls, the network trained in flight
+                between wavefront iterations, paths ended at their second vertex with its
+                prediction; a 16-spp warm-up counts towards its time). Error evaluation is
+                outside the clock.
   cpu_baseline  the CPU oracle (this project's C++ restatement of the reference shaders,
-                OpenMP) on a bounded sample of the same frame: every 36th row, 1 spp.
+                OpenMP) timed on whole 1-spp frames of the same workload for ~12 s.
 """
 import argparse
 import json
@@ -67,6 +73,64 @@ def cpu_baseline(scene, cam, threads, target_s=12.0):
                       % (frames, paths, rays, dt)}
 
 
+def time_to_threshold(tr, cam, ref, chunk, limit_spp, limit_s, threshold, before=None):
+    """Renders chunk-spp batches until the relative MSE against ref is <= threshold; returns
+    (seconds of rendering incl. before(), spp, final rel-MSE, trajectory). Readback and the
+    error evaluation are outside the clock."""
+    tr.set_camera(cam)
+    tr.synchronize()
+    el = 0.0
+    if before is not None:
+        t = time.perf_counter()
+        before()
+        tr.synchronize()
+        el += time.perf_counter() - t
+        tr.set_camera(cam)  # restart the accumulation
+    done, e, traj = 0, float("inf"), []
+    while done < limit_spp and el < limit_s:
+        t = time.perf_counter()
+        tr.render(chunk)
+        tr.synchronize()
+        el += time.perf_counter() - t
+        done += chunk
+        e = pt_amd.rel_mse(tr.read(), ref)
+        traj.append((done, round(el, 4), float("%.4e" % e)))
+        if e <= threshold:
+            break
+    return el, done, e, traj
+
+
+def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
+    """Seconds to rel-MSE <= threshold against a ref_spp plain reference, plain vs cached."""
+    t = time.perf_counter()
+    tr.set_camera(cam)
+    for _ in range(ref_spp // 512):
+        tr.render(512)
+    tr.synchronize()
+    ref_s = time.perf_counter() - t
+    ref = tr.read()
+    log("convergence: %d-spp reference in %.1f s" % (ref_spp, ref_s))
+    plain_s, plain_spp, plain_e, plain_traj = time_to_threshold(tr, cam, ref, 16, 4096, 60.0, threshold)
+    log("convergence: plain %.3f s, %d spp, rel-MSE %.3e" % (plain_s, plain_spp, plain_e))
+    # in-flight training (two Adam steps of 8192 records after every wavefront iteration), one
+    # path in 16 trains; the others end at their second vertex with the prediction
+    cfg = dict(query=True, train_stride=16, train_steps=2, batch=8192, learning_rate=1e-3, frame_spp=0,
+               continue_prob=0.0, seed=1)
+    tr.cache_configure(**cfg)
+    warm = 16
+    cached_s, cached_spp, cached_e, cached_traj = time_to_threshold(
+        tr, cam, ref, 16, 4096, max(60.0, 3 * plain_s), threshold, before=lambda: tr.render(warm))
+    cs = tr.cache_stats()
+    tr.cache_configure(enable=False)
+    log("convergence: cached %.3f s, %d spp, rel-MSE %.3e" % (cached_s, cached_spp, cached_e))
+    return {"threshold": threshold, "reference_spp": ref_spp, "reference_s": round(ref_s, 3),
+            "plain_s": round(plain_s, 4), "plain_spp": plain_spp, "plain_relmse": plain_e,
+            "cached_s": round(cached_s, 4), "cached_spp": cached_spp, "cached_relmse": cached_e,
+            "cached_over_plain_time": round(cached_s / plain_s, 3) if plain_s > 0 else None,
+            "cache_config": dict(cfg, warmup_spp=warm, steps_done=cs["steps"]),
+            "plain_trajectory": plain_traj[-4:], "cached_trajectory": cached_traj[-4:]}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of a stage from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", "k_extend_instanced_1080p_pmc_traffic.txt")
@@ -87,6 +151,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     a = ap.parse_args()
     if a.gpus != 1:
@@ -161,6 +226,11 @@ def main():
                 "bytes_per_launch": d["bytes"] / max(1, d["launches"]),
                 "avg_launch_ms": d["ms"] / max(1, d["launches"])}
 
+    conv = None
+    if not a.no_convergence:
+        tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 0)
+        conv = convergence(tr, cam)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, cam, a.cpu_threads)
@@ -180,6 +250,7 @@ def main():
         "stages_ms": {k: round(v["ms"], 3) for k, v in stages.items()},
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
         "roofline": roofline,
+        "convergence": conv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -196,10 +196,19 @@ typedef struct pt_cache_config {
     uint32_t enable;       /* 0: free the cache, plain path tracing */
     uint32_t query;        /* != 0: end non-training paths at the 2nd vertex with the prediction */
     uint32_t train_stride; /* one path in train_stride is a training path (>= 1) */
-    uint32_t train_steps;  /* optimiser steps after each render job */
+    uint32_t train_steps;  /* optimiser steps after each frame (frame_spp > 0) or after each
+                            * wavefront iteration (frame_spp == 0) */
     uint32_t batch;        /* samples per optimiser step */
     float learning_rate;   /* Adam step size */
     uint32_t seed;         /* weight initialisation (He-uniform from a Splitmix32 stream) */
+    uint32_t frame_spp;    /* > 0: render jobs of at most frame_spp samples per pixel, training
+                            * on each job's completed records between jobs. 0: jobs as large as
+                            * without the cache, training in flight on the records completed so
+                            * far, between wavefront iterations (no host synchronisation) */
+    float continue_prob;   /* 0: a queried path ends with the prediction (biased). p in (0, 1]:
+                            * it goes on with probability p and weight 1/p while the prediction
+                            * is added with weight 1 - 1/p, a control variate (unbiased; per-sample
+                            * sums may then be negative and are accumulated as they are) */
     uint32_t reserved;
 } pt_cache_config;
 

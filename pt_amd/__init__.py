@@ -91,7 +91,8 @@ PT_CACHE_FEATURES = 96
 class CacheConfig(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("query", C.c_uint32), ("train_stride", C.c_uint32),
                 ("train_steps", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
-                ("seed", C.c_uint32), ("reserved", C.c_uint32)]
+                ("seed", C.c_uint32), ("frame_spp", C.c_uint32), ("continue_prob", C.c_float),
+                ("reserved", C.c_uint32)]
 
 
 class CacheStats(C.Structure):
@@ -373,8 +374,9 @@ class Tracer:
     # ---- learned radiance cache (no counterpart in the reference)
 
     def cache_configure(self, enable=True, query=True, train_stride=32, train_steps=4, batch=16384,
-                        learning_rate=1e-3, seed=1):
-        cfg = CacheConfig(int(enable), int(query), train_stride, train_steps, batch, learning_rate, seed, 0)
+                        learning_rate=1e-3, seed=1, frame_spp=1, continue_prob=0.0):
+        cfg = CacheConfig(int(enable), int(query), train_stride, train_steps, batch, learning_rate, seed, frame_spp,
+                          continue_prob, 0)
         self._check(self._lib.pt_cache_configure(self._h, C.byref(cfg)))
 
     def cache_stats(self):

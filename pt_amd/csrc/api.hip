@@ -592,7 +592,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
         c.stats_fresh = true;
         return PT_OK;
     }
-    if (cfg->train_stride == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f) || cfg->batch > (1u << 24))
+    if (cfg->train_stride == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f) || cfg->batch > (1u << 24) ||
+        !(cfg->continue_prob >= 0.0f && cfg->continue_prob <= 1.0f))
         return fail(&c, PT_EINVAL, "invalid cache configuration");
     CacheState& cs = c.cache;
     cs.query = cfg->query != 0;
@@ -601,6 +602,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
     cs.batch = cfg->batch;
     cs.lr = cfg->learning_rate;
     cs.seed = cfg->seed;
+    cs.frame_spp = cfg->frame_spp;
+    cs.cont_prob = cfg->continue_prob;
     cs.records_total = 0;
     cs.last_records = 0;
     // one training record per training path of a job
@@ -631,6 +634,11 @@ int pt_cache_get_stats(pt_ctx* h, pt_cache_stats* out)
     if (cs.configured) {
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
         HIP_TRY(&c, hipMemcpy(&s.last_loss, cs.loss_acc, 4, hipMemcpyDeviceToHost), "cache loss");
+        if (cs.frame_spp == 0) {  // in-flight training: records of the current job
+            uint32_t n = 0;
+            HIP_TRY(&c, hipMemcpy(&n, cs.counters, 4, hipMemcpyDeviceToHost), "cache records");
+            s.last_records = std::min(n, cs.record_capacity);
+        }
     }
     for (int k = 0; k < 3; k++) {
         s.bounds_lo[k] = cs.bounds_lo[k];

@@ -58,6 +58,8 @@ struct CacheState {
     uint32_t batch = 16384;
     float lr = 1e-3f;
     uint32_t seed = 1;
+    uint32_t frame_spp = 1;      // samples per render job while training
+    float cont_prob = 0.0f;      // control-variate continuation probability (0: terminate)
     uint32_t steps_done = 0;
     float last_loss = 0.0f;
     uint64_t records_total = 0;

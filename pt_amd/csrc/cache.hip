@@ -16,6 +16,7 @@
 
 #include "cache.h"
 #include "common.h"
+#include "shading.h"
 
 namespace ptk {
 
@@ -144,37 +145,57 @@ struct CacheArgs {
     float lo[3], inv_ext[3];
 };
 
+// Where the training samples of a batch come from: records (sample i is record
+// start + i * stride mod n_rec, or, with n_rec_dev, a hashed draw among the min(*n_rec_dev,
+// cap) records written so far, of which only the completed ones count) or, when features
+// != nullptr, explicit feature rows and targets (n x 96 and n x 3 floats).
+struct TrainSrc {
+    const float* records;
+    uint32_t n_rec;
+    const uint32_t* n_rec_dev;
+    uint32_t cap;
+    uint32_t start, stride;
+    const float* features;
+    const float* targets;
+};
+
 // Training: forward + backward over n samples; gradients (summed over the batch, already
-// scaled by the loss normalisation) are added into grads. Samples come from records
-// (perm index start + i * stride mod n_rec) or, when features != nullptr, from explicit
-// feature rows and targets (n x 96 and n x 3 floats).
-__global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, const float* records, uint32_t n_rec, uint32_t start,
-                                                   uint32_t stride, const float* features, const float* targets,
-                                                   uint32_t n)
+// scaled by the loss normalisation) are added into grads.
+__global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uint32_t n)
 {
     __shared__ __attribute__((aligned(16))) uint16_t lds[(MLP_LAYERS + 2) * WS * ST];
     uint16_t* const gbase = lds + MLP_LAYERS * WS * ST;  // two gradient buffers
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
     const float inv_n3 = 1.0f / (3.0f * float(n));
     float loss = 0.0f;
+    const uint32_t n_rec = src.n_rec_dev ? min(*src.n_rec_dev, src.cap) : src.n_rec;
+    if (!src.features && n_rec == 0)
+        return;
 
     for (uint32_t base = blockIdx.x * WS; base < n; base += gridDim.x * WS) {
         float tgt[3] = {0, 0, 0};
+        float use = 0.0f;  // 1 for a sample that counts
         const uint32_t s = base + uint32_t(lane);
         if (lane < WS) {
             uint16_t* row = lds + lane * ST;
             if (s < n) {
-                if (features) {
+                if (src.features) {
                     for (int f = 0; f < MLP_IN; f++)
-                        row[f] = f2bf(features[size_t(s) * MLP_IN + f]);
+                        row[f] = f2bf(src.features[size_t(s) * MLP_IN + f]);
                     for (int c = 0; c < 3; c++)
-                        tgt[c] = targets[size_t(s) * 3 + c];
+                        tgt[c] = src.targets[size_t(s) * 3 + c];
+                    use = 1.0f;
                 } else {
-                    const uint32_t j = uint32_t((uint64_t(start) + uint64_t(s) * stride) % n_rec);
-                    const float* rec = records + size_t(j) * MLP_REC_FLOATS;
+                    uint32_t j;
+                    if (src.n_rec_dev)
+                        j = hash_u32(src.start + s * 0x9E3779B9u) % n_rec;
+                    else
+                        j = uint32_t((uint64_t(src.start) + uint64_t(s) * src.stride) % n_rec);
+                    const float* rec = src.records + size_t(j) * MLP_REC_FLOATS;
                     encode_row(rec, a.lo, a.inv_ext, row);
                     for (int c = 0; c < 3; c++)
                         tgt[c] = rec[13 + c];
+                    use = rec[19] != 0.0f ? 1.0f : 0.0f;  // the target is complete
                 }
             } else {
                 for (int f = 0; f < MLP_IN; f++)
@@ -195,9 +216,10 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, const float* reco
                 float t_c = 0.0f;
                 // fetch target component r of sample srow from the lane that loaded it
                 const float t0 = __shfl(tgt[0], srow), t1 = __shfl(tgt[1], srow), t2 = __shfl(tgt[2], srow);
+                const float u = __shfl(use, srow);
                 t_c = r == 0 ? t0 : (r == 1 ? t1 : t2);
                 float gr = 0.0f;
-                if (r < 3 && base + uint32_t(srow) < n) {
+                if (r < 3 && u != 0.0f) {
                     const float yv = y[mt][i];
                     const float den = yv * yv + 0.01f;
                     const float d = yv - t_c;
@@ -268,7 +290,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, const float* reco
 }
 
 // Inference. Query mode (features == nullptr): each query record's prediction, clamped at
-// zero and weighted by the record's throughput, is added to the radiance of its slot.
+// zero and weighted by the record's throughput and weight, is added to the radiance of its slot.
 // Test mode: predictions of explicit feature rows are written to out (n x 3).
 __global__ __launch_bounds__(256) void k_mlp_infer(CacheArgs a, const float* queries, const uint32_t* query_slot,
                                                    const uint32_t* count_ptr, float4* rad, const float* features,
@@ -307,9 +329,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(CacheArgs a, const float* que
                 if (features) {
                     out[size_t(q) * 3 + r] = y[mt][i];
                 } else {
-                    const float T = queries[size_t(q) * MLP_REC_FLOATS + 13 + r];
+                    const float* rec = queries + size_t(q) * MLP_REC_FLOATS;
                     float* dst = reinterpret_cast<float*>(&rad[query_slot[q]]) + r;
-                    *dst += T * fmaxf(y[mt][i], 0.0f);
+                    *dst += rec[19] * rec[13 + r] * fmaxf(y[mt][i], 0.0f);
                 }
             }
         __builtin_amdgcn_s_waitcnt(0);
@@ -516,8 +538,8 @@ hipError_t cache_backward(CacheState& c, const float* d_features, const float* d
     if (e != hipSuccess)
         return e;
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + WS - 1) / WS, 2048));
-    hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), c.records, n_rec, start,
-                       stride, d_features, d_targets, n);
+    TrainSrc src{c.records, n_rec, nullptr, c.record_capacity, start, stride, d_features, d_targets};
+    hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), src, n);
     return hipGetLastError();
 }
 
@@ -531,6 +553,26 @@ hipError_t cache_adam(CacheState& c, hipStream_t st)
     if (e != hipSuccess)
         return e;
     return cache_sync_weights(c, st);
+}
+
+// Steps on the records written so far in the current job (count read on the device; only
+// completed records count): lets the network learn while a job's paths are still in flight.
+hipError_t cache_train_inflight(CacheState& c, hipStream_t st)
+{
+    hipError_t e;
+    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(c.batch) + WS - 1) / WS, 2048));
+    for (uint32_t k = 0; k < c.train_steps; k++) {
+        if ((e = hipMemsetAsync(c.loss_acc, 0, 4, st)) != hipSuccess)
+            return e;
+        TrainSrc src{c.records, 0, c.counters, c.record_capacity, (c.steps_done + 1) * 0x6C8E9CF5u, 0, nullptr,
+                     nullptr};
+        hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), src, c.batch);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        if ((e = cache_adam(c, st)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 // Trains on the records collected by the last job: steps minibatches of the batch size,

@@ -62,6 +62,7 @@ struct CacheDev {
     uint32_t train_stride;
     uint32_t collect;         // record training paths at the 2nd bounce
     uint32_t query;           // end other paths there with the cache's prediction
+    float cont_prob;          // > 0: queried paths go on with this probability (control variate)
 };
 
 enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_CACHE = 3, KERNEL_COUNT = 4 };
@@ -186,5 +187,6 @@ hipError_t cache_backward(CacheState& c, const float* d_features, const float* d
                           uint32_t start, uint32_t stride, hipStream_t st);
 hipError_t cache_adam(CacheState& c, hipStream_t st);
 hipError_t cache_train_records(CacheState& c, uint32_t n_records, hipStream_t st);
+hipError_t cache_train_inflight(CacheState& c, hipStream_t st);
 
 }  // namespace ptk

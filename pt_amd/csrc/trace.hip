@@ -336,6 +336,7 @@ struct JobParams {
     uint32_t sample_base;   // sample index of job-local sample 0
     uint32_t paths;         // npix * samples of the job
     uint32_t max_bounces;
+    uint32_t signed_radiance;  // keep negative per-sample sums (control-variate estimates)
 };
 
 // Starts path p in slot s: sampler, camera ray (pathtrace.rg.hlsl:18-33), fresh payload.
@@ -452,6 +453,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
         const bool valid = i < count;
         bool cont = false, shadow = false;
         bool want_train = false, want_query = false;
+        float qf = 1.0f;  // weight of the cache's prediction
         float cr[13];
         float3 cT = f3(0, 0, 0), cL = f3(0, 0, 0);
         uint32_t slot = 0;
@@ -584,8 +586,16 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
                             want_train = true;
                         } else if (cd.query) {
                             want_query = true;
-                            cached = true;
-                            terminated = true;
+                            const float uc = float(hash_u32(hash_u32(p ^ 0xA511E9B3u) + jp.sample_base) >> 8) * 0x1p-24f;
+                            if (uc < cd.cont_prob) {
+                                // control variate: the path goes on with weight 1/p and the
+                                // prediction is added with weight 1 - 1/p (unbiased)
+                                qf = 1.0f - 1.0f / cd.cont_prob;
+                                T = T * (1.0f / cd.cont_prob);
+                            } else {
+                                cached = true;
+                                terminated = true;
+                            }
                         }
                     }
                     if (o.z <= 0.0f) {
@@ -691,6 +701,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
                 for (int k = 0; k < 13; k++)
                     rec[k] = cr[k];
                 rec[13] = cT.x, rec[14] = cT.y, rec[15] = cT.z;
+                rec[19] = qf;
                 cd.query_slot[qpos] = slot;
             }
         }
@@ -721,6 +732,7 @@ __global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams j
                         const float tc = rec[13 + c];
                         rec[13 + c] = tc > 0.0f ? (lv[c] - rec[16 + c]) / tc : 0.0f;
                     }
+                    rec[19] = 1.0f;  // complete
                 }
             }
         }
@@ -759,9 +771,15 @@ __global__ __launch_bounds__(256) void k_accumulate(JobParams jp, const float4* 
         float4 acc = accum[pix];
         for (uint32_t sl = 0; sl < samples; sl++) {
             const float4 r4 = rad[size_t(sl) * jp.npix + pix];
-            const float rx = correct_component(r4.x * exposure);
-            const float ry = correct_component(r4.y * exposure);
-            const float rz = correct_component(r4.z * exposure);
+            float rx, ry, rz;
+            if (jp.signed_radiance) {
+                rx = r4.x * exposure, ry = r4.y * exposure, rz = r4.z * exposure;
+                rx = isfinite(rx) ? rx : 0.0f, ry = isfinite(ry) ? ry : 0.0f, rz = isfinite(rz) ? rz : 0.0f;
+            } else {
+                rx = correct_component(r4.x * exposure);
+                ry = correct_component(r4.y * exposure);
+                rz = correct_component(r4.z * exposure);
+            }
             const uint32_t si = jp.sample_base + sl;
             if (si == 0) {
                 acc = make_float4(rx, ry, rz, 1.0f);
@@ -894,7 +912,11 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
 {
     Wavefront& wf = ctx.wf;
     const uint32_t npix = ctx.width * ctx.height;
-    const uint32_t per_job = std::max<uint32_t>(1, wf.rad_out_capacity / npix);
+    uint32_t per_job = std::max<uint32_t>(1, wf.rad_out_capacity / npix);
+    const bool train_between = ctx.cache.configured && ctx.cache.train_steps > 0 && ctx.cache.frame_spp > 0;
+    const bool train_inflight = ctx.cache.configured && ctx.cache.train_steps > 0 && ctx.cache.frame_spp == 0;
+    if (train_between)
+        per_job = std::min(per_job, ctx.cache.frame_spp);  // train between frames
     hipStream_t st = ctx.stream;
     hipError_t e;
     uint32_t done = 0;
@@ -919,10 +941,13 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
             cd.train_stride = std::max<uint32_t>(1, cs.train_stride);
             cd.collect = 1;
             cd.query = (cs.query && cs.steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
+            cd.cont_prob = cs.cont_prob;
+            jp.signed_radiance = (cd.query && cs.cont_prob > 0.0f) ? 1u : 0u;
         }
         hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min(jp.paths, wf.capacity), 256, 65535u * 4)), dim3(256),
                            0, st, ctx.cam, jp, wf, cd);
-        const unsigned shade_blocks = blocks_for(wf.capacity, 256, 65535u * 4);
+        // grid-stride stages: enough blocks to fill the chip, not one per pool slot
+        const unsigned shade_blocks = blocks_for(wf.capacity, 256, ctx.grid_blocks * 2);
         int q = 0;
         // Iterate until the queue an iteration leaves behind is empty. The count of iteration
         // i is read back (pinned memory, event) while iteration i+1 is already queued.
@@ -952,7 +977,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
             if (cd.query) {
                 if ((e = ctx.timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
                     return e;
-                if ((e = cache_query(cs, &cs.counters[1], wf.rad, 4096, st)) != hipSuccess)
+                if ((e = cache_query(cs, &cs.counters[1], wf.rad, ctx.grid_blocks, st)) != hipSuccess)
                     return e;
                 if ((e = ctx.timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
                     return e;
@@ -960,6 +985,14 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
             hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, ctx.cam, jp, wf, q, cd);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
+            if (train_inflight) {
+                if ((e = ctx.timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
+                    return e;
+                if ((e = cache_train_inflight(cs, st)) != hipSuccess)
+                    return e;
+                if ((e = ctx.timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
+                    return e;
+            }
             if ((e = hipMemcpyAsync(&wf.host_counts[it & 1], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st)) !=
                 hipSuccess)
                 return e;
@@ -977,7 +1010,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
                            ctx.accum, ctx.cam.exposure);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        if (cd.collect && cs.train_steps > 0) {
+        if (cd.collect && train_between) {
             // online training on the path suffixes this job recorded
             uint32_t n_rec = 0;
             if ((e = hipMemcpyAsync(&n_rec, &cs.counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)

@@ -150,6 +150,39 @@ def test_training_only_leaves_the_image_unchanged(box):
     plain.close()
 
 
+def test_inflight_training_leaves_the_image_unchanged(box):
+    scene, cam, tr = box
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    plain.render(16)
+    tr.cache_configure(query=False, seed=5, train_stride=4, train_steps=1, batch=2048, frame_spp=0)
+    tr.set_camera(cam)
+    tr.render(16)
+    np.testing.assert_array_equal(tr.read(), plain.read())
+    st = tr.cache_stats()
+    assert st["steps"] >= 9  # one step after every wavefront iteration of the 16-spp job
+    assert st["last_records"] > 0.1 * 16 * W * H / 4
+    assert np.isfinite(st["last_loss"]) and st["last_loss"] > 0
+    plain.close()
+
+
+def test_control_variate_with_certain_continuation_is_the_plain_estimate(box):
+    """continue_prob = 1: every queried path goes on with weight 1 and the prediction enters
+    with weight 0, so the image is the plain one."""
+    scene, cam, tr = box
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    tr.cache_configure(query=True, seed=6, train_stride=8, train_steps=2, batch=4096, continue_prob=1.0)
+    tr.set_camera(cam)
+    tr.render(2)  # trains; the first frame is not queried
+    tr.set_camera(cam)
+    for _ in range(8):
+        tr.render(1)
+    plain.render(8)
+    np.testing.assert_array_equal(tr.read(), plain.read())
+    plain.close()
+
+
 def test_cached_estimate_gains_at_equal_samples(box):
     """The cached estimator (paths end at their 2nd vertex with the prediction) has a lower
     error against the converged plain image than the plain estimator at the same sample
