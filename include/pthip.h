@@ -138,6 +138,8 @@ typedef struct pt_stats {
  * instrumentation is the wall-clock print of src/pathtrace/ptdispatch.cpp:65-70). */
 #define PT_OPT_PROFILE_KERNELS 1u /* value != 0: time every stage launch with HIP events */
 #define PT_OPT_COUNT_FETCHES 2u   /* value != 0: count BVH node / triangle fetches per ray */
+#define PT_OPT_GRAPHS 3u          /* value != 0 (default): replay each wavefront iteration from a
+                                   * captured HIP graph (not while kernels are being timed) */
 
 typedef struct pt_ctx pt_ctx;
 
@@ -182,6 +184,39 @@ int pt_reset_stats(pt_ctx* ctx);
 
 /* Sets an instrumentation option (PT_OPT_*). */
 int pt_set_option(pt_ctx* ctx, uint32_t option, int64_t value);
+
+/* ---- Batches: many views of one or more scenes in one persistent wavefront run -----------
+ * No counterpart in the reference, which renders one camera of one scene per PathTracer
+ * (src/pathtrace/pathtrace.h:43-103). A batch takes the scenes of existing contexts (their
+ * device-built BVHs are shared, not copied; the contexts must outlive the batch) and a list
+ * of views; pt_batch_render traces every view's spp further samples with all views' paths
+ * streaming through one path pool, each view accumulating exactly as pt_render would
+ * (same sampler seeding per pixel and sample index, same running mean). */
+typedef struct pt_view {
+    uint32_t scene;       /* index into the batch's scene list */
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;         /* samples per pixel added by each pt_batch_render (> 0) */
+    uint32_t max_bounces; /* PathTracer::set_max_bounces */
+    uint32_t reserved;
+    pt_camera camera;     /* PathTracer::set_camera content */
+} pt_view;
+
+typedef struct pt_batch pt_batch;
+
+int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* views, uint32_t view_count,
+                    pt_batch** out);
+/* Replaces a view's camera, scene, spp or depth (not its size) and restarts its accumulation. */
+int pt_batch_set_view(pt_batch* batch, uint32_t view, const pt_view* v);
+/* Asynchronous on the batch's stream. */
+int pt_batch_render(pt_batch* batch);
+int pt_batch_read(pt_batch* batch, uint32_t view, float* rgba, size_t row_pitch);
+/* Counters summed over all views (and BVH figures over all scenes); width/height are 0. */
+int pt_batch_get_stats(pt_batch* batch, pt_stats* out);
+int pt_batch_set_option(pt_batch* batch, uint32_t option, int64_t value);
+int pt_batch_reset_stats(pt_batch* batch);
+const char* pt_batch_last_error(const pt_batch* batch);
+void pt_batch_destroy(pt_batch* batch);
 
 /* ---- Learned radiance cache ---------------------------------------------------------------
  * No counterpart in the reference: an optional estimator on top of its path tracer. A small

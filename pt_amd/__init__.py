@@ -85,7 +85,13 @@ class Stats(C.Structure):
 
 PT_OPT_PROFILE_KERNELS = 1
 PT_OPT_COUNT_FETCHES = 2
+PT_OPT_GRAPHS = 3
 PT_CACHE_FEATURES = 96
+
+
+class View(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("reserved", C.c_uint32), ("camera", Camera)]
 
 
 class CacheConfig(C.Structure):
@@ -161,6 +167,16 @@ def hip_lib():
         lib.pt_last_error.argtypes = [P]
         lib.pt_last_error.restype = C.c_char_p
         lib.pt_destroy.argtypes = [P]
+        lib.pt_batch_create.argtypes = [C.POINTER(P), C.c_uint32, C.POINTER(View), C.c_uint32, C.POINTER(P)]
+        lib.pt_batch_set_view.argtypes = [P, C.c_uint32, C.POINTER(View)]
+        lib.pt_batch_render.argtypes = [P]
+        lib.pt_batch_read.argtypes = [P, C.c_uint32, C.c_void_p, C.c_size_t]
+        lib.pt_batch_get_stats.argtypes = [P, C.POINTER(Stats)]
+        lib.pt_batch_set_option.argtypes = [P, C.c_uint32, C.c_int64]
+        lib.pt_batch_reset_stats.argtypes = [P]
+        lib.pt_batch_last_error.argtypes = [P]
+        lib.pt_batch_last_error.restype = C.c_char_p
+        lib.pt_batch_destroy.argtypes = [P]
         lib.pt_cache_configure.argtypes = [P, C.POINTER(CacheConfig)]
         lib.pt_cache_get_stats.argtypes = [P, C.POINTER(CacheStats)]
         lib.pt_cache_param_count.argtypes = []
@@ -418,3 +434,70 @@ class Tracer:
         self._check(self._lib.pt_cache_train_batch(self._h, f.ctypes.data, t.ctypes.data, len(f), int(apply),
                                                    grads.ctypes.data, loss.ctypes.data))
         return float(loss[0]), grads
+
+
+class Batch:
+    """Many views of one or more scenes rendered in one persistent wavefront run
+    (pt_batch_* in include/pthip.h). tracers: the Tracer objects whose scenes the views use
+    (kept alive by the batch); views: (scene index, width, height, spp, max_bounces, Camera)."""
+
+    def __init__(self, tracers, views):
+        lib = hip_lib()
+        self._lib = lib
+        self.tracers = list(tracers)
+        self.views = [tuple(v) for v in views]
+        arr = (C.c_void_p * len(self.tracers))(*[t._h.value for t in self.tracers])
+        varr = (View * len(self.views))()
+        for i, v in enumerate(self.views):
+            varr[i] = self._view(v)
+        h = C.c_void_p()
+        code = lib.pt_batch_create(C.cast(arr, C.POINTER(C.c_void_p)), len(self.tracers), varr, len(self.views),
+                                   C.byref(h))
+        if code != PT_OK:
+            raise PtError(code, lib.pt_batch_last_error(None).decode())
+        self._h = h
+
+    @staticmethod
+    def _view(v):
+        scene, w, h, spp, bounces, cam = v
+        return View(scene, w, h, spp, bounces, 0, cam)
+
+    def _check(self, code):
+        if code != PT_OK:
+            raise PtError(code, self._lib.pt_batch_last_error(self._h).decode())
+
+    def render(self):
+        self._check(self._lib.pt_batch_render(self._h))
+
+    def set_view(self, i, v):
+        vv = self._view(v)
+        self._check(self._lib.pt_batch_set_view(self._h, i, C.byref(vv)))
+        self.views[i] = tuple(v)
+
+    def read(self, i):
+        _, w, h, _, _, _ = self.views[i]
+        out = np.zeros((h, w, 4), np.float32)
+        self._check(self._lib.pt_batch_read(self._h, i, out.ctypes.data, w * 16))
+        return out
+
+    def stats(self):
+        s = Stats()
+        self._check(self._lib.pt_batch_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def set_option(self, option, value):
+        self._check(self._lib.pt_batch_set_option(self._h, option, int(value)))
+
+    def reset_stats(self):
+        self._check(self._lib.pt_batch_reset_stats(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

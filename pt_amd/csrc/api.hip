@@ -64,6 +64,15 @@ void release(Ctx& c)
     free_bvh(c.bvh);
     wavefront_free(c.wf);
     cache_free(c.cache);
+    c.graphs.destroy();
+    if (c.d_views)
+        (void)hipFree(c.d_views);
+    if (c.d_job)
+        (void)hipFree(c.d_job);
+    if (c.h_view)
+        (void)hipHostFree(c.h_view);
+    if (c.h_job)
+        (void)hipHostFree(c.h_job);
     c.timer.destroy();
     if (c.ev_start)
         (void)hipEventDestroy(c.ev_start);
@@ -260,6 +269,109 @@ struct DevBufs {
 
 }  // namespace
 
+struct pt_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::string error;
+    std::vector<pt_ctx*> scenes;
+    std::vector<pt_view> views;
+    std::vector<uint32_t> sample_index;
+    std::vector<size_t> accum_offset;  // in pixels
+    float4* accum = nullptr;
+    DeviceScene* d_scenes = nullptr;   // scene table (several scenes only)
+    ViewDev* d_views = nullptr;
+    ViewDev* h_views = nullptr;        // pinned staging
+    JobDev* d_job = nullptr;
+    JobDev* h_job = nullptr;
+    Wavefront wf;
+    uint32_t grid_blocks = 0;
+    bool count_fetches = false;
+    bool use_graphs = true;
+    KernelTimer timer;
+    IterGraphs graphs;
+    bool stats_fresh = true;
+    unsigned long long last_paths = 0;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+int bfail(pt_batch* b, int code, const std::string& msg)
+{
+    if (b)
+        b->error = msg;
+    else
+        g_batch_error = msg;
+    return code;
+}
+
+int bhip_fail(pt_batch* b, hipError_t e, const char* what)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PT_ENOMEM : PT_EDEVICE;
+    return bfail(b, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BHIP_TRY(b, expr, what)                \
+    do {                                       \
+        hipError_t e_ = (expr);                \
+        if (e_ != hipSuccess)                  \
+            return bhip_fail((b), e_, (what)); \
+    } while (0)
+
+void batch_release(pt_batch& b)
+{
+    if (b.stream)
+        (void)hipStreamSynchronize(b.stream);
+    void* ptrs[] = {b.accum, b.d_scenes, b.d_views, b.d_job};
+    for (void* p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    if (b.h_views)
+        (void)hipHostFree(b.h_views);
+    if (b.h_job)
+        (void)hipHostFree(b.h_job);
+    wavefront_free(b.wf);
+    b.timer.destroy();
+    b.graphs.destroy();
+    if (b.ev_start)
+        (void)hipEventDestroy(b.ev_start);
+    if (b.ev_stop)
+        (void)hipEventDestroy(b.ev_stop);
+    if (b.stream)
+        (void)hipStreamDestroy(b.stream);
+}
+
+int check_view(const pt_view& v, uint32_t scene_count, std::string& why)
+{
+    if (v.scene >= scene_count)
+        return (why = "view scene index out of range", PT_EINVAL);
+    if (v.width == 0 || v.height == 0 || uint64_t(v.width) * v.height > (1ull << 27))
+        return (why = "invalid view size", PT_EINVAL);
+    if (v.spp == 0)
+        return (why = "Sample count was zero.", PT_EINVAL);
+    if (v.max_bounces > 1024)
+        return (why = "max_bounces too large", PT_EINVAL);
+    return PT_OK;
+}
+
+void view_dev(const pt_batch& b, uint32_t i, ViewDev& d)
+{
+    const pt_view& v = b.views[i];
+    static_assert(sizeof(CameraBlock) == sizeof(pt_camera), "camera block layout");
+    memcpy(&d.cam, &v.camera, sizeof(CameraBlock));
+    d.accum = b.accum + b.accum_offset[i];
+    d.width = v.width;
+    d.height = v.height;
+    d.npix = v.width * v.height;
+    d.scene = b.scenes.size() > 1 ? v.scene : 0u;
+    d.max_bounces = v.max_bounces;
+    d.pad = 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out)
@@ -374,6 +486,12 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              const uint32_t job_spp = std::max<uint32_t>(1, std::min<uint32_t>(1024, (1u << 27) / npix));
              HIP_TRY(&c, wavefront_alloc(c.wf, pool, npix * job_spp, c.grid_blocks * TRACE_BLOCK),
                      "wavefront buffers");
+             HIP_TRY(&c, hipMalloc(&c.d_views, sizeof(ViewDev)), "view table");
+             HIP_TRY(&c, hipMalloc(&c.d_job, sizeof(JobDev)), "job table");
+             HIP_TRY(&c, hipHostMalloc((void**)&c.h_view, sizeof(ViewDev), hipHostMallocDefault), "view table");
+             HIP_TRY(&c, hipHostMalloc((void**)&c.h_job, sizeof(JobDev), hipHostMallocDefault), "job table");
+             if (const char* env = getenv("PT_GRAPHS"))
+                 c.use_graphs = atoi(env) != 0;
              HIP_TRY(&c, hipStreamSynchronize(st), "upload");
              return PT_OK;
          }()) != PT_OK)
@@ -573,6 +691,10 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
     case PT_OPT_COUNT_FETCHES:
         c.count_fetches = value != 0;
         return PT_OK;
+    case PT_OPT_GRAPHS:
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.use_graphs = value != 0;
+        return PT_OK;
     default:
         return fail(&c, PT_EINVAL, "unknown option " + std::to_string(option));
     }
@@ -755,6 +877,251 @@ int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets,
         HIP_TRY(&c, hipMemsetAsync(c.cache.grads, 0, size_t(mlp_param_count()) * 4, c.stream), "cache gradients");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache train");
     return PT_OK;
+}
+
+int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* views, uint32_t view_count,
+                    pt_batch** out)
+{
+    if (!out)
+        return bfail(nullptr, PT_EINVAL, "out is null");
+    *out = nullptr;
+    if (!scenes || scene_count == 0 || !views || view_count == 0)
+        return bfail(nullptr, PT_EINVAL, "a batch needs at least one scene and one view");
+    if (view_count > 65536)
+        return bfail(nullptr, PT_EINVAL, "too many views");
+    for (uint32_t i = 0; i < scene_count; i++)
+        if (!scenes[i])
+            return bfail(nullptr, PT_EINVAL, "scene context is null");
+    std::string why;
+    for (uint32_t i = 0; i < view_count; i++)
+        if (int v = check_view(views[i], scene_count, why))
+            return bfail(nullptr, v, "view " + std::to_string(i) + ": " + why);
+    const int device = scenes[0]->c.device;
+    for (uint32_t i = 1; i < scene_count; i++)
+        if (scenes[i]->c.device != device)
+            return bfail(nullptr, PT_EINVAL, "scenes live on different devices");
+    pt_batch* b = new (std::nothrow) pt_batch;
+    if (!b)
+        return bfail(nullptr, PT_ENOMEM, "out of host memory");
+    int code = [&]() -> int {
+        b->device = device;
+        BHIP_TRY(b, hipSetDevice(device), "hipSetDevice");
+        BHIP_TRY(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        BHIP_TRY(b, hipEventCreate(&b->ev_start), "hipEventCreate");
+        BHIP_TRY(b, hipEventCreate(&b->ev_stop), "hipEventCreate");
+        b->scenes.assign(scenes, scenes + scene_count);
+        b->views.assign(views, views + view_count);
+        b->sample_index.assign(view_count, 0);
+        size_t pix = 0;
+        for (uint32_t i = 0; i < view_count; i++) {
+            b->accum_offset.push_back(pix);
+            pix += size_t(views[i].width) * views[i].height;
+        }
+        BHIP_TRY(b, hipMalloc(&b->accum, pix * 16), "accumulation images");
+        BHIP_TRY(b, hipMemsetAsync(b->accum, 0, pix * 16, b->stream), "accumulation images");
+        if (scene_count > 1) {
+            std::vector<DeviceScene> tab(scene_count);
+            for (uint32_t i = 0; i < scene_count; i++)
+                tab[i] = scenes[i]->c.scene;
+            BHIP_TRY(b, hipMalloc(&b->d_scenes, scene_count * sizeof(DeviceScene)), "scene table");
+            BHIP_TRY(b,
+                     hipMemcpyAsync(b->d_scenes, tab.data(), scene_count * sizeof(DeviceScene), hipMemcpyHostToDevice,
+                                    b->stream),
+                     "scene table");
+        }
+        BHIP_TRY(b, hipMalloc(&b->d_views, view_count * sizeof(ViewDev)), "view table");
+        BHIP_TRY(b, hipHostMalloc((void**)&b->h_views, view_count * sizeof(ViewDev), hipHostMallocDefault),
+                 "view table");
+        BHIP_TRY(b, hipMalloc(&b->d_job, sizeof(JobDev)), "job table");
+        BHIP_TRY(b, hipHostMalloc((void**)&b->h_job, sizeof(JobDev), hipHostMallocDefault), "job table");
+        for (uint32_t i = 0; i < view_count; i++)
+            view_dev(*b, i, b->h_views[i]);
+        BHIP_TRY(b,
+                 hipMemcpyAsync(b->d_views, b->h_views, view_count * sizeof(ViewDev), hipMemcpyHostToDevice,
+                                b->stream),
+                 "view table");
+        hipDeviceProp_t prop;
+        BHIP_TRY(b, hipGetDeviceProperties(&prop, device), "device properties");
+        const uint32_t cus = std::max(1, prop.multiProcessorCount);
+        uint32_t blocks_per_cu = 4;
+        if (const char* env = getenv("PT_BLOCKS_PER_CU"))
+            blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
+        b->grid_blocks = ((cus * blocks_per_cu + 7) / 8) * 8;
+        uint32_t pool = std::min<uint64_t>(8u << 20, std::max<uint64_t>(pix * 4, 65536));
+        if (const char* env = getenv("PT_POOL_SLOTS"))
+            pool = uint32_t(std::max(1024, atoi(env)));
+        BHIP_TRY(b, wavefront_alloc(b->wf, pool, 1u << 27, b->grid_blocks * TRACE_BLOCK), "wavefront buffers");
+        if (const char* env = getenv("PT_GRAPHS"))
+            b->use_graphs = atoi(env) != 0;
+        BHIP_TRY(b, hipStreamSynchronize(b->stream), "upload");
+        return PT_OK;
+    }();
+    if (code != PT_OK) {
+        g_batch_error = b->error;
+        batch_release(*b);
+        delete b;
+        return code;
+    }
+    *out = b;
+    return PT_OK;
+}
+
+int pt_batch_set_view(pt_batch* b, uint32_t view, const pt_view* v)
+{
+    if (!b)
+        return PT_EINVAL;
+    if (!v || view >= b->views.size())
+        return bfail(b, PT_EINVAL, "no such view");
+    const pt_view& old = b->views[view];
+    if (v->width != old.width || v->height != old.height)
+        return bfail(b, PT_EINVAL, "a view keeps its image size");
+    std::string why;
+    if (int c = check_view(*v, uint32_t(b->scenes.size()), why))
+        return bfail(b, c, why);
+    BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+    b->views[view] = *v;
+    b->sample_index[view] = 0;
+    view_dev(*b, view, b->h_views[view]);
+    BHIP_TRY(b,
+             hipMemcpyAsync(b->d_views + view, b->h_views + view, sizeof(ViewDev), hipMemcpyHostToDevice, b->stream),
+             "view table");
+    b->stats_fresh = true;
+    return PT_OK;
+}
+
+int pt_batch_render(pt_batch* b)
+{
+    if (!b)
+        return PT_EINVAL;
+    std::vector<ViewWork> work;
+    unsigned long long paths = 0;
+    for (uint32_t i = 0; i < b->views.size(); i++) {
+        const pt_view& v = b->views[i];
+        if (uint64_t(b->sample_index[i]) + v.spp > 0xFFFFFFFFull)
+            return bfail(b, PT_EINVAL, "sample index overflow");
+        work.push_back(ViewWork{i, v.width * v.height, b->sample_index[i], v.spp});
+        paths += (unsigned long long)v.width * v.height * v.spp;
+    }
+    if (b->stats_fresh) {
+        b->timer.reset();
+        BHIP_TRY(b, hipMemsetAsync(b->wf.totals, 0, 8 * 8, b->stream), "stats");
+        BHIP_TRY(b, hipEventRecord(b->ev_start, b->stream), "event");
+        b->last_paths = 0;
+        b->stats_fresh = false;
+    }
+    RenderIO io{b->stream,   &b->wf,   b->grid_blocks, b->count_fetches,
+                &b->timer,   b->scenes[0]->c.scene,    b->d_scenes,    b->d_views,
+                b->d_job,    b->h_job, b->use_graphs ? &b->graphs : nullptr,
+                nullptr};
+    BHIP_TRY(b, render_views(io, work), "render");
+    BHIP_TRY(b, hipEventRecord(b->ev_stop, b->stream), "event");
+    for (uint32_t i = 0; i < b->views.size(); i++)
+        b->sample_index[i] += b->views[i].spp;
+    b->last_paths += paths;
+    return PT_OK;
+}
+
+int pt_batch_read(pt_batch* b, uint32_t view, float* rgba, size_t row_pitch)
+{
+    if (!b)
+        return PT_EINVAL;
+    if (view >= b->views.size())
+        return bfail(b, PT_EINVAL, "no such view");
+    const pt_view& v = b->views[view];
+    if (!rgba || row_pitch < size_t(v.width) * 16)
+        return bfail(b, PT_EINVAL, "bad output buffer");
+    BHIP_TRY(b,
+             hipMemcpy2DAsync(rgba, row_pitch, b->accum + b->accum_offset[view], size_t(v.width) * 16,
+                              size_t(v.width) * 16, v.height, hipMemcpyDeviceToHost, b->stream),
+             "readback");
+    BHIP_TRY(b, hipStreamSynchronize(b->stream), "readback");
+    return PT_OK;
+}
+
+int pt_batch_get_stats(pt_batch* b, pt_stats* out)
+{
+    if (!b)
+        return PT_EINVAL;
+    if (!out)
+        return bfail(b, PT_EINVAL, "out is null");
+    BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+    unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t counters[4] = {0, 0, 0, 0};
+    BHIP_TRY(b, hipMemcpy(totals, b->wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
+    BHIP_TRY(b, hipMemcpy(counters, b->wf.counters, sizeof(counters), hipMemcpyDeviceToHost), "stats");
+    float ms = 0.0f;
+    if (b->last_paths)
+        BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev_start, b->ev_stop), "event");
+    pt_stats s{};
+    s.extension_rays = totals[0];
+    s.shadow_rays = totals[1];
+    s.paths = b->last_paths;
+    s.render_ms = ms;
+    for (pt_ctx* c : b->scenes) {
+        s.bvh_build_ms += c->c.bvh_build_ms;
+        s.bvh_bytes += uint64_t(c->c.bvh.node_count) * sizeof(Node4) + uint64_t(c->c.bvh.tri_count) * sizeof(TriRec);
+        s.node_count += c->c.bvh.node_count;
+        s.tri_record_count += c->c.bvh.tri_count;
+    }
+    s.ext_node_fetches = totals[2];
+    s.ext_tri_fetches = totals[3];
+    s.shadow_node_fetches = totals[4];
+    s.shadow_tri_fetches = totals[5];
+    if (b->timer.enabled) {
+        double kms[KERNEL_COUNT];
+        uint32_t kl[KERNEL_COUNT];
+        BHIP_TRY(b, b->timer.collect(kms, kl), "kernel timing");
+        s.extend_ms = kms[KERNEL_EXTEND];
+        s.shade_ms = kms[KERNEL_SHADE];
+        s.shadow_ms = kms[KERNEL_SHADOW];
+        s.extend_launches = kl[KERNEL_EXTEND];
+        s.shade_launches = kl[KERNEL_SHADE];
+        s.shadow_launches = kl[KERNEL_SHADOW];
+    }
+    *out = s;
+    if (counters[3])
+        return bfail(b, PT_EDEVICE, "traversal stack overflow");
+    return PT_OK;
+}
+
+int pt_batch_set_option(pt_batch* b, uint32_t option, int64_t value)
+{
+    if (!b)
+        return PT_EINVAL;
+    switch (option) {
+    case PT_OPT_PROFILE_KERNELS:
+        BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+        b->timer.enabled = value != 0;
+        b->stats_fresh = true;
+        return PT_OK;
+    case PT_OPT_COUNT_FETCHES:
+        b->count_fetches = value != 0;
+        return PT_OK;
+    case PT_OPT_GRAPHS:
+        BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+        b->use_graphs = value != 0;
+        return PT_OK;
+    default:
+        return bfail(b, PT_EINVAL, "unknown option " + std::to_string(option));
+    }
+}
+
+int pt_batch_reset_stats(pt_batch* b)
+{
+    if (!b)
+        return PT_EINVAL;
+    b->stats_fresh = true;
+    return PT_OK;
+}
+
+const char* pt_batch_last_error(const pt_batch* b) { return b ? b->error.c_str() : g_batch_error.c_str(); }
+
+void pt_batch_destroy(pt_batch* b)
+{
+    if (!b)
+        return;
+    batch_release(*b);
+    delete b;
 }
 
 int pt_synchronize(pt_ctx* h)

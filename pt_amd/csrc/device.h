@@ -23,19 +23,24 @@ constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to
 // until the job's last paths drain.
 struct Wavefront {
     uint32_t capacity = 0;         // slots
-    uint32_t* path_id = nullptr;   // path of each slot: sample * npix + pixel (job-relative)
+    uint32_t* path_id = nullptr;   // path of each slot: its index within the job
+    uint32_t* view_of = nullptr;   // view of each slot's path
+    uint32_t* pix_of = nullptr;    // pixel of each slot's path within its view
+    uint32_t* sample_of = nullptr; // sample index of each slot's path
     uint32_t* seg = nullptr;       // current segment of each slot
     float4* thr_pdf = nullptr;     // throughput rgb, brdf pdf
     float4* rad = nullptr;         // radiance rgb
     uint32_t* q_slot[2] = {nullptr, nullptr};  // extension queues (ping-pong)
     float4* q_o[2] = {nullptr, nullptr};       // origin, tmin
     float4* q_d[2] = {nullptr, nullptr};       // direction, tmax
+    uint32_t* q_scene[2] = {nullptr, nullptr}; // scene of each queued ray (scene batches)
     float4* hit = nullptr;         // t, u, v, instance bits (by queue position)
     uint32_t* hit_prim = nullptr;
     uint32_t* s_slot = nullptr;    // shadow rays
     float4* s_o = nullptr;         // origin, tmax
     float4* s_d = nullptr;         // direction
     float4* s_c = nullptr;         // unoccluded contribution
+    uint32_t* s_scene = nullptr;   // scene of each shadow ray (scene batches)
     uint32_t* retire = nullptr;    // slots whose path ended this iteration
     float4* rad_out = nullptr;     // radiance of every path of the job, by path id
     uint32_t rad_out_capacity = 0; // paths
@@ -63,6 +68,51 @@ struct CacheDev {
     uint32_t collect;         // record training paths at the 2nd bounce
     uint32_t query;           // end other paths there with the cache's prediction
     float cont_prob;          // > 0: queried paths go on with this probability (control variate)
+};
+
+// One view of a render: its camera, image and accumulation buffer (device table).
+struct ViewDev {
+    CameraBlock cam;
+    float4* accum;
+    uint32_t width, height, npix;
+    uint32_t scene;        // index into the scene table
+    uint32_t max_bounces;
+    uint32_t pad;
+};
+
+// A job is a run of the path pool over consecutive segments; a segment is a range of
+// samples of one view, its paths numbered sample-major from path_offset.
+constexpr uint32_t MAX_SEGS = 1024;
+struct SegDev {
+    uint32_t view;
+    uint32_t first_sample;  // sample index of the segment's first sample
+    uint32_t samples;
+    uint32_t path_offset;   // job path index of (first_sample, pixel 0)
+    uint32_t pix_offset;    // sum of the pixel counts of the earlier segments
+    uint32_t pad[3];
+};
+struct JobDev {
+    uint32_t paths;            // paths of the job
+    uint32_t nseg;
+    uint32_t signed_radiance;  // keep negative per-sample sums (control-variate estimates)
+    uint32_t pix_total;        // sum of the segments' pixel counts
+    uint32_t pad[4];
+    SegDev seg[MAX_SEGS];
+};
+
+// Executable graphs of one wavefront iteration (queue parity 0 and 1), captured on first use
+// and kept while the launch configuration they were captured with stays the same.
+struct IterGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    uint64_t sig[2] = {0, 0};
+    void destroy()
+    {
+        for (int k = 0; k < 2; k++)
+            if (exec[k])
+                (void)hipGraphExecDestroy(exec[k]);
+        exec[0] = exec[1] = nullptr;
+        sig[0] = sig[1] = 0;
+    }
 };
 
 enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_CACHE = 3, KERNEL_COUNT = 4 };
@@ -154,6 +204,12 @@ struct Ctx {
     KernelTimer timer;
     bool stats_fresh = true;  // counters restart at the next pt_render
     CacheState cache;
+    ViewDev* d_views = nullptr;  // the context's one view
+    ViewDev* h_view = nullptr;   // pinned staging of it
+    JobDev* d_job = nullptr;
+    JobDev* h_job = nullptr;     // pinned staging of the job table
+    IterGraphs graphs;
+    bool use_graphs = true;
 
     // stats of the last render
     unsigned long long last_ext = 0, last_shadow = 0, last_paths = 0;
@@ -162,6 +218,30 @@ struct Ctx {
 
 hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_paths, uint32_t spill_threads);
 void wavefront_free(Wavefront& wf);
+
+// What a render needs of its owner (a context or a batch).
+struct RenderIO {
+    hipStream_t st;
+    Wavefront* wf;
+    uint32_t grid_blocks;
+    bool count_fetches;
+    KernelTimer* timer;
+    DeviceScene scene;              // the scene when there is one
+    const DeviceScene* d_scenes;    // scene table when there are several, else nullptr
+    const ViewDev* d_views;
+    JobDev* d_job;
+    JobDev* h_job;
+    IterGraphs* graphs;             // nullptr: launch every kernel directly
+    CacheState* cache;              // nullptr: no radiance cache
+};
+
+// Samples [sample_index, sample_index + spp) of one view.
+struct ViewWork {
+    uint32_t view, npix, sample_index, spp;
+};
+
+// Renders the given samples of every view, all views' paths streaming through one pool.
+hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work);
 
 // Renders samples [ctx.sample_index, ctx.sample_index + spp) into ctx.accum.
 hipError_t render_samples(Ctx& ctx, uint32_t spp);

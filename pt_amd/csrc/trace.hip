@@ -331,47 +331,67 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
 
 // --- kernels -------------------------------------------------------------------------------
 
-struct JobParams {
-    uint32_t width, height, npix;
-    uint32_t sample_base;   // sample index of job-local sample 0
-    uint32_t paths;         // npix * samples of the job
-    uint32_t max_bounces;
-    uint32_t signed_radiance;  // keep negative per-sample sums (control-variate estimates)
-};
+// Segment of a job path: the last segment whose first path is <= p.
+__device__ __forceinline__ uint32_t find_segment(const JobDev* job, uint32_t p)
+{
+    uint32_t lo = 0, hi = job->nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (job->seg[mid].path_offset <= p)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
-// Starts path p in slot s: sampler, camera ray (pathtrace.rg.hlsl:18-33), fresh payload.
-__device__ __forceinline__ void start_path(const CameraBlock& cam, const JobParams& jp, const Wavefront& wf,
-                                           const CacheDev& cd, uint32_t s, uint32_t p, float3& o, float3& d)
+// Starts job path p in slot s: sampler, camera ray of its view (pathtrace.rg.hlsl:18-33),
+// fresh payload. Returns the ray's [tmin, tmax] and scene.
+__device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, const Wavefront& wf,
+                                           const CacheDev& cd, uint32_t s, uint32_t p, float3& o, float3& d,
+                                           float& tmin, float& tmax, uint32_t& scene)
 {
     if (cd.collect)
         cd.train_rec[s] = NONE;
-    const uint32_t pix = p % jp.npix, sl = p / jp.npix;
-    const uint32_t px = pix % jp.width, py = pix / jp.width;
+    const SegDev sg = job->seg[find_segment(job, p)];
+    const ViewDev& vw = views[sg.view];
+    const uint32_t local = p - sg.path_offset;
+    const uint32_t pix = local % vw.npix, sample = sg.first_sample + local / vw.npix;
+    const uint32_t px = pix % vw.width, py = pix / vw.width;
     Sampler smp;
-    smp.start(px, py, jp.sample_base + sl, 0);
+    smp.start(px, py, sample, 0);
     const float2 up = smp.s2();
     const float2 ul = smp.s2();
-    camera_ray(cam, px, py, jp.width, jp.height, up, ul, o, d);
+    camera_ray(vw.cam, px, py, vw.width, vw.height, up, ul, o, d);
+    tmin = vw.cam.near_clip;
+    tmax = vw.cam.far_clip;
+    scene = vw.scene;
     wf.path_id[s] = p;
+    wf.view_of[s] = sg.view;
+    wf.pix_of[s] = pix;
+    wf.sample_of[s] = sample;
     wf.seg[s] = 0;
     wf.thr_pdf[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     wf.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Fills the pool with the job's first paths; their camera rays form queue 0.
-__global__ __launch_bounds__(256) void k_start_job(CameraBlock cam, JobParams jp, Wavefront wf, CacheDev cd)
+__global__ __launch_bounds__(256) void k_start_job(const ViewDev* views, const JobDev* job, Wavefront wf, CacheDev cd)
 {
-    const uint32_t n = min(jp.paths, wf.capacity);
+    const uint32_t n = min(job->paths, wf.capacity);
     if (blockIdx.x == 0 && threadIdx.x == 0 && cd.collect) {
         cd.counters[0] = 0;
         cd.counters[1] = 0;
     }
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         float3 o, d;
-        start_path(cam, jp, wf, cd, s, s, o, d);
+        float tmin, tmax;
+        uint32_t scene;
+        start_path(views, job, wf, cd, s, s, o, d, tmin, tmax, scene);
         wf.q_slot[0][s] = s;
-        wf.q_o[0][s] = make_float4(o.x, o.y, o.z, cam.near_clip);
-        wf.q_d[0][s] = make_float4(d.x, d.y, d.z, cam.far_clip);
+        wf.q_o[0][s] = make_float4(o.x, o.y, o.z, tmin);
+        wf.q_d[0][s] = make_float4(d.x, d.y, d.z, tmax);
+        wf.q_scene[0][s] = scene;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         wf.counters[0] = n;
@@ -382,8 +402,8 @@ __global__ __launch_bounds__(256) void k_start_job(CameraBlock cam, JobParams jp
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront wf, int q)
+template <bool COUNT, bool MULTI>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
     const uint32_t count = wf.counters[q];
@@ -393,6 +413,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
+        const DeviceScene& s = MULTI ? scenes[wf.q_scene[q][i]] : s0;
         HitOut h;
         float trans;
         traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &n_nodes, &n_tris);
@@ -409,8 +430,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s, Wavefront
         atomicAdd(&wf.totals[0], (unsigned long long)count);
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront wf)
+template <bool COUNT, bool MULTI>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const DeviceScene* scenes, Wavefront wf)
 {
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
     const uint32_t count = wf.counters[2];
@@ -420,6 +441,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
+        const DeviceScene& s = MULTI ? scenes[wf.s_scene[i]] : s0;
         HitOut h;
         float trans;
         const bool occluded = traverse<true, COUNT>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
@@ -445,7 +467,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s, Wavefront
 
 // Miss / closest-hit / Russian roulette of one segment for every ray of queue q. Continuing
 // paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
-__global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wavefront wf, int q, CacheDev cd)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
+                                               const JobDev* job, Wavefront wf, int q, CacheDev cd)
 {
     const uint32_t count = wf.counters[q];
     const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
@@ -461,9 +485,13 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
         float3 sh_o = f3(0, 0, 0), sh_d = f3(0, 0, 0), sh_c = f3(0, 0, 0);
         float sh_tmax = 0.0f;
         uint32_t segment = 0;
+        uint32_t sid = 0;
         if (valid) {
             slot = wf.q_slot[q][i];
-            const uint32_t p = wf.path_id[slot];
+            const ViewDev& vw = views[wf.view_of[slot]];
+            sid = vw.scene;
+            const DeviceScene& s = MULTI ? scenes[sid] : s0;
+            const uint32_t pix = wf.pix_of[slot], sample = wf.sample_of[slot];
             segment = wf.seg[slot];
             const float4 ro4 = wf.q_o[q][i], rd4 = wf.q_d[q][i];
             const float3 ro = xyz(ro4), rd = xyz(rd4);
@@ -474,11 +502,10 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
             float brdf_pdf = tp.w;
             float4 rr = wf.rad[slot];
             float3 L = xyz(rr);
-            const bool final_seg = segment >= jp.max_bounces;
-            const uint32_t pix = p % jp.npix, sl = p / jp.npix;
-            const uint32_t px = pix % jp.width, py = pix / jp.width;
+            const bool final_seg = segment >= vw.max_bounces;
+            const uint32_t px = pix % vw.width, py = pix / vw.width;
             Sampler smp;
-            smp.start(px, py, jp.sample_base + sl, segment + 1);
+            smp.start(px, py, sample, segment + 1);
             bool terminated = false;
 
             if (hinst == NONE) {
@@ -582,11 +609,11 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
                         cr[10] = mat.base_colour.x, cr[11] = mat.base_colour.y, cr[12] = mat.base_colour.z;
                         cT = T;
                         cL = L;
-                        if (hash_u32(p ^ (jp.sample_base * 0x9E3779B9u)) % cd.train_stride == 0) {
+                        if (hash_u32(pix ^ (sample * 0x9E3779B9u)) % cd.train_stride == 0) {
                             want_train = true;
                         } else if (cd.query) {
                             want_query = true;
-                            const float uc = float(hash_u32(hash_u32(p ^ 0xA511E9B3u) + jp.sample_base) >> 8) * 0x1p-24f;
+                            const float uc = float(hash_u32(hash_u32(pix ^ 0xA511E9B3u) + sample) >> 8) * 0x1p-24f;
                             if (uc < cd.cont_prob) {
                                 // control variate: the path goes on with weight 1/p and the
                                 // prediction is added with weight 1 - 1/p (unbiased)
@@ -672,6 +699,8 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
             wf.q_slot[1 - q][pos] = slot;
             wf.q_o[1 - q][pos] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
             wf.q_d[1 - q][pos] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
+            if (MULTI)
+                wf.q_scene[1 - q][pos] = sid;
         }
         const uint32_t rpos = wave_append(valid && !cont, &wf.counters[4]);
         if (valid && !cont)
@@ -682,6 +711,8 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
             wf.s_o[spos] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
             wf.s_d[spos] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
             wf.s_c[spos] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
+            if (MULTI)
+                wf.s_scene[spos] = sid;
         }
         if (cd.collect) {
             // training record: inputs, throughput and radiance so far (target filled at the end)
@@ -710,8 +741,10 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s, JobParams jp, Wave
 
 // Stores the radiance of every ended path and refills its slot with the job's next path,
 // whose camera ray joins queue 1-q.
-__global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams jp, Wavefront wf, int q, CacheDev cd)
+__global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const JobDev* job, Wavefront wf, int q,
+                                                    CacheDev cd)
 {
+    const uint32_t paths = job->paths;
     const uint32_t count = wf.counters[4];
     const uint32_t rounded = (count + 63u) & ~63u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
@@ -738,15 +771,18 @@ __global__ __launch_bounds__(256) void k_regenerate(CameraBlock cam, JobParams j
         }
         // one atomic per wave hands out the next path ids
         const uint32_t first = wave_append(valid, &wf.counters[5]);
-        const bool start = valid && first < jp.paths;
+        const bool start = valid && first < paths;
         float3 o = f3(0, 0, 0), d = f3(0, 0, 0);
+        float tmin = 0.0f, tmax = 0.0f;
+        uint32_t scene = 0;
         if (start)
-            start_path(cam, jp, wf, cd, slot, first, o, d);
+            start_path(views, job, wf, cd, slot, first, o, d, tmin, tmax, scene);
         const uint32_t pos = wave_append(start, &wf.counters[1 - q]);
         if (start) {
             wf.q_slot[1 - q][pos] = slot;
-            wf.q_o[1 - q][pos] = make_float4(o.x, o.y, o.z, cam.near_clip);
-            wf.q_d[1 - q][pos] = make_float4(d.x, d.y, d.z, cam.far_clip);
+            wf.q_o[1 - q][pos] = make_float4(o.x, o.y, o.z, tmin);
+            wf.q_d[1 - q][pos] = make_float4(d.x, d.y, d.z, tmax);
+            wf.q_scene[1 - q][pos] = scene;
         }
     }
 }
@@ -763,16 +799,30 @@ __global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
     }
 }
 
-// Running mean per pixel in sample order (pathtrace.rg.hlsl:61-67, one sample per dispatch).
-__global__ __launch_bounds__(256) void k_accumulate(JobParams jp, const float4* rad, float4* accum, float exposure)
+// Running mean per pixel in sample order (pathtrace.rg.hlsl:61-67, one sample per dispatch),
+// for every segment of the job.
+__global__ __launch_bounds__(256) void k_accumulate(const ViewDev* views, const JobDev* job, const float4* rad)
 {
-    const uint32_t samples = jp.paths / jp.npix;
-    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < jp.npix; pix += gridDim.x * blockDim.x) {
-        float4 acc = accum[pix];
-        for (uint32_t sl = 0; sl < samples; sl++) {
-            const float4 r4 = rad[size_t(sl) * jp.npix + pix];
+    const uint32_t total = job->pix_total, nseg = job->nseg;
+    const bool signed_radiance = job->signed_radiance != 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (job->seg[mid].pix_offset <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const SegDev sg = job->seg[lo];
+        const ViewDev& vw = views[sg.view];
+        const uint32_t pix = t - sg.pix_offset;
+        const float exposure = vw.cam.exposure;
+        float4 acc = vw.accum[pix];
+        for (uint32_t sl = 0; sl < sg.samples; sl++) {
+            const float4 r4 = rad[sg.path_offset + size_t(sl) * vw.npix + pix];
             float rx, ry, rz;
-            if (jp.signed_radiance) {
+            if (signed_radiance) {
                 rx = r4.x * exposure, ry = r4.y * exposure, rz = r4.z * exposure;
                 rx = isfinite(rx) ? rx : 0.0f, ry = isfinite(ry) ? ry : 0.0f, rz = isfinite(rz) ? rz : 0.0f;
             } else {
@@ -780,7 +830,7 @@ __global__ __launch_bounds__(256) void k_accumulate(JobParams jp, const float4* 
                 ry = correct_component(r4.y * exposure);
                 rz = correct_component(r4.z * exposure);
             }
-            const uint32_t si = jp.sample_base + sl;
+            const uint32_t si = sg.first_sample + sl;
             if (si == 0) {
                 acc = make_float4(rx, ry, rz, 1.0f);
             } else {
@@ -790,7 +840,7 @@ __global__ __launch_bounds__(256) void k_accumulate(JobParams jp, const float4* 
                 acc.z = acc.z + (rz - 1.0f * acc.z) / n;
             }
         }
-        accum[pix] = acc;
+        vw.accum[pix] = acc;
     }
 }
 
@@ -862,6 +912,10 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
             e = hipMalloc((void**)&p, bytes);
     };
     A(wf.path_id, c * 4);
+    A(wf.view_of, c * 4);
+    A(wf.pix_of, c * 4);
+    A(wf.sample_of, c * 4);
+    A(wf.s_scene, c * 4);
     A(wf.seg, c * 4);
     A(wf.thr_pdf, c * 16);
     A(wf.rad, c * 16);
@@ -869,6 +923,7 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
         A(wf.q_slot[k], c * 4);
         A(wf.q_o[k], c * 16);
         A(wf.q_d[k], c * 16);
+        A(wf.q_scene[k], c * 4);
     }
     A(wf.hit, c * 16);
     A(wf.hit_prim, c * 4);
@@ -896,7 +951,8 @@ void wavefront_free(Wavefront& wf)
 {
     void* ptrs[] = {wf.path_id, wf.seg,   wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
                     wf.q_o[1],  wf.q_d[0], wf.q_d[1], wf.hit,      wf.hit_prim,  wf.s_slot,    wf.s_o,
-                    wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill};
+                    wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill,
+                    wf.view_of, wf.pix_of, wf.sample_of, wf.s_scene, wf.q_scene[0], wf.q_scene[1]};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -908,124 +964,273 @@ void wavefront_free(Wavefront& wf)
     wf = Wavefront{};
 }
 
-hipError_t render_samples(Ctx& ctx, uint32_t spp)
+namespace {
+
+// One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
+// cache queries, retire and refill, and copy the count of the next queue to pinned memory.
+hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
 {
-    Wavefront& wf = ctx.wf;
-    const uint32_t npix = ctx.width * ctx.height;
-    uint32_t per_job = std::max<uint32_t>(1, wf.rad_out_capacity / npix);
-    const bool train_between = ctx.cache.configured && ctx.cache.train_steps > 0 && ctx.cache.frame_spp > 0;
-    const bool train_inflight = ctx.cache.configured && ctx.cache.train_steps > 0 && ctx.cache.frame_spp == 0;
-    if (train_between)
-        per_job = std::min(per_job, ctx.cache.frame_spp);  // train between frames
-    hipStream_t st = ctx.stream;
+    Wavefront& wf = *io.wf;
+    hipStream_t st = io.st;
+    KernelTimer& timer = *io.timer;
+    const bool multi = io.d_scenes != nullptr;
+    const DeviceScene* tab = io.d_scenes;
     hipError_t e;
-    uint32_t done = 0;
-    while (done < spp) {
-        JobParams jp{};
-        jp.width = ctx.width;
-        jp.height = ctx.height;
-        jp.npix = npix;
-        jp.sample_base = ctx.sample_index + done;
-        const uint32_t samples = std::min(per_job, spp - done);
-        jp.paths = npix * samples;
-        jp.max_bounces = ctx.max_bounces;
-        CacheState& cs = ctx.cache;
-        CacheDev cd{};
-        if (cs.configured) {
-            cd.records = cs.records;
-            cd.counters = cs.counters;
-            cd.record_capacity = cs.record_capacity;
-            cd.queries = cs.queries;
-            cd.query_slot = cs.query_slot;
-            cd.train_rec = cs.train_rec;
-            cd.train_stride = std::max<uint32_t>(1, cs.train_stride);
-            cd.collect = 1;
-            cd.query = (cs.query && cs.steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
-            cd.cont_prob = cs.cont_prob;
-            jp.signed_radiance = (cd.query && cs.cont_prob > 0.0f) ? 1u : 0u;
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd);
+    if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
+        return e;
+    const dim3 tg(io.grid_blocks), tb(TRACE_BLOCK);
+    void (*extend)(DeviceScene, const DeviceScene*, Wavefront, int) =
+        io.count_fetches ? (multi ? k_extend<true, true> : k_extend<true, false>)
+                         : (multi ? k_extend<false, true> : k_extend<false, false>);
+    hipLaunchKernelGGL(extend, tg, tb, 0, st, io.scene, tab, wf, q);
+    if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
+        return e;
+    if ((e = timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
+        return e;
+    if (multi)
+        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
+                           wf, q, cd);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
+                           wf, q, cd);
+    if ((e = timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
+        return e;
+    if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
+        return e;
+    void (*shadow)(DeviceScene, const DeviceScene*, Wavefront) =
+        io.count_fetches ? (multi ? k_shadow<true, true> : k_shadow<true, false>)
+                         : (multi ? k_shadow<false, true> : k_shadow<false, false>);
+    hipLaunchKernelGGL(shadow, tg, tb, 0, st, io.scene, tab, wf);
+    if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
+        return e;
+    if (cd.query) {
+        if ((e = timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
+            return e;
+        if ((e = cache_query(*io.cache, &io.cache->counters[1], wf.rad, io.grid_blocks, st)) != hipSuccess)
+            return e;
+        if ((e = timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
+            return e;
+    }
+    hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, wf, q, cd);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    return hipMemcpyAsync(&wf.host_counts[q], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st);
+}
+
+// Launch configuration an iteration graph depends on.
+uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    mix(io.count_fetches);
+    mix(reinterpret_cast<uintptr_t>(io.d_scenes));
+    mix(reinterpret_cast<uintptr_t>(io.d_views));
+    mix(reinterpret_cast<uintptr_t>(io.d_job));
+    mix(reinterpret_cast<uintptr_t>(io.wf->path_id));
+    mix(io.wf->capacity);
+    mix(io.grid_blocks);
+    mix(io.cache ? reinterpret_cast<uintptr_t>(io.cache->params) : 0);
+    mix(cd.collect);
+    mix(cd.query);
+    uint32_t cp;
+    memcpy(&cp, &cd.cont_prob, 4);
+    mix(cp);
+    mix(cd.train_stride);
+    mix(reinterpret_cast<uintptr_t>(cd.records));
+    mix(reinterpret_cast<uintptr_t>(cd.queries));
+    return h | 1ull;
+}
+
+// Runs one iteration through its captured graph (captured on first use).
+hipError_t graph_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
+{
+    IterGraphs& g = *io.graphs;
+    const uint64_t sig = graph_signature(io, cd);
+    hipError_t e;
+    if (!g.exec[q] || g.sig[q] != sig) {
+        if (g.exec[q]) {
+            (void)hipGraphExecDestroy(g.exec[q]);
+            g.exec[q] = nullptr;
         }
-        hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min(jp.paths, wf.capacity), 256, 65535u * 4)), dim3(256),
-                           0, st, ctx.cam, jp, wf, cd);
-        // grid-stride stages: enough blocks to fill the chip, not one per pool slot
-        const unsigned shade_blocks = blocks_for(wf.capacity, 256, ctx.grid_blocks * 2);
+        if ((e = hipStreamBeginCapture(io.st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return e;
+        hipError_t le = launch_iteration(io, cd, q, shade_blocks);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(io.st, &graph);
+        if (le != hipSuccess || e != hipSuccess) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return le != hipSuccess ? le : e;
+        }
+        e = hipGraphInstantiate(&g.exec[q], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.exec[q] = nullptr;
+            return e;
+        }
+        g.sig[q] = sig;
+    }
+    return hipGraphLaunch(g.exec[q], io.st);
+}
+
+// Cuts the work into jobs of at most wf.rad_out_capacity paths (and, when the cache trains
+// between frames, at most max_samples samples of one view per job).
+struct JobPlan {
+    std::vector<SegDev> segs;
+    uint64_t paths = 0, pix = 0;
+};
+
+std::vector<JobPlan> plan_jobs(const std::vector<ViewWork>& work, uint32_t cap, uint32_t max_samples)
+{
+    std::vector<JobPlan> jobs;
+    JobPlan cur;
+    auto close = [&]() {
+        if (!cur.segs.empty())
+            jobs.push_back(cur);
+        cur = JobPlan{};
+    };
+    for (const ViewWork& w : work) {
+        uint32_t left = w.spp, sample = w.sample_index;
+        while (left > 0) {
+            uint64_t fit = (cap - cur.paths) / w.npix;
+            fit = std::min<uint64_t>(fit, left);
+            if (max_samples)
+                fit = std::min<uint64_t>(fit, max_samples);
+            if (fit == 0 || cur.segs.size() == MAX_SEGS) {
+                close();
+                continue;
+            }
+            SegDev sg{};
+            sg.view = w.view;
+            sg.first_sample = sample;
+            sg.samples = uint32_t(fit);
+            sg.path_offset = uint32_t(cur.paths);
+            sg.pix_offset = uint32_t(cur.pix);
+            cur.segs.push_back(sg);
+            cur.paths += fit * w.npix;
+            cur.pix += w.npix;
+            left -= uint32_t(fit);
+            sample += uint32_t(fit);
+            if (max_samples)
+                close();  // one frame per job: the cache trains in between
+        }
+    }
+    close();
+    return jobs;
+}
+
+}  // namespace
+
+hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
+{
+    Wavefront& wf = *io.wf;
+    hipStream_t st = io.st;
+    hipError_t e;
+    CacheState* cs = io.cache && io.cache->configured ? io.cache : nullptr;
+    const bool train_between = cs && cs->train_steps > 0 && cs->frame_spp > 0;
+    const bool train_inflight = cs && cs->train_steps > 0 && cs->frame_spp == 0;
+    for (const ViewWork& w : work)
+        if (w.npix == 0 || w.npix > wf.rad_out_capacity)
+            return hipErrorInvalidValue;
+    const auto jobs = plan_jobs(work, wf.rad_out_capacity, train_between ? cs->frame_spp : 0);
+    // grid-stride stages: enough blocks to fill the chip, not one per pool slot
+    const unsigned shade_blocks = blocks_for(wf.capacity, 256, io.grid_blocks * 2);
+    for (const JobPlan& plan : jobs) {
+        const auto& segs = plan.segs;
+        JobDev& jh = *io.h_job;
+        jh.nseg = uint32_t(segs.size());
+        for (size_t k = 0; k < segs.size(); k++)
+            jh.seg[k] = segs[k];
+        jh.paths = uint32_t(plan.paths);
+        jh.pix_total = uint32_t(plan.pix);
+        CacheDev cd{};
+        if (cs) {
+            cd.records = cs->records;
+            cd.counters = cs->counters;
+            cd.record_capacity = cs->record_capacity;
+            cd.queries = cs->queries;
+            cd.query_slot = cs->query_slot;
+            cd.train_rec = cs->train_rec;
+            cd.train_stride = std::max<uint32_t>(1, cs->train_stride);
+            cd.collect = 1;
+            cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
+            cd.cont_prob = cs->cont_prob;
+        }
+        jh.signed_radiance = (cd.query && cd.cont_prob > 0.0f) ? 1u : 0u;
+        const size_t job_bytes = offsetof(JobDev, seg) + segs.size() * sizeof(SegDev);
+        if ((e = hipMemcpyAsync(io.d_job, io.h_job, job_bytes, hipMemcpyHostToDevice, st)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min<uint32_t>(jh.paths, wf.capacity), 256, 65535u * 4)),
+                           dim3(256), 0, st, io.d_views, io.d_job, wf, cd);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        const bool graphs = io.graphs && !io.timer->enabled && !train_inflight;
         int q = 0;
         // Iterate until the queue an iteration leaves behind is empty. The count of iteration
         // i is read back (pinned memory, event) while iteration i+1 is already queued.
         for (uint32_t it = 0; it < 1000000; it++) {
-            hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd);
-            if ((e = ctx.timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
-                return e;
-            if (ctx.count_fetches)
-                hipLaunchKernelGGL(k_extend<true>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
-            else
-                hipLaunchKernelGGL(k_extend<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf, q);
-            if ((e = ctx.timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
-                return e;
-            if ((e = ctx.timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
-                return e;
-            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, st, ctx.scene, jp, wf, q, cd);
-            if ((e = ctx.timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
-                return e;
-            if ((e = ctx.timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
-                return e;
-            if (ctx.count_fetches)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
-            else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(ctx.grid_blocks), dim3(TRACE_BLOCK), 0, st, ctx.scene, wf);
-            if ((e = ctx.timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
-                return e;
-            if (cd.query) {
-                if ((e = ctx.timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
-                    return e;
-                if ((e = cache_query(cs, &cs.counters[1], wf.rad, ctx.grid_blocks, st)) != hipSuccess)
-                    return e;
-                if ((e = ctx.timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
-                    return e;
-            }
-            hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, ctx.cam, jp, wf, q, cd);
-            if ((e = hipGetLastError()) != hipSuccess)
+            e = graphs ? graph_iteration(io, cd, q, shade_blocks) : launch_iteration(io, cd, q, shade_blocks);
+            if (e != hipSuccess)
                 return e;
             if (train_inflight) {
-                if ((e = ctx.timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
+                if ((e = io.timer->mark(st, KERNEL_CACHE, true)) != hipSuccess)
                     return e;
-                if ((e = cache_train_inflight(cs, st)) != hipSuccess)
+                if ((e = cache_train_inflight(*cs, st)) != hipSuccess)
                     return e;
-                if ((e = ctx.timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
+                if ((e = io.timer->mark(st, KERNEL_CACHE, false)) != hipSuccess)
                     return e;
             }
-            if ((e = hipMemcpyAsync(&wf.host_counts[it & 1], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st)) !=
-                hipSuccess)
-                return e;
-            if ((e = hipEventRecord(wf.count_ready[it & 1], st)) != hipSuccess)
+            if ((e = hipEventRecord(wf.count_ready[q], st)) != hipSuccess)
                 return e;
             q = 1 - q;
             if (it >= 1) {
-                if ((e = hipEventSynchronize(wf.count_ready[(it - 1) & 1])) != hipSuccess)
+                // q now names the parity of iteration it - 1
+                if ((e = hipEventSynchronize(wf.count_ready[q])) != hipSuccess)
                     return e;
-                if (wf.host_counts[(it - 1) & 1] == 0)
+                if (wf.host_counts[q] == 0)
                     break;  // iteration it-1 left nothing to trace; iteration it was empty
             }
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256, 65535u * 4)), dim3(256), 0, st, jp, wf.rad_out,
-                           ctx.accum, ctx.cam.exposure);
+        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(jh.pix_total, 256, 65535u * 4)), dim3(256), 0, st, io.d_views,
+                           io.d_job, wf.rad_out);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        if (cd.collect && train_between) {
+        if (train_between) {
             // online training on the path suffixes this job recorded
             uint32_t n_rec = 0;
-            if ((e = hipMemcpyAsync(&n_rec, &cs.counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            if ((e = hipMemcpyAsync(&n_rec, &cs->counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
                 return e;
             if ((e = hipStreamSynchronize(st)) != hipSuccess)
                 return e;
-            n_rec = std::min(n_rec, cs.record_capacity);
-            cs.records_total += n_rec;
-            cs.last_records = n_rec;
-            if ((e = cache_train_records(cs, n_rec, st)) != hipSuccess)
+            n_rec = std::min(n_rec, cs->record_capacity);
+            cs->records_total += n_rec;
+            cs->last_records = n_rec;
+            if ((e = cache_train_records(*cs, n_rec, st)) != hipSuccess)
                 return e;
         }
-        done += samples;
     }
     return hipSuccess;
+}
+
+hipError_t render_samples(Ctx& ctx, uint32_t spp)
+{
+    hipError_t e;
+    ViewDev& v = *ctx.h_view;
+    v.cam = ctx.cam;
+    v.accum = ctx.accum;
+    v.width = ctx.width;
+    v.height = ctx.height;
+    v.npix = ctx.width * ctx.height;
+    v.scene = 0;
+    v.max_bounces = ctx.max_bounces;
+    if ((e = hipMemcpyAsync(ctx.d_views, ctx.h_view, sizeof(v), hipMemcpyHostToDevice, ctx.stream)) != hipSuccess)
+        return e;
+    RenderIO io{ctx.stream,       &ctx.wf,     ctx.grid_blocks, ctx.count_fetches,
+                &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
+                ctx.d_job,        ctx.h_job,   ctx.use_graphs ? &ctx.graphs : nullptr,
+                &ctx.cache};
+    return render_views(io, {ViewWork{0, v.npix, ctx.sample_index, spp}});
 }
 
 hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits)
