@@ -11,7 +11,11 @@ src/pathtrace/ptdispatch.cpp:101); the default 8 steps are the 64-spp frame. Inp
 resident in HBM before the timed region (scene, BVH, camera); the timed region is K
 back-to-back pt_render(8) calls bracketed by stream synchronisation.
 
-The JSON line also carries:
+Other workloads (--workload views | scenes): 64 views of the box scene, or 16 instanced
+scenes of 10k-20M triangles with 4 views each, traced in one persistent wavefront run
+(pt_batch_*); a step is one render of every view.
+
+The JSON line of the default workload also carries:
   roofline      the dominant stage's algorithmic bytes per launch / its average HIP-event
                 duration, against the 8 TB/s HBM peak. Bytes per ray: 32 (ray) + 20 (hit)
                 + 128 x node fetches + 48 x triangle-record fetches (extension) and
@@ -143,22 +147,7 @@ def measured_traffic(kernel):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--spp-per-step", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-convergence", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print("bench.py: this build measures one GPU; every box here has one, so --gpus %d is not run"
-              % a.gpus, file=sys.stderr)
-        return 2
-
+def run_instanced(a):
     t0 = time.time()
     scn = generate.cached_scene("instanced", root=a.scene_dir)
     scene = pt_amd.Scene(scn)
@@ -190,8 +179,7 @@ def main():
     log("fetches/ray: extension %.1f nodes %.1f tris (%.0f B), shadow %.1f nodes %.1f tris (%.0f B)"
         % (n_node_ext, n_tri_ext, bytes_ext, n_node_sh, n_tri_sh, bytes_sh))
 
-    # timed region: exactly K steps
-    tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 1)
+    # timed region: exactly K steps, production settings (iterations replayed from HIP graphs)
     tr.set_camera(cam)
     tr.synchronize()
     t_start = time.perf_counter()
@@ -202,20 +190,29 @@ def main():
     st = tr.stats()
     rays = st["extension_rays"] + st["shadow_rays"]
     value = rays / elapsed / 1e6
-    log("timed: %d steps in %.3f s, %d ext + %d shadow rays; stages: extend %.1f ms, shade %.1f ms, shadow %.1f ms"
-        % (a.steps, elapsed, st["extension_rays"], st["shadow_rays"], st["extend_ms"], st["shade_ms"],
-           st["shadow_ms"]))
+    log("timed: %d steps in %.3f s, %d ext + %d shadow rays" % (a.steps, elapsed, st["extension_rays"],
+                                                               st["shadow_rays"]))
+
+    # the same steps again with every stage launch timed by HIP events (direct launches)
+    tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 1)
+    tr.set_camera(cam)
+    for _ in range(a.steps):
+        tr.render(a.spp_per_step)
+    sp = tr.stats()
+    tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 0)
+    log("profiled pass: stages extend %.1f ms, shade %.1f ms, shadow %.1f ms (render %.1f ms)"
+        % (sp["extend_ms"], sp["shade_ms"], sp["shadow_ms"], sp["render_ms"]))
 
     # the 8-bounce shade stage reads per hit: hit 20 + path state 32 + instance 128 + material
     # 96 + indices 12 + 3 vertices 96 + light bins 64 + emitter vertices 36, writes next ray
     # 36 + path state 32 + shadow record 64.
     stages = {
-        "extend": {"ms": st["extend_ms"], "launches": st["extend_launches"],
-                   "bytes": st["extension_rays"] * bytes_ext},
-        "shadow": {"ms": st["shadow_ms"], "launches": st["shadow_launches"],
-                   "bytes": st["shadow_rays"] * bytes_sh},
-        "shade": {"ms": st["shade_ms"], "launches": st["shade_launches"],
-                  "bytes": st["extension_rays"] * 616},
+        "extend": {"ms": sp["extend_ms"], "launches": sp["extend_launches"],
+                   "bytes": sp["extension_rays"] * bytes_ext},
+        "shadow": {"ms": sp["shadow_ms"], "launches": sp["shadow_launches"],
+                   "bytes": sp["shadow_rays"] * bytes_sh},
+        "shade": {"ms": sp["shade_ms"], "launches": sp["shade_launches"],
+                  "bytes": sp["extension_rays"] * 616},
     }
     dom = max(stages, key=lambda k: stages[k]["ms"])
     d = stages[dom]
@@ -228,14 +225,13 @@ def main():
 
     conv = None
     if not a.no_convergence:
-        tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 0)
         conv = convergence(tr, cam)
 
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, cam, a.cpu_threads)
 
-    out = {
+    return {
         "metric": "Mrays/s", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
@@ -253,6 +249,151 @@ def main():
         "convergence": conv,
         "cpu_baseline": cpu,
     }
+
+
+def _random_dir(rng):
+    z = rng.uniform(-1.0, 1.0)
+    phi = rng.uniform(0.0, 2 * np.pi)
+    r = (1 - z * z) ** 0.5
+    return (r * np.cos(phi), r * np.sin(phi), z)
+
+
+def _timed_batch(batch, steps, warmup):
+    for _ in range(warmup):
+        batch.render()
+    batch.reset_stats()
+    t = time.perf_counter()
+    for _ in range(steps):
+        batch.render()
+    st = batch.stats()  # synchronises
+    el = time.perf_counter() - t
+    return el, st
+
+
+def run_views(a):
+    """64 views of the box scene in one persistent wavefront run: resolution log-uniform in
+    128^2..1024^2 pixels, aspect U(0.5, 2), spp log-uniform in 4..256, eye uniform in the box,
+    direction uniform on the sphere, depth 8. A step is one pt_batch_render of all views."""
+    from pt_amd.scenes.rng import Xshiro128
+    scene = pt_amd.Scene(generate.cached_scene("box", root=a.scene_dir, seed=1))
+    tr = pt_amd.Tracer(scene, 64, 64)
+    rng = Xshiro128.seeded(3)
+    views = []
+    for _ in range(64):
+        area = rng.log_uniform(128.0 ** 2, 1024.0 ** 2)
+        aspect = rng.uniform(0.5, 2.0)
+        w = max(1, int(round((area * aspect) ** 0.5)))
+        h = max(1, int(round(area / w)))
+        spp = int(round(rng.log_uniform(4.0, 256.0)))
+        eye = (rng.uniform(-1, 1), rng.uniform(-2, 0), rng.uniform(-1, 1))
+        fwd = _random_dir(rng)
+        # the reference's camera looks along -lookdir from -position (src/scene/camera.h:16)
+        cam = pt_amd.camera_block([-e for e in eye], [-f for f in fwd], fov_deg=60.0, aspect=w / h,
+                                  focus_dist=1.0, lens_radius=0.01)
+        views.append((0, w, h, spp, DEPTH, cam))
+    paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
+    log("views: 64 views, %d pixel-samples per step" % paths)
+    batch = pt_amd.Batch([tr], views)
+    el, st = _timed_batch(batch, a.steps, a.warmup)
+    rays = st["extension_rays"] + st["shadow_rays"]
+    batch.close()
+    # the same views one after another, each its own one-view batch
+    seq = 0.0
+    for v in views:
+        b = pt_amd.Batch([tr], [v])
+        b.render()
+        b.stats()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            b.render()
+        b.stats()
+        seq += time.perf_counter() - t
+        b.close()
+    return {
+        "metric": "Mrays/s", "value": round(rays / el / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "64 views of the box scene in one batch (res 128^2-1024^2, spp 4-256, depth 8)",
+                   "pixel_samples_per_step": paths},
+        "pixel_samples_per_s": round(paths * a.steps / el, 1),
+        "sequential_views_s": round(seq, 4), "batched_s": round(el, 4),
+        "batched_speedup": round(seq / el, 3),
+    }
+
+
+def run_scenes(a):
+    """16 seeded instanced scenes with triangle totals log-uniform in 10k..20M, each with its
+    own device-built BVH and 4 views (log-uniform 256^2..1920x1080 pixels, 16-128 spp, depth
+    8), all traced in one persistent wavefront run."""
+    from pt_amd.scenes.rng import Xshiro128
+    rng = Xshiro128.seeded(4)
+    tracers, scenes_info, views = [], [], []
+    for k in range(16):
+        target = rng.log_uniform(1e4, 2e7)
+        counts = generate.counts_for_triangles(target)
+        t = time.time()
+        scn = generate.cached_scene("instanced", root=a.scene_dir, seed=100 + k, counts=counts)
+        scene = pt_amd.Scene(scn)
+        tr = pt_amd.Tracer(scene, 64, 64)
+        s0 = tr.stats()
+        info = scene.info()
+        scenes_info.append({"triangles": info["total_triangles"], "instances": info["instances"],
+                            "bvh_build_ms": round(s0["bvh_build_ms"], 2), "bvh_mb": round(s0["bvh_bytes"] / 1e6, 1)})
+        log("scene %d: %d triangles, %d instances, BVH %.1f ms (%.1f s incl. load)"
+            % (k, info["total_triangles"], info["instances"], s0["bvh_build_ms"], time.time() - t))
+        tracers.append(tr)
+        for _ in range(4):
+            area = rng.log_uniform(256.0 ** 2, 1920.0 * 1080.0)
+            aspect = rng.uniform(0.5, 2.0)
+            w = max(1, int(round((area * aspect) ** 0.5)))
+            h = max(1, int(round(area / w)))
+            spp = int(round(rng.log_uniform(16.0, 128.0)))
+            eye = [rng.uniform(-20, 20) for _ in range(3)]
+            fwd = _random_dir(rng)
+            cam = pt_amd.camera_block([-e for e in eye], [-f for f in fwd], fov_deg=60.0, aspect=w / h,
+                                      focus_dist=30.0, lens_radius=0.03)
+            views.append((k, w, h, spp, DEPTH, cam))
+    paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
+    log("scenes: 16 scenes, 64 views, %d pixel-samples per step" % paths)
+    batch = pt_amd.Batch(tracers, views)
+    el, st = _timed_batch(batch, a.steps, a.warmup)
+    rays = st["extension_rays"] + st["shadow_rays"]
+    batch.close()
+    # each scene's views on their own
+    for k in range(16):
+        vk = [(0,) + v[1:] for v in views if v[0] == k]
+        b = pt_amd.Batch([tracers[k]], vk)
+        ek, sk = _timed_batch(b, 1, 1)
+        scenes_info[k]["mrays_s"] = round((sk["extension_rays"] + sk["shadow_rays"]) / ek / 1e6, 2)
+        b.close()
+    return {
+        "metric": "Mrays/s", "value": round(rays / el / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": "16 instanced scenes (10k-20M triangles) x 4 views in one batch, depth 8",
+                   "pixel_samples_per_step": paths},
+        "pixel_samples_per_s": round(paths * a.steps / el, 1),
+        "scenes": scenes_info,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--spp-per-step", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["instanced", "views", "scenes"], default="instanced")
+    ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-convergence", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; every box here has one, so --gpus %d is not run"
+              % a.gpus, file=sys.stderr)
+        return 2
+    out = {"instanced": run_instanced, "views": run_views, "scenes": run_scenes}[a.workload](a)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -343,10 +343,35 @@ def instanced_scene(out_dir, seed=2, counts=None, emitters=INSTANCED_EMITTERS, g
     return scn
 
 
+MESH_TRIS = {"grid": 2 * 512 * 512, "soup": 200000, "ico6": 81920, "ico4": 5120, "ico2": 320}
+
+
+def counts_for_triangles(total):
+    """Instance counts of the five mesh kinds whose triangles (with the 20 emissive
+    level-2 spheres) add up to about `total`: the default scene's mix of kinds, scaled, and
+    the remainder filled with the largest spheres that fit."""
+    default_total = sum(INSTANCED_COUNTS[k] * MESH_TRIS[k] for k in INSTANCED_COUNTS)
+    left = max(0, int(total) - INSTANCED_EMITTERS * MESH_TRIS["ico2"])
+    counts = {}
+    for kind in ("grid", "soup", "ico6", "ico4"):
+        share = INSTANCED_COUNTS[kind] * MESH_TRIS[kind] / default_total
+        counts[kind] = int(left * share // MESH_TRIS[kind])
+    counts["ico2"] = 0
+    rem = left - sum(counts[k] * MESH_TRIS[k] for k in counts)
+    for kind in ("ico6", "ico4", "ico2"):  # the remainder, largest meshes first
+        n = rem // MESH_TRIS[kind]
+        counts[kind] += n
+        rem -= n * MESH_TRIS[kind]
+    counts["ico2"] = max(1, counts["ico2"])
+    return counts
+
+
 def cached_scene(kind, root=None, **kw):
     """Generates a scene once under root (default $PT_SCENE_DIR or /tmp/pt_scenes)."""
     root = root or os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes")
-    tag = kind + "".join("_%s%s" % (k, v) for k, v in sorted(kw.items()))
+    def fmt(v):
+        return "-".join("%s%s" % kv for kv in sorted(v.items())) if isinstance(v, dict) else str(v)
+    tag = kind + "".join("_%s%s" % (k, fmt(v)) for k, v in sorted(kw.items()))
     out = os.path.join(root, tag)
     name = {"box": "box.scn", "instanced": "instanced.scn"}[kind]
     path = os.path.join(out, name)
