@@ -140,6 +140,13 @@ typedef struct pt_stats {
 #define PT_OPT_COUNT_FETCHES 2u   /* value != 0: count BVH node / triangle fetches per ray */
 #define PT_OPT_GRAPHS 3u          /* value != 0 (default): replay each wavefront iteration from a
                                    * captured HIP graph (not while kernels are being timed) */
+/* Launch configuration (no counterpart in the reference, whose traversal belongs to the
+ * driver). Images do not depend on them, except that another leaf size can resolve exact
+ * ties between equally distant triangles differently. */
+#define PT_OPT_BLOCKS_PER_CU 4u   /* traversal grid: blocks of 256 threads per CU (1..64, default 4) */
+#define PT_OPT_POOL_SLOTS 5u      /* path pool slots (1024..2^26; default min(8M, max(4 px, 64K))) */
+#define PT_OPT_LDS_STACK 6u       /* traversal stack entries per lane kept in LDS (8, 16, 32) */
+#define PT_OPT_LEAF_SIZE 7u       /* max triangles per BLAS leaf (1..16, default 4): rebuilds the BVH */
 
 typedef struct pt_ctx pt_ctx;
 
@@ -286,6 +293,23 @@ int pt_cache_encode(pt_ctx* ctx, const float* records, uint32_t n, float* featur
  * apply != 0 one Adam step follows. Synchronises. */
 int pt_cache_train_batch(pt_ctx* ctx, const float* features, const float* targets, uint32_t n, int apply,
                          float* grads, float* loss);
+
+/* Autotuner (no counterpart in the reference): times spp samples of the current view under
+ * candidate launch configurations (coordinate descent over blocks per CU, LDS stack depth,
+ * pool slots and BLAS leaf size; the node width is 4, the only one this build has), keeps the
+ * fastest (device time per traced ray) and clears the accumulation (sample_index 0).
+ * Not with the radiance cache or instrumentation options on. Synchronises. */
+typedef struct pt_tune_result {
+    uint32_t blocks_per_cu;
+    uint32_t pool_slots;
+    uint32_t lds_stack;
+    uint32_t leaf_size;
+    uint32_t node_width;
+    uint32_t trials;
+    double ns_per_ray_start; /* the configuration the tuner started from */
+    double ns_per_ray_best;
+} pt_tune_result;
+int pt_autotune(pt_ctx* ctx, uint32_t spp, pt_tune_result* out);
 
 /* Waits for all work queued on the context's stream. */
 int pt_synchronize(pt_ctx* ctx);

@@ -86,6 +86,16 @@ class Stats(C.Structure):
 PT_OPT_PROFILE_KERNELS = 1
 PT_OPT_COUNT_FETCHES = 2
 PT_OPT_GRAPHS = 3
+PT_OPT_BLOCKS_PER_CU = 4
+PT_OPT_POOL_SLOTS = 5
+PT_OPT_LDS_STACK = 6
+PT_OPT_LEAF_SIZE = 7
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("pool_slots", C.c_uint32), ("lds_stack", C.c_uint32),
+                ("leaf_size", C.c_uint32), ("node_width", C.c_uint32), ("trials", C.c_uint32),
+                ("ns_per_ray_start", C.c_double), ("ns_per_ray_best", C.c_double)]
 PT_CACHE_FEATURES = 96
 
 
@@ -167,6 +177,7 @@ def hip_lib():
         lib.pt_last_error.argtypes = [P]
         lib.pt_last_error.restype = C.c_char_p
         lib.pt_destroy.argtypes = [P]
+        lib.pt_autotune.argtypes = [P, C.c_uint32, C.POINTER(TuneResult)]
         lib.pt_batch_create.argtypes = [C.POINTER(P), C.c_uint32, C.POINTER(View), C.c_uint32, C.POINTER(P)]
         lib.pt_batch_set_view.argtypes = [P, C.c_uint32, C.POINTER(View)]
         lib.pt_batch_render.argtypes = [P]
@@ -386,6 +397,11 @@ class Tracer:
         s = Stats()
         self._check(self._lib.pt_get_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def autotune(self, spp=2):
+        r = TuneResult()
+        self._check(self._lib.pt_autotune(self._h, spp, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     # ---- learned radiance cache (no counterpart in the reference)
 

@@ -267,6 +267,43 @@ struct DevBufs {
 };
 
 
+// (Re)builds the BLAS+TLAS with the context's leaf size and checks the hierarchy.
+int build_scene_bvh(Ctx& c)
+{
+    hipStream_t st = c.stream;
+    free_bvh(c.bvh);
+    HIP_TRY(&c, hipEventRecord(c.ev_start, st), "event");
+    BvhBuildParams bp;
+    bp.blas_leaf_size = c.leaf_size;
+    HIP_TRY(&c,
+            build_bvh(st, c.d_vertices, c.d_indices, c.d_meshes, c.meshes, c.d_instances, c.d_instance_mesh,
+                      c.instance_count, c.d_materials, bp, c.bvh),
+            "BVH build");
+    HIP_TRY(&c, hipEventRecord(c.ev_stop, st), "event");
+    HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
+    float ms = 0.0f;
+    HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
+    c.bvh_build_ms = ms;
+    std::string why;
+    if (int v = validate_bvh(c, c.meshes, why))
+        return fail(&c, v, "BVH validation: " + why);
+    c.scene.nodes = c.bvh.nodes;
+    c.scene.tris = c.bvh.tris;
+    c.scene.inst_inv = c.bvh.inst_inv;
+    c.scene.inst_blas_root = c.bvh.inst_blas_root;
+    c.scene.tlas_root = c.bvh.tlas_root;
+    c.graphs.destroy();
+    return PT_OK;
+}
+
+// Grid and pool for the context's launch configuration (buffers reallocated).
+hipError_t apply_launch(Ctx& c)
+{
+    c.grid_blocks = ((c.cus * c.blocks_per_cu + 7) / 8) * 8;
+    c.graphs.destroy();
+    return wavefront_alloc(c.wf, c.pool_slots, c.job_paths, c.grid_blocks * TRACE_BLOCK);
+}
+
 }  // namespace
 
 struct pt_batch {
@@ -288,6 +325,7 @@ struct pt_batch {
     uint32_t grid_blocks = 0;
     bool count_fetches = false;
     bool use_graphs = true;
+    uint32_t lds_stack = LDS_STACK;
     KernelTimer timer;
     IterGraphs graphs;
     bool stats_fresh = true;
@@ -429,25 +467,11 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              HIP_TRY(&c, upload(c.d_inst_opacity, opacity.data(), opacity.size() * 4, st), "opacity");
              c.instance_count = scene->instance_count;
 
-             std::vector<pt_mesh_desc> meshes(scene->meshes, scene->meshes + scene->mesh_count);
-             HIP_TRY(&c, hipEventRecord(c.ev_start, st), "event");
-             BvhBuildParams bp;
+             c.meshes.assign(scene->meshes, scene->meshes + scene->mesh_count);
              if (const char* env = getenv("PT_BVH_LEAF"))
-                 bp.blas_leaf_size = uint32_t(atoi(env));
-             HIP_TRY(&c,
-                     build_bvh(st, c.d_vertices, c.d_indices, c.d_meshes, meshes, c.d_instances, c.d_instance_mesh,
-                               scene->instance_count, c.d_materials, bp, c.bvh),
-                     "BVH build");
-             HIP_TRY(&c, hipEventRecord(c.ev_stop, st), "event");
-             HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
-             float ms = 0.0f;
-             HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
-             c.bvh_build_ms = ms;
-             {
-                 std::string why;
-                 if (int v = validate_bvh(c, meshes, why))
-                     return fail(&c, v, "BVH validation: " + why);
-             }
+                 c.leaf_size = uint32_t(std::max(1, std::min(16, atoi(env))));
+             if (int v = build_scene_bvh(c))
+                 return v;
 
              DeviceScene& s = c.scene;
              s.vertices = c.d_vertices;
@@ -472,20 +496,18 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
 
              hipDeviceProp_t prop;
              HIP_TRY(&c, hipGetDeviceProperties(&prop, c.device), "device properties");
-             const uint32_t cus = std::max(1, prop.multiProcessorCount);
-             uint32_t blocks_per_cu = 4;
+             c.cus = std::max(1, prop.multiProcessorCount);
              if (const char* env = getenv("PT_BLOCKS_PER_CU"))
-                 blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
-             c.grid_blocks = ((cus * blocks_per_cu + 7) / 8) * 8;
+                 c.blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
              const uint32_t npix = width * height;
              // Path pool: 8M slots (fewer for small images); per-job radiance buffer for up to
              // 2^27 paths (64 spp at 1920x1080).
-             uint32_t pool = std::min<uint32_t>(8u << 20, std::max<uint32_t>(npix * 4u, 65536u));
+             c.pool_slots = std::min<uint32_t>(8u << 20, std::max<uint32_t>(npix * 4u, 65536u));
              if (const char* env = getenv("PT_POOL_SLOTS"))
-                 pool = uint32_t(std::max(1024, atoi(env)));
+                 c.pool_slots = uint32_t(std::max(1024, atoi(env)));
              const uint32_t job_spp = std::max<uint32_t>(1, std::min<uint32_t>(1024, (1u << 27) / npix));
-             HIP_TRY(&c, wavefront_alloc(c.wf, pool, npix * job_spp, c.grid_blocks * TRACE_BLOCK),
-                     "wavefront buffers");
+             c.job_paths = npix * job_spp;
+             HIP_TRY(&c, apply_launch(c), "wavefront buffers");
              HIP_TRY(&c, hipMalloc(&c.d_views, sizeof(ViewDev)), "view table");
              HIP_TRY(&c, hipMalloc(&c.d_job, sizeof(JobDev)), "job table");
              HIP_TRY(&c, hipHostMalloc((void**)&c.h_view, sizeof(ViewDev), hipHostMallocDefault), "view table");
@@ -694,6 +716,35 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
     case PT_OPT_GRAPHS:
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
         c.use_graphs = value != 0;
+        return PT_OK;
+    case PT_OPT_BLOCKS_PER_CU:
+    case PT_OPT_POOL_SLOTS: {
+        if (option == PT_OPT_BLOCKS_PER_CU && (value < 1 || value > 64))
+            return fail(&c, PT_EINVAL, "blocks per CU must be 1..64");
+        if (option == PT_OPT_POOL_SLOTS && (value < 1024 || value > (1 << 26)))
+            return fail(&c, PT_EINVAL, "pool slots must be 1024..2^26");
+        if (c.cache.configured)
+            return fail(&c, PT_ESTATE, "disable the radiance cache before changing the path pool");
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        (option == PT_OPT_BLOCKS_PER_CU ? c.blocks_per_cu : c.pool_slots) = uint32_t(value);
+        HIP_TRY(&c, apply_launch(c), "wavefront buffers");
+        c.stats_fresh = true;
+        return PT_OK;
+    }
+    case PT_OPT_LDS_STACK:
+        if (value != 8 && value != 16 && value != 32)
+            return fail(&c, PT_EINVAL, "LDS stack depth must be 8, 16 or 32");
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.lds_stack = uint32_t(value);
+        return PT_OK;
+    case PT_OPT_LEAF_SIZE:
+        if (value < 1 || value > 16)
+            return fail(&c, PT_EINVAL, "leaf size must be 1..16");
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.leaf_size = uint32_t(value);
+        if (int v = build_scene_bvh(c))
+            return v;
+        c.stats_fresh = true;
         return PT_OK;
     default:
         return fail(&c, PT_EINVAL, "unknown option " + std::to_string(option));
@@ -1012,7 +1063,7 @@ int pt_batch_render(pt_batch* b)
     RenderIO io{b->stream,   &b->wf,   b->grid_blocks, b->count_fetches,
                 &b->timer,   b->scenes[0]->c.scene,    b->d_scenes,    b->d_views,
                 b->d_job,    b->h_job, b->use_graphs ? &b->graphs : nullptr,
-                nullptr};
+                nullptr,     b->lds_stack};
     BHIP_TRY(b, render_views(io, work), "render");
     BHIP_TRY(b, hipEventRecord(b->ev_stop, b->stream), "event");
     for (uint32_t i = 0; i < b->views.size(); i++)
@@ -1122,6 +1173,110 @@ void pt_batch_destroy(pt_batch* b)
         return;
     batch_release(*b);
     delete b;
+}
+
+int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (spp == 0)
+        return fail(&c, PT_EINVAL, "Sample count was zero.");
+    if (!c.have_camera)
+        return fail(&c, PT_ESTATE, "no camera set");
+    if (c.cache.configured)
+        return fail(&c, PT_ESTATE, "disable the radiance cache before tuning");
+    if (c.timer.enabled || c.count_fetches)
+        return fail(&c, PT_ESTATE, "instrumentation options are on");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    struct Cfg {
+        int64_t bpc, pool, lds, leaf;
+    };
+    const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size};
+    uint32_t trials = 0;
+    // device nanoseconds per traced ray of spp samples of the current view (best of two runs
+    // after a first one that warms the caches and captures the graphs)
+    auto measure = [&](const Cfg& g, double& ns) -> int {
+        if (g.bpc != int64_t(c.blocks_per_cu))
+            if (int v = pt_set_option(h, PT_OPT_BLOCKS_PER_CU, g.bpc))
+                return v;
+        if (g.pool != int64_t(c.pool_slots))
+            if (int v = pt_set_option(h, PT_OPT_POOL_SLOTS, g.pool))
+                return v;
+        if (int v = pt_set_option(h, PT_OPT_LDS_STACK, g.lds))
+            return v;
+        if (g.leaf != int64_t(c.leaf_size))
+            if (int v = pt_set_option(h, PT_OPT_LEAF_SIZE, g.leaf))
+                return v;
+        ns = 1e300;
+        for (int rep = 0; rep < 3; rep++) {
+            unsigned long long t0[2], t1[2];
+            HIP_TRY(&c, hipMemcpy(t0, c.wf.totals, sizeof(t0), hipMemcpyDeviceToHost), "stats");
+            c.sample_index = 0;
+            HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
+            HIP_TRY(&c, render_samples(c, spp), "render");
+            HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
+            HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
+            float ms = 0.0f;
+            HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
+            HIP_TRY(&c, hipMemcpy(t1, c.wf.totals, sizeof(t1), hipMemcpyDeviceToHost), "stats");
+            const double rays = double(t1[0] - t0[0]) + double(t1[1] - t0[1]);
+            if (rep > 0 && rays > 0)
+                ns = std::min(ns, ms * 1e6 / rays);
+        }
+        trials++;
+        return PT_OK;
+    };
+    double base_ns = 0.0;
+    if (int v = measure(start, base_ns))
+        return v;
+    Cfg best = start;
+    double best_ns = base_ns;
+    const uint64_t npix = uint64_t(c.width) * c.height;
+    const std::vector<int64_t> bpcs = {2, 3, 4, 5, 6, 8};
+    const std::vector<int64_t> ldss = {8, 16, 32};
+    std::vector<int64_t> pools;
+    for (int64_t p : {1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20})
+        if (uint64_t(p) <= std::max<uint64_t>(npix * spp, 1u << 20))
+            pools.push_back(p);
+    const std::vector<int64_t> leaves = {2, 4, 8};
+    // coordinate descent over (blocks per CU, LDS stack depth, pool slots, leaf size)
+    for (int param = 0; param < 4; param++) {
+        const std::vector<int64_t>& vals = param == 0 ? bpcs : param == 1 ? ldss : param == 2 ? pools : leaves;
+        for (int64_t v : vals) {
+            Cfg g = best;
+            (param == 0 ? g.bpc : param == 1 ? g.lds : param == 2 ? g.pool : g.leaf) = v;
+            if (g.bpc == best.bpc && g.lds == best.lds && g.pool == best.pool && g.leaf == best.leaf)
+                continue;
+            double ns;
+            if (int e = measure(g, ns))
+                return e;
+            if (ns < best_ns) {
+                best_ns = ns;
+                best = g;
+            }
+        }
+    }
+    double final_ns;
+    if (int v = measure(best, final_ns))
+        return v;
+    c.sample_index = 0;
+    c.stats_fresh = true;
+    HIP_TRY(&c, hipMemsetAsync(c.accum, 0, npix * 16, c.stream), "accumulation image");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    if (out) {
+        pt_tune_result r{};
+        r.blocks_per_cu = uint32_t(best.bpc);
+        r.pool_slots = uint32_t(best.pool);
+        r.lds_stack = uint32_t(best.lds);
+        r.leaf_size = uint32_t(best.leaf);
+        r.node_width = 4;
+        r.trials = trials;
+        r.ns_per_ray_start = base_ns;
+        r.ns_per_ray_best = best_ns;
+        *out = r;
+    }
+    return PT_OK;
 }
 
 int pt_synchronize(pt_ctx* h)

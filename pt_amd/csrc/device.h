@@ -210,6 +210,14 @@ struct Ctx {
     JobDev* h_job = nullptr;     // pinned staging of the job table
     IterGraphs graphs;
     bool use_graphs = true;
+    // launch configuration (PT_OPT_BLOCKS_PER_CU / POOL_SLOTS / LDS_STACK / LEAF_SIZE)
+    uint32_t cus = 1;
+    uint32_t blocks_per_cu = 4;
+    uint32_t pool_slots = 0;
+    uint32_t lds_stack = LDS_STACK;
+    uint32_t leaf_size = 4;
+    uint32_t job_paths = 0;  // capacity of the per-job radiance buffer
+    std::vector<pt_mesh_desc> meshes;  // host copy (BVH rebuilds)
 
     // stats of the last render
     unsigned long long last_ext = 0, last_shadow = 0, last_paths = 0;
@@ -233,6 +241,7 @@ struct RenderIO {
     JobDev* h_job;
     IterGraphs* graphs;             // nullptr: launch every kernel directly
     CacheState* cache;              // nullptr: no radiance cache
+    uint32_t lds_stack;             // LDS stack entries per lane of the traversal kernels
 };
 
 // Samples [sample_index, sample_index + spp) of one view.

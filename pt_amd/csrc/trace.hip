@@ -124,7 +124,11 @@ __device__ __forceinline__ bool intersect_tri(const RayPre& r, float3 v0, float3
     return true;
 }
 
-struct Stack {
+// Per-lane traversal stack: the newest D entries in an LDS ring (this lane's column of the
+// block's array), older ones spilled to global memory.
+template <uint32_t D>
+struct StackT {
+    static_assert((D & (D - 1)) == 0, "LDS stack depth is a power of two");
     uint32_t* lds;    // this lane's column: entry k at lds[k * TRACE_BLOCK]
     uint32_t* spill;  // this thread's spill area: entry k at spill[k * spill_stride]
     uint32_t spill_stride;
@@ -133,27 +137,28 @@ struct Stack {
 
     __device__ __forceinline__ void push(uint32_t v)
     {
-        if (sp >= LDS_STACK) {
-            const uint32_t deep = sp - LDS_STACK;
+        if (sp >= D) {
+            const uint32_t deep = sp - D;
             if (deep >= SPILL_STACK) {
                 overflow = true;
                 return;
             }
-            spill[size_t(deep) * spill_stride] = lds[(sp & (LDS_STACK - 1)) * TRACE_BLOCK];
+            spill[size_t(deep) * spill_stride] = lds[(sp & (D - 1)) * TRACE_BLOCK];
         }
-        lds[(sp & (LDS_STACK - 1)) * TRACE_BLOCK] = v;
+        lds[(sp & (D - 1)) * TRACE_BLOCK] = v;
         sp++;
     }
     __device__ __forceinline__ uint32_t pop()
     {
         sp--;
-        const uint32_t slot = (sp & (LDS_STACK - 1)) * TRACE_BLOCK;
+        const uint32_t slot = (sp & (D - 1)) * TRACE_BLOCK;
         const uint32_t v = lds[slot];
-        if (sp >= LDS_STACK)
-            lds[slot] = spill[size_t(sp - LDS_STACK) * spill_stride];
+        if (sp >= D)
+            lds[slot] = spill[size_t(sp - D) * spill_stride];
         return v;
     }
 };
+using Stack = StackT<LDS_STACK>;
 
 struct HitOut {
     float t, u, v;
@@ -162,8 +167,8 @@ struct HitOut {
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
 // transmittance through transparent instances is returned in trans (shadowray.ah.hlsl).
-template <bool ANY, bool COUNT = false>
-__device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stack& stk, HitOut& hit,
+template <bool ANY, bool COUNT = false, typename Stk>
+__device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
                          float& trans, uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr)
 {
     hit.inst = NONE;
@@ -402,14 +407,14 @@ __global__ __launch_bounds__(256) void k_start_job(const ViewDev* views, const J
     }
 }
 
-template <bool COUNT, bool MULTI>
+template <bool COUNT, bool MULTI, uint32_t D>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
-    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    __shared__ uint32_t lds[D * TRACE_BLOCK];
     const uint32_t count = wf.counters[q];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
@@ -430,14 +435,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s0, const De
         atomicAdd(&wf.totals[0], (unsigned long long)count);
 }
 
-template <bool COUNT, bool MULTI>
+template <bool COUNT, bool MULTI, uint32_t D>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const DeviceScene* scenes, Wavefront wf)
 {
-    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    __shared__ uint32_t lds[D * TRACE_BLOCK];
     const uint32_t count = wf.counters[2];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
@@ -966,6 +971,33 @@ void wavefront_free(Wavefront& wf)
 
 namespace {
 
+typedef void (*ExtendFn)(DeviceScene, const DeviceScene*, Wavefront, int);
+typedef void (*ShadowFn)(DeviceScene, const DeviceScene*, Wavefront);
+
+template <uint32_t D>
+ExtendFn pick_extend_d(bool count, bool multi)
+{
+    return count ? (multi ? k_extend<true, true, D> : k_extend<true, false, D>)
+                 : (multi ? k_extend<false, true, D> : k_extend<false, false, D>);
+}
+template <uint32_t D>
+ShadowFn pick_shadow_d(bool count, bool multi)
+{
+    return count ? (multi ? k_shadow<true, true, D> : k_shadow<true, false, D>)
+                 : (multi ? k_shadow<false, true, D> : k_shadow<false, false, D>);
+}
+// Traversal kernel variants by LDS stack depth (8, 16 or 32 entries per lane).
+ExtendFn pick_extend(bool count, bool multi, uint32_t depth)
+{
+    return depth == 8 ? pick_extend_d<8>(count, multi)
+                      : (depth == 32 ? pick_extend_d<32>(count, multi) : pick_extend_d<16>(count, multi));
+}
+ShadowFn pick_shadow(bool count, bool multi, uint32_t depth)
+{
+    return depth == 8 ? pick_shadow_d<8>(count, multi)
+                      : (depth == 32 ? pick_shadow_d<32>(count, multi) : pick_shadow_d<16>(count, multi));
+}
+
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
 // cache queries, retire and refill, and copy the count of the next queue to pinned memory.
 hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
@@ -980,10 +1012,7 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
     if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
         return e;
     const dim3 tg(io.grid_blocks), tb(TRACE_BLOCK);
-    void (*extend)(DeviceScene, const DeviceScene*, Wavefront, int) =
-        io.count_fetches ? (multi ? k_extend<true, true> : k_extend<true, false>)
-                         : (multi ? k_extend<false, true> : k_extend<false, false>);
-    hipLaunchKernelGGL(extend, tg, tb, 0, st, io.scene, tab, wf, q);
+    hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
     if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
@@ -998,10 +1027,7 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
         return e;
     if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
         return e;
-    void (*shadow)(DeviceScene, const DeviceScene*, Wavefront) =
-        io.count_fetches ? (multi ? k_shadow<true, true> : k_shadow<true, false>)
-                         : (multi ? k_shadow<false, true> : k_shadow<false, false>);
-    hipLaunchKernelGGL(shadow, tg, tb, 0, st, io.scene, tab, wf);
+    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf);
     if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
         return e;
     if (cd.query) {
@@ -1024,6 +1050,8 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     uint64_t h = 1469598103934665603ull;
     auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
     mix(io.count_fetches);
+    mix(io.lds_stack);
+    mix(reinterpret_cast<uintptr_t>(io.scene.nodes));
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
     mix(reinterpret_cast<uintptr_t>(io.d_views));
     mix(reinterpret_cast<uintptr_t>(io.d_job));
@@ -1229,7 +1257,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
     RenderIO io{ctx.stream,       &ctx.wf,     ctx.grid_blocks, ctx.count_fetches,
                 &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
                 ctx.d_job,        ctx.h_job,   ctx.use_graphs ? &ctx.graphs : nullptr,
-                &ctx.cache};
+                &ctx.cache,       ctx.lds_stack};
     return render_views(io, {ViewWork{0, v.npix, ctx.sample_index, spp}});
 }
 

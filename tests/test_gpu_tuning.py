@@ -1,0 +1,87 @@
+"""Launch-configuration options and the autotuner: the image must not depend on them."""
+import numpy as np
+import pytest
+
+import pt_amd
+
+pytestmark = pytest.mark.gpu
+
+W, H = 160, 90
+
+
+@pytest.fixture(scope="module")
+def inst(scene_dir):
+    from pt_amd.scenes import generate
+    scene = pt_amd.Scene(generate.cached_scene("instanced", root=scene_dir))
+    cam = scene.camera(W, H)
+    t = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    t.set_camera(cam)
+    t.render(4)
+    return scene, cam, t.read()
+
+
+def _render(scene, cam, opts, spp=4):
+    t = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    for k, v in opts:
+        t.set_option(k, v)
+    t.set_camera(cam)
+    t.render(spp)
+    img = t.read()
+    st = t.stats()
+    t.close()
+    return img, st
+
+
+@pytest.mark.parametrize("opts", [
+    [(pt_amd.PT_OPT_BLOCKS_PER_CU, 2)],
+    [(pt_amd.PT_OPT_BLOCKS_PER_CU, 7), (pt_amd.PT_OPT_LDS_STACK, 32)],
+    [(pt_amd.PT_OPT_POOL_SLOTS, 5000), (pt_amd.PT_OPT_LDS_STACK, 8)],
+    [(pt_amd.PT_OPT_POOL_SLOTS, 1 << 20)],
+])
+def test_launch_options_leave_the_image_unchanged(inst, opts):
+    scene, cam, ref = inst
+    img, _ = _render(scene, cam, opts)
+    np.testing.assert_array_equal(img, ref)
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 8, 16])
+def test_leaf_size_rebuild(inst, leaf):
+    scene, cam, ref = inst
+    img, st = _render(scene, cam, [(pt_amd.PT_OPT_LEAF_SIZE, leaf)])
+    assert st["tri_record_count"] == scene.info()["total_triangles"]
+    # another hierarchy may only resolve exact ties differently
+    same = np.all(img == ref, axis=2)
+    assert same.mean() >= 0.999, same.mean()
+
+
+def test_option_checks(inst):
+    scene, cam, _ = inst
+    t = pt_amd.Tracer(scene, 16, 16)
+    for k, v in [(pt_amd.PT_OPT_BLOCKS_PER_CU, 0), (pt_amd.PT_OPT_BLOCKS_PER_CU, 65),
+                 (pt_amd.PT_OPT_POOL_SLOTS, 100), (pt_amd.PT_OPT_LDS_STACK, 12), (pt_amd.PT_OPT_LEAF_SIZE, 17)]:
+        with pytest.raises(pt_amd.PtError):
+            t.set_option(k, v)
+    with pytest.raises(pt_amd.PtError, match="no camera"):
+        t.autotune(1)
+    t.cache_configure(query=False)
+    t.set_camera(cam)
+    with pytest.raises(pt_amd.PtError, match="radiance cache"):
+        t.set_option(pt_amd.PT_OPT_POOL_SLOTS, 4096)
+    t.close()
+
+
+def test_autotune_picks_a_configuration(inst):
+    scene, cam, ref = inst
+    t = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    t.set_camera(cam)
+    t.render(1)
+    r = t.autotune(2)
+    assert r["trials"] >= 10 and r["node_width"] == 4
+    assert r["blocks_per_cu"] in (2, 3, 4, 5, 6, 8) and r["lds_stack"] in (8, 16, 32)
+    assert r["leaf_size"] in (2, 4, 8)
+    assert 0 < r["ns_per_ray_best"] <= r["ns_per_ray_start"]
+    assert t.stats()["sample_index"] == 0
+    t.render(4)  # the accumulation restarted: the tuned renderer gives the same image
+    same = np.all(t.read() == ref, axis=2)
+    assert same.mean() >= (1.0 if r["leaf_size"] == 4 else 0.999)
+    t.close()
