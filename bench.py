@@ -159,6 +159,16 @@ def run_instanced(a):
     st0 = tr.stats()
     log("BVH: %.1f ms build, %d nodes, %.1f MB" % (st0["bvh_build_ms"], st0["node_count"], st0["bvh_bytes"] / 1e6))
 
+    # launch configuration picked by the autotuner (untimed set-up, like the BVH build)
+    tune = None
+    if not a.no_autotune:
+        tr.render(1)
+        tune = tr.autotune(4)
+        log("autotune: %d trials, blocks/CU %d, LDS stack %d, leaf %d, pool %d: %.3f -> %.3f ns/ray"
+            % (tune["trials"], tune["blocks_per_cu"], tune["lds_stack"], tune["leaf_size"], tune["pool_slots"],
+               tune["ns_per_ray_start"], tune["ns_per_ray_best"]))
+        st0 = tr.stats()
+
     # warm-up
     for _ in range(max(1, a.warmup)):
         tr.render(a.spp_per_step)
@@ -245,6 +255,7 @@ def run_instanced(a):
                             "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)},
         "stages_ms": {k: round(v["ms"], 3) for k, v in stages.items()},
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
+        "autotune": tune,
         "roofline": roofline,
         "convergence": conv,
         "cpu_baseline": cpu,
@@ -387,6 +398,7 @@ def main():
     ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     a = ap.parse_args()
     if a.gpus != 1:
