@@ -294,6 +294,16 @@ int pt_cache_encode(pt_ctx* ctx, const float* records, uint32_t n, float* featur
 int pt_cache_train_batch(pt_ctx* ctx, const float* features, const float* targets, uint32_t n, int apply,
                          float* grads, float* loss);
 
+/* Device build of the light sampler's alias tables: TableLightSampler's light table and the
+ * per-emissive-mesh triangle tables (src/scene/lightsample.cpp:5-178, src/aliastable.h:21-67),
+ * recomputed on the device from the scene on the GPU (material emission, point/directional
+ * light colours, triangle areas) and written over the words pt_create uploaded; the result is
+ * bit-identical to the host packing. Resets sample_index. PT_ESTATE when the words passed to
+ * pt_create do not follow that packing. */
+int pt_build_light_tables(pt_ctx* ctx);
+/* Copies the light-sampler words (light_word_count of the scene) back to host memory. */
+int pt_read_light_words(pt_ctx* ctx, uint32_t* words, size_t count);
+
 /* Autotuner (no counterpart in the reference): times spp samples of the current view under
  * candidate launch configurations (coordinate descent over blocks per CU, LDS stack depth,
  * pool slots and BLAS leaf size; the node width is 4, the only one this build has), keeps the

@@ -178,6 +178,8 @@ def hip_lib():
         lib.pt_last_error.restype = C.c_char_p
         lib.pt_destroy.argtypes = [P]
         lib.pt_autotune.argtypes = [P, C.c_uint32, C.POINTER(TuneResult)]
+        lib.pt_build_light_tables.argtypes = [P]
+        lib.pt_read_light_words.argtypes = [P, C.c_void_p, C.c_size_t]
         lib.pt_batch_create.argtypes = [C.POINTER(P), C.c_uint32, C.POINTER(View), C.c_uint32, C.POINTER(P)]
         lib.pt_batch_set_view.argtypes = [P, C.c_uint32, C.POINTER(View)]
         lib.pt_batch_render.argtypes = [P]
@@ -397,6 +399,15 @@ class Tracer:
         s = Stats()
         self._check(self._lib.pt_get_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def build_light_tables(self):
+        self._check(self._lib.pt_build_light_tables(self._h))
+
+    def light_words(self):
+        n = int(self.scene.blob.contents.light_word_count)
+        out = np.zeros(n, np.uint32)
+        self._check(self._lib.pt_read_light_words(self._h, out.ctypes.data, n))
+        return out
 
     def autotune(self, spp=2):
         r = TuneResult()

@@ -267,6 +267,36 @@ struct DevBufs {
 };
 
 
+// The emitter triangle tables follow the light table in the order their meshes first appear
+// among the emitting lights (lightsample.cpp:89-178 as the host packs it). False when the words
+// do not have that layout.
+bool parse_emitter_tables(const pt_scene_blob& b, std::vector<EmitterTable>& out)
+{
+    out.clear();
+    if (b.light_word_count < uint64_t(b.light_count) * PT_LIGHT_BIN_WORDS)
+        return false;
+    uint64_t ofs = uint64_t(b.light_count) * PT_LIGHT_BIN_WORDS;
+    for (uint32_t i = 0; i < b.light_count; i++) {
+        const uint32_t* w = b.light_words + size_t(i) * PT_LIGHT_BIN_WORDS;
+        if (w[3] != 3u)  // not an emitting object
+            continue;
+        const uint32_t inst = w[6];
+        if (inst >= b.instance_count)
+            return false;
+        const uint32_t mesh = b.instance_mesh[inst];
+        if (mesh >= b.mesh_count || w[5] != b.meshes[mesh].tri_count)
+            return false;
+        bool seen = false;
+        for (const EmitterTable& t : out)
+            seen |= t.mesh == mesh;
+        if (!seen) {
+            out.push_back(EmitterTable{mesh, uint32_t(ofs), b.meshes[mesh].tri_count});
+            ofs += uint64_t(b.meshes[mesh].tri_count) * PT_EMITTER_BIN_WORDS;
+        }
+    }
+    return ofs == b.light_word_count;
+}
+
 // (Re)builds the BLAS+TLAS with the context's leaf size and checks the hierarchy.
 int build_scene_bvh(Ctx& c)
 {
@@ -468,6 +498,9 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              c.instance_count = scene->instance_count;
 
              c.meshes.assign(scene->meshes, scene->meshes + scene->mesh_count);
+             c.light_count = scene->light_count;
+             c.light_word_count = scene->light_word_count;
+             c.light_layout_known = parse_emitter_tables(*scene, c.emitter_tables);
              if (const char* env = getenv("PT_BVH_LEAF"))
                  c.leaf_size = uint32_t(std::max(1, std::min(16, atoi(env))));
              if (int v = build_scene_bvh(c))
@@ -1276,6 +1309,33 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         r.ns_per_ray_best = best_ns;
         *out = r;
     }
+    return PT_OK;
+}
+
+int pt_build_light_tables(pt_ctx* h)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!c.light_layout_known)
+        return fail(&c, PT_ESTATE, "the light-sampler words do not have the packed layout");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    HIP_TRY(&c, build_light_tables(c), "light tables");
+    c.sample_index = 0;
+    c.stats_fresh = true;
+    return PT_OK;
+}
+
+int pt_read_light_words(pt_ctx* h, uint32_t* words, size_t count)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!words || count != c.light_word_count)
+        return fail(&c, PT_EINVAL, "the buffer must hold the scene's light_word_count words");
+    if (count)
+        HIP_TRY(&c, hipMemcpyAsync(words, c.d_light, count * 4, hipMemcpyDeviceToHost, c.stream), "readback");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "readback");
     return PT_OK;
 }
 

@@ -170,6 +170,11 @@ struct KernelTimer {
     }
 };
 
+// An emissive mesh's triangle alias table inside the light-sampler words.
+struct EmitterTable {
+    uint32_t mesh, word_offset, tris;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -218,6 +223,10 @@ struct Ctx {
     uint32_t leaf_size = 4;
     uint32_t job_paths = 0;  // capacity of the per-job radiance buffer
     std::vector<pt_mesh_desc> meshes;  // host copy (BVH rebuilds)
+    uint32_t light_count = 0;
+    uint64_t light_word_count = 0;
+    std::vector<EmitterTable> emitter_tables;  // in word order
+    bool light_layout_known = false;           // the words follow the packing of the host build
 
     // stats of the last render
     unsigned long long last_ext = 0, last_shadow = 0, last_paths = 0;
@@ -262,6 +271,9 @@ hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits);
 hipError_t trace_rays(Ctx& ctx, const pt_ray* d_rays, uint32_t count, pt_hit* d_hits, bool any_hit);
 
 hipError_t upload_sobol_table();
+
+// Rebuilds the light sampler's alias tables on the device (lights.hip).
+hipError_t build_light_tables(Ctx& c);
 
 // Radiance cache (cache.hip)
 int mlp_param_count();

@@ -85,3 +85,17 @@ def test_autotune_picks_a_configuration(inst):
     same = np.all(t.read() == ref, axis=2)
     assert same.mean() >= (1.0 if r["leaf_size"] == 4 else 0.999)
     t.close()
+
+
+@pytest.mark.parametrize("kind", ["box", "inst"])
+def test_device_light_tables_equal_host_packing(kind, box_scn, scene_dir):
+    from pt_amd.scenes import generate
+    scn = box_scn if kind == "box" else generate.cached_scene("instanced", root=scene_dir)
+    scene = pt_amd.Scene(scn)
+    host = scene.arrays()["light_words"].view(np.uint32).copy()
+    t = pt_amd.Tracer(scene, 32, 32, max_bounces=4)
+    np.testing.assert_array_equal(t.light_words(), host)
+    t.build_light_tables()  # clears p, q, alias and the triangle payloads, then rebuilds them
+    dev = t.light_words()
+    np.testing.assert_array_equal(dev, host)
+    t.close()
