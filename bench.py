@@ -5,11 +5,11 @@ Workload (the real-shape configuration): the instanced scene - 300 instances of 
 meshes (icospheres L2/L4/L6, a 512x512 noise terrain, a 200k-triangle soup; ~2.95M
 triangles, per-instance materials, 20 emissive spheres) - at 1920x1080, max depth 8, fp32.
 
-A step is one batch of 8 samples per pixel over the whole 1920x1080 frame (pt_render(8):
-8 sample passes, each traced exactly as the reference's 1-spp submits,
-src/pathtrace/ptdispatch.cpp:101); the default 8 steps are the 64-spp frame. Inputs are
-resident in HBM before the timed region (scene, BVH, camera); the timed region is K
-back-to-back pt_render(8) calls bracketed by stream synchronisation.
+A step is one 64-spp frame at 1920x1080 (pt_render(64): 64 sample passes, each traced
+exactly as the reference's 1-spp submits, src/pathtrace/ptdispatch.cpp:101, as the
+reference's headless `-s 64` run renders it). Inputs are resident in HBM before the timed
+region (scene, BVH, camera); the timed region is K back-to-back pt_render(64) calls
+bracketed by stream synchronisation.
 
 Other workloads (--workload views | scenes): 64 views of the box scene, or 16 instanced
 scenes of 10k-20M triangles with 4 views each, traced in one persistent wavefront run
@@ -169,6 +169,8 @@ def run_instanced(a):
                tune["ns_per_ray_start"], tune["ns_per_ray_best"]))
         st0 = tr.stats()
 
+    tr.set_option(pt_amd.PT_OPT_SORT_HITS, a.sort_hits)
+
     # warm-up
     for _ in range(max(1, a.warmup)):
         tr.render(a.spp_per_step)
@@ -256,6 +258,7 @@ def run_instanced(a):
         "stages_ms": {k: round(v["ms"], 3) for k, v in stages.items()},
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
         "autotune": tune,
+        "sort_hits": bool(a.sort_hits),
         "roofline": roofline,
         "convergence": conv,
         "cpu_baseline": cpu,
@@ -391,14 +394,15 @@ def run_scenes(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--spp-per-step", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--spp-per-step", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["instanced", "views", "scenes"], default="instanced")
     ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     a = ap.parse_args()
     if a.gpus != 1:

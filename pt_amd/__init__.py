@@ -90,6 +90,7 @@ PT_OPT_BLOCKS_PER_CU = 4
 PT_OPT_POOL_SLOTS = 5
 PT_OPT_LDS_STACK = 6
 PT_OPT_LEAF_SIZE = 7
+PT_OPT_SORT_HITS = 8
 
 
 class TuneResult(C.Structure):

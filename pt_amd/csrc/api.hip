@@ -356,6 +356,7 @@ struct pt_batch {
     bool count_fetches = false;
     bool use_graphs = true;
     uint32_t lds_stack = LDS_STACK;
+    bool sort_hits = false;
     KernelTimer timer;
     IterGraphs graphs;
     bool stats_fresh = true;
@@ -764,6 +765,10 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
         c.stats_fresh = true;
         return PT_OK;
     }
+    case PT_OPT_SORT_HITS:
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.sort_hits = value != 0;
+        return PT_OK;
     case PT_OPT_LDS_STACK:
         if (value != 8 && value != 16 && value != 32)
             return fail(&c, PT_EINVAL, "LDS stack depth must be 8, 16 or 32");
@@ -1096,7 +1101,7 @@ int pt_batch_render(pt_batch* b)
     RenderIO io{b->stream,   &b->wf,   b->grid_blocks, b->count_fetches,
                 &b->timer,   b->scenes[0]->c.scene,    b->d_scenes,    b->d_views,
                 b->d_job,    b->h_job, b->use_graphs ? &b->graphs : nullptr,
-                nullptr,     b->lds_stack};
+                nullptr,     b->lds_stack, b->sort_hits};
     BHIP_TRY(b, render_views(io, work), "render");
     BHIP_TRY(b, hipEventRecord(b->ev_stop, b->stream), "event");
     for (uint32_t i = 0; i < b->views.size(); i++)
@@ -1184,6 +1189,16 @@ int pt_batch_set_option(pt_batch* b, uint32_t option, int64_t value)
     case PT_OPT_GRAPHS:
         BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
         b->use_graphs = value != 0;
+        return PT_OK;
+    case PT_OPT_SORT_HITS:
+        BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+        b->sort_hits = value != 0;
+        return PT_OK;
+    case PT_OPT_LDS_STACK:
+        if (value != 8 && value != 16 && value != 32)
+            return bfail(b, PT_EINVAL, "LDS stack depth must be 8, 16 or 32");
+        BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+        b->lds_stack = uint32_t(value);
         return PT_OK;
     default:
         return bfail(b, PT_EINVAL, "unknown option " + std::to_string(option));

@@ -51,6 +51,11 @@ struct Wavefront {
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
     unsigned long long* totals = nullptr;
     uint32_t* spill = nullptr;     // traversal stack spill, SPILL_STACK words per grid thread
+    // material sort of the extension hits (allocated on first use)
+    uint32_t* sort_keys[2] = {nullptr, nullptr};
+    uint32_t* sort_vals[2] = {nullptr, nullptr};
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
     uint32_t spill_threads = 0;
     uint32_t* host_counts = nullptr;  // pinned: queue counts read back with a one-iteration lag
     hipEvent_t count_ready[2] = {nullptr, nullptr};
@@ -221,6 +226,7 @@ struct Ctx {
     uint32_t pool_slots = 0;
     uint32_t lds_stack = LDS_STACK;
     uint32_t leaf_size = 4;
+    bool sort_hits = false;
     uint32_t job_paths = 0;  // capacity of the per-job radiance buffer
     std::vector<pt_mesh_desc> meshes;  // host copy (BVH rebuilds)
     uint32_t light_count = 0;
@@ -251,6 +257,7 @@ struct RenderIO {
     IterGraphs* graphs;             // nullptr: launch every kernel directly
     CacheState* cache;              // nullptr: no radiance cache
     uint32_t lds_stack;             // LDS stack entries per lane of the traversal kernels
+    bool sort_hits;                 // shade the extension hits in (scene, material) order
 };
 
 // Samples [sample_index, sample_index + spp) of one view.

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "device.h"
 
 namespace ptk {
@@ -474,7 +476,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const De
 // paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
-                                               const JobDev* job, Wavefront wf, int q, CacheDev cd)
+                                               const JobDev* job, Wavefront wf, int q, CacheDev cd,
+                                               const uint32_t* order)
 {
     const uint32_t count = wf.counters[q];
     const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
@@ -492,15 +495,16 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
         uint32_t segment = 0;
         uint32_t sid = 0;
         if (valid) {
-            slot = wf.q_slot[q][i];
+            const uint32_t qi = order ? order[i] : i;  // material-sorted order when given
+            slot = wf.q_slot[q][qi];
             const ViewDev& vw = views[wf.view_of[slot]];
             sid = vw.scene;
             const DeviceScene& s = MULTI ? scenes[sid] : s0;
             const uint32_t pix = wf.pix_of[slot], sample = wf.sample_of[slot];
             segment = wf.seg[slot];
-            const float4 ro4 = wf.q_o[q][i], rd4 = wf.q_d[q][i];
+            const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
             const float3 ro = xyz(ro4), rd = xyz(rd4);
-            const float4 hh = wf.hit[i];
+            const float4 hh = wf.hit[qi];
             const uint32_t hinst = float_as_u32(hh.w);
             float4 tp = wf.thr_pdf[slot];
             float3 T = xyz(tp);
@@ -517,7 +521,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                 L = L + T * s.envcol;  // pathtrace.ms.hlsl:7-11
                 terminated = true;
             } else {
-                const uint32_t prim = wf.hit_prim[i];
+                const uint32_t prim = wf.hit_prim[qi];
                 const float t = hh.x, bu = hh.y, bv = hh.z;
                 const InstanceData& in = s.instances[hinst];
                 const MaterialRec mat = s.materials[in.material_index];
@@ -744,6 +748,32 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
     }
 }
 
+// Sort keys of the extension hits: (scene, material) of the hit instance; misses after every
+// material, entries past the queue's end last. (A counting sort by material with one global
+// atomic per distinct key and wave was tried instead of the radix sort: the few hot buckets
+// serialise the atomics and it cost more than the shading it saved.)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_hit_keys(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q,
+                                                  uint32_t n)
+{
+    const uint32_t count = wf.counters[q];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (i < count) {
+            const uint32_t inst = float_as_u32(wf.hit[i].w);
+            if (inst == NONE) {
+                key = 0xFFFFFFFEu;
+            } else {
+                const uint32_t sid = MULTI ? wf.q_scene[q][i] : 0u;
+                const DeviceScene& s = MULTI ? scenes[sid] : s0;
+                key = (sid << 20) | min(s.instances[inst].material_index, 0xFFFFFu);
+            }
+        }
+        wf.sort_keys[0][i] = key;
+        wf.sort_vals[0][i] = i;
+    }
+}
+
 // Stores the radiance of every ended path and refills its slot with the job's next path,
 // whose camera ray joins queue 1-q.
 __global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const JobDev* job, Wavefront wf, int q,
@@ -957,7 +987,8 @@ void wavefront_free(Wavefront& wf)
     void* ptrs[] = {wf.path_id, wf.seg,   wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
                     wf.q_o[1],  wf.q_d[0], wf.q_d[1], wf.hit,      wf.hit_prim,  wf.s_slot,    wf.s_o,
                     wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill,
-                    wf.view_of, wf.pix_of, wf.sample_of, wf.s_scene, wf.q_scene[0], wf.q_scene[1]};
+                    wf.view_of, wf.pix_of, wf.sample_of, wf.s_scene, wf.q_scene[0], wf.q_scene[1],
+                    wf.sort_keys[0], wf.sort_keys[1], wf.sort_vals[0], wf.sort_vals[1], wf.sort_temp};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1015,14 +1046,30 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
     hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
     if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
         return e;
+    const uint32_t* order = nullptr;
+    if (io.sort_hits) {
+        // shade in (scene, material) order: radix sort of the hit keys over the whole pool
+        if (multi)
+            hipLaunchKernelGGL(k_hit_keys<true>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, wf, q, wf.capacity);
+        else
+            hipLaunchKernelGGL(k_hit_keys<false>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, wf, q,
+                               wf.capacity);
+        hipcub::DoubleBuffer<uint32_t> dk(wf.sort_keys[0], wf.sort_keys[1]);
+        hipcub::DoubleBuffer<uint32_t> dv(wf.sort_vals[0], wf.sort_vals[1]);
+        size_t bytes = wf.sort_temp_bytes;
+        if ((e = hipcub::DeviceRadixSort::SortPairs(wf.sort_temp, bytes, dk, dv, int(wf.capacity), 0, 32, st)) !=
+            hipSuccess)
+            return e;
+        order = dv.Current();
+    }
     if ((e = timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
         return e;
     if (multi)
         hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
-                           wf, q, cd);
+                           wf, q, cd, order);
     else
         hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
-                           wf, q, cd);
+                           wf, q, cd, order);
     if ((e = timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
@@ -1051,6 +1098,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
     mix(io.count_fetches);
     mix(io.lds_stack);
+    mix(io.sort_hits);
     mix(reinterpret_cast<uintptr_t>(io.scene.nodes));
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
     mix(reinterpret_cast<uintptr_t>(io.d_views));
@@ -1150,11 +1198,35 @@ std::vector<JobPlan> plan_jobs(const std::vector<ViewWork>& work, uint32_t cap, 
 
 }  // namespace
 
+hipError_t wavefront_sort_buffers(Wavefront& wf)
+{
+    if (wf.sort_temp)
+        return hipSuccess;
+    const size_t c = std::max<uint32_t>(wf.capacity, 1);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = hipMalloc(&wf.sort_keys[k], c * 4);
+        if (e == hipSuccess)
+            e = hipMalloc(&wf.sort_vals[k], c * 4);
+    }
+    if (e != hipSuccess)
+        return e;
+    hipcub::DoubleBuffer<uint32_t> dk(wf.sort_keys[0], wf.sort_keys[1]);
+    hipcub::DoubleBuffer<uint32_t> dv(wf.sort_vals[0], wf.sort_vals[1]);
+    size_t bytes = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, dk, dv, int(wf.capacity), 0, 32)) != hipSuccess)
+        return e;
+    wf.sort_temp_bytes = bytes;
+    return hipMalloc(&wf.sort_temp, std::max<size_t>(bytes, 16));
+}
+
 hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
 {
     Wavefront& wf = *io.wf;
     hipStream_t st = io.st;
     hipError_t e;
+    if (io.sort_hits && (e = wavefront_sort_buffers(wf)) != hipSuccess)
+        return e;
     CacheState* cs = io.cache && io.cache->configured ? io.cache : nullptr;
     const bool train_between = cs && cs->train_steps > 0 && cs->frame_spp > 0;
     const bool train_inflight = cs && cs->train_steps > 0 && cs->frame_spp == 0;
@@ -1193,7 +1265,8 @@ hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
                            dim3(256), 0, st, io.d_views, io.d_job, wf, cd);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        const bool graphs = io.graphs && !io.timer->enabled && !train_inflight;
+        // (the library radix sort is launched directly: it may record memsets)
+        const bool graphs = io.graphs && !io.timer->enabled && !train_inflight && !io.sort_hits;
         int q = 0;
         // Iterate until the queue an iteration leaves behind is empty. The count of iteration
         // i is read back (pinned memory, event) while iteration i+1 is already queued.
@@ -1257,7 +1330,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
     RenderIO io{ctx.stream,       &ctx.wf,     ctx.grid_blocks, ctx.count_fetches,
                 &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
                 ctx.d_job,        ctx.h_job,   ctx.use_graphs ? &ctx.graphs : nullptr,
-                &ctx.cache,       ctx.lds_stack};
+                &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
     return render_views(io, {ViewWork{0, v.npix, ctx.sample_index, spp}});
 }
 

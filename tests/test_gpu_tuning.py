@@ -99,3 +99,9 @@ def test_device_light_tables_equal_host_packing(kind, box_scn, scene_dir):
     dev = t.light_words()
     np.testing.assert_array_equal(dev, host)
     t.close()
+
+
+def test_material_sorted_shading_leaves_the_image_unchanged(inst):
+    scene, cam, ref = inst
+    img, st = _render(scene, cam, [(pt_amd.PT_OPT_SORT_HITS, 1)])
+    np.testing.assert_array_equal(img, ref)
