@@ -149,6 +149,8 @@ def host_lib():
         lib.pth_to_srgb.argtypes = [C.c_float]
         lib.pth_to_srgb.restype = C.c_float
         lib.pth_free.argtypes = [C.c_void_p]
+        lib.pth_decode_image.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32), C.POINTER(C.POINTER(C.c_uint8))]
         lib.pth_alias_table.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.pth_last_error.restype = C.c_char_p
         _host = lib

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "hdrio.h"
+#include "png.h"
 #include "pthost.h"
 #include "scene.h"
 
@@ -150,6 +151,26 @@ int pth_read_hdr(const char* path, float** out, uint32_t* width, uint32_t* heigh
             throw std::bad_alloc();
         memcpy(buf, img.data(), img.size() * sizeof(float));
         *out = buf;
+    });
+}
+
+int pth_decode_image(const uint8_t* data, size_t size, uint32_t* width, uint32_t* height, uint32_t* channels,
+                     uint8_t** px)
+{
+    if (!data || !width || !height || !channels || !px) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] {
+        pth::Image img = pth::load_texture_image(data, size);
+        uint8_t* buf = (uint8_t*)std::malloc(std::max<size_t>(img.px.size(), 1));
+        if (!buf)
+            throw std::bad_alloc();
+        memcpy(buf, img.px.data(), img.px.size());
+        *width = img.width;
+        *height = img.height;
+        *channels = img.channels;
+        *px = buf;
     });
 }
 
