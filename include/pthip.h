@@ -50,6 +50,19 @@ typedef struct pt_mesh_desc {
     uint32_t vertex_count;
 } pt_mesh_desc;
 
+/* One texture of the packed scene (SceneTextures, src/scene/texturedata.cpp:30-46): 8-bit
+ * channels at texels + offset, rows top to bottom; channels 1 (R8), 2 (R8G8) or 4 (RGBA8);
+ * srgb != 0 for base-colour and emission maps with four channels (VK_FORMAT_R8G8B8A8_SRGB),
+ * decoded to linear per texel before filtering. Sampled bilinearly with REPEAT addressing at
+ * mip level 0 (src/graphics/sampler.cpp:8-22). */
+typedef struct pt_texture_desc {
+    uint64_t offset;
+    uint32_t width;
+    uint32_t height;
+    uint32_t channels;
+    uint32_t srgb;
+} pt_texture_desc;
+
 /* The packed scene, exactly the regions of the reference's SceneBuffer
  * (src/scene/scenebuffer.cpp:65-168) plus the instance->mesh map the TLAS needs
  * (src/pathtrace/accstruct.cpp:197-216). Host memory; only read during pt_create. */
@@ -69,6 +82,11 @@ typedef struct pt_scene_blob {
     const pt_mesh_desc* meshes;
     const uint32_t* instance_mesh; /* instance_count entries: mesh (BLAS) of each instance */
     float envcol[4];               /* linear environment colour (UBO environment_colour) */
+    const pt_texture_desc* textures; /* texture_count entries (the materials' map indices) */
+    uint32_t texture_count;
+    uint32_t reserved;
+    const uint8_t* texels;
+    uint64_t texel_bytes;
 } pt_scene_blob;
 
 /* Camera block: the same content PathTracer::set_camera writes into the uniform block

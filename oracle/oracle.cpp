@@ -550,8 +550,84 @@ struct oracle_scene {
     std::vector<uint32_t> instance_mesh;
     f3 envcol{0, 0, 0};
 
+    std::vector<pt_texture_desc> tex;
+    std::vector<uint8_t> texels;
+    float srgb_lut[256];
+
     std::vector<MeshAccel> mesh_accel;
     std::vector<double> inst_inv;  // 12 doubles per instance: world->object 3x4 (row-major)
+
+    // --- textures (material.hlsli:23-106): Texture2D.SampleLevel(sampler, uv, 0) with the
+    // reference's sampler (src/graphics/sampler.cpp:8-22): bilinear, REPEAT, mip 0. Texel
+    // centres at (i + 0.5) / size; sRGB textures decode each texel before filtering.
+    static uint32_t bits_of(float f)
+    {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        return u;
+    }
+    f4 texel(const pt_texture_desc& d, uint32_t x, uint32_t y) const
+    {
+        const uint8_t* p = texels.data() + d.offset + (size_t(y) * d.width + x) * d.channels;
+        const float k = 1.0f / 255.0f;
+        if (d.channels == 4) {
+            if (d.srgb)
+                return {srgb_lut[p[0]], srgb_lut[p[1]], srgb_lut[p[2]], float(p[3]) * k};
+            return {float(p[0]) * k, float(p[1]) * k, float(p[2]) * k, float(p[3]) * k};
+        }
+        if (d.channels == 2)
+            return {float(p[0]) / 255.0f, float(p[1]) / 255.0f, 0.0f, 1.0f};
+        return {float(p[0]) / 255.0f, 0.0f, 0.0f, 1.0f};
+    }
+    f4 sample_tex(uint32_t t, f2 uv) const
+    {
+        const pt_texture_desc& d = tex.at(t);
+        const float u = uv.x - std::floor(uv.x), v = uv.y - std::floor(uv.y);
+        const float x = u * float(d.width) - 0.5f, y = v * float(d.height) - 0.5f;
+        const float fx = std::floor(x), fy = std::floor(y);
+        const float ax = x - fx, ay = y - fy;
+        const int64_t ix = int64_t(fx), iy = int64_t(fy);
+        const uint32_t x0 = uint32_t((ix % int64_t(d.width) + d.width) % d.width);
+        const uint32_t y0 = uint32_t((iy % int64_t(d.height) + d.height) % d.height);
+        const uint32_t x1 = (x0 + 1) % d.width, y1 = (y0 + 1) % d.height;
+        const f4 a = texel(d, x0, y0), b = texel(d, x1, y0), c = texel(d, x0, y1), e = texel(d, x1, y1);
+        const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay), w01 = (1.0f - ax) * ay, w11 = ax * ay;
+        return {w00 * a.x + w10 * b.x + w01 * c.x + w11 * e.x, w00 * a.y + w10 * b.y + w01 * c.y + w11 * e.y,
+                w00 * a.z + w10 * b.z + w01 * c.z + w11 * e.z, w00 * a.w + w10 * b.w + w01 * c.w + w11 * e.w};
+    }
+    f4 base_colour_at(const Material& m, f2 uv) const  // get_base_colour
+    {
+        if (bits_of(m.rmnb.w) & 1u)
+            return sample_tex(bits_of(m.base_colour.x), uv);
+        return m.base_colour;
+    }
+    f3 emission_at(const Material& m, f2 uv) const  // get_emission
+    {
+        if (bits_of(m.rmnb.w) & 2u) {
+            const f4 t = sample_tex(bits_of(m.emission.x), uv);
+            return mk(t.x, t.y, t.z) * m.emission.w;
+        }
+        return mk(m.emission.x, m.emission.y, m.emission.z) * m.emission.w;
+    }
+    // get_roughness_metalness_normal
+    void rmn_at(const Material& m, f2 uv, float& roughness, float& metalness, f3& normal) const
+    {
+        const uint32_t b = bits_of(m.rmnb.w);
+        if (b & (1u << 4)) {
+            const f4 t = sample_tex(bits_of(m.rmnb.x), uv);
+            roughness = t.y;
+            metalness = t.z;
+        } else {
+            roughness = (b & (1u << 2)) ? sample_tex(bits_of(m.rmnb.x), uv).x : m.rmnb.x;
+            metalness = (b & (1u << 3)) ? sample_tex(bits_of(m.rmnb.y), uv).x : m.rmnb.y;
+        }
+        if (b & (1u << 12)) {
+            const f4 t = sample_tex(bits_of(m.rmnb.z), uv);
+            normal = normalize(mk(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f));
+        } else {
+            normal = mk(0.0f, 0.0f, 1.0f);
+        }
+    }
     Bvh top;
     std::vector<AABBd> inst_box;
 
@@ -692,8 +768,16 @@ struct oracle_scene {
                         if (!(t >= double(tmin) && t <= best))
                             continue;
                         if (any && transparent && trans) {
-                            // shadowray.ah.hlsl:12-40: T *= 1 - alpha, accept when T < 1e-5.
-                            *trans *= (1.0f - mat.base_colour.w);
+                            // shadowray.ah.hlsl:12-40: T *= 1 - alpha (of the base colour at the
+                            // hit), accept when T < 1e-5.
+                            const uint32_t ix = instances[ii].index_index + p * 3;
+                            const Vtx& a0 = vertices[instances[ii].vertex_index + indices[ix]];
+                            const Vtx& a1 = vertices[instances[ii].vertex_index + indices[ix + 1]];
+                            const Vtx& a2 = vertices[instances[ii].vertex_index + indices[ix + 2]];
+                            const float fb1 = float(b1), fb2 = float(b2), fwa = 1.0f - fb1 - fb2;
+                            const f2 huv = {fwa * a0.uv.x + fb1 * a1.uv.x + fb2 * a2.uv.x,
+                                            fwa * a0.uv.y + fb1 * a1.uv.y + fb2 * a2.uv.y};
+                            *trans *= (1.0f - base_colour_at(mat, huv).w);
                             if (*trans < 1e-5f) {
                                 hit = {t, float(b1), float(b2), ii, p};
                                 return true;
@@ -804,7 +888,9 @@ f3 sample_light(const oracle_scene& s, f3 u, f3 pt, f3& light_dir, float& light_
         is_delta = false;
         if (!std::isfinite(pdf))
             return mk(0, 0, 0);
-        return emission_of(s.materials[in.material_index]);
+        const f2 luv = {bary.x * va.uv.x + bary.y * vb.uv.x + bary.z * vc.uv.x,
+                        bary.x * va.uv.y + bary.y * vb.uv.y + bary.z * vc.uv.y};
+        return s.emission_at(s.materials[in.material_index], luv);
     } else if (type == 1) {
         const f3 lp = mk(as_float(s.light[bo + 1]), as_float(s.light[bo + 2]), as_float(s.light[bo + 3]));
         light_dir = lp - pt;
@@ -912,7 +998,7 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     const f3 obj_n = normalize(alpha_b * va.nrm + h.u * vb.nrm + h.v * vc.nrm);
     const f2 uv = {alpha_b * va.uv.x + h.u * vb.uv.x + h.v * vc.uv.x, alpha_b * va.uv.y + h.u * vb.uv.y + h.v * vc.uv.y};
 
-    const f3 emission = emission_of(mat);
+    const f3 emission = s.emission_at(mat, uv);
     // ObjectRayDirection() = world->object linear part applied to the world direction.
     const double* iv = &s.inst_inv[size_t(h.inst) * 12];
     const f3 obj_dir = {float(iv[0] * rd.x + iv[1] * rd.y + iv[2] * rd.z), float(iv[4] * rd.x + iv[5] * rd.y + iv[6] * rd.z),
@@ -938,7 +1024,7 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     if (pl.final_segment)
         return;
 
-    const f4 base = mat.base_colour;
+    const f4 base = s.base_colour_at(mat, uv);
     if (base.w < pl.smp.s1()) {
         pl.next_origin = ro + rd * (t + ORIGIN_OFFSET);
         pl.next_dir = rd;
@@ -946,8 +1032,9 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     }
     BrdfInput bi;
     bi.base_colour = mk(base.x, base.y, base.z);
-    float roughness = mat.rmnb.x;
-    bi.metalness = mat.rmnb.y;
+    float roughness;
+    f3 mapped_n;
+    s.rmn_at(mat, uv, roughness, bi.metalness, mapped_n);
     bi.specular = mk(mat.specular.x, mat.specular.y, mat.specular.z);
     bi.clearcoat = mat.cc.x;
     const float cc_rough = mat.cc.y;
@@ -986,7 +1073,22 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     f3 twn = normalize(xform_normal(in, true_n));
     if (dot(rd, twn) > 0.0f)
         twn = -twn;
-    const f3 fw[3] = {wx, wy, wn};  // rows of from_world_space; mapped normal is (0,0,1)
+    if (mapped_n.z < 0.999f) {
+        // from_world_space = float3x3(mx, my, mn) * float3x3(wx, wy, wn) (ch.hlsl:321-326), with
+        // the orthonormal basis of Duff et al. 2017 (util.hlsli:57-64)
+        const float sg = mapped_n.z < 0.0f ? -1.0f : 1.0f;
+        const float a = -1.0f / (sg + mapped_n.z);
+        const float b = mapped_n.x * mapped_n.y * a;
+        const f3 mx = mk(1.0f + sg * mapped_n.x * mapped_n.x * a, sg * b, -sg * mapped_n.x);
+        const f3 my = mk(b, sg + mapped_n.y * mapped_n.y * a, -mapped_n.y);
+        const f3 fx = mx.x * wx + mx.y * wy + mx.z * wn;
+        const f3 fy = my.x * wx + my.y * wy + my.z * wn;
+        const f3 fn = mapped_n.x * wx + mapped_n.y * wy + mapped_n.z * wn;
+        wx = fx;
+        wy = fy;
+        wn = fn;
+    }
+    const f3 fw[3] = {wx, wy, wn};  // rows of from_world_space
     const f3 nrd = -rd;
     const f3 o = normalize(mk(dot(wx, nrd), dot(wy, nrd), dot(wn, nrd)));
     if (o.z <= 0.0f) {
@@ -1087,6 +1189,14 @@ oracle_scene* make_scene(const pt_scene_blob* b)
     s->meshes.assign(b->meshes, b->meshes + b->mesh_count);
     s->instance_mesh.assign(b->instance_mesh, b->instance_mesh + b->instance_count);
     s->envcol = {b->envcol[0], b->envcol[1], b->envcol[2]};
+    if (b->texture_count)
+        s->tex.assign(b->textures, b->textures + b->texture_count);
+    if (b->texel_bytes)
+        s->texels.assign(b->texels, b->texels + b->texel_bytes);
+    for (int i = 0; i < 256; i++) {  // the sRGB transfer function VK_FORMAT_R8G8B8A8_SRGB decodes with
+        const double c = i / 255.0;
+        s->srgb_lut[i] = float(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
     s->build();
     return s.release();
 }

@@ -97,6 +97,19 @@ int main(int argc, char** argv)
     printf("],\n");
 
     // --- aliastable.h ----------------------------------------------------------------
+    {
+        // The seeding expression of calculate_object_intensity (src/scene/lightsample.cpp:43-46),
+        // compiled as the reference compiles it (g++): the order in which the four
+        // seeder.next() arguments are evaluated decides the generator's state.
+        Splitmix32 seeder(0xbd7a0d48u);
+        Xshiro128 rng(seeder.next(), seeder.next(), seeder.next(), seeder.next());
+        std::vector<float> fl;
+        for (int i = 0; i < 8; i++)
+            fl.push_back(rng.next_float());
+        printf("\"intensity_rng_float_bits\": ");
+        print_floats(fl.data(), fl.size());
+        printf(",\n");
+    }
     printf("\"alias\": [\n");
     std::vector<std::vector<float>> weight_sets = {
         { 1.0f },

@@ -34,13 +34,20 @@ class MeshDesc(C.Structure):
                 ("vertex_count", C.c_uint32)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
+                ("srgb", C.c_uint32)]
+
+
 class SceneBlob(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.POINTER(C.c_uint32)),
                 ("index_count", C.c_uint64), ("instances", C.c_void_p), ("instance_count", C.c_uint32),
                 ("materials", C.c_void_p), ("material_count", C.c_uint32),
                 ("light_words", C.POINTER(C.c_uint32)), ("light_word_count", C.c_uint64),
                 ("light_count", C.c_uint32), ("mesh_count", C.c_uint32), ("meshes", C.POINTER(MeshDesc)),
-                ("instance_mesh", C.POINTER(C.c_uint32)), ("envcol", C.c_float * 4)]
+                ("instance_mesh", C.POINTER(C.c_uint32)), ("envcol", C.c_float * 4),
+                ("textures", C.POINTER(TextureDesc)), ("texture_count", C.c_uint32), ("reserved", C.c_uint32),
+                ("texels", C.POINTER(C.c_uint8)), ("texel_bytes", C.c_uint64)]
 
 
 class Camera(C.Structure):

@@ -57,7 +57,7 @@ hipError_t upload(T*& dst, const void* src, size_t bytes, hipStream_t st)
 void release(Ctx& c)
 {
     void* ptrs[] = {c.d_vertices, c.d_indices, c.d_instances, c.d_materials, c.d_light,
-                    c.d_meshes, c.d_instance_mesh, c.d_inst_opacity, c.accum};
+                    c.d_meshes, c.d_instance_mesh, c.d_inst_opacity, c.accum, c.d_tex, c.d_texels};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -137,6 +137,28 @@ int validate_blob(const pt_scene_blob* b, std::string& why)
     for (uint32_t i = 0; i < b->instance_count; i++)
         if (inst[i].emitter_index != PT_NO_EMITTER && uint64_t(inst[i].emitter_index) >= b->light_count)
             return (why = "instance " + std::to_string(i) + " has an emitter index beyond the light table", PT_EINVAL);
+    // textures: descriptors inside the texel bytes, 4-byte aligned RGBA rows
+    if ((b->texture_count && !b->textures) || (b->texel_bytes && !b->texels))
+        return (why = "texture array pointer is null", PT_EINVAL);
+    for (uint32_t t = 0; t < b->texture_count; t++) {
+        const pt_texture_desc& d = b->textures[t];
+        const uint64_t bytes = uint64_t(d.width) * d.height * d.channels;
+        if (d.width == 0 || d.height == 0 || (d.channels != 1 && d.channels != 2 && d.channels != 4) ||
+            d.offset % 4 != 0 || d.offset + bytes > b->texel_bytes || (d.srgb && d.channels != 4))
+            return (why = "texture " + std::to_string(t) + " is malformed", PT_EINVAL);
+    }
+    // every map a material names exists
+    const auto* mats = static_cast<const MaterialRec*>(b->materials);
+    for (uint32_t m = 0; m < b->material_count; m++) {
+        const uint32_t bits = float_as_u32(mats[m].rmnb.w);
+        const float* slots[] = {&mats[m].base_colour.x, &mats[m].emission.x, &mats[m].rmnb.x, &mats[m].rmnb.y,
+                                &mats[m].rmnb.x, &mats[m].specular.x, &mats[m].sheen.x, &mats[m].rmnb.z};
+        const uint32_t flags[] = {MAP_BASE_COLOUR, MAP_EMISSION, MAP_ROUGHNESS, MAP_METALNESS, MAP_ROUGHNESS_METALNESS,
+                                  MAP_SPECULAR, MAP_SHEEN, MAP_NORMAL};
+        for (int k = 0; k < 8; k++)
+            if ((bits & flags[k]) && float_as_u32(*slots[k]) >= b->texture_count)
+                return (why = "material " + std::to_string(m) + " names a missing texture", PT_EINVAL);
+    }
     return PT_OK;
 }
 
@@ -492,9 +514,12 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              for (uint32_t i = 0; i < scene->instance_count; i++) {
                  const MaterialRec& m = mats[inst[i].material_index];
                  const uint32_t bits = float_as_u32(m.rmnb.w);
-                 if (bits & (1u << 13))
-                     opacity[i] = m.base_colour.w;
+                 if (bits & (1u << 13))  // transparent: constant alpha, or -1 for the map's
+                     opacity[i] = (bits & MAP_BASE_COLOUR) ? -1.0f : m.base_colour.w;
              }
+             HIP_TRY(&c, upload(c.d_tex, scene->textures, size_t(scene->texture_count) * sizeof(pt_texture_desc), st),
+                     "textures");
+             HIP_TRY(&c, upload(c.d_texels, scene->texels, scene->texel_bytes, st), "textures");
              HIP_TRY(&c, upload(c.d_inst_opacity, opacity.data(), opacity.size() * 4, st), "opacity");
              c.instance_count = scene->instance_count;
 
@@ -522,6 +547,9 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              s.inst_opacity = c.d_inst_opacity;
              s.tlas_root = c.bvh.tlas_root;
              s.envcol = make_float3(scene->envcol[0], scene->envcol[1], scene->envcol[2]);
+             s.tex = c.d_tex;
+             s.texels = c.d_texels;
+             s.tex_count = scene->texture_count;
 
              c.width = width;
              c.height = height;

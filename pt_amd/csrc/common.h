@@ -64,9 +64,13 @@ struct DeviceScene {
     const TriRec* tris;
     const InstXform* inst_inv;
     const uint32_t* inst_blas_root;   // root node of each instance's BLAS (CHILD_EMPTY: none)
-    const float* inst_opacity;        // base-colour alpha of transparent instances, 1 = opaque
+    const float* inst_opacity;        // base-colour alpha of transparent instances, 1 = opaque,
+                                      // -1 = alpha from the base-colour map at the hit
     uint32_t tlas_root;               // CHILD_EMPTY when the scene has no instances
     float3 envcol;
+    const pt_texture_desc* tex;       // textures (pthip.h pt_texture_desc)
+    const uint8_t* texels;
+    uint32_t tex_count;
 };
 
 __host__ __device__ inline uint32_t float_as_u32(float f)

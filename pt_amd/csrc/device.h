@@ -195,6 +195,8 @@ struct Ctx {
     pt_mesh_desc* d_meshes = nullptr;
     uint32_t* d_instance_mesh = nullptr;
     float* d_inst_opacity = nullptr;
+    pt_texture_desc* d_tex = nullptr;
+    uint8_t* d_texels = nullptr;
     BvhBuffers bvh;
     DeviceScene scene{};
     double bvh_build_ms = 0.0;

@@ -334,6 +334,126 @@ __device__ __forceinline__ float3 emission_of(const MaterialRec& m)
     return f3(m.emission.x, m.emission.y, m.emission.z) * m.emission.w;
 }
 
+// --- textures (material.hlsli:23-106; sampler: bilinear, REPEAT, mip 0, src/graphics/sampler.cpp:8-22)
+
+extern __constant__ float c_srgb[256];  // sRGB-encoded byte -> linear (VK_FORMAT_R8G8B8A8_SRGB fetch)
+
+enum : uint32_t {
+    MAP_BASE_COLOUR = 1u << 0,
+    MAP_EMISSION = 1u << 1,
+    MAP_ROUGHNESS = 1u << 2,
+    MAP_METALNESS = 1u << 3,
+    MAP_ROUGHNESS_METALNESS = 1u << 4,
+    MAP_SPECULAR = 1u << 5,
+    MAP_SHEEN = 1u << 7,
+    MAP_NORMAL = 1u << 12,
+};
+
+__device__ __forceinline__ uint32_t map_bits(const MaterialRec& m) { return float_as_u32(m.rmnb.w); }
+
+__device__ __forceinline__ float4 texel(const DeviceScene& s, const pt_texture_desc& d, uint32_t x, uint32_t y)
+{
+    const uint8_t* p = s.texels + d.offset + (size_t(y) * d.width + x) * d.channels;
+    if (d.channels == 4) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);  // 4-byte aligned (texel offsets are)
+        const uint32_t r = w & 255u, g = (w >> 8) & 255u, b = (w >> 16) & 255u, a = w >> 24;
+        const float k = 1.0f / 255.0f;
+        if (d.srgb)
+            return make_float4(c_srgb[r], c_srgb[g], c_srgb[b], float(a) * k);
+        return make_float4(float(r) * k, float(g) * k, float(b) * k, float(a) * k);
+    }
+    if (d.channels == 2)
+        return make_float4(float(p[0]) / 255.0f, float(p[1]) / 255.0f, 0.0f, 1.0f);
+    return make_float4(float(p[0]) / 255.0f, 0.0f, 0.0f, 1.0f);
+}
+
+// Texture2D.SampleLevel(sampler, uv, 0): texel centres at (i + 0.5) / size, four texels weighted
+// bilinearly, coordinates wrapped (REPEAT).
+__device__ __forceinline__ float4 sample_texture(const DeviceScene& s, uint32_t t, float2 uv)
+{
+    const pt_texture_desc d = s.tex[t];
+    const float u = uv.x - floorf(uv.x), v = uv.y - floorf(uv.y);
+    const float x = u * float(d.width) - 0.5f, y = v * float(d.height) - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    int ix = int(fx), iy = int(fy);
+    const uint32_t x0 = ix < 0 ? d.width - 1 : min(uint32_t(ix), d.width - 1);
+    const uint32_t y0 = iy < 0 ? d.height - 1 : min(uint32_t(iy), d.height - 1);
+    const uint32_t x1 = x0 + 1 >= d.width ? 0 : x0 + 1;
+    const uint32_t y1 = y0 + 1 >= d.height ? 0 : y0 + 1;
+    const float4 t00 = texel(s, d, x0, y0), t10 = texel(s, d, x1, y0);
+    const float4 t01 = texel(s, d, x0, y1), t11 = texel(s, d, x1, y1);
+    const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay), w01 = (1.0f - ax) * ay, w11 = ax * ay;
+    return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
+                       w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+                       w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z,
+                       w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w);
+}
+
+// get_emission / get_base_colour (material.hlsli:23-41)
+__device__ __forceinline__ float3 material_emission(const DeviceScene& s, const MaterialRec& m, float2 uv)
+{
+    if (map_bits(m) & MAP_EMISSION)
+        return xyz(sample_texture(s, float_as_u32(m.emission.x), uv)) * m.emission.w;
+    return emission_of(m);
+}
+
+__device__ __forceinline__ float4 material_base_colour(const DeviceScene& s, const MaterialRec& m, float2 uv)
+{
+    if (map_bits(m) & MAP_BASE_COLOUR)
+        return sample_texture(s, float_as_u32(m.base_colour.x), uv);
+    return m.base_colour;
+}
+
+// get_roughness_metalness_normal (material.hlsli:43-70); normal is the tangent-space map normal
+// or (0, 0, 1).
+__device__ __forceinline__ void material_rmn(const DeviceScene& s, const MaterialRec& m, float2 uv, float& roughness,
+                                             float& metalness, float3& normal)
+{
+    const uint32_t b = map_bits(m);
+    if (b & MAP_ROUGHNESS_METALNESS) {
+        const float4 t = sample_texture(s, float_as_u32(m.rmnb.x), uv);
+        roughness = t.y;
+        metalness = t.z;
+    } else {
+        roughness = (b & MAP_ROUGHNESS) ? sample_texture(s, float_as_u32(m.rmnb.x), uv).x : m.rmnb.x;
+        metalness = (b & MAP_METALNESS) ? sample_texture(s, float_as_u32(m.rmnb.y), uv).x : m.rmnb.y;
+    }
+    if (b & MAP_NORMAL) {
+        const float4 t = sample_texture(s, float_as_u32(m.rmnb.z), uv);
+        normal = normalize3(f3(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f));
+    } else {
+        normal = f3(0.0f, 0.0f, 1.0f);
+    }
+}
+
+__device__ __forceinline__ float3 material_specular(const DeviceScene& s, const MaterialRec& m, float2 uv)
+{
+    return (map_bits(m) & MAP_SPECULAR) ? xyz(sample_texture(s, float_as_u32(m.specular.x), uv)) : xyz(m.specular);
+}
+
+__device__ __forceinline__ float3 material_sheen(const DeviceScene& s, const MaterialRec& m, float2 uv)
+{
+    return (map_bits(m) & MAP_SHEEN) ? xyz(sample_texture(s, float_as_u32(m.sheen.x), uv)) : xyz(m.sheen);
+}
+
+// Orthonormal basis around a unit vector (Duff et al. 2017, util.hlsli:57-64).
+__device__ __forceinline__ void onb(float3 n, float3& x, float3& y)
+{
+    const float sg = n.z < 0.0f ? -1.0f : 1.0f;
+    const float a = -1.0f / (sg + n.z);
+    const float b = n.x * n.y * a;
+    x = f3(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x);
+    y = f3(b, sg + n.y * n.y * a, -n.y);
+}
+
+// Texture coordinates of a triangle point (geometry.hlsli:39-48).
+__device__ __forceinline__ float2 vertex_uv(const DeviceScene& s, uint32_t v)
+{
+    const float4 t = s.vertices[size_t(v) * 2 + 1];
+    return make_float2(t.z, t.w);
+}
+
 __device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, float3 pt, float3& light_dir,
                                                float& light_dist, float& pdf, bool& is_delta)
 {
@@ -354,6 +474,8 @@ __device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, f
         const float3 pc = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 2]));
         const float3 bary = random_barycentric(make_float2(u.y, u.z));
         const float3 on_tri = bary.x * pa + bary.y * pb + bary.z * pc;
+        const uint32_t va = in.vertex_index + s.light[bo + 0], vb = in.vertex_index + s.light[bo + 1],
+                       vc = in.vertex_index + s.light[bo + 2];
         light_dir = on_tri - pt;
         light_dist = 1.0f;
         const float3 tn = cross3(pb - pa, pc - pa);
@@ -364,7 +486,13 @@ __device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, f
         is_delta = false;
         if (!isfinite(pdf))
             return f3(0, 0, 0);
-        return emission_of(s.materials[in.material_index]);
+        const MaterialRec& m = s.materials[in.material_index];
+        if (!(map_bits(m) & MAP_EMISSION))
+            return emission_of(m);
+        const float2 ua = vertex_uv(s, va), ub = vertex_uv(s, vb), uc = vertex_uv(s, vc);
+        const float2 uv = make_float2(bary.x * ua.x + bary.y * ub.x + bary.z * uc.x,
+                                      bary.x * ua.y + bary.y * ub.y + bary.z * uc.y);
+        return material_emission(s, m, uv);
     } else if (type == 1u) {
         const float3 lp = f3(u32_as_float(s.light[bo + 1]), u32_as_float(s.light[bo + 2]), u32_as_float(s.light[bo + 3]));
         light_dir = lp - pt;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -22,6 +23,7 @@
 namespace ptk {
 
 __constant__ uint32_t c_sobol[4][32];
+__constant__ float c_srgb[256];
 
 hipError_t upload_sobol_table()
 {
@@ -47,7 +49,16 @@ hipError_t upload_sobol_table()
         for (int k = 0; k < 32; k++)
             v[d][k] = m[k] << (31 - k);
     }
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), v, sizeof(v));
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), v, sizeof(v));
+    if (e != hipSuccess)
+        return e;
+    // sRGB transfer function of 8-bit texels (the decode VK_FORMAT_R8G8B8A8_SRGB applies)
+    float lut[256];
+    for (int i = 0; i < 256; i++) {
+        const double c = i / 255.0;
+        lut[i] = float(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut));
 }
 
 namespace {
@@ -247,7 +258,20 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                         continue;
                     if (ANY) {
                         if (opacity < 1.0f) {
-                            trans *= (1.0f - opacity);
+                            float alpha = opacity;
+                            if (alpha < 0.0f) {
+                                // base-colour map alpha at the hit (shadowray.ah.hlsl:21-33)
+                                const InstanceData& id = s.instances[inst];
+                                const uint32_t pr = float_as_u32(tr.v0.w), ix = id.index_index + pr * 3;
+                                const float2 ua = vertex_uv(s, id.vertex_index + s.indices[ix]);
+                                const float2 ub = vertex_uv(s, id.vertex_index + s.indices[ix + 1]);
+                                const float2 uc = vertex_uv(s, id.vertex_index + s.indices[ix + 2]);
+                                const float wa = 1.0f - b1 - b2;
+                                const float2 uv = make_float2(wa * ua.x + b1 * ub.x + b2 * uc.x,
+                                                              wa * ua.y + b1 * ub.y + b2 * uc.y);
+                                alpha = material_base_colour(s, s.materials[id.material_index], uv).w;
+                            }
+                            trans *= (1.0f - alpha);
                             if (trans < 1e-5f) {
                                 hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
                                 return true;
@@ -536,7 +560,8 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                 // get_vertex_attributes (geometry.hlsli:39-48)
                 const float wa = 1.0f - bu - bv;
                 const float3 obj_n = normalize3(wa * na + bu * nb + bv * nc);
-                const float3 emission = emission_of(mat);
+                const float2 uv = make_float2(wa * uva.x + bu * uvb.x + bv * uvc.x, wa * uva.y + bu * uvb.y + bv * uvc.y);
+                const float3 emission = material_emission(s, mat, uv);
                 // ObjectRayDirection(): the world->object transform applied to the direction
                 const InstXform x = s.inst_inv[hinst];
                 const float3 odir = f3(x.r0.x * rd.x + x.r0.y * rd.y + x.r0.z * rd.z, x.r1.x * rd.x + x.r1.y * rd.y + x.r1.z * rd.z,
@@ -559,19 +584,22 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                         }
                     }
                 }
+                const float4 base = final_seg ? make_float4(0, 0, 0, 1) : material_base_colour(s, mat, uv);
                 if (final_seg) {
                     terminated = true;
-                } else if (mat.base_colour.w < smp.s1()) {
+                } else if (base.w < smp.s1()) {
                     // stochastic transparency: continue straight through (ch.hlsl:279-283)
                     next_o = ro + rd * (t + ORIGIN_OFFSET);
                     next_d = rd;
                 } else {
                     BrdfInput bi;
-                    bi.base_colour = xyz(mat.base_colour);
-                    float roughness = mat.rmnb.x;
-                    bi.metalness = mat.rmnb.y;
-                    bi.specular = xyz(mat.specular);
-                    bi.sheen = xyz(mat.sheen);
+                    bi.base_colour = xyz(base);
+                    float roughness;
+                    float3 mapped_n;
+                    material_rmn(s, mat, uv, roughness, bi.metalness, mapped_n);
+                    const float rough_param = roughness;
+                    bi.specular = material_specular(s, mat, uv);
+                    bi.sheen = material_sheen(s, mat, uv);
                     const float anisotropy = mat.cc.z;
                     roughness = fmaxf(0.00001f, roughness * roughness);
                     const float aspect = sqrtf(1.0f - 0.9f * fabsf(anisotropy));
@@ -603,6 +631,18 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                     float3 twn = normalize3(xform_normal(in, true_n));
                     if (dot3(rd, twn) > 0.0f)
                         twn = -twn;
+                    if (mapped_n.z < 0.999f) {
+                        // normal map: the shading frame becomes (mx, my, mn) expressed in the
+                        // interpolated frame (ch.hlsl:321-326)
+                        float3 mx, my;
+                        onb(mapped_n, mx, my);
+                        const float3 fx = mx.x * wx + mx.y * wy + mx.z * wn;
+                        const float3 fy = my.x * wx + my.y * wy + my.z * wn;
+                        const float3 fn = mapped_n.x * wx + mapped_n.y * wy + mapped_n.z * wn;
+                        wx = fx;
+                        wy = fy;
+                        wn = fn;
+                    }
                     const float3 nrd = -rd;
                     const float3 o = normalize3(f3(dot3(wx, nrd), dot3(wy, nrd), dot3(wn, nrd)));
                     bool cached = false;
@@ -614,8 +654,8 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                         cr[0] = hp.x, cr[1] = hp.y, cr[2] = hp.z;
                         cr[3] = -rd.x, cr[4] = -rd.y, cr[5] = -rd.z;
                         cr[6] = wn.x, cr[7] = wn.y, cr[8] = wn.z;
-                        cr[9] = mat.rmnb.x;
-                        cr[10] = mat.base_colour.x, cr[11] = mat.base_colour.y, cr[12] = mat.base_colour.z;
+                        cr[9] = rough_param;
+                        cr[10] = base.x, cr[11] = base.y, cr[12] = base.z;
                         cT = T;
                         cL = L;
                         if (hash_u32(pix ^ (sample * 0x9E3779B9u)) % cd.train_stride == 0) {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <map>
 #include <numeric>
@@ -13,6 +14,7 @@
 #include <unordered_map>
 
 #include "json.h"
+#include "png.h"
 
 namespace pth {
 
@@ -33,6 +35,25 @@ bool Material::is_emitter() const
     return ((map_bits() & EMISSION_MAP_BIT) ||
             std::max(std::max(emission[0], emission[1]), emission[2]) > 0.0f) &&
            emission[3] > 0.0f;
+}
+
+void Texture::colour_at(uint32_t x, uint32_t y, float out[4]) const
+{
+    const uint8_t* p = px.data() + (size_t(y) * width + x) * channels;
+    out[0] = p[0] / 255.0f;
+    out[1] = channels >= 2 ? p[1] / 255.0f : 0.0f;
+    out[2] = channels == 4 ? p[2] / 255.0f : 0.0f;
+    out[3] = channels == 4 ? p[3] / 255.0f : 1.0f;
+}
+
+bool Texture::has_alpha_less_than_one() const
+{
+    if (channels != 4)
+        return false;
+    for (size_t i = 0; i < size_t(width) * height; i++)
+        if (px[i * 4 + 3] < 255)
+            return true;
+    return false;
 }
 
 // The thresholds are the reference's own (colour.cpp:4 uses 0.00031308 for the encode).
@@ -249,40 +270,77 @@ M4 node_matrix(const Json& node)
 
 bool has_texture(const Json& j) { return j.kind == Json::OBJ && j.has("index"); }
 
-Material convert_material(const Json& m)
+float index_bits(uint32_t index)
+{
+    float f;
+    memcpy(&f, &index, 4);
+    return f;
+}
+
+// tex(ref, srgb) returns the scene texture index of a glTF texture reference, loading the image
+// on its first use (the first use decides sRGB, scene.cpp:53-72).
+Material convert_material(const Json& m, const std::function<uint32_t(const Json&, bool)>& tex)
 {
     // Field defaults of src/scene/scene.cpp:74-295 where the importer provides no value; the
     // glTF core factors are always present after import (spec defaults 1/1/1 and 0 emission).
+    // Maps are looked up in the order process_materials does (scene.cpp:374-379): base colour,
+    // emission, metallic-roughness, normal.
     const Json& pbr = m["pbrMetallicRoughness"];
-    if (has_texture(pbr["baseColorTexture"]) || has_texture(pbr["metallicRoughnessTexture"]) ||
-        has_texture(m["emissiveTexture"]) || has_texture(m["normalTexture"]))
-        throw std::runtime_error("glTF materials with textures are not supported by this loader");
+    const Json& ext = m["extensions"];
+    for (const char* e : {"KHR_materials_specular", "KHR_materials_sheen", "KHR_materials_clearcoat",
+                          "KHR_materials_anisotropy"})
+        for (const char* t : {"specularTexture", "specularColorTexture", "sheenColorTexture",
+                              "sheenRoughnessTexture", "clearcoatTexture", "clearcoatRoughnessTexture",
+                              "clearcoatNormalTexture", "anisotropyTexture"})
+            if (has_texture(ext[e][t]))
+                throw std::runtime_error(std::string("glTF: texture slot ") + e + "." + t +
+                                         " is not supported by this loader");
     Material out;
     uint32_t bits = 0;
-    float bc[4] = {1, 1, 1, 1};
-    if (pbr.has("baseColorFactor"))
-        for (int i = 0; i < 4; i++)
-            bc[i] = float(pbr["baseColorFactor"][size_t(i)].number(1));
-    V3 lin = from_srgb(V3(bc[0], bc[1], bc[2]));
-    out.base_colour[0] = lin.x;
-    out.base_colour[1] = lin.y;
-    out.base_colour[2] = lin.z;
-    out.base_colour[3] = bc[3];
+    if (has_texture(pbr["baseColorTexture"])) {  // process_base_colour (scene.cpp:74-111)
+        bits |= BASE_COLOUR_MAP_BIT;
+        out.base_colour[0] = index_bits(tex(pbr["baseColorTexture"], true));
+    } else {
+        float bc[4] = {1, 1, 1, 1};
+        if (pbr.has("baseColorFactor"))
+            for (int i = 0; i < 4; i++)
+                bc[i] = float(pbr["baseColorFactor"][size_t(i)].number(1));
+        V3 lin = from_srgb(V3(bc[0], bc[1], bc[2]));
+        out.base_colour[0] = lin.x;
+        out.base_colour[1] = lin.y;
+        out.base_colour[2] = lin.z;
+        out.base_colour[3] = bc[3];
+    }
 
-    V3 e(0, 0, 0);
-    if (m.has("emissiveFactor"))
-        e = V3(float(m["emissiveFactor"][0].number(0)), float(m["emissiveFactor"][1].number(0)),
-               float(m["emissiveFactor"][2].number(0)));
-    V3 le = from_srgb(e);
-    out.emission[0] = le.x;
-    out.emission[1] = le.y;
-    out.emission[2] = le.z;
-    const Json& ext = m["extensions"];
+    if (has_texture(m["emissiveTexture"])) {  // process_emission (scene.cpp:113-147)
+        bits |= EMISSION_MAP_BIT;
+        out.emission[0] = index_bits(tex(m["emissiveTexture"], true));
+    } else {
+        V3 e(0, 0, 0);
+        if (m.has("emissiveFactor"))
+            e = V3(float(m["emissiveFactor"][0].number(0)), float(m["emissiveFactor"][1].number(0)),
+                   float(m["emissiveFactor"][2].number(0)));
+        V3 le = from_srgb(e);
+        out.emission[0] = le.x;
+        out.emission[1] = le.y;
+        out.emission[2] = le.z;
+    }
     out.emission[3] = float(ext["KHR_materials_emissive_strength"]["emissiveStrength"].number(1.0));
 
-    out.rough_metal_normal_bits[0] = float(pbr["roughnessFactor"].number(1.0));
-    out.rough_metal_normal_bits[1] = float(pbr["metallicFactor"].number(1.0));
-    out.rough_metal_normal_bits[2] = 0.0f;
+    // process_roughness_metalness_normal (scene.cpp:149-207)
+    if (has_texture(pbr["metallicRoughnessTexture"])) {
+        bits |= ROUGHNESS_METALNESS_MAP_BIT;
+        out.rough_metal_normal_bits[0] = index_bits(tex(pbr["metallicRoughnessTexture"], false));
+    } else {
+        out.rough_metal_normal_bits[0] = float(pbr["roughnessFactor"].number(1.0));
+        out.rough_metal_normal_bits[1] = float(pbr["metallicFactor"].number(1.0));
+    }
+    if (has_texture(m["normalTexture"])) {
+        bits |= NORMAL_MAP_BIT;
+        out.rough_metal_normal_bits[2] = index_bits(tex(m["normalTexture"], false));
+    } else {
+        out.rough_metal_normal_bits[2] = 0.0f;
+    }
 
     const Json& spec = ext["KHR_materials_specular"];
     if (spec.kind == Json::OBJ && spec.has("specularColorFactor")) {
@@ -299,11 +357,25 @@ Material convert_material(const Json& m)
     out.cc_ccrgh_aniso[1] = float(cc["clearcoatRoughnessFactor"].number(0.5));
     out.cc_ccrgh_aniso[2] = float(ext["KHR_materials_anisotropy"]["anisotropyStrength"].number(0));
 
-    // has_transparency (scene.cpp:344-354): constant base-colour alpha below one.
-    if (out.base_colour[3] < 1.0f)
-        bits |= MATERIAL_IS_TRANSPARENT_BIT;
     out.set_map_bits(bits);
     return out;
+}
+
+// has_transparency (scene.cpp:344-354): a base-colour map with a texel alpha below one, or a
+// constant base-colour alpha below one.
+void mark_transparency(Material& m, const std::vector<Texture>& textures)
+{
+    uint32_t bits = m.map_bits();
+    bool t;
+    if (bits & BASE_COLOUR_MAP_BIT) {
+        uint32_t idx;
+        memcpy(&idx, &m.base_colour[0], 4);
+        t = textures.at(idx).has_alpha_less_than_one();
+    } else {
+        t = m.base_colour[3] < 1.0f;
+    }
+    if (t)
+        m.set_map_bits(bits | MATERIAL_IS_TRANSPARENT_BIT);
 }
 
 void walk_nodes(const Json& doc, size_t node_index, const M4& parent, std::vector<std::vector<M4>>& mesh_instances,
@@ -335,9 +407,59 @@ Scene load_gltf(const std::string& path)
     Scene scene;
     scene.resource_directory = g.dir;
 
+    // Textures of this file, keyed like Assimp's texture paths (the image URI, or "*n" for an
+    // embedded image): one entry per distinct image, in order of first use.
+    std::map<std::string, uint32_t> tex_index;
+    auto tex = [&](const Json& ref, bool srgb) -> uint32_t {
+        const Json& t = g.doc["textures"][size_t(ref["index"].number(-1))];
+        if (t.kind != Json::OBJ || !t.has("source"))
+            throw std::runtime_error("glTF: texture without an image source");
+        const size_t ii = size_t(t["source"].number(-1));
+        const Json& img = g.doc["images"][ii];
+        if (img.kind != Json::OBJ)
+            throw std::runtime_error("glTF: bad image index");
+        const bool external = img.has("uri") && img["uri"].str.compare(0, 5, "data:") != 0;
+        const std::string key = external ? img["uri"].str : "*" + std::to_string(ii);
+        auto it = tex_index.find(key);
+        if (it != tex_index.end())
+            return it->second;
+        std::vector<uint8_t> bytes;
+        if (external) {
+            const std::string d = read_file(g.dir + img["uri"].str, true);
+            bytes.assign(d.begin(), d.end());
+        } else if (img.has("uri")) {
+            const std::string& uri = img["uri"].str;
+            const size_t comma = uri.find(',');
+            if (comma == std::string::npos || uri.find(";base64") == std::string::npos)
+                throw std::runtime_error("glTF: unsupported image data URI");
+            bytes = decode_base64(uri.substr(comma + 1));
+        } else if (img.has("bufferView")) {
+            const Json& bv = g.doc["bufferViews"][size_t(img["bufferView"].number(-1))];
+            const size_t b = size_t(bv["buffer"].number(-1));
+            const size_t off = size_t(bv["byteOffset"].number(0)), len = size_t(bv["byteLength"].number(0));
+            if (b >= g.buffers.size() || off + len > g.buffers[b].size())
+                throw std::runtime_error("glTF: image bufferView exceeds its buffer");
+            bytes.assign(g.buffers[b].begin() + long(off), g.buffers[b].begin() + long(off + len));
+        } else {
+            throw std::runtime_error("glTF: image without data");
+        }
+        Image im = load_texture_image(bytes.data(), bytes.size());
+        Texture tx;
+        tx.width = im.width;
+        tx.height = im.height;
+        tx.channels = im.channels;
+        tx.srgb = srgb && im.channels == 4;  // only RGBA8 has an sRGB format (texturedata.cpp:6-18)
+        tx.px = std::move(im.px);
+        const uint32_t index = uint32_t(scene.textures.size());
+        scene.textures.push_back(std::move(tx));
+        tex_index[key] = index;
+        return index;
+    };
     const Json& mats = g.doc["materials"];
     for (size_t i = 0; i < mats.size(); i++)
-        scene.materials.push_back(convert_material(mats[i]));
+        scene.materials.push_back(convert_material(mats[i], tex));
+    for (Material& m : scene.materials)
+        mark_transparency(m, scene.textures);
 
     const Json& meshes = g.doc["meshes"];
     std::vector<std::vector<uint32_t>> mesh_prims(meshes.size());
@@ -678,7 +800,30 @@ Scene load_scn(const std::string& path)
                     v.instances.push_back(si.transform * oi);
             scene.variants.push_back(std::move(v));
         }
-        scene.materials.insert(scene.materials.end(), loaded.materials.begin(), loaded.materials.end());
+        // PbrMaterial::offset_maps_by (src/scene/material.cpp:16-42) for the maps this loader sets
+        const uint32_t texture_offset = uint32_t(scene.textures.size());
+        for (Material m : loaded.materials) {
+            const uint32_t b = m.map_bits();
+            auto shift = [&](float& f) {
+                uint32_t v;
+                memcpy(&v, &f, 4);
+                v += texture_offset;
+                memcpy(&f, &v, 4);
+            };
+            if (b & BASE_COLOUR_MAP_BIT)
+                shift(m.base_colour[0]);
+            if (b & EMISSION_MAP_BIT)
+                shift(m.emission[0]);
+            if (b & (ROUGHNESS_MAP_BIT | ROUGHNESS_METALNESS_MAP_BIT))
+                shift(m.rough_metal_normal_bits[0]);
+            if (b & METALNESS_MAP_BIT)
+                shift(m.rough_metal_normal_bits[1]);
+            if (b & NORMAL_MAP_BIT)
+                shift(m.rough_metal_normal_bits[2]);
+            scene.materials.push_back(m);
+        }
+        for (Texture& t : loaded.textures)
+            scene.textures.push_back(std::move(t));
     }
     return scene;
 }
@@ -736,6 +881,70 @@ std::vector<uint32_t> build_alias_table(const std::vector<float>& weights, const
     return words;
 }
 
+// calculate_object_intensity (src/scene/lightsample.cpp:41-87): for an emission map, a Monte
+// Carlo mean of r + g + b over 20 uniform points per triangle (texel nearest to the
+// interpolated UV, raw channels / 255), times the strength; otherwise (r + g + b) * strength.
+// The generator is Xshiro128 seeded by Splitmix32(0xbd7a0d48) with its four arguments
+// evaluated right to left, as the reference's g++ build does (its first state word is the
+// seeder's fourth output; pinned by the oracle/_ref golden "intensity_rng_float_bits").
+float object_intensity(const Mesh& mesh, const Material& mat, const std::vector<Texture>& textures)
+{
+    if (!(mat.map_bits() & EMISSION_MAP_BIT))
+        return (mat.emission[0] + mat.emission[1] + mat.emission[2]) * mat.emission[3];
+    uint32_t ti;
+    memcpy(&ti, &mat.emission[0], 4);
+    const Texture& tex = textures.at(ti);
+    uint32_t sm = 0xbd7a0d48u;
+    auto split = [&]() {
+        uint32_t z = (sm += 0x9e3779b9u);
+        z = (z ^ (z >> 16)) * 0x85ebca6bu;
+        z = (z ^ (z >> 13)) * 0xc2b2ae35u;
+        return z ^ (z >> 16);
+    };
+    uint32_t d[4];
+    for (int k = 0; k < 4; k++)
+        d[k] = split();
+    uint32_t st[4] = {d[3], d[2], d[1], d[0]};
+    auto rotl = [](uint32_t x, int k) { return (x << k) | (x >> (32 - k)); };
+    auto next_float = [&]() {
+        const uint32_t result = rotl(st[0] + st[3], 7) + st[0];
+        const uint32_t t = st[1] << 9;
+        st[2] ^= st[0];
+        st[3] ^= st[1];
+        st[1] ^= st[2];
+        st[0] ^= st[3];
+        st[2] ^= t;
+        st[3] = rotl(st[3], 11);
+        return float(result) / 4294967296.0f;
+    };
+    const size_t spt = 20;
+    double total = 0.0;
+    for (const Tri& tri : mesh.tris) {
+        const Vertex& a = mesh.vertices[tri.i[0]];
+        const Vertex& b = mesh.vertices[tri.i[1]];
+        const Vertex& c = mesh.vertices[tri.i[2]];
+        double sum = 0.0;
+        for (size_t j = 0; j < spt; j++) {
+            const float beta = 1.0f - std::sqrt(next_float());
+            const float gamma = (1.0f - beta) * next_float();
+            const float alpha = 1.0f - beta - gamma;
+            const float u = alpha * a.uv[0] + beta * b.uv[0] + gamma * c.uv[0];
+            const float v = alpha * a.uv[1] + beta * b.uv[1] + gamma * c.uv[1];
+            float ip;
+            // a negative fraction converts to an out-of-range unsigned value, clamped to the edge
+            const float fx = std::round(std::modf(u, &ip) * float(tex.width));
+            const float fy = std::round(std::modf(v, &ip) * float(tex.height));
+            const uint32_t x = fx < 0.0f ? tex.width - 1 : std::min(uint32_t(fx), tex.width - 1);
+            const uint32_t y = fy < 0.0f ? tex.height - 1 : std::min(uint32_t(fy), tex.height - 1);
+            float col[4];
+            tex.colour_at(x, y, col);
+            sum += col[0] + col[1] + col[2];
+        }
+        total += sum;
+    }
+    return float(total * mat.emission[3] / double(mesh.tris.size() * spt));
+}
+
 std::vector<uint32_t> build_light_words(const Scene& scene, const std::vector<uint32_t>& instance_offsets,
                                         uint32_t* light_count)
 {
@@ -751,8 +960,7 @@ std::vector<uint32_t> build_light_words(const Scene& scene, const std::vector<ui
         const Material& mat = scene.materials[var.material];
         if (!mat.is_emitter())
             continue;
-        if (mat.map_bits() & EMISSION_MAP_BIT)
-            throw std::runtime_error("textured emitters are not supported by this loader");
+
         size_t table_index;
         auto it = mesh_table.find(var.mesh);
         if (it == mesh_table.end()) {
@@ -778,8 +986,7 @@ std::vector<uint32_t> build_light_words(const Scene& scene, const std::vector<ui
         } else {
             table_index = it->second;
         }
-        // calculate_object_intensity for an untextured emitter (lightsample.cpp:85-86)
-        const float intensity = (mat.emission[0] + mat.emission[1] + mat.emission[2]) * mat.emission[3];
+        const float intensity = object_intensity(scene.meshes[var.mesh], mat, scene.textures);
         weights.push_back(intensity);
         uint32_t payload[7] = {LIGHT_TYPE_EMITTER,
                                0u,  // table offset, fixed below
@@ -842,6 +1049,10 @@ void PackedScene::bind_blob()
     blob.instance_mesh = instance_mesh.data();
     for (int i = 0; i < 4; i++)
         blob.envcol[i] = envcol[i];
+    blob.textures = textures.data();
+    blob.texture_count = uint32_t(textures.size());
+    blob.texels = texels.data();
+    blob.texel_bytes = texels.size();
 }
 
 PackedScene pack_scene(const Scene& scene)
@@ -898,6 +1109,17 @@ PackedScene pack_scene(const Scene& scene)
     ps.envcol[1] = scene.envcol.y;
     ps.envcol[2] = scene.envcol.z;
     ps.envcol[3] = 0.0f;
+    for (const Texture& t : scene.textures) {
+        pt_texture_desc d{};
+        d.offset = ps.texels.size();
+        d.width = t.width;
+        d.height = t.height;
+        d.channels = t.channels;
+        d.srgb = t.srgb ? 1u : 0u;
+        ps.textures.push_back(d);
+        ps.texels.insert(ps.texels.end(), t.px.begin(), t.px.end());
+        ps.texels.resize((ps.texels.size() + 3) & ~size_t(3), 0);  // 4-byte aligned starts
+    }
     ps.bind_blob();
     return ps;
 }

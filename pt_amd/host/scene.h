@@ -81,7 +81,19 @@ struct CameraSection {  // parse_camera defaults, src/scene/scenebuild.cpp:138-1
     float exposure = 1.0f;
 };
 
+// A decoded texture (TextureData, src/scene/texturedata.cpp): 8-bit channels, rows top to
+// bottom; srgb as the first material slot that used it decided (scene.cpp:42-72).
+struct Texture {
+    uint32_t width = 0, height = 0, channels = 0;
+    bool srgb = false;
+    std::vector<uint8_t> px;
+    // TextureData::colour_at (texturedata.cpp:96-121): raw channels / 255, no sRGB decode.
+    void colour_at(uint32_t x, uint32_t y, float out[4]) const;
+    bool has_alpha_less_than_one() const;  // texturedata.cpp:138-154
+};
+
 struct Scene {
+    std::vector<Texture> textures;
     std::vector<Variant> variants;
     std::vector<Mesh> meshes;
     std::vector<Material> materials;
@@ -112,6 +124,8 @@ struct PackedScene {
     std::vector<pt_mesh_desc> meshes;
     std::vector<uint32_t> instance_mesh;
     float envcol[4] = {0, 0, 0, 0};
+    std::vector<pt_texture_desc> textures;
+    std::vector<uint8_t> texels;
     pt_scene_blob blob{};
     void bind_blob();
 };

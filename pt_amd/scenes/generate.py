@@ -191,12 +191,32 @@ def write_bin(path, mesh):
             "min": pos.min(0).tolist(), "max": pos.max(0).tolist()}
 
 
-def write_gltf(path, prims):
-    """prims: list of (bin_layout, material dict). One node, one mesh, one primitive each."""
+def write_png(path, px):
+    """8-bit RGB or RGBA PNG (H x W x 3|4 uint8), no filtering, zlib-compressed."""
+    import struct
+    import zlib
+    px = np.ascontiguousarray(px, np.uint8)
+    h, w, n = px.shape
+    raw = b"".join(b"\x00" + px[y].tobytes() for y in range(h))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    data = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2 if n == 3 else 6, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+    with open(path, "wb") as f:
+        f.write(data)
+
+
+def write_gltf(path, prims, images=()):
+    """prims: list of (bin_layout, material dict). One node, one mesh, one primitive each.
+    images: PNG file names; texture i of the materials is image i."""
     doc = {"asset": {"version": "2.0", "generator": "pt_amd.scenes"}, "scene": 0,
            "scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
            "meshes": [{"name": os.path.splitext(os.path.basename(path))[0], "primitives": []}],
            "materials": [], "buffers": [], "bufferViews": [], "accessors": []}
+    if images:
+        doc["images"] = [{"uri": im} for im in images]
+        doc["textures"] = [{"source": i} for i in range(len(images))]
     buffers = {}
     for layout, mat in prims:
         if layout["file"] not in buffers:
@@ -366,6 +386,75 @@ def counts_for_triangles(total):
     return counts
 
 
+def textured_scene(out_dir, seed=3):
+    """The box scene with textured materials: a checker floor (sRGB base-colour map repeated 4x,
+    metallic-roughness map, normal map), a back wall with an RGBA base-colour map, a floating
+    quad whose base-colour map has alpha stripes (stochastic transparency and alpha shadows),
+    and a ceiling light with an emission map. Returns the .scn path."""
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    checker = (((xx // 8) + (yy // 8)) % 2).astype(np.float64)
+    base = np.stack([40 + 180 * checker, 60 + 120 * (1 - checker), 90 + 40 * checker], -1)
+    base = (base + rng.integers(0, 30, (n, n, 3))).clip(0, 255).astype(np.uint8)
+    write_png(os.path.join(out_dir, "floor_base.png"), base)
+    mr = np.stack([np.zeros((n, n)), 60 + 190 * (xx / (n - 1)), 255 * (yy > n // 2)], -1).astype(np.uint8)
+    write_png(os.path.join(out_dir, "floor_mr.png"), mr)
+    bump = np.stack([0.35 * np.sin(xx * 2 * np.pi / 16), 0.35 * np.cos(yy * 2 * np.pi / 16), np.ones((n, n))], -1)
+    bump = bump / np.linalg.norm(bump, axis=-1, keepdims=True)
+    write_png(os.path.join(out_dir, "floor_normal.png"), ((bump * 0.5 + 0.5) * 255).round().astype(np.uint8))
+    wall = np.stack([xx * 4, yy * 4, 255 - xx * 2, np.full((n, n), 255)], -1).clip(0, 255).astype(np.uint8)
+    write_png(os.path.join(out_dir, "wall_base.png"), wall)
+    stripes = np.stack([np.full((n, n), 230), np.full((n, n), 120), np.full((n, n), 40),
+                        np.where((xx // 6) % 2 == 0, 255, np.where((xx // 6) % 3 == 0, 0, 90))], -1).astype(np.uint8)
+    write_png(os.path.join(out_dir, "veil_base.png"), stripes)
+    glow = np.stack([200 + 55 * checker, 180 + 40 * (1 - checker), 120 + 100 * (xx / (n - 1))], -1).astype(np.uint8)
+    write_png(os.path.join(out_dir, "light_emission.png"), glow)
+
+    floor = quad((-1, -2, 1), (2, 0, 0), (0, 0, -2), (0, 1, 0))
+    floor = (floor[0], floor[1], floor[2] * 4.0, floor[3])  # REPEAT addressing
+    back = quad((-1, -2, -1), (2, 0, 0), (0, 2, 0), (0, 0, 1))
+    ceil = quad((-1, 0, -1), (2, 0, 0), (0, 0, 2), (0, -1, 0))
+    left = quad((-1, -2, 1), (0, 0, -2), (0, 2, 0), (1, 0, 0))
+    right = quad((1, -2, -1), (0, 0, 2), (0, 2, 0), (-1, 0, 0))
+    veil = quad((-0.6, -0.8, -0.5), (1.2, 0, 0), (0, 0, 1.0), (0, 1, 0))
+    sp, sn, su, si = icosphere(3, 0.45)
+    sphere = (sp + np.array([0.35, -1.55, -0.2], np.float32), sn, su, si)
+    light = quad((-0.3, -0.001, -0.3), (0.6, 0, 0), (0, 0, 0.6), (0, -1, 0))
+    images = ["floor_base.png", "floor_mr.png", "floor_normal.png", "wall_base.png", "veil_base.png",
+              "light_emission.png"]
+
+    def tex(i):
+        return {"index": i}
+    m_floor = {"name": "floor", "pbrMetallicRoughness": {"baseColorTexture": tex(0), "metallicRoughnessTexture": tex(1)},
+               "normalTexture": tex(2)}
+    m_back = {"name": "back", "pbrMetallicRoughness": {"baseColorTexture": tex(3), "roughnessFactor": 0.8,
+                                                        "metallicFactor": 0.0}}
+    m_veil = {"name": "veil", "pbrMetallicRoughness": {"baseColorTexture": tex(4), "roughnessFactor": 0.6,
+                                                        "metallicFactor": 0.0}}
+    m_light = {"name": "light", "pbrMetallicRoughness": {"baseColorFactor": [0, 0, 0, 1], "roughnessFactor": 1.0,
+                                                         "metallicFactor": 0.0},
+               "emissiveTexture": tex(5), "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 15.0}}}
+    prims = []
+    for name, mesh, mat in [
+        ("floor", floor, m_floor), ("back", back, m_back),
+        ("ceil", ceil, _material("ceil", (0.7, 0.7, 0.7), 1.0, 0.0)),
+        ("left", left, _material("left", (0.6, 0.25, 0.2), 1.0, 0.0)),
+        ("right", right, _material("right", (0.2, 0.4, 0.6), 1.0, 0.0)),
+        ("veil", veil, m_veil),
+        ("sphere", sphere, _material("sphere", (0.7, 0.7, 0.7), 0.4, 1.0)),
+        ("light", light, m_light),
+    ]:
+        prims.append((write_bin(os.path.join(out_dir, "tex_%s.bin" % name), mesh), mat))
+    write_gltf(os.path.join(out_dir, "textured.gltf"), prims, images)
+    scn = os.path.join(out_dir, "textured.scn")
+    with open(scn, "w") as f:
+        f.write("camera:\n position 0 1 -3\n lookdir 0 0 1\n fov 60\n exposure 1\n envcol 0.375 0.45 0.9\n\n"
+                "objects:\n tex textured.gltf\n\n tex\n")
+    return scn
+
+
 def cached_scene(kind, root=None, **kw):
     """Generates a scene once under root (default $PT_SCENE_DIR or /tmp/pt_scenes)."""
     root = root or os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes")
@@ -373,11 +462,11 @@ def cached_scene(kind, root=None, **kw):
         return "-".join("%s%s" % kv for kv in sorted(v.items())) if isinstance(v, dict) else str(v)
     tag = kind + "".join("_%s%s" % (k, fmt(v)) for k, v in sorted(kw.items()))
     out = os.path.join(root, tag)
-    name = {"box": "box.scn", "instanced": "instanced.scn"}[kind]
+    name = {"box": "box.scn", "instanced": "instanced.scn", "textured": "textured.scn"}[kind]
     path = os.path.join(out, name)
     done = os.path.join(out, ".done")
     if not os.path.exists(done):
-        fn = {"box": box_scene, "instanced": instanced_scene}[kind]
+        fn = {"box": box_scene, "instanced": instanced_scene, "textured": textured_scene}[kind]
         fn(out, **kw)
         open(done, "w").close()
     return path
