@@ -367,6 +367,20 @@ def run_scenes(a):
             cam = pt_amd.camera_block([-e for e in eye], [-f for f in fwd], fov_deg=60.0, aspect=w / h,
                                       focus_dist=30.0, lens_radius=0.03)
             views.append((k, w, h, spp, DEPTH, cam))
+    if a.only_scene >= 0:
+        # one scene's four views alone (per-scene profiling)
+        k = a.only_scene
+        vk = [(0,) + v[1:] for v in views if v[0] == k]
+        b = pt_amd.Batch([tracers[k]], vk)
+        ek, sk = _timed_batch(b, a.steps, a.warmup)
+        rays = sk["extension_rays"] + sk["shadow_rays"]
+        b.close()
+        return {"metric": "Mrays/s", "value": round(rays / ek / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+                "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ek / a.steps * 1e3, 3),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic (seeded procedural scenes)",
+                "config": {"workload": "scene %d of the 16-scene batch alone (4 views, depth 8)" % k},
+                "scene": scenes_info[k]}
     paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
     log("scenes: 16 scenes, 64 views, %d pixel-samples per step" % paths)
     batch = pt_amd.Batch(tracers, views)
@@ -402,6 +416,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     a = ap.parse_args()

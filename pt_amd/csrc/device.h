@@ -23,11 +23,8 @@ constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to
 // until the job's last paths drain.
 struct Wavefront {
     uint32_t capacity = 0;         // slots
-    uint32_t* path_id = nullptr;   // path of each slot: its index within the job
-    uint32_t* view_of = nullptr;   // view of each slot's path
-    uint32_t* pix_of = nullptr;    // pixel of each slot's path within its view
-    uint32_t* sample_of = nullptr; // sample index of each slot's path
-    uint32_t* seg = nullptr;       // current segment of each slot
+    // per slot: job path index, pixel within its view, sample index, view << 16 | segment
+    uint4* pstate = nullptr;
     float4* thr_pdf = nullptr;     // throughput rgb, brdf pdf
     float4* rad = nullptr;         // radiance rgb
     uint32_t* q_slot[2] = {nullptr, nullptr};  // extension queues (ping-pong)

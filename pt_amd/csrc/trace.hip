@@ -397,11 +397,7 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     tmin = vw.cam.near_clip;
     tmax = vw.cam.far_clip;
     scene = vw.scene;
-    wf.path_id[s] = p;
-    wf.view_of[s] = sg.view;
-    wf.pix_of[s] = pix;
-    wf.sample_of[s] = sample;
-    wf.seg[s] = 0;
+    wf.pstate[s] = make_uint4(p, pix, sample, sg.view << 16);  // segment 0
     wf.thr_pdf[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     wf.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -521,11 +517,12 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
         if (valid) {
             const uint32_t qi = order ? order[i] : i;  // material-sorted order when given
             slot = wf.q_slot[q][qi];
-            const ViewDev& vw = views[wf.view_of[slot]];
+            const uint4 ps = wf.pstate[slot];  // job path, pixel, sample, view << 16 | segment
+            const ViewDev& vw = views[ps.w >> 16];
             sid = vw.scene;
             const DeviceScene& s = MULTI ? scenes[sid] : s0;
-            const uint32_t pix = wf.pix_of[slot], sample = wf.sample_of[slot];
-            segment = wf.seg[slot];
+            const uint32_t pix = ps.y, sample = ps.z;
+            segment = ps.w & 0xFFFFu;
             const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
             const float3 ro = xyz(ro4), rd = xyz(rd4);
             const float4 hh = wf.hit[qi];
@@ -741,7 +738,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
             wf.thr_pdf[slot] = make_float4(T.x, T.y, T.z, brdf_pdf);
             wf.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
             if (cont)
-                wf.seg[slot] = segment + 1;
+                reinterpret_cast<uint32_t*>(wf.pstate + slot)[3] += 1u;  // next segment
         }
         const uint32_t pos = wave_append(cont, &wf.counters[1 - q]);
         if (cont) {
@@ -828,7 +825,7 @@ __global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const 
         if (valid) {
             slot = wf.retire[i];
             const float4 lf = wf.rad[slot];
-            wf.rad_out[wf.path_id[slot]] = lf;
+            wf.rad_out[wf.pstate[slot].x] = lf;
             if (cd.collect) {
                 // training target: radiance gathered after the 2nd-bounce vertex, divided by
                 // the throughput that reached it
@@ -986,12 +983,8 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
         if (e == hipSuccess)
             e = hipMalloc((void**)&p, bytes);
     };
-    A(wf.path_id, c * 4);
-    A(wf.view_of, c * 4);
-    A(wf.pix_of, c * 4);
-    A(wf.sample_of, c * 4);
+    A(wf.pstate, c * 16);
     A(wf.s_scene, c * 4);
-    A(wf.seg, c * 4);
     A(wf.thr_pdf, c * 16);
     A(wf.rad, c * 16);
     for (int k = 0; k < 2; k++) {
@@ -1024,10 +1017,10 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
 
 void wavefront_free(Wavefront& wf)
 {
-    void* ptrs[] = {wf.path_id, wf.seg,   wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
+    void* ptrs[] = {wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
                     wf.q_o[1],  wf.q_d[0], wf.q_d[1], wf.hit,      wf.hit_prim,  wf.s_slot,    wf.s_o,
                     wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill,
-                    wf.view_of, wf.pix_of, wf.sample_of, wf.s_scene, wf.q_scene[0], wf.q_scene[1],
+                    wf.pstate, wf.s_scene, wf.q_scene[0], wf.q_scene[1],
                     wf.sort_keys[0], wf.sort_keys[1], wf.sort_vals[0], wf.sort_vals[1], wf.sort_temp};
     for (void* p : ptrs)
         if (p)
@@ -1143,7 +1136,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
     mix(reinterpret_cast<uintptr_t>(io.d_views));
     mix(reinterpret_cast<uintptr_t>(io.d_job));
-    mix(reinterpret_cast<uintptr_t>(io.wf->path_id));
+    mix(reinterpret_cast<uintptr_t>(io.wf->pstate));
     mix(io.wf->capacity);
     mix(io.grid_blocks);
     mix(io.cache ? reinterpret_cast<uintptr_t>(io.cache->params) : 0);
