@@ -158,9 +158,10 @@ typedef struct pt_stats {
 #define PT_OPT_COUNT_FETCHES 2u   /* value != 0: count BVH node / triangle fetches per ray */
 #define PT_OPT_GRAPHS 3u          /* value != 0 (default): replay each wavefront iteration from a
                                    * captured HIP graph (not while kernels are being timed) */
-/* Launch configuration (no counterpart in the reference, whose traversal belongs to the
- * driver). Images do not depend on them, except that another leaf size can resolve exact
- * ties between equally distant triangles differently. */
+/* Launch and hierarchy configuration (no counterpart in the reference, whose traversal
+ * belongs to the driver). Images do not depend on them, except that another hierarchy (leaf
+ * size, node width, builder) can resolve exact ties between equally distant triangles
+ * differently. */
 #define PT_OPT_BLOCKS_PER_CU 4u   /* traversal grid: blocks of 256 threads per CU (1..64, default 4) */
 #define PT_OPT_POOL_SLOTS 5u      /* path pool slots (1024..2^26; default min(8M, max(4 px, 64K))) */
 #define PT_OPT_LDS_STACK 6u       /* traversal stack entries per lane kept in LDS (8, 16, 32) */
@@ -168,6 +169,10 @@ typedef struct pt_stats {
 #define PT_OPT_SORT_HITS 8u       /* value != 0: shade extension hits in (scene, material) order
                                    * after a radix sort (default 0; iterations then launch
                                    * without graphs) */
+#define PT_OPT_NODE_WIDTH 9u      /* children per BVH node the SAH collapse may use (2..4, default 4;
+                                   * every node is one 128-byte line): rebuilds the BVH */
+#define PT_OPT_BVH_BUILDER 10u    /* binary tree under the SAH collapse: 0 PLOC clustering (default),
+                                   * 1 Karras radix tree over the Morton codes: rebuilds the BVH */
 
 typedef struct pt_ctx pt_ctx;
 
@@ -327,8 +332,8 @@ int pt_read_light_words(pt_ctx* ctx, uint32_t* words, size_t count);
 
 /* Autotuner (no counterpart in the reference): times spp samples of the current view under
  * candidate launch configurations (coordinate descent over blocks per CU, LDS stack depth,
- * pool slots and BLAS leaf size; the node width is 4, the only one this build has), keeps the
- * fastest (device time per traced ray) and clears the accumulation (sample_index 0).
+ * pool slots, BVH node width and BLAS leaf size), keeps the fastest (device time per traced
+ * ray) and clears the accumulation (sample_index 0).
  * Not with the radiance cache or instrumentation options on. Synchronises. */
 typedef struct pt_tune_result {
     uint32_t blocks_per_cu;

@@ -98,6 +98,8 @@ PT_OPT_POOL_SLOTS = 5
 PT_OPT_LDS_STACK = 6
 PT_OPT_LEAF_SIZE = 7
 PT_OPT_SORT_HITS = 8
+PT_OPT_NODE_WIDTH = 9
+PT_OPT_BVH_BUILDER = 10
 
 
 class TuneResult(C.Structure):

@@ -175,7 +175,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     if (bv.tri_count &&
         hipMemcpy(tris.data(), bv.tris, size_t(bv.tri_count) * sizeof(TriRec), hipMemcpyDeviceToHost) != hipSuccess)
         return (why = "BVH readback failed", PT_EDEVICE);
-    std::vector<uint8_t> tri_seen(bv.tri_count, 0), inst_seen(c.instance_count, 0);
+    std::vector<uint8_t> tri_seen(bv.tri_count, 0), prim_seen(bv.tri_count, 0), inst_seen(c.instance_count, 0);
     std::vector<uint32_t> stack;
     uint32_t tri_base = 0;
     for (uint32_t m = 0; m < meshes.size(); m++) {
@@ -198,8 +198,11 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
                         for (uint32_t q = f; q < f + cnt; q++) {
                             if (tri_seen[q]++)
                                 return (why = "triangle referenced twice", PT_EDEVICE);
-                            if (float_as_u32(tris[q].v0.w) >= nt)
+                            const uint32_t prim = float_as_u32(tris[q].v0.w);
+                            if (prim >= nt)
                                 return (why = "triangle record with a bad primitive index", PT_EDEVICE);
+                            if (prim_seen[tri_base + prim]++)
+                                return (why = "primitive recorded twice", PT_EDEVICE);
                         }
                     } else {
                         if (ch >= bv.node_count || ch < meshes.size())
@@ -327,6 +330,10 @@ int build_scene_bvh(Ctx& c)
     HIP_TRY(&c, hipEventRecord(c.ev_start, st), "event");
     BvhBuildParams bp;
     bp.blas_leaf_size = c.leaf_size;
+    bp.node_width = c.node_width;
+    bp.builder = c.bvh_builder;
+    if (const char* env = getenv("PT_BVH_SAH_TRI_COST"))  // tuning of the SAH cost model
+        bp.sah_tri_cost = float(std::max(0.01, std::min(100.0, atof(env))));
     HIP_TRY(&c,
             build_bvh(st, c.d_vertices, c.d_indices, c.d_meshes, c.meshes, c.d_instances, c.d_instance_mesh,
                       c.instance_count, c.d_materials, bp, c.bvh),
@@ -804,14 +811,23 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
         c.lds_stack = uint32_t(value);
         return PT_OK;
     case PT_OPT_LEAF_SIZE:
-        if (value < 1 || value > 16)
+    case PT_OPT_NODE_WIDTH:
+    case PT_OPT_BVH_BUILDER: {
+        if (option == PT_OPT_LEAF_SIZE && (value < 1 || value > 16))
             return fail(&c, PT_EINVAL, "leaf size must be 1..16");
+        if (option == PT_OPT_NODE_WIDTH && (value < 2 || value > 4))
+            return fail(&c, PT_EINVAL, "node width must be 2..4");
+        if (option == PT_OPT_BVH_BUILDER && value != BVH_BUILDER_PLOC && value != BVH_BUILDER_LBVH)
+            return fail(&c, PT_EINVAL, "BVH builder must be 0 (PLOC) or 1 (radix tree)");
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
-        c.leaf_size = uint32_t(value);
+        uint32_t& field = option == PT_OPT_LEAF_SIZE ? c.leaf_size : option == PT_OPT_NODE_WIDTH ? c.node_width
+                                                                                                 : c.bvh_builder;
+        field = uint32_t(value);
         if (int v = build_scene_bvh(c))
             return v;
         c.stats_fresh = true;
         return PT_OK;
+    }
     default:
         return fail(&c, PT_EINVAL, "unknown option " + std::to_string(option));
     }
@@ -1266,9 +1282,13 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         return fail(&c, PT_ESTATE, "instrumentation options are on");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     struct Cfg {
-        int64_t bpc, pool, lds, leaf;
+        int64_t bpc, pool, lds, leaf, width;
+        bool operator==(const Cfg& o) const
+        {
+            return bpc == o.bpc && pool == o.pool && lds == o.lds && leaf == o.leaf && width == o.width;
+        }
     };
-    const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size};
+    const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size, c.node_width};
     uint32_t trials = 0;
     // device nanoseconds per traced ray of spp samples of the current view (best of two runs
     // after a first one that warms the caches and captures the graphs)
@@ -1283,6 +1303,9 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
             return v;
         if (g.leaf != int64_t(c.leaf_size))
             if (int v = pt_set_option(h, PT_OPT_LEAF_SIZE, g.leaf))
+                return v;
+        if (g.width != int64_t(c.node_width))
+            if (int v = pt_set_option(h, PT_OPT_NODE_WIDTH, g.width))
                 return v;
         ns = 1e300;
         for (int rep = 0; rep < 3; rep++) {
@@ -1316,13 +1339,15 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         if (uint64_t(p) <= std::max<uint64_t>(npix * spp, 1u << 20))
             pools.push_back(p);
     const std::vector<int64_t> leaves = {2, 4, 8};
-    // coordinate descent over (blocks per CU, LDS stack depth, pool slots, leaf size)
-    for (int param = 0; param < 4; param++) {
-        const std::vector<int64_t>& vals = param == 0 ? bpcs : param == 1 ? ldss : param == 2 ? pools : leaves;
+    const std::vector<int64_t> widths = {4, 3, 2};
+    // coordinate descent over (blocks per CU, LDS stack depth, pool slots, node width, leaf size)
+    for (int param = 0; param < 5; param++) {
+        const std::vector<int64_t>& vals =
+            param == 0 ? bpcs : param == 1 ? ldss : param == 2 ? pools : param == 3 ? widths : leaves;
         for (int64_t v : vals) {
             Cfg g = best;
-            (param == 0 ? g.bpc : param == 1 ? g.lds : param == 2 ? g.pool : g.leaf) = v;
-            if (g.bpc == best.bpc && g.lds == best.lds && g.pool == best.pool && g.leaf == best.leaf)
+            (param == 0 ? g.bpc : param == 1 ? g.lds : param == 2 ? g.pool : param == 3 ? g.width : g.leaf) = v;
+            if (g == best)
                 continue;
             double ns;
             if (int e = measure(g, ns))
@@ -1346,7 +1371,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         r.pool_slots = uint32_t(best.pool);
         r.lds_stack = uint32_t(best.lds);
         r.leaf_size = uint32_t(best.leaf);
-        r.node_width = 4;
+        r.node_width = uint32_t(best.width);
         r.trials = trials;
         r.ns_per_ray_start = base_ns;
         r.ns_per_ray_best = best_ns;

@@ -1,6 +1,6 @@
-// Device-built LBVH: Morton codes -> radix sort -> Karras (2012) binary radix tree ->
-// bottom-up bounds -> collapse into 4-wide 128-byte nodes. One BLAS per mesh (all meshes in
-// one batched build) and a TLAS over the instances.
+// Device-built BVH: Morton codes -> radix sort -> binary tree (PLOC clustering, or Karras'
+// (2012) radix tree) -> SAH-optimal collapse into 4-wide 128-byte nodes. One BLAS per mesh
+// (all meshes in one batched build) and a TLAS over the instances.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,8 +11,17 @@
 
 namespace ptk {
 
+constexpr uint32_t BVH_BUILDER_PLOC = 0;  // parallel locally-ordered clustering (Meister & Bittner 2018)
+constexpr uint32_t BVH_BUILDER_LBVH = 1;  // Karras (2012) binary radix tree over the Morton codes
+
 struct BvhBuildParams {
     uint32_t blas_leaf_size = 4;  // max triangles per BLAS leaf (1..16)
+    uint32_t node_width = 4;      // children per node (2..4) the SAH collapse may use
+    uint32_t builder = BVH_BUILDER_PLOC;
+    // SAH cost of entering a node, testing a triangle, entering an instance
+    float sah_node_cost = 1.0f;
+    float sah_tri_cost = 0.5f;
+    float sah_instance_cost = 1.0f;
 };
 
 struct BvhBuffers {

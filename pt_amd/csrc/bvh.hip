@@ -1,11 +1,13 @@
-// Device LBVH builder (see bvh.h).
+// Device BVH builder (see bvh.h).
 //
 // Per build set (all meshes' triangles at once, or the instances): primitive bounds ->
 // 30-bit Morton code of the centroid within its segment's centroid bounds -> one 64-bit
-// radix sort of (segment << 32 | morton) -> Karras' binary radix tree per segment ->
-// bottom-up bounds with atomic visit counters -> top-down collapse into 4-wide nodes, each
-// step opening the child with the largest surface area, subtrees of <= leaf_size
-// primitives becoming leaves (contiguous ranges of the sorted order).
+// radix sort of (segment << 32 | morton) -> a binary tree per segment, built either by
+// PLOC (parallel locally-ordered clustering over the Morton order, the default) or as
+// Karras' binary radix tree -> SAH cost of every binary node as a subtree of a W-wide tree
+// (W = 2..4 children) -> top-down collapse into 4-wide 128-byte nodes following the
+// SAH decisions; a triangle leaf is a contiguous range of the triangle records, which are
+// written in leaf order.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -41,31 +43,6 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v)
     v = (v | (v << 2)) & 0x09249249u;
     return v;
 }
-
-struct BuildSet {
-    uint32_t n = 0;          // primitives
-    uint32_t segs = 0;       // segments
-    float4* lo = nullptr;    // prim bounds
-    float4* hi = nullptr;
-    uint32_t* seg = nullptr;        // segment of each (unsorted) primitive
-    const uint32_t* seg_begin = nullptr;  // segs+1 prefix offsets (device)
-    uint32_t* cmin = nullptr;  // 3 ordered uints per segment (centroid bounds)
-    uint32_t* cmax = nullptr;
-    uint64_t* keys = nullptr;
-    uint64_t* keys_alt = nullptr;
-    uint32_t* vals = nullptr;
-    uint32_t* vals_alt = nullptr;
-    // binary tree, indexed by global sorted position
-    uint32_t* left = nullptr;
-    uint32_t* right = nullptr;
-    uint32_t* first = nullptr;
-    uint32_t* last = nullptr;
-    uint32_t* parent = nullptr;       // of internal nodes
-    uint32_t* leaf_parent = nullptr;  // of leaves
-    uint32_t* visit = nullptr;
-    float* ibox = nullptr;  // 6 floats per internal node
-    float* lbox = nullptr;  // 6 floats per leaf
-};
 
 __global__ void k_centroid_bounds(uint32_t n, const float4* lo, const float4* hi, const uint32_t* seg,
                                   uint32_t* cmin, uint32_t* cmax)
@@ -127,6 +104,166 @@ __global__ void k_morton(uint32_t n, const float4* lo, const float4* hi, const u
     }
 }
 
+struct BuildSet {
+    uint32_t n = 0;          // primitives
+    uint32_t segs = 0;       // segments
+    float4* lo = nullptr;    // prim bounds
+    float4* hi = nullptr;
+    uint32_t* seg = nullptr;        // segment of each (unsorted) primitive
+    const uint32_t* seg_begin = nullptr;  // segs+1 prefix offsets (device)
+    uint32_t* cmin = nullptr;  // 3 ordered uints per segment (centroid bounds)
+    uint32_t* cmax = nullptr;
+    uint64_t* keys = nullptr;
+    uint64_t* keys_alt = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* vals_alt = nullptr;
+    // Binary tree. A reference with BREF_LEAF is a sorted position (one primitive), any
+    // other an internal node.
+    uint32_t* left = nullptr;
+    uint32_t* right = nullptr;
+    uint32_t* cnt = nullptr;    // primitives under an internal node
+    uint32_t* dec = nullptr;    // collapse decisions of an internal node (sah_node)
+    float4* cost = nullptr;     // SAH cost of an internal node with 1..4 slots
+    float* ibox = nullptr;      // 6 floats per internal node
+    float* lbox = nullptr;      // 6 floats per leaf
+    uint32_t* root = nullptr;   // per segment: reference of its root (NONE: no primitives)
+    float* segbox = nullptr;    // per segment: 6 floats
+    uint32_t* perm = nullptr;   // sorted position of each output position (leaf order)
+};
+
+// SAH cost model of the collapse into wide nodes: entering a node costs c_node, testing a
+// primitive (a triangle, or entering an instance) c_leaf, each weighted by the surface area
+// of the box through which a ray reaches it.
+struct Sah {
+    uint32_t width;     // children per wide node (2..4)
+    uint32_t max_leaf;  // primitives per leaf (1..16)
+    float c_node;
+    float c_leaf;
+};
+
+__device__ __forceinline__ float half_area(const float* b)
+{
+    const float dx = fmaxf(b[3] - b[0], 0.0f), dy = fmaxf(b[4] - b[1], 0.0f), dz = fmaxf(b[5] - b[2], 0.0f);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ void box_union(const float* a, const float* b, float* u)
+{
+    for (int k = 0; k < 3; k++) {
+        u[k] = fminf(a[k], b[k]);
+        u[k + 3] = fmaxf(a[k + 3], b[k + 3]);
+    }
+}
+
+__device__ __forceinline__ void load_box(const float* b, float* out)
+{
+    for (int k = 0; k < 6; k++)
+        out[k] = __hip_atomic_load(&b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_box(float* b, const float* v)
+{
+    for (int k = 0; k < 6; k++)
+        __hip_atomic_store(&b[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Cost of a child subtree when it may take 1..4 slots of its parent wide node.
+__device__ __forceinline__ void child_cost(const Sah& sah, uint32_t ref, const float* box, const float4* cost,
+                                           float* c)
+{
+    if (ref & BREF_LEAF) {
+        const float v = sah.c_leaf * half_area(box);
+        c[0] = c[1] = c[2] = c[3] = v;
+        return;
+    }
+    const float* q = reinterpret_cast<const float*>(&cost[ref]);
+    for (int k = 0; k < 4; k++)
+        c[k] = __hip_atomic_load(&q[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The dynamic programme of Ylitie, Karras and Laine (HPG 2017) at one binary node with
+// box `box`, `count` primitives and children costs cl, cr:
+//   dist(i)  = min over kl of cl(kl) + cr(i - kl)          (i slots shared by the children)
+//   C(n, 1)  = min(leaf: A c_leaf count, wide node: A c_node + dist(W))
+//   C(n, i)  = min(C(n, i - 1), dist(i))                    (i = 2..W)
+// dec bit 0: the node as one slot is a leaf; bits 4(i-1)..4(i-1)+3 (i = 2..4): how its
+// subtree fills i slots (left slots | right slots << 2; 0: the node itself takes one slot);
+// bits 16-17: left slots of the node's own wide node (the right child gets W minus them).
+__device__ inline void sah_node(const Sah& sah, const float* box, uint32_t count, const float* cl, const float* cr,
+                                float4& cost, uint32_t& dec)
+{
+    float dist[5];
+    uint32_t split[5];
+    for (uint32_t i = 2; i <= 4; i++) {
+        float best = INFINITY;
+        uint32_t bk = 1;
+        for (uint32_t kl = 1; kl < i; kl++) {
+            const float v = cl[kl - 1] + cr[i - kl - 1];
+            if (v < best) {
+                best = v;
+                bk = kl;
+            }
+        }
+        dist[i] = best;
+        split[i] = bk;
+    }
+    const float a = half_area(box);
+    const float c_int = a * sah.c_node + dist[sah.width];
+    const float c_leaf = count <= sah.max_leaf ? a * sah.c_leaf * float(count) : INFINITY;
+    float c[4];
+    c[0] = fminf(c_leaf, c_int);
+    uint32_t d = c_leaf <= c_int ? 1u : 0u;
+    uint32_t code = 0;
+    for (uint32_t i = 2; i <= 4; i++) {
+        if (i <= sah.width && dist[i] < c[i - 2]) {
+            c[i - 1] = dist[i];
+            code = split[i] | ((i - split[i]) << 2);
+        } else {
+            c[i - 1] = c[i - 2];
+        }
+        d |= code << (4 * (i - 1));
+    }
+    dec = d | (split[sah.width] << 16);
+    cost = make_float4(c[0], c[1], c[2], c[3]);
+}
+
+// Leaf boxes in sorted order, and the initial PLOC clusters (one per primitive).
+struct Cluster {
+    float4 lo;  // box min, segment (bits of w)
+    float4 hi;  // box max, binary reference (bits of w)
+};
+
+__global__ void k_leaves(uint32_t n, const uint32_t* vals, const uint64_t* keys, const float4* lo, const float4* hi,
+                         float* lbox, Cluster* cl)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = vals[i];
+        const float4 a = lo[p], b = hi[p];
+        float* o = &lbox[size_t(i) * 6];
+        o[0] = a.x;
+        o[1] = a.y;
+        o[2] = a.z;
+        o[3] = b.x;
+        o[4] = b.y;
+        o[5] = b.z;
+        if (cl)
+            cl[i] = Cluster{make_float4(a.x, a.y, a.z, u32_as_float(uint32_t(keys[i] >> 32))),
+                            make_float4(b.x, b.y, b.z, u32_as_float(i | BREF_LEAF))};
+    }
+}
+
+__global__ void k_empty_boxes(uint32_t n, float* box, uint32_t* root)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; k++) {
+            box[size_t(i) * 6 + k] = INFINITY;
+            box[size_t(i) * 6 + k + 3] = -INFINITY;
+        }
+        root[i] = NONE;
+    }
+}
+
+// ---- Karras (2012) binary radix tree
+
 __device__ __forceinline__ int delta(const uint64_t* keys, uint32_t base, int n, int a, int b)
 {
     if (b < 0 || b >= n)
@@ -137,8 +274,9 @@ __device__ __forceinline__ int delta(const uint64_t* keys, uint32_t base, int n,
     return __clz(ka ^ kb);
 }
 
+// Internal node g = base + j of a segment starting at base; its leaves are sorted positions.
 __global__ void k_karras(uint32_t n, const uint64_t* keys, const uint32_t* seg_begin, uint32_t* left, uint32_t* right,
-                         uint32_t* first, uint32_t* last, uint32_t* parent, uint32_t* leaf_parent)
+                         uint32_t* cnt, uint32_t* parent, uint32_t* leaf_parent)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t s = uint32_t(keys[i] >> 32);
@@ -189,32 +327,16 @@ __global__ void k_karras(uint32_t n, const uint64_t* keys, const uint32_t* seg_b
             right[g] = rc;
             parent[rc] = g;
         }
-        first[g] = base + uint32_t(lo_i);
-        last[g] = base + uint32_t(hi_i);
+        cnt[g] = uint32_t(hi_i - lo_i + 1);
     }
 }
 
-__device__ __forceinline__ void load_box(const float* b, float* out)
-{
-    for (int k = 0; k < 6; k++)
-        out[k] = __hip_atomic_load(&b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_box(float* b, const float* v)
-{
-    for (int k = 0; k < 6; k++)
-        __hip_atomic_store(&b[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void k_bounds_up(uint32_t n, const uint32_t* vals, const float4* lo, const float4* hi,
-                            const uint32_t* left, const uint32_t* right, const uint32_t* parent,
-                            const uint32_t* leaf_parent, uint32_t* visit, float* ibox, float* lbox)
+// Bottom-up bounds and SAH costs: the second child to finish completes its parent.
+__global__ void k_bounds_up(uint32_t n, const uint32_t* left, const uint32_t* right, const uint32_t* cnt,
+                            const uint32_t* parent, const uint32_t* leaf_parent, uint32_t* visit, float* ibox,
+                            const float* lbox, float4* cost, uint32_t* dec, Sah sah)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p0 = vals[i];
-        const float4 a = lo[p0], b = hi[p0];
-        const float box[6] = {a.x, a.y, a.z, b.x, b.y, b.z};
-        store_box(&lbox[size_t(i) * 6], box);
-        __threadfence();
         uint32_t p = leaf_parent[i];
         while (p != NONE) {
             const uint32_t old = atomicAdd(&visit[p], 1u);
@@ -226,67 +348,220 @@ __global__ void k_bounds_up(uint32_t n, const uint32_t* vals, const float4* lo, 
             load_box((l & BREF_LEAF) ? &lbox[size_t(l & ~BREF_LEAF) * 6] : &ibox[size_t(l) * 6], bl);
             load_box((r & BREF_LEAF) ? &lbox[size_t(r & ~BREF_LEAF) * 6] : &ibox[size_t(r) * 6], br);
             float u[6];
-            for (int k = 0; k < 3; k++) {
-                u[k] = fminf(bl[k], br[k]);
-                u[k + 3] = fmaxf(bl[k + 3], br[k + 3]);
-            }
+            box_union(bl, br, u);
+            float cl[4], cr[4];
+            child_cost(sah, l, bl, cost, cl);
+            child_cost(sah, r, br, cost, cr);
+            float4 cv;
+            uint32_t dv;
+            sah_node(sah, u, cnt[p], cl, cr, cv, dv);
             store_box(&ibox[size_t(p) * 6], u);
+            float* q = reinterpret_cast<float*>(&cost[p]);
+            const float cva[4] = {cv.x, cv.y, cv.z, cv.w};
+            for (int k = 0; k < 4; k++)
+                __hip_atomic_store(&q[k], cva[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            dec[p] = dv;
             __threadfence();
             p = parent[p];
         }
     }
 }
 
-struct CollapseItem {
-    uint32_t bref;  // binary reference (leaf flag) whose subtree becomes this node
-    uint32_t slot;  // output node index
-};
-
-__device__ __forceinline__ float box_area(const float* b)
+__global__ void k_lbvh_roots(uint32_t segs, const uint32_t* seg_begin, const float* ibox, const float* lbox,
+                             uint32_t* root, float* segbox)
 {
-    const float dx = fmaxf(b[3] - b[0], 0.0f), dy = fmaxf(b[4] - b[1], 0.0f), dz = fmaxf(b[5] - b[2], 0.0f);
-    return 2.0f * (dx * dy + dy * dz + dz * dx);
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < segs; s += gridDim.x * blockDim.x) {
+        const uint32_t b = seg_begin[s], e = seg_begin[s + 1];
+        if (e == b)
+            continue;
+        const float* src = (e - b == 1) ? &lbox[size_t(b) * 6] : &ibox[size_t(b) * 6];
+        root[s] = (e - b == 1) ? (b | BREF_LEAF) : b;
+        for (int k = 0; k < 6; k++)
+            segbox[size_t(s) * 6 + k] = src[k];
+    }
 }
 
-// tlas != 0: leaves hold one instance each (value = original primitive = instance index).
+// ---- PLOC (Meister and Bittner, IEEE TVCG 2018): each cluster finds its nearest neighbour
+// among the PLOC_R clusters either side of it in Morton order (distance: surface area of
+// the merged box), mutual nearest neighbours merge, the survivors are compacted in order.
+// Pairs compare by (distance, index gap, parity of the lower index, lower index): a total
+// order, so the closest pair of every segment is mutual and each pass merges at least once
+// per unfinished segment; the parity term pairs (2k, 2k+1) among equal boxes.
+
+constexpr int PLOC_R = 16;
+constexpr int PLOC_B = 256;
+
+__global__ __launch_bounds__(PLOC_B) void k_ploc_nn(uint32_t n, const Cluster* cl, uint32_t* nn)
+{
+    __shared__ float4 slo[PLOC_B + 2 * PLOC_R], shi[PLOC_B + 2 * PLOC_R];
+    const int64_t b0 = int64_t(blockIdx.x) * PLOC_B;
+    for (int t = threadIdx.x; t < PLOC_B + 2 * PLOC_R; t += PLOC_B) {
+        const int64_t g = b0 - PLOC_R + t;
+        if (g >= 0 && g < int64_t(n)) {
+            slo[t] = cl[g].lo;
+            shi[t] = cl[g].hi;
+        } else {
+            slo[t] = make_float4(0.0f, 0.0f, 0.0f, u32_as_float(NONE));  // matches no segment
+            shi[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    __syncthreads();
+    const int64_t i = b0 + threadIdx.x;
+    if (i >= int64_t(n))
+        return;
+    const int li = int(threadIdx.x) + PLOC_R;
+    const float4 a0 = slo[li], a1 = shi[li];
+    const uint32_t sg = float_as_u32(a0.w);
+    uint64_t best = ~0ull;
+    uint32_t bj = NONE;
+    for (int k = 1; k <= PLOC_R; k++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int lj = side ? li + k : li - k;
+            const float4 c0 = slo[lj];
+            if (float_as_u32(c0.w) != sg)
+                continue;
+            const float4 c1 = shi[lj];
+            const float dx = fmaxf(a1.x, c1.x) - fminf(a0.x, c0.x);
+            const float dy = fmaxf(a1.y, c1.y) - fminf(a0.y, c0.y);
+            const float dz = fmaxf(a1.z, c1.z) - fminf(a0.z, c0.z);
+            const float d = fmaxf(dx * dy + dy * dz + dz * dx, 0.0f);
+            const uint32_t lower = uint32_t(side ? i : i - k);
+            const uint64_t key = (uint64_t(float_as_u32(d)) << 32) | (uint32_t(k) << 2) | ((lower & 1u) << 1) |
+                                 uint32_t(side);
+            if (key < best) {
+                best = key;
+                bj = uint32_t(side ? i + k : i - k);
+            }
+        }
+    }
+    nn[i] = bj;
+}
+
+// flags: bit 0 = the cluster survives this pass (unpaired, or the lower of a pair), bit 32 =
+// it becomes a new internal node.
+__global__ void k_ploc_flags(uint32_t n, const uint32_t* nn, uint64_t* flags)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t j = nn[i];
+        const bool mutual = j != NONE && nn[j] == i;
+        flags[i] = uint64_t(!mutual || i < j) | (uint64_t(mutual && i < j) << 32);
+    }
+}
+
+__global__ void k_ploc_merge(uint32_t n, const Cluster* cin, const uint32_t* nn, const uint64_t* flags,
+                             const uint64_t* scan, uint32_t node_base, Cluster* cout, uint32_t* left, uint32_t* right,
+                             uint32_t* cnt, float* ibox, float4* cost, uint32_t* dec, uint32_t* totals, Sah sah)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t f = flags[i], s = scan[i];
+        if (i == n - 1) {
+            totals[0] = uint32_t(s) + uint32_t(f);
+            totals[1] = uint32_t(s >> 32) + uint32_t(f >> 32);
+        }
+        if (!(f & 1u))
+            continue;
+        const uint32_t p = uint32_t(s);
+        if (!(f >> 32)) {
+            cout[p] = cin[i];
+            continue;
+        }
+        const Cluster a = cin[i], b = cin[nn[i]];
+        const uint32_t id = node_base + uint32_t(s >> 32);
+        const uint32_t ra = float_as_u32(a.hi.w), rb = float_as_u32(b.hi.w);
+        const float ba[6] = {a.lo.x, a.lo.y, a.lo.z, a.hi.x, a.hi.y, a.hi.z};
+        const float bb[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+        float u[6];
+        box_union(ba, bb, u);
+        float cl[4], cr[4];
+        child_cost(sah, ra, ba, cost, cl);
+        child_cost(sah, rb, bb, cost, cr);
+        const uint32_t count = ((ra & BREF_LEAF) ? 1u : cnt[ra]) + ((rb & BREF_LEAF) ? 1u : cnt[rb]);
+        float4 cv;
+        uint32_t dv;
+        sah_node(sah, u, count, cl, cr, cv, dv);
+        left[id] = ra;
+        right[id] = rb;
+        cnt[id] = count;
+        for (int k = 0; k < 6; k++)
+            ibox[size_t(id) * 6 + k] = u[k];
+        cost[id] = cv;
+        dec[id] = dv;
+        cout[p] = Cluster{make_float4(u[0], u[1], u[2], a.lo.w), make_float4(u[3], u[4], u[5], u32_as_float(id))};
+    }
+}
+
+// After the last pass: one cluster per non-empty segment.
+__global__ void k_ploc_roots(uint32_t n, const Cluster* cl, uint32_t* root, float* segbox)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Cluster c = cl[i];
+        const uint32_t s = float_as_u32(c.lo.w);
+        root[s] = float_as_u32(c.hi.w);
+        float* o = &segbox[size_t(s) * 6];
+        o[0] = c.lo.x;
+        o[1] = c.lo.y;
+        o[2] = c.lo.z;
+        o[3] = c.hi.x;
+        o[4] = c.hi.y;
+        o[5] = c.hi.z;
+    }
+}
+
+// ---- collapse into 4-wide nodes
+
+struct CollapseItem {
+    uint32_t bref;    // binary reference whose subtree becomes this node
+    uint32_t slot;    // output node index
+    uint32_t offset;  // first output position of the subtree's primitives
+    uint32_t pad;
+};
+
+// One wide node per item: its W child slots are filled as the decisions of sah_node say;
+// a child that is a leaf gets the next output positions of its primitives (depth first,
+// left before right), a child that is an internal node becomes an item of the next level.
+// tlas != 0: every leaf holds one instance (its original primitive index).
 __global__ void k_collapse(const CollapseItem* in, const uint32_t* in_count, CollapseItem* out, uint32_t* out_count,
                            uint32_t* node_counter, uint32_t node_capacity, Node4* nodes, const uint32_t* left,
-                           const uint32_t* right, const uint32_t* first, const uint32_t* last, const float* ibox,
-                           const float* lbox, const uint32_t* vals, uint32_t leaf_size, int tlas)
+                           const uint32_t* right, const uint32_t* cnt, const uint32_t* dec, const float* ibox,
+                           const float* lbox, const uint32_t* vals, uint32_t* perm, uint32_t n, uint32_t width,
+                           int tlas)
 {
-    const uint32_t cnt = *in_count;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < cnt; w += gridDim.x * blockDim.x) {
+    const uint32_t items = *in_count;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < items; w += gridDim.x * blockDim.x) {
         const CollapseItem it = in[w];
-        uint32_t c[4];
-        int nc;
+        uint32_t c[4], off[4];
+        int nc = 0;
         if (it.bref & BREF_LEAF) {  // a single-primitive set
             c[0] = it.bref;
+            off[0] = it.offset;
             nc = 1;
         } else {
-            c[0] = left[it.bref];
-            c[1] = right[it.bref];
-            nc = 2;
-        }
-        auto size_of = [&](uint32_t r) -> uint32_t {
-            return (r & BREF_LEAF) ? 1u : (last[r] - first[r] + 1u);
-        };
-        while (nc < 4) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int k = 0; k < nc; k++) {
-                if ((c[k] & BREF_LEAF) || size_of(c[k]) <= leaf_size)
+            struct Open {
+                uint32_t ref, slots, offset;
+            };
+            Open st[4];
+            int sp = 0;
+            const uint32_t kl = (dec[it.bref] >> 16) & 3u;
+            const uint32_t l = left[it.bref], r = right[it.bref];
+            st[sp++] = Open{r, width - kl, it.offset + ((l & BREF_LEAF) ? 1u : cnt[l])};
+            st[sp++] = Open{l, kl, it.offset};
+            // slots sum to at most width <= 4, so at most 4 entries are ever pending
+            while (sp > 0 && nc < 4) {
+                const Open e = st[--sp];
+                uint32_t code = 0;
+                if (!(e.ref & BREF_LEAF) && e.slots >= 2)
+                    code = (dec[e.ref] >> (4 * (e.slots - 1))) & 15u;
+                if (code == 0 || sp > 2) {
+                    c[nc] = e.ref;
+                    off[nc] = e.offset;
+                    nc++;
                     continue;
-                const float a = box_area(&ibox[size_t(c[k]) * 6]);
-                if (a > best_area) {
-                    best_area = a;
-                    best = k;
                 }
+                const uint32_t el = left[e.ref], er = right[e.ref];
+                st[sp++] = Open{er, code >> 2, e.offset + ((el & BREF_LEAF) ? 1u : cnt[el])};
+                st[sp++] = Open{el, code & 3u, e.offset};
             }
-            if (best < 0)
-                break;
-            const uint32_t r = c[best];
-            c[best] = left[r];
-            c[nc++] = right[r];
         }
         Node4 node;
         for (int k = 0; k < 4; k++) {
@@ -304,18 +579,38 @@ __global__ void k_collapse(const CollapseItem* in, const uint32_t* in_count, Col
             node.hi_x[k] = b[3];
             node.hi_y[k] = b[4];
             node.hi_z[k] = b[5];
-            const uint32_t sz = size_of(r);
-            if ((r & BREF_LEAF) || sz <= leaf_size) {
-                const uint32_t f = (r & BREF_LEAF) ? (r & ~BREF_LEAF) : first[r];
-                if (tlas)
-                    node.child[k] = CHILD_LEAF | vals[f];
-                else
-                    node.child[k] = CHILD_LEAF | ((sz - 1u) << 27) | f;
+            if (r & BREF_LEAF) {
+                const uint32_t pos = r & ~BREF_LEAF;
+                if (tlas) {
+                    node.child[k] = CHILD_LEAF | vals[pos];
+                } else {
+                    if (off[k] < n)
+                        perm[off[k]] = pos;
+                    node.child[k] = CHILD_LEAF | off[k];
+                }
+            } else if (!tlas && (dec[r] & 1u)) {
+                // a leaf of cnt[r] <= 16 primitives: its subtree's leaves in order
+                uint32_t st2[16];
+                int sp2 = 0;
+                uint32_t o = off[k];
+                st2[sp2++] = r;
+                while (sp2 > 0) {
+                    const uint32_t x = st2[--sp2];
+                    if (x & BREF_LEAF) {
+                        if (o < n)
+                            perm[o] = x & ~BREF_LEAF;
+                        o++;
+                    } else if (sp2 <= 14) {
+                        st2[sp2++] = right[x];
+                        st2[sp2++] = left[x];
+                    }
+                }
+                node.child[k] = CHILD_LEAF | ((cnt[r] - 1u) << 27) | off[k];
             } else {
                 const uint32_t slot = atomicAdd(node_counter, 1u);
                 if (slot < node_capacity) {
                     const uint32_t q = atomicAdd(out_count, 1u);
-                    out[q] = CollapseItem{r, slot};
+                    out[q] = CollapseItem{r, slot, off[k], 0u};
                     node.child[k] = slot;
                 } else {
                     node.child[k] = CHILD_EMPTY;  // capacity guard; the host reports the overflow
@@ -353,12 +648,13 @@ __global__ void k_tri_bounds(uint32_t n, const float4* vertices, const uint32_t*
     }
 }
 
-__global__ void k_tri_records(uint32_t n, const uint32_t* vals, const uint32_t* seg, const float4* vertices,
-                              const uint32_t* indices, const pt_mesh_desc* meshes, const uint32_t* tri_begin,
-                              TriRec* tris)
+// Record i holds the primitive at sorted position perm[i] (leaf order).
+__global__ void k_tri_records(uint32_t n, const uint32_t* perm, const uint32_t* vals, const uint32_t* seg,
+                              const float4* vertices, const uint32_t* indices, const pt_mesh_desc* meshes,
+                              const uint32_t* tri_begin, TriRec* tris)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p = vals[i];
+        const uint32_t p = vals[perm[i]];
         const uint32_t m = seg[p];
         const pt_mesh_desc md = meshes[m];
         const uint32_t t = p - tri_begin[m];
@@ -375,8 +671,8 @@ __global__ void k_tri_records(uint32_t n, const uint32_t* vals, const uint32_t* 
 
 // World bounds of each instance from its BLAS root box (8 transformed corners).
 __global__ void k_instance_bounds(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
-                                  const uint32_t* tri_begin, const float* ibox, const float* lbox,
-                                  const uint32_t* mesh_tris, float4* lo, float4* hi, uint32_t* seg)
+                                  const float* mesh_box, const uint32_t* mesh_tris, float4* lo, float4* hi,
+                                  uint32_t* seg)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t m = inst_mesh[i];
@@ -387,8 +683,7 @@ __global__ void k_instance_bounds(uint32_t n, const InstanceData* inst, const ui
             hi[i] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
             continue;
         }
-        const uint32_t root = tri_begin[m];
-        const float* b = nt == 1 ? &lbox[size_t(root) * 6] : &ibox[size_t(root) * 6];
+        const float* b = &mesh_box[size_t(m) * 6];
         const InstanceData& d = inst[i];
         float3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
         for (int c = 0; c < 8; c++) {
@@ -439,11 +734,12 @@ struct Scratch {
     }
 };
 
-// Builds the radix tree and collapses it into nodes[slot0 + s] (root of segment s) and
-// further slots from *node_counter (device). Prim bounds / seg must be filled.
+// Builds the binary tree of every segment (PLOC or Karras), its SAH collapse decisions, and
+// collapses it into nodes[slot0 + s] (root of segment s) and further slots from
+// *node_counter (device). Prim bounds / seg must be filled. BLAS sets also get bs.perm.
 hipError_t build_set(hipStream_t st, Scratch& sc, BuildSet& bs, const std::vector<uint32_t>& seg_begin_host,
-                     uint32_t leaf_size, int tlas, Node4* nodes, uint32_t slot0, uint32_t* d_node_counter,
-                     uint32_t node_capacity)
+                     const Sah& sah, uint32_t builder, int tlas, Node4* nodes, uint32_t slot0,
+                     uint32_t* d_node_counter, uint32_t node_capacity)
 {
     const uint32_t n = bs.n;
     const unsigned B = 256;
@@ -470,31 +766,83 @@ hipError_t build_set(hipStream_t st, Scratch& sc, BuildSet& bs, const std::vecto
     CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, dk, dv, int(n), 0, end_bit, st));
     uint64_t* keys = dk.Current();
     uint32_t* vals = dv.Current();
-
-    CHECK(sc.alloc(bs.left, n));
-    CHECK(sc.alloc(bs.right, n));
-    CHECK(sc.alloc(bs.first, n));
-    CHECK(sc.alloc(bs.last, n));
-    CHECK(sc.alloc(bs.parent, n));
-    CHECK(sc.alloc(bs.leaf_parent, n));
-    CHECK(sc.alloc(bs.visit, n));
-    CHECK(sc.alloc(bs.ibox, size_t(n) * 6));
-    CHECK(sc.alloc(bs.lbox, size_t(n) * 6));
-    CHECK(hipMemsetAsync(bs.visit, 0, size_t(n) * 4, st));
-    hipLaunchKernelGGL(k_karras, dim3(grid_for(n)), dim3(B), 0, st, n, keys, bs.seg_begin, bs.left, bs.right, bs.first,
-                       bs.last, bs.parent, bs.leaf_parent);
-    hipLaunchKernelGGL(k_bounds_up, dim3(grid_for(n)), dim3(B), 0, st, n, vals, bs.lo, bs.hi, bs.left, bs.right,
-                       bs.parent, bs.leaf_parent, bs.visit, bs.ibox, bs.lbox);
     bs.vals = vals;
     bs.keys = keys;
 
+    CHECK(sc.alloc(bs.left, n));
+    CHECK(sc.alloc(bs.right, n));
+    CHECK(sc.alloc(bs.cnt, n));
+    CHECK(sc.alloc(bs.dec, n));
+    CHECK(sc.alloc(bs.cost, n));
+    CHECK(sc.alloc(bs.ibox, size_t(n) * 6));
+    CHECK(sc.alloc(bs.lbox, size_t(n) * 6));
+    CHECK(sc.alloc(bs.root, bs.segs));
+    CHECK(sc.alloc(bs.segbox, size_t(bs.segs) * 6));
+    hipLaunchKernelGGL(k_empty_boxes, dim3(grid_for(bs.segs)), dim3(B), 0, st, bs.segs, bs.segbox, bs.root);
+
+    if (builder == BVH_BUILDER_LBVH) {
+        uint32_t *parent, *leaf_parent, *visit;
+        CHECK(sc.alloc(parent, n));
+        CHECK(sc.alloc(leaf_parent, n));
+        CHECK(sc.alloc(visit, n));
+        CHECK(hipMemsetAsync(visit, 0, size_t(n) * 4, st));
+        hipLaunchKernelGGL(k_leaves, dim3(grid_for(n)), dim3(B), 0, st, n, vals, keys, bs.lo, bs.hi, bs.lbox,
+                           (Cluster*)nullptr);
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(n)), dim3(B), 0, st, n, keys, bs.seg_begin, bs.left, bs.right,
+                           bs.cnt, parent, leaf_parent);
+        hipLaunchKernelGGL(k_bounds_up, dim3(grid_for(n)), dim3(B), 0, st, n, bs.left, bs.right, bs.cnt, parent,
+                           leaf_parent, visit, bs.ibox, bs.lbox, bs.cost, bs.dec, sah);
+        hipLaunchKernelGGL(k_lbvh_roots, dim3(grid_for(bs.segs)), dim3(B), 0, st, bs.segs, bs.seg_begin, bs.ibox,
+                           bs.lbox, bs.root, bs.segbox);
+        CHECK(hipGetLastError());
+    } else {
+        Cluster *ca, *cb;
+        uint32_t *nn, *totals;
+        uint64_t *flags, *scan;
+        CHECK(sc.alloc(ca, n));
+        CHECK(sc.alloc(cb, n));
+        CHECK(sc.alloc(nn, n));
+        CHECK(sc.alloc(flags, n));
+        CHECK(sc.alloc(scan, n));
+        CHECK(sc.alloc(totals, 2));
+        size_t scan_bytes = 0;
+        CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags, scan, int(n), st));
+        uint8_t* scan_temp;
+        CHECK(sc.alloc(scan_temp, scan_bytes));
+        hipLaunchKernelGGL(k_leaves, dim3(grid_for(n)), dim3(B), 0, st, n, vals, keys, bs.lo, bs.hi, bs.lbox, ca);
+        uint32_t m = n, node_base = 0;
+        // every pass merges at least one pair of each segment that still has two clusters
+        for (uint32_t pass = 0; pass < n; pass++) {
+            hipLaunchKernelGGL(k_ploc_nn, dim3((m + PLOC_B - 1) / PLOC_B), dim3(PLOC_B), 0, st, m, ca, nn);
+            hipLaunchKernelGGL(k_ploc_flags, dim3(grid_for(m)), dim3(B), 0, st, m, nn, flags);
+            CHECK(hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, flags, scan, int(m), st));
+            hipLaunchKernelGGL(k_ploc_merge, dim3(grid_for(m)), dim3(B), 0, st, m, ca, nn, flags, scan, node_base, cb,
+                               bs.left, bs.right, bs.cnt, bs.ibox, bs.cost, bs.dec, totals, sah);
+            CHECK(hipGetLastError());
+            uint32_t t[2];
+            CHECK(hipMemcpyAsync(t, totals, 8, hipMemcpyDeviceToHost, st));
+            CHECK(hipStreamSynchronize(st));
+            std::swap(ca, cb);
+            m = t[0];
+            node_base += t[1];
+            if (t[1] == 0)
+                break;
+        }
+        hipLaunchKernelGGL(k_ploc_roots, dim3(grid_for(m)), dim3(B), 0, st, m, ca, bs.root, bs.segbox);
+        CHECK(hipGetLastError());
+    }
+
     // Top-down collapse, one launch per level.
+    std::vector<uint32_t> root(bs.segs);
+    CHECK(hipMemcpyAsync(root.data(), bs.root, size_t(bs.segs) * 4, hipMemcpyDeviceToHost, st));
+    CHECK(hipStreamSynchronize(st));
     std::vector<CollapseItem> roots;
-    for (uint32_t s = 0; s < bs.segs; s++) {
-        const uint32_t b = seg_begin_host[s], e = seg_begin_host[s + 1];
-        if (e == b)
-            continue;
-        roots.push_back(CollapseItem{(e - b == 1) ? (b | BREF_LEAF) : b, slot0 + s});
+    for (uint32_t s = 0; s < bs.segs; s++)
+        if (root[s] != NONE)
+            roots.push_back(CollapseItem{root[s], slot0 + s, seg_begin_host[s], 0u});
+    if (!tlas) {
+        CHECK(sc.alloc(bs.perm, n));
+        CHECK(hipMemsetAsync(bs.perm, 0, size_t(n) * 4, st));  // a hole would repeat primitive 0 (validation)
     }
     CollapseItem *qa, *qb;
     uint32_t* counts;  // [count_a, count_b]
@@ -507,20 +855,20 @@ hipError_t build_set(hipStream_t st, Scratch& sc, BuildSet& bs, const std::vecto
     uint32_t host_count = uint32_t(roots.size());
     CHECK(hipMemcpyAsync(&counts[0], &host_count, 4, hipMemcpyHostToDevice, st));
     int cur = 0;
-    for (int level = 0; level < 128 && host_count > 0; level++) {
+    for (int level = 0; level < 4096 && host_count > 0; level++) {
         CollapseItem* qin = cur ? qb : qa;
         CollapseItem* qout = cur ? qa : qb;
         CHECK(hipMemsetAsync(&counts[1 - cur], 0, 4, st));
         hipLaunchKernelGGL(k_collapse, dim3(grid_for(host_count)), dim3(B), 0, st, qin, &counts[cur], qout,
-                           &counts[1 - cur], d_node_counter, node_capacity, nodes, bs.left, bs.right, bs.first,
-                           bs.last, bs.ibox, bs.lbox, vals, leaf_size, tlas);
+                           &counts[1 - cur], d_node_counter, node_capacity, nodes, bs.left, bs.right, bs.cnt, bs.dec,
+                           bs.ibox, bs.lbox, vals, bs.perm, n, sah.width, tlas);
         CHECK(hipGetLastError());
         CHECK(hipMemcpyAsync(&host_count, &counts[1 - cur], 4, hipMemcpyDeviceToHost, st));
         CHECK(hipStreamSynchronize(st));
         cur = 1 - cur;
     }
     if (host_count > 0)
-        return hipErrorUnknown;  // tree deeper than the level cap (cannot happen for 64-bit keys)
+        return hipErrorUnknown;  // deeper than the level cap
     return hipSuccess;
 }
 
@@ -550,6 +898,9 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
     }
     const uint32_t T = tri_begin[M];
     const uint32_t leaf = std::min<uint32_t>(std::max<uint32_t>(params.blas_leaf_size, 1), LEAF_MAX_TRIS);
+    const uint32_t width = std::min<uint32_t>(std::max<uint32_t>(params.node_width, 2), 4);
+    const Sah blas_sah{width, leaf, params.sah_node_cost, params.sah_tri_cost};
+    const Sah tlas_sah{width, 1, params.sah_node_cost, params.sah_instance_cost};
 
     // Node budget: BLAS roots + at most one node per binary internal node; TLAS likewise.
     const uint32_t blas_cap = M + T + 1;
@@ -584,8 +935,7 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
     CHECK(hipMemcpyAsync(d_counter, &counter, 4, hipMemcpyHostToDevice, st));
 
     BuildSet blas;
-    float* blas_ibox = nullptr;
-    float* blas_lbox = nullptr;
+    float* mesh_box = nullptr;  // root box of each mesh's BLAS
     if (T > 0) {
         blas.n = T;
         blas.segs = M;
@@ -595,11 +945,10 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         CHECK(sc.alloc(blas.seg, T));
         hipLaunchKernelGGL(k_tri_bounds, dim3(grid_for(T)), dim3(256), 0, st, T, d_vertices, d_indices, d_meshes,
                            d_tri_begin, M, blas.lo, blas.hi, blas.seg);
-        CHECK(build_set(st, sc, blas, tri_begin, leaf, 0, out.nodes, 0, d_counter, blas_cap));
-        hipLaunchKernelGGL(k_tri_records, dim3(grid_for(T)), dim3(256), 0, st, T, blas.vals, blas.seg, d_vertices,
-                           d_indices, d_meshes, d_tri_begin, out.tris);
-        blas_ibox = blas.ibox;
-        blas_lbox = blas.lbox;
+        CHECK(build_set(st, sc, blas, tri_begin, blas_sah, params.builder, 0, out.nodes, 0, d_counter, blas_cap));
+        hipLaunchKernelGGL(k_tri_records, dim3(grid_for(T)), dim3(256), 0, st, T, blas.perm, blas.vals, blas.seg,
+                           d_vertices, d_indices, d_meshes, d_tri_begin, out.tris);
+        mesh_box = blas.segbox;
     }
     CHECK(hipMemcpyAsync(&counter, d_counter, 4, hipMemcpyDeviceToHost, st));
     CHECK(hipStreamSynchronize(st));
@@ -674,12 +1023,13 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         float* dummy_box = nullptr;
         CHECK(sc.alloc(dummy_box, 6));
         hipLaunchKernelGGL(k_instance_bounds, dim3(grid_for(instance_count)), dim3(256), 0, st, instance_count,
-                           d_instances, d_instance_mesh, d_tri_begin, blas_ibox ? blas_ibox : dummy_box,
-                           blas_lbox ? blas_lbox : dummy_box, d_mesh_tris, tl.lo, tl.hi, tl.seg);
+                           d_instances, d_instance_mesh, mesh_box ? mesh_box : dummy_box, d_mesh_tris, tl.lo, tl.hi,
+                           tl.seg);
         const uint32_t tlas_slot = blas_nodes;
         counter = blas_nodes + 1;
         CHECK(hipMemcpyAsync(d_counter, &counter, 4, hipMemcpyHostToDevice, st));
-        CHECK(build_set(st, sc, tl, sb, 1, 1, out.nodes, tlas_slot, d_counter, out.node_capacity));
+        CHECK(build_set(st, sc, tl, sb, tlas_sah, params.builder, 1, out.nodes, tlas_slot, d_counter,
+                        out.node_capacity));
         CHECK(hipMemcpyAsync(&counter, d_counter, 4, hipMemcpyDeviceToHost, st));
         CHECK(hipStreamSynchronize(st));
         if (counter > out.node_capacity)
