@@ -177,9 +177,9 @@ typedef struct pt_stats {
 typedef struct pt_ctx pt_ctx;
 
 /* PathTracer::PathTracer + SceneBuffer upload + AccelerationStructure build
- * (src/pathtrace/pathtrace.cpp:81-116, accstruct.cpp:288-296). Builds one LBVH per mesh and
- * a TLAS over the instances on the device. width/height size the accumulation image
- * (PathTraceDispatcher ctor, src/pathtrace/ptdispatch.cpp:26-27). */
+ * (src/pathtrace/pathtrace.cpp:81-116, accstruct.cpp:288-296). Builds one BVH per mesh and
+ * a TLAS over the instances on the device. width/height (1..65535 each) size the
+ * accumulation image (PathTraceDispatcher ctor, src/pathtrace/ptdispatch.cpp:26-27). */
 int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out);
 
 /* PathTracer::set_camera (src/pathtrace/pathtrace.cpp:381-396). Resets sample_index to 0. */

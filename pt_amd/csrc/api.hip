@@ -445,7 +445,8 @@ int check_view(const pt_view& v, uint32_t scene_count, std::string& why)
 {
     if (v.scene >= scene_count)
         return (why = "view scene index out of range", PT_EINVAL);
-    if (v.width == 0 || v.height == 0 || uint64_t(v.width) * v.height > (1ull << 27))
+    if (v.width == 0 || v.height == 0 || v.width > 65535 || v.height > 65535 ||
+        uint64_t(v.width) * v.height > (1ull << 27))
         return (why = "invalid view size", PT_EINVAL);
     if (v.spp == 0)
         return (why = "Sample count was zero.", PT_EINVAL);
@@ -477,8 +478,8 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
     if (!out)
         return fail(nullptr, PT_EINVAL, "out is null");
     *out = nullptr;
-    if (width == 0 || height == 0 || uint64_t(width) * height > 0x7FFFFFFFull)
-        return fail(nullptr, PT_EINVAL, "invalid image size");
+    if (width == 0 || height == 0 || width > 65535 || height > 65535 || uint64_t(width) * height > 0x7FFFFFFFull)
+        return fail(nullptr, PT_EINVAL, "invalid image size (sides 1..65535)");
     std::string why;
     if (int v = validate_blob(scene, why))
         return fail(nullptr, v, why);
@@ -720,9 +721,9 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
         return fail(&c, PT_EINVAL, "out is null");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
-    uint32_t counters[4] = {0, 0, 0, 0};
+    uint32_t error_flags = 0;
     HIP_TRY(&c, hipMemcpy(totals, c.wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
-    HIP_TRY(&c, hipMemcpy(counters, c.wf.counters, sizeof(counters), hipMemcpyDeviceToHost), "stats");
+    HIP_TRY(&c, hipMemcpy(&error_flags, c.wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "stats");
     float ms = 0.0f;
     if (c.last_paths)
         HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
@@ -755,7 +756,7 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
         s.shadow_launches = kl[KERNEL_SHADOW];
     }
     *out = s;
-    if (counters[3])
+    if (error_flags)
         return fail(&c, PT_EDEVICE, "traversal stack overflow");
     return PT_OK;
 }
@@ -1179,9 +1180,9 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
         return bfail(b, PT_EINVAL, "out is null");
     BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
     unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
-    uint32_t counters[4] = {0, 0, 0, 0};
+    uint32_t error_flags = 0;
     BHIP_TRY(b, hipMemcpy(totals, b->wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
-    BHIP_TRY(b, hipMemcpy(counters, b->wf.counters, sizeof(counters), hipMemcpyDeviceToHost), "stats");
+    BHIP_TRY(b, hipMemcpy(&error_flags, b->wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "stats");
     float ms = 0.0f;
     if (b->last_paths)
         BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev_start, b->ev_stop), "event");
@@ -1212,7 +1213,7 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
         s.shadow_launches = kl[KERNEL_SHADOW];
     }
     *out = s;
-    if (counters[3])
+    if (error_flags)
         return bfail(b, PT_EDEVICE, "traversal stack overflow");
     return PT_OK;
 }

@@ -21,9 +21,14 @@ constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to
 // pixel x every sample of a pt_render chunk) stream through the pool: a slot whose path
 // ends is refilled with the next path of the job, so every extend/shadow launch stays full
 // until the job's last paths drain.
+// The wavefront counters sit 4 KB apart: the atomics of the queue appends then go to
+// different memory channels instead of queueing on one line.
+constexpr uint32_t CTR_STRIDE = 1024;
+__host__ __device__ constexpr uint32_t CTR(uint32_t k) { return k * CTR_STRIDE; }
+
 struct Wavefront {
     uint32_t capacity = 0;         // slots
-    // per slot: job path index, pixel within its view, sample index, view << 16 | segment
+    // per slot: job path index, py << 16 | px of its pixel, sample index, view << 16 | segment
     uint4* pstate = nullptr;
     float4* thr_pdf = nullptr;     // throughput rgb, brdf pdf
     float4* rad = nullptr;         // radiance rgb
@@ -41,8 +46,8 @@ struct Wavefront {
     uint32_t* retire = nullptr;    // slots whose path ended this iteration
     float4* rad_out = nullptr;     // radiance of every path of the job, by path id
     uint32_t rad_out_capacity = 0; // paths
-    // [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] retire queue,
-    // [5] next path to start
+    // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] retire
+    // queue, [5] next path to start
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)

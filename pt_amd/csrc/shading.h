@@ -454,28 +454,77 @@ __device__ __forceinline__ float2 vertex_uv(const DeviceScene& s, uint32_t v)
     return make_float2(t.z, t.w);
 }
 
-__device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, float3 pt, float3& light_dir,
-                                               float& light_dist, float& pdf, bool& is_delta)
+// sample_light (lightsample.hlsli:7-143) split at its dependent loads, so that the shade
+// stage keeps its own loads in flight between them: alias_pick / alias_follow are the two
+// halves of sample_alias_table (the bin u falls in, then that bin or its alias, u rescaled
+// each time), load_alias_bin reads one bin, light_sample_at evaluates the chosen light at the
+// shading point from what the walk read.
+template <int PAYLOAD>
+struct AliasBin {
+    float p, q;
+    uint32_t alias;
+    uint32_t w[PAYLOAD];  // the bin's payload words (light: type and 6 words; emitter: 3 vertices)
+};
+
+template <int PAYLOAD>
+__device__ __forceinline__ AliasBin<PAYLOAD> load_alias_bin(const uint32_t* b)
 {
-    uint32_t bo = sample_alias_table(s.light, s.light_count, PT_LIGHT_BIN_WORDS, 0, u.x, pdf);
-    const uint32_t type = s.light[bo + 0];
+    AliasBin<PAYLOAD> r;
+    r.p = u32_as_float(b[0]);
+    r.q = u32_as_float(b[1]);
+    r.alias = b[2];
+    for (int k = 0; k < PAYLOAD; k++)
+        r.w[k] = b[3 + k];
+    return r;
+}
+
+__device__ __forceinline__ uint32_t alias_pick(uint32_t bin_count, float& u)
+{
+    const uint32_t i = min(bin_count - 1, uint32_t(u * float(bin_count)));
+    u = fminf((u * float(bin_count)) - float(i), just_below_one());
+    return i;
+}
+
+// True when the alias is taken.
+__device__ __forceinline__ bool alias_follow(float q, float& u)
+{
+    if (u < q) {
+        u = fminf(u / q, just_below_one());
+        return false;
+    }
+    u = fminf((u - q) / (1.0f - q), just_below_one());
+    return true;
+}
+
+__device__ __forceinline__ float3 xform_point_rows(float4 r0, float4 r1, float4 r2, float3 p)
+{
+    return f3(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
+              r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
+}
+
+__device__ __forceinline__ float3 xform_rows(float4 r0, float4 r1, float4 r2, float3 v)
+{
+    return f3(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
+              r2.x * v.x + r2.y * v.y + r2.z * v.z);
+}
+
+// lb: the chosen light bin; for an emitting object (type 3) also eb: the chosen triangle bin,
+// eh / m0-m2: the emitter instance's indices and object->world rows, v0-v2: the triangle's
+// vertices, emission / rmnb: the emitter material's emission and map bits.
+__device__ __forceinline__ float3 light_sample_at(const DeviceScene& s, const AliasBin<7>& lb, const AliasBin<3>& eb,
+                                                  uint4 eh, float4 m0, float4 m1, float4 m2, float4 v0, float4 v1,
+                                                  float4 v2, float4 emission, float4 rmnb, float2 u12, float3 pt,
+                                                  float3& light_dir, float& light_dist, float& pdf, bool& is_delta)
+{
+    pdf = lb.p;
+    const uint32_t type = lb.w[0];
     if (type == 3u) {
-        const uint32_t table_offset = s.light[bo + 1];
-        const uint32_t tri_count = s.light[bo + 2];
-        const uint32_t inst_off = s.light[bo + 3];
-        const uint32_t inst_count = s.light[bo + 4];
-        float p;
-        bo = sample_alias_table(s.light, tri_count, PT_EMITTER_BIN_WORDS, table_offset, u.x, p);
-        pdf *= p;
-        const uint32_t inst = inst_off + min(inst_count - 1, uint32_t(u.x * float(inst_count)));
-        const InstanceData& in = s.instances[inst];
-        const float3 pa = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 0]));
-        const float3 pb = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 1]));
-        const float3 pc = xform_point(in, vertex_pos(s, in.vertex_index + s.light[bo + 2]));
-        const float3 bary = random_barycentric(make_float2(u.y, u.z));
+        pdf *= eb.p;
+        const float3 pa = xform_point_rows(m0, m1, m2, xyz(v0));
+        const float3 pb = xform_point_rows(m0, m1, m2, xyz(v1));
+        const float3 pc = xform_point_rows(m0, m1, m2, xyz(v2));
+        const float3 bary = random_barycentric(u12);
         const float3 on_tri = bary.x * pa + bary.y * pb + bary.z * pc;
-        const uint32_t va = in.vertex_index + s.light[bo + 0], vb = in.vertex_index + s.light[bo + 1],
-                       vc = in.vertex_index + s.light[bo + 2];
         light_dir = on_tri - pt;
         light_dist = 1.0f;
         const float3 tn = cross3(pb - pa, pc - pa);
@@ -486,25 +535,25 @@ __device__ __forceinline__ float3 sample_light(const DeviceScene& s, float3 u, f
         is_delta = false;
         if (!isfinite(pdf))
             return f3(0, 0, 0);
-        const MaterialRec& m = s.materials[in.material_index];
-        if (!(map_bits(m) & MAP_EMISSION))
-            return emission_of(m);
-        const float2 ua = vertex_uv(s, va), ub = vertex_uv(s, vb), uc = vertex_uv(s, vc);
+        if (!(float_as_u32(rmnb.w) & MAP_EMISSION))
+            return f3(emission.x, emission.y, emission.z) * emission.w;
+        const float2 ua = vertex_uv(s, eh.x + eb.w[0]), ub = vertex_uv(s, eh.x + eb.w[1]),
+                     uc = vertex_uv(s, eh.x + eb.w[2]);
         const float2 uv = make_float2(bary.x * ua.x + bary.y * ub.x + bary.z * uc.x,
                                       bary.x * ua.y + bary.y * ub.y + bary.z * uc.y);
-        return material_emission(s, m, uv);
+        return material_emission(s, s.materials[eh.w], uv);
     } else if (type == 1u) {
-        const float3 lp = f3(u32_as_float(s.light[bo + 1]), u32_as_float(s.light[bo + 2]), u32_as_float(s.light[bo + 3]));
+        const float3 lp = f3(u32_as_float(lb.w[1]), u32_as_float(lb.w[2]), u32_as_float(lb.w[3]));
         light_dir = lp - pt;
         light_dist = 1.0f;
         is_delta = true;
         return (1.0f / dot3(light_dir, light_dir)) *
-               f3(u32_as_float(s.light[bo + 4]), u32_as_float(s.light[bo + 5]), u32_as_float(s.light[bo + 6]));
+               f3(u32_as_float(lb.w[4]), u32_as_float(lb.w[5]), u32_as_float(lb.w[6]));
     } else if (type == 2u) {
-        light_dir = f3(u32_as_float(s.light[bo + 1]), u32_as_float(s.light[bo + 2]), u32_as_float(s.light[bo + 3]));
+        light_dir = f3(u32_as_float(lb.w[1]), u32_as_float(lb.w[2]), u32_as_float(lb.w[3]));
         light_dist = INFINITY;
         is_delta = true;
-        return f3(u32_as_float(s.light[bo + 4]), u32_as_float(s.light[bo + 5]), u32_as_float(s.light[bo + 6]));
+        return f3(u32_as_float(lb.w[4]), u32_as_float(lb.w[5]), u32_as_float(lb.w[6]));
     }
     light_dist = 0.0f;
     is_delta = true;

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -344,20 +345,47 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Appends `want` lanes of the wave to a queue with one atomic per wave; returns the slot.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
+// Positions in NC queues for the threads of a block that want one, with one atomic per
+// queue and block: device-scope atomics on one address serialise across the whole chip, so
+// the shade and refill stages aggregate over the block instead of the wave. Every thread of
+// the block must call it (block-uniform control flow).
+template <int NC>
+__device__ __forceinline__ void block_append(const bool (&want)[NC], uint32_t* const (&counter)[NC],
+                                             uint32_t (&pos)[NC])
 {
-    const unsigned long long mask = __ballot(want);
-    const uint32_t lane = lane_id();
-    const uint32_t n = __popcll(mask);
-    uint32_t base = 0;
-    if (n) {
-        const uint32_t leader = __ffsll((long long)mask) - 1;
-        if (lane == leader)
-            base = atomicAdd(counter, n);
-        base = __shfl(base, leader);
+    __shared__ uint32_t s_wave[NC][TRACE_BLOCK / 64];
+    __shared__ uint32_t s_base[NC];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    unsigned long long mask[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        mask[k] = __ballot(want[k]);
+        if (lane == 0)
+            s_wave[k][wave] = uint32_t(__popcll(mask[k]));
     }
-    return base + __popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (threadIdx.x < NC) {
+        const int k = int(threadIdx.x);
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < waves; w++) {
+            const uint32_t c = s_wave[k][w];
+            s_wave[k][w] = total;
+            total += c;
+        }
+        s_base[k] = total ? atomicAdd(counter[k], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+        pos[k] = s_base[k] + s_wave[k][wave] + uint32_t(__popcll(mask[k] & ((1ull << lane) - 1ull)));
+    __syncthreads();  // the next call reuses s_wave / s_base
+}
+
+__device__ __forceinline__ uint32_t block_append1(bool want, uint32_t* counter)
+{
+    uint32_t pos[1];
+    block_append<1>({want}, {counter}, pos);
+    return pos[0];
 }
 
 // --- kernels -------------------------------------------------------------------------------
@@ -397,7 +425,7 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     tmin = vw.cam.near_clip;
     tmax = vw.cam.far_clip;
     scene = vw.scene;
-    wf.pstate[s] = make_uint4(p, pix, sample, sg.view << 16);  // segment 0
+    wf.pstate[s] = make_uint4(p, (py << 16) | px, sample, sg.view << 16);  // segment 0
     wf.thr_pdf[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     wf.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -421,11 +449,11 @@ __global__ __launch_bounds__(256) void k_start_job(const ViewDev* views, const J
         wf.q_scene[0][s] = scene;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        wf.counters[0] = n;
-        wf.counters[1] = 0;
-        wf.counters[2] = 0;
-        wf.counters[4] = 0;
-        wf.counters[5] = n;
+        wf.counters[CTR(0)] = n;
+        wf.counters[CTR(1)] = 0;
+        wf.counters[CTR(2)] = 0;
+        wf.counters[CTR(4)] = 0;
+        wf.counters[CTR(5)] = n;
     }
 }
 
@@ -433,7 +461,7 @@ template <bool COUNT, bool MULTI, uint32_t D>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[D * TRACE_BLOCK];
-    const uint32_t count = wf.counters[q];
+    const uint32_t count = wf.counters[CTR(q)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
@@ -447,7 +475,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(DeviceScene s0, const De
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
         if (stk.overflow)
-            atomicOr(&wf.counters[3], 1u);
+            atomicOr(&wf.counters[CTR(3)], 1u);
     }
     if (COUNT && (n_nodes | n_tris)) {
         atomicAdd(&wf.totals[2], (unsigned long long)n_nodes);
@@ -461,7 +489,7 @@ template <bool COUNT, bool MULTI, uint32_t D>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const DeviceScene* scenes, Wavefront wf)
 {
     __shared__ uint32_t lds[D * TRACE_BLOCK];
-    const uint32_t count = wf.counters[2];
+    const uint32_t count = wf.counters[CTR(2)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
@@ -482,7 +510,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const De
             wf.rad[slot] = r;
         }
         if (stk.overflow)
-            atomicOr(&wf.counters[3], 1u);
+            atomicOr(&wf.counters[CTR(3)], 1u);
     }
     if (COUNT && (n_nodes | n_tris)) {
         atomicAdd(&wf.totals[4], (unsigned long long)n_nodes);
@@ -494,13 +522,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DeviceScene s0, const De
 
 // Miss / closest-hit / Russian roulette of one segment for every ray of queue q. Continuing
 // paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
+// WPE: waves per SIMD the register allocation must allow (3 = no limit beyond the block).
+template <bool MULTI, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_shade(
+    DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
                                                const JobDev* job, Wavefront wf, int q, CacheDev cd,
                                                const uint32_t* order)
 {
-    const uint32_t count = wf.counters[q];
-    const uint32_t rounded = (count + 63u) & ~63u;  // whole waves take part in the compaction
+    const uint32_t count = wf.counters[CTR(q)];
+    const uint32_t rounded = (count + 255u) & ~255u;  // whole blocks take part in the compaction
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
         const bool valid = i < count;
         bool cont = false, shadow = false;
@@ -516,41 +546,67 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
         uint32_t sid = 0;
         if (valid) {
             const uint32_t qi = order ? order[i] : i;  // material-sorted order when given
+            // Loads are grouped by the round trip that resolves their addresses, and every
+            // group is issued before the data of the previous one are used: (1) the queue
+            // entry; (2) path state and the hit instance; (3) view, material, triangle indices
+            // and the light table's first level; (4) vertices and the chosen light bin; then,
+            // for next-event estimation, (5-7) an emitting object's triangle table, emitter
+            // instance, vertices and emission.
             slot = wf.q_slot[q][qi];
-            const uint4 ps = wf.pstate[slot];  // job path, pixel, sample, view << 16 | segment
-            const ViewDev& vw = views[ps.w >> 16];
-            sid = vw.scene;
-            const DeviceScene& s = MULTI ? scenes[sid] : s0;
-            const uint32_t pix = ps.y, sample = ps.z;
-            segment = ps.w & 0xFFFFu;
-            const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
-            const float3 ro = xyz(ro4), rd = xyz(rd4);
             const float4 hh = wf.hit[qi];
+            const uint32_t prim = wf.hit_prim[qi];
+            const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
+            if (MULTI)
+                sid = wf.q_scene[q][qi];
+            const DeviceScene& s = MULTI ? scenes[sid] : s0;
             const uint32_t hinst = float_as_u32(hh.w);
-            float4 tp = wf.thr_pdf[slot];
+            const uint4 ps = wf.pstate[slot];  // job path, py << 16 | px, sample, view << 16 | segment
+            const float4 tp = wf.thr_pdf[slot];
+            const float4 rr = wf.rad[slot];
+            const ViewDev& vw = views[ps.w >> 16];
             float3 T = xyz(tp);
             float brdf_pdf = tp.w;
-            float4 rr = wf.rad[slot];
             float3 L = xyz(rr);
-            const bool final_seg = segment >= vw.max_bounces;
-            const uint32_t px = pix % vw.width, py = pix / vw.width;
-            Sampler smp;
-            smp.start(px, py, sample, segment + 1);
+            const float3 ro = xyz(ro4), rd = xyz(rd4);
+            const uint32_t px = ps.y & 0xFFFFu, py = ps.y >> 16, sample = ps.z;
+            segment = ps.w & 0xFFFFu;
             bool terminated = false;
 
             if (hinst == NONE) {
                 L = L + T * s.envcol;  // pathtrace.ms.hlsl:7-11
                 terminated = true;
             } else {
-                const uint32_t prim = wf.hit_prim[qi];
-                const float t = hh.x, bu = hh.y, bv = hh.z;
-                const InstanceData& in = s.instances[hinst];
-                const MaterialRec mat = s.materials[in.material_index];
-                const uint32_t ib = in.index_index + prim * 3;
+                // (2) the hit instance: vertex / index / emitter / material indices, normal matrix
+                const float4* ip = reinterpret_cast<const float4*>(s.instances + hinst);
+                const uint4 ih = *reinterpret_cast<const uint4*>(ip);
+                const float4 nm0 = ip[5], nm1 = ip[6], nm2 = ip[7];
+                Sampler smp;
+                smp.start(px, py, sample, segment + 1);
+                // draws in the order the path consumes them: the alpha test, then the light
+                Sampler s_alpha = smp;
+                const float u_alpha = s_alpha.s1();
+                Sampler s_light = s_alpha;
+                const float lu0 = s_light.s1();
+                const float2 lu12 = s_light.s2();
+                // (3)
+                const uint32_t max_bounces = vw.max_bounces, vw_width = vw.width;
+                const MaterialRec mat = s.materials[ih.w];
+                const uint32_t ib = ih.y + prim * 3;
                 const uint32_t ia = s.indices[ib], ibb = s.indices[ib + 1], ic = s.indices[ib + 2];
-                const float4 a0 = s.vertices[size_t(in.vertex_index + ia) * 2], a1 = s.vertices[size_t(in.vertex_index + ia) * 2 + 1];
-                const float4 b0 = s.vertices[size_t(in.vertex_index + ibb) * 2], b1 = s.vertices[size_t(in.vertex_index + ibb) * 2 + 1];
-                const float4 c0 = s.vertices[size_t(in.vertex_index + ic) * 2], c1 = s.vertices[size_t(in.vertex_index + ic) * 2 + 1];
+                const uint32_t lc = s.light_count;
+                float lux = lu0;
+                AliasBin<7> lbin;
+                if (lc > 0)
+                    lbin = load_alias_bin<7>(s.light + size_t(alias_pick(lc, lux)) * PT_LIGHT_BIN_WORDS);
+                // (4)
+                const float4* vp = s.vertices;
+                const float4 a0 = vp[size_t(ih.x + ia) * 2], a1 = vp[size_t(ih.x + ia) * 2 + 1];
+                const float4 b0 = vp[size_t(ih.x + ibb) * 2], b1 = vp[size_t(ih.x + ibb) * 2 + 1];
+                const float4 c0 = vp[size_t(ih.x + ic) * 2], c1 = vp[size_t(ih.x + ic) * 2 + 1];
+                if (lc > 0 && alias_follow(lbin.q, lux))
+                    lbin = load_alias_bin<7>(s.light + size_t(lbin.alias) * PT_LIGHT_BIN_WORDS);
+                const float t = hh.x, bu = hh.y, bv = hh.z;
+                const bool final_seg = segment >= max_bounces;
                 const float3 pa = xyz(a0), pb = xyz(b0), pc = xyz(c0);
                 const float3 na = f3(a0.w, a1.x, a1.y), nb = f3(b0.w, b1.x, b1.y), nc = f3(c0.w, c1.x, c1.y);
                 const float2 uva = make_float2(a1.z, a1.w), uvb = make_float2(b1.z, b1.w), uvc = make_float2(c1.z, c1.w);
@@ -559,32 +615,36 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                 const float3 obj_n = normalize3(wa * na + bu * nb + bv * nc);
                 const float2 uv = make_float2(wa * uva.x + bu * uvb.x + bv * uvc.x, wa * uva.y + bu * uvb.y + bv * uvc.y);
                 const float3 emission = material_emission(s, mat, uv);
-                // ObjectRayDirection(): the world->object transform applied to the direction
-                const InstXform x = s.inst_inv[hinst];
-                const float3 odir = f3(x.r0.x * rd.x + x.r0.y * rd.y + x.r0.z * rd.z, x.r1.x * rd.x + x.r1.y * rd.y + x.r1.z * rd.z,
-                                       x.r2.x * rd.x + x.r2.y * rd.y + x.r2.z * rd.z);
-                if (maxc3(emission) > 0.0f && dot3(odir, obj_n) < 0.0f) {
-                    if (brdf_pdf < 0.0f) {
-                        L = L + T * emission;
-                    } else {
-                        const float3 wab = xform_vec(in, pb - pa);
-                        const float3 wac = xform_vec(in, pc - pa);
-                        const float3 sn = cross3(wab, wac);
-                        const float area = sqrtf(dot3(sn, sn)) * 0.5f;
-                        const float dist2 = t * t;
-                        const float cosine = fabsf(dot3(rd, normalize3(sn)));
-                        float pdf_l = light_pdf(s.light, in.emitter_index, prim);
-                        pdf_l *= dist2 / (area * cosine);
-                        if (pdf_l > 0.0f && dist2 > 0.0f) {
-                            const float w = brdf_pdf / (brdf_pdf + pdf_l);
-                            L = L + T * (w * emission);
+                if (maxc3(emission) > 0.0f) {
+                    // ObjectRayDirection(): the world->object transform applied to the direction
+                    const InstXform x = s.inst_inv[hinst];
+                    const float3 odir = f3(x.r0.x * rd.x + x.r0.y * rd.y + x.r0.z * rd.z,
+                                           x.r1.x * rd.x + x.r1.y * rd.y + x.r1.z * rd.z,
+                                           x.r2.x * rd.x + x.r2.y * rd.y + x.r2.z * rd.z);
+                    if (dot3(odir, obj_n) < 0.0f) {
+                        if (brdf_pdf < 0.0f) {
+                            L = L + T * emission;
+                        } else {
+                            const InstanceData& in = s.instances[hinst];
+                            const float3 wab = xform_vec(in, pb - pa);
+                            const float3 wac = xform_vec(in, pc - pa);
+                            const float3 sn = cross3(wab, wac);
+                            const float area = sqrtf(dot3(sn, sn)) * 0.5f;
+                            const float dist2 = t * t;
+                            const float cosine = fabsf(dot3(rd, normalize3(sn)));
+                            float pdf_l = light_pdf(s.light, ih.z, prim);
+                            pdf_l *= dist2 / (area * cosine);
+                            if (pdf_l > 0.0f && dist2 > 0.0f) {
+                                const float w = brdf_pdf / (brdf_pdf + pdf_l);
+                                L = L + T * (w * emission);
+                            }
                         }
                     }
                 }
                 const float4 base = final_seg ? make_float4(0, 0, 0, 1) : material_base_colour(s, mat, uv);
                 if (final_seg) {
                     terminated = true;
-                } else if (base.w < smp.s1()) {
+                } else if ((smp = s_alpha, base.w < u_alpha)) {
                     // stochastic transparency: continue straight through (ch.hlsl:279-283)
                     next_o = ro + rd * (t + ORIGIN_OFFSET);
                     next_d = rd;
@@ -617,15 +677,15 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                         tangent = tangent - dot3(tangent, obj_n) * obj_n;
                     }
                     const float3 bitangent = cross3(obj_n, tangent);
-                    float3 wx = normalize3(xform_normal(in, tangent));
-                    float3 wy = normalize3(xform_normal(in, bitangent));
-                    float3 wn = normalize3(xform_normal(in, obj_n));
+                    float3 wx = normalize3(xform_rows(nm0, nm1, nm2, tangent));
+                    float3 wy = normalize3(xform_rows(nm0, nm1, nm2, bitangent));
+                    float3 wn = normalize3(xform_rows(nm0, nm1, nm2, obj_n));
                     if (dot3(wn, rd) > 0.0f) {
                         wx = -wx;
                         wy = -wy;
                         wn = -wn;
                     }
-                    float3 twn = normalize3(xform_normal(in, true_n));
+                    float3 twn = normalize3(xform_rows(nm0, nm1, nm2, true_n));
                     if (dot3(rd, twn) > 0.0f)
                         twn = -twn;
                     if (mapped_n.z < 0.999f) {
@@ -647,6 +707,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                         // Second-bounce vertex: a training path records the cache input here
                         // (its target is completed when the path ends); any other path ends
                         // here with the cache's prediction of the radiance leaving it.
+                        const uint32_t pix = py * vw_width + px;
                         const float3 hp = ro + rd * t;
                         cr[0] = hp.x, cr[1] = hp.y, cr[2] = hp.z;
                         cr[3] = -rd.x, cr[4] = -rd.y, cr[5] = -rd.z;
@@ -676,13 +737,43 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                     } else if (!cached) {
                         const float3 xp = ro + rd * t + twn * ORIGIN_OFFSET;
                         // next-event estimation (ch.hlsl:159-225): the shadow ray is queued
-                        if (s.light_count > 0) {
-                            const float u0 = smp.s1();
-                            const float2 u12 = smp.s2();
+                        if (lc > 0) {
+                            smp = s_light;
+                            // (5) an emitting object (payload: type 3, table offset, triangles,
+                            // first instance, instances): its triangle table
+                            const bool object_light = lbin.w[0] == 3u;
+                            const uint32_t* etab = s.light + lbin.w[1];
+                            float eux = lux;
+                            AliasBin<3> ebin;
+                            if (object_light)
+                                ebin = load_alias_bin<3>(etab + size_t(alias_pick(lbin.w[2], eux)) * PT_EMITTER_BIN_WORDS);
+                            // (6) the table's alias and the emitter instance
+                            uint4 eh = make_uint4(0, 0, 0, 0);
+                            float4 em0 = make_float4(0, 0, 0, 0), em1 = em0, em2 = em0;
+                            if (object_light) {
+                                if (alias_follow(ebin.q, eux))
+                                    ebin = load_alias_bin<3>(etab + size_t(ebin.alias) * PT_EMITTER_BIN_WORDS);
+                                const uint32_t einst = lbin.w[3] + min(lbin.w[4] - 1, uint32_t(eux * float(lbin.w[4])));
+                                const float4* ep = reinterpret_cast<const float4*>(s.instances + einst);
+                                eh = *reinterpret_cast<const uint4*>(ep);
+                                em0 = ep[1], em1 = ep[2], em2 = ep[3];
+                            }
+                            // (7) the emitter triangle's vertices and emission
+                            float4 ev0 = make_float4(0, 0, 0, 0), ev1 = ev0, ev2 = ev0;
+                            float4 e_emission = ev0, e_rmnb = ev0;
+                            if (object_light) {
+                                ev0 = vp[size_t(eh.x + ebin.w[0]) * 2];
+                                ev1 = vp[size_t(eh.x + ebin.w[1]) * 2];
+                                ev2 = vp[size_t(eh.x + ebin.w[2]) * 2];
+                                e_emission = s.materials[eh.w].emission;
+                                e_rmnb = s.materials[eh.w].rmnb;
+                            }
                             float3 light_dir;
                             float light_dist, lpdf;
                             bool is_delta;
-                            float3 le = sample_light(s, f3(u0, u12.x, u12.y), xp, light_dir, light_dist, lpdf, is_delta);
+                            const float3 le =
+                                light_sample_at(s, lbin, ebin, eh, em0, em1, em2, ev0, ev1, ev2, e_emission, e_rmnb,
+                                                lu12, xp, light_dir, light_dist, lpdf, is_delta);
                             if (maxc3(le) > 0.0f) {
                                 float3 li = f3(dot3(wx, light_dir), dot3(wy, light_dir), dot3(wn, light_dir));
                                 if (li.z > 0.0f) {
@@ -721,17 +812,17 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                         }
                     }
                 }
-            }
-            // Russian roulette after segments >= 1 (pathtrace.rg.hlsl:53, pathtrace.hlsli:118-129)
-            if (!terminated && segment >= 1) {
-                const float mt = maxc3(T);
-                const float u = smp.s1();
-                if (mt < 1.0f) {
-                    const float qq = fmaxf(0.0f, 1.0f - mt);
-                    if (u < qq)
-                        terminated = true;
-                    else
-                        T = T * (1.0f / (1.0f - qq));
+                // Russian roulette after segments >= 1 (pathtrace.rg.hlsl:53, pathtrace.hlsli:118-129)
+                if (!terminated && segment >= 1) {
+                    const float mt = maxc3(T);
+                    const float u = smp.s1();
+                    if (mt < 1.0f) {
+                        const float qq = fmaxf(0.0f, 1.0f - mt);
+                        if (u < qq)
+                            terminated = true;
+                        else
+                            T = T * (1.0f / (1.0f - qq));
+                    }
                 }
             }
             cont = !terminated;
@@ -740,7 +831,19 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
             if (cont)
                 reinterpret_cast<uint32_t*>(wf.pstate + slot)[3] += 1u;  // next segment
         }
-        const uint32_t pos = wave_append(cont, &wf.counters[1 - q]);
+        uint32_t apos[5] = {0, 0, 0, 0, 0};
+        if (cd.collect)  // uniform: training records and cache queries are appended too
+            block_append<5>({cont, valid && !cont, shadow, want_train, want_query},
+                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(4)], &wf.counters[CTR(2)], &cd.counters[0],
+                             &cd.counters[1]},
+                            apos);
+        else {
+            uint32_t p3[3];
+            block_append<3>({cont, valid && !cont, shadow},
+                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(4)], &wf.counters[CTR(2)]}, p3);
+            apos[0] = p3[0], apos[1] = p3[1], apos[2] = p3[2];
+        }
+        const uint32_t pos = apos[0], rpos = apos[1], spos = apos[2];
         if (cont) {
             wf.q_slot[1 - q][pos] = slot;
             wf.q_o[1 - q][pos] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
@@ -748,10 +851,8 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
             if (MULTI)
                 wf.q_scene[1 - q][pos] = sid;
         }
-        const uint32_t rpos = wave_append(valid && !cont, &wf.counters[4]);
         if (valid && !cont)
             wf.retire[rpos] = slot;
-        const uint32_t spos = wave_append(shadow, &wf.counters[2]);
         if (shadow) {
             wf.s_slot[spos] = slot;
             wf.s_o[spos] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
@@ -762,7 +863,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
         }
         if (cd.collect) {
             // training record: inputs, throughput and radiance so far (target filled at the end)
-            const uint32_t tpos = wave_append(want_train, &cd.counters[0]);
+            const uint32_t tpos = apos[3];
             if (want_train && tpos < cd.record_capacity) {
                 float* rec = cd.records + size_t(tpos) * MLP_REC_FLOATS;
                 for (int k = 0; k < 13; k++)
@@ -772,7 +873,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene s0, const DeviceScene
                 rec[19] = 0.0f;
                 cd.train_rec[slot] = tpos;
             }
-            const uint32_t qpos = wave_append(want_query, &cd.counters[1]);
+            const uint32_t qpos = apos[4];
             if (want_query) {
                 float* rec = cd.queries + size_t(qpos) * MLP_REC_FLOATS;
                 for (int k = 0; k < 13; k++)
@@ -793,7 +894,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_hit_keys(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q,
                                                   uint32_t n)
 {
-    const uint32_t count = wf.counters[q];
+    const uint32_t count = wf.counters[CTR(q)];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t key = 0xFFFFFFFFu;
         if (i < count) {
@@ -817,8 +918,8 @@ __global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const 
                                                     CacheDev cd)
 {
     const uint32_t paths = job->paths;
-    const uint32_t count = wf.counters[4];
-    const uint32_t rounded = (count + 63u) & ~63u;
+    const uint32_t count = wf.counters[CTR(4)];
+    const uint32_t rounded = (count + 255u) & ~255u;  // whole blocks take part in the appends
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
         const bool valid = i < count;
         uint32_t slot = 0;
@@ -841,15 +942,15 @@ __global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const 
                 }
             }
         }
-        // one atomic per wave hands out the next path ids
-        const uint32_t first = wave_append(valid, &wf.counters[5]);
+        // one atomic per block hands out the next path ids
+        const uint32_t first = block_append1(valid, &wf.counters[CTR(5)]);
         const bool start = valid && first < paths;
         float3 o = f3(0, 0, 0), d = f3(0, 0, 0);
         float tmin = 0.0f, tmax = 0.0f;
         uint32_t scene = 0;
         if (start)
             start_path(views, job, wf, cd, slot, first, o, d, tmin, tmax, scene);
-        const uint32_t pos = wave_append(start, &wf.counters[1 - q]);
+        const uint32_t pos = block_append1(start, &wf.counters[CTR(1 - q)]);
         if (start) {
             wf.q_slot[1 - q][pos] = slot;
             wf.q_o[1 - q][pos] = make_float4(o.x, o.y, o.z, tmin);
@@ -863,9 +964,9 @@ __global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const 
 __global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
 {
     if (threadIdx.x == 0) {
-        wf.counters[1 - q] = 0;
-        wf.counters[2] = 0;
-        wf.counters[4] = 0;
+        wf.counters[CTR(1 - q)] = 0;
+        wf.counters[CTR(2)] = 0;
+        wf.counters[CTR(4)] = 0;
         if (cd.collect)
             cd.counters[1] = 0;
     }
@@ -937,7 +1038,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, Cam
         traverse<false>(s, o, d, cam.near_clip, cam.far_clip, stk, h, trans);
         hits[p] = h.inst == NONE ? pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT} : pt_hit{h.t, h.u, h.v, h.inst, h.prim};
         if (stk.overflow)
-            atomicOr(&wf.counters[3], 1u);
+            atomicOr(&wf.counters[CTR(3)], 1u);
     }
 }
 
@@ -959,7 +1060,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene s, const
             found = traverse<false>(s, o, d, r.tmin, r.tmax, stk, h, trans);
         hits[p] = found ? pt_hit{h.t, h.u, h.v, h.inst, h.prim} : pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT};
         if (stk.overflow)
-            atomicOr(&wf.counters[3], 1u);
+            atomicOr(&wf.counters[CTR(3)], 1u);
     }
 }
 
@@ -1001,7 +1102,7 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
     A(wf.s_c, c * 16);
     A(wf.retire, c * 4);
     A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 16);
-    A(wf.counters, 16 * 4);
+    A(wf.counters, size_t(CTR(16)) * 4);
     A(wf.totals, 8 * 8);
     A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
     if (e == hipSuccess)
@@ -1009,7 +1110,7 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
     for (int k = 0; k < 2 && e == hipSuccess; k++)
         e = hipEventCreateWithFlags(&wf.count_ready[k], hipEventDisableTiming);
     if (e == hipSuccess)
-        e = hipMemset(wf.counters, 0, 16 * 4);
+        e = hipMemset(wf.counters, 0, size_t(CTR(16)) * 4);
     if (e == hipSuccess)
         e = hipMemset(wf.totals, 0, 8 * 8);
     return e;
@@ -1062,6 +1163,59 @@ ShadowFn pick_shadow(bool count, bool multi, uint32_t depth)
                       : (depth == 32 ? pick_shadow_d<32>(count, multi) : pick_shadow_d<16>(count, multi));
 }
 
+using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
+                        const uint32_t*);
+
+// Register budget of the shade stage: PT_SHADE_WAVES (2..6 waves per SIMD; tuning) or 3,
+// the fastest measured (168 registers, hardly any spilling).
+int shade_waves()
+{
+    static const int w = [] {
+        const char* env = getenv("PT_SHADE_WAVES");
+        return env ? std::max(2, std::min(6, atoi(env))) : 3;
+    }();
+    return w;
+}
+
+template <bool MULTI>
+ShadeFn pick_shade_m()
+{
+    switch (shade_waves()) {
+    case 2:
+        return k_shade<MULTI, 2>;
+    case 3:
+        return k_shade<MULTI, 3>;
+    case 5:
+        return k_shade<MULTI, 5>;
+    case 6:
+        return k_shade<MULTI, 6>;
+    default:
+        return k_shade<MULTI, 4>;
+    }
+}
+
+ShadeFn pick_shade(bool multi) { return multi ? pick_shade_m<true>() : pick_shade_m<false>(); }
+
+// Blocks of 256 threads of the shade stage resident on the whole chip at once: a
+// grid-stride launch of exactly that many finishes in one round, with no partial round.
+unsigned shade_grid(bool multi)
+{
+    static unsigned g[2] = {0, 0};
+    unsigned& v = g[multi ? 1 : 0];
+    if (v == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pick_shade(multi)), 256,
+                                                         0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 2;
+        v = unsigned(cus) * unsigned(per_cu);
+    }
+    return v;
+}
+
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
 // cache queries, retire and refill, and copy the count of the next queue to pinned memory.
 hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
@@ -1097,12 +1251,8 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
     }
     if ((e = timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
         return e;
-    if (multi)
-        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
-                           wf, q, cd, order);
-    else
-        hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
-                           wf, q, cd, order);
+    hipLaunchKernelGGL(pick_shade(multi), dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
+                       wf, q, cd, order);
     if ((e = timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
@@ -1121,7 +1271,7 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
     hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, wf, q, cd);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    return hipMemcpyAsync(&wf.host_counts[q], &wf.counters[1 - q], 4, hipMemcpyDeviceToHost, st);
+    return hipMemcpyAsync(&wf.host_counts[q], &wf.counters[CTR(1 - q)], 4, hipMemcpyDeviceToHost, st);
 }
 
 // Launch configuration an iteration graph depends on.
@@ -1267,8 +1417,8 @@ hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
         if (w.npix == 0 || w.npix > wf.rad_out_capacity)
             return hipErrorInvalidValue;
     const auto jobs = plan_jobs(work, wf.rad_out_capacity, train_between ? cs->frame_spp : 0);
-    // grid-stride stages: enough blocks to fill the chip, not one per pool slot
-    const unsigned shade_blocks = blocks_for(wf.capacity, 256, io.grid_blocks * 2);
+    // grid-stride stages: the blocks that fill the chip in one round, not one per pool slot
+    const unsigned shade_blocks = blocks_for(wf.capacity, 256, shade_grid(io.d_scenes != nullptr));
     for (const JobPlan& plan : jobs) {
         const auto& segs = plan.segs;
         JobDev& jh = *io.h_job;
