@@ -159,7 +159,13 @@ def run_instanced(a):
     st0 = tr.stats()
     log("BVH: %.1f ms build, %d nodes, %.1f MB" % (st0["bvh_build_ms"], st0["node_count"], st0["bvh_bytes"] / 1e6))
 
-    # launch configuration picked by the autotuner (untimed set-up, like the BVH build)
+    # launch configuration picked by the autotuner (untimed set-up, like the BVH build),
+    # starting from any configuration given on the command line
+    if a.leaf_size:
+        tr.set_option(pt_amd.PT_OPT_LEAF_SIZE, a.leaf_size)
+    if a.blocks_per_cu:
+        tr.set_option(pt_amd.PT_OPT_BLOCKS_PER_CU, a.blocks_per_cu)
+    st0 = tr.stats()
     tune = None
     if not a.no_autotune:
         tr.render(1)
@@ -212,8 +218,8 @@ def run_instanced(a):
         tr.render(a.spp_per_step)
     sp = tr.stats()
     tr.set_option(pt_amd.PT_OPT_PROFILE_KERNELS, 0)
-    log("profiled pass: stages extend %.1f ms, shade %.1f ms, shadow %.1f ms (render %.1f ms)"
-        % (sp["extend_ms"], sp["shade_ms"], sp["shadow_ms"], sp["render_ms"]))
+    log("profiled pass: stages extend %.1f ms, shade %.1f ms, shadow %.1f ms, refill %.1f ms (render %.1f ms)"
+        % (sp["extend_ms"], sp["shade_ms"], sp["shadow_ms"], sp["refill_ms"], sp["render_ms"]))
 
     # the 8-bounce shade stage reads per hit: hit 20 + path state 32 + instance 128 + material
     # 96 + indices 12 + 3 vertices 96 + light bins 64 + emitter vertices 36, writes next ray
@@ -231,7 +237,7 @@ def run_instanced(a):
     achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_" + dom),
-                "kernel": "k_" + dom,
+                "kernel": "k_" + dom, "launches": d["launches"],
                 "bytes_per_launch": d["bytes"] / max(1, d["launches"]),
                 "avg_launch_ms": d["ms"] / max(1, d["launches"])}
 
@@ -255,7 +261,8 @@ def run_instanced(a):
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "fetches_per_ray": {"ext_nodes": round(n_node_ext, 3), "ext_tris": round(n_tri_ext, 3),
                             "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)},
-        "stages_ms": {k: round(v["ms"], 3) for k, v in stages.items()},
+        "stages_ms": dict({k: round(v["ms"], 3) for k, v in stages.items()}, refill=round(sp["refill_ms"], 3)),
+        "stage_launches": dict({k: v["launches"] for k, v in stages.items()}, refill=sp["refill_launches"]),
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
         "autotune": tune,
         "sort_hits": bool(a.sort_hits),
@@ -416,6 +423,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--leaf-size", type=int, default=0, help="BLAS leaf size to start from (0: the library's)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="traversal blocks per CU to start from")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))

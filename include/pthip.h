@@ -142,14 +142,16 @@ typedef struct pt_stats {
     uint64_t shadow_node_fetches;
     uint64_t shadow_tri_fetches;
     /* With PT_OPT_PROFILE_KERNELS: summed HIP-event durations and launch counts of the
-     * extend (closest hit), shade and shadow (any hit) stages of the last pt_render. */
+     * extend (closest hit), shade, shadow (any hit) and refill (ended paths out, next paths
+     * in) stages of the last pt_render. */
     double extend_ms;
     double shade_ms;
     double shadow_ms;
     uint32_t extend_launches;
     uint32_t shade_launches;
     uint32_t shadow_launches;
-    uint32_t reserved;
+    uint32_t refill_launches;
+    double refill_ms;
 } pt_stats;
 
 /* Options for pt_set_option (instrumentation; no counterpart in the reference, whose only
@@ -162,7 +164,7 @@ typedef struct pt_stats {
  * belongs to the driver). Images do not depend on them, except that another hierarchy (leaf
  * size, node width, builder) can resolve exact ties between equally distant triangles
  * differently. */
-#define PT_OPT_BLOCKS_PER_CU 4u   /* traversal grid: blocks of 256 threads per CU (1..64, default 4) */
+#define PT_OPT_BLOCKS_PER_CU 4u   /* traversal grid: blocks of 256 threads per CU (1..64, default 6) */
 #define PT_OPT_POOL_SLOTS 5u      /* path pool slots (1024..2^26; default min(8M, max(4 px, 64K))) */
 #define PT_OPT_LDS_STACK 6u       /* traversal stack entries per lane kept in LDS (8, 16, 32) */
 #define PT_OPT_LEAF_SIZE 7u       /* max triangles per BLAS leaf (1..16, default 4): rebuilds the BVH */

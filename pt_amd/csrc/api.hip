@@ -754,6 +754,8 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
         s.extend_launches = kl[KERNEL_EXTEND];
         s.shade_launches = kl[KERNEL_SHADE];
         s.shadow_launches = kl[KERNEL_SHADOW];
+        s.refill_ms = kms[KERNEL_REFILL];
+        s.refill_launches = kl[KERNEL_REFILL];
     }
     *out = s;
     if (error_flags)
@@ -1077,7 +1079,7 @@ int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* 
         hipDeviceProp_t prop;
         BHIP_TRY(b, hipGetDeviceProperties(&prop, device), "device properties");
         const uint32_t cus = std::max(1, prop.multiProcessorCount);
-        uint32_t blocks_per_cu = 4;
+        uint32_t blocks_per_cu = 6;
         if (const char* env = getenv("PT_BLOCKS_PER_CU"))
             blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
         b->grid_blocks = ((cus * blocks_per_cu + 7) / 8) * 8;
@@ -1211,6 +1213,8 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
         s.extend_launches = kl[KERNEL_EXTEND];
         s.shade_launches = kl[KERNEL_SHADE];
         s.shadow_launches = kl[KERNEL_SHADOW];
+        s.refill_ms = kms[KERNEL_REFILL];
+        s.refill_launches = kl[KERNEL_REFILL];
     }
     *out = s;
     if (error_flags)

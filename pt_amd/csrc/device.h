@@ -122,7 +122,7 @@ struct IterGraphs {
     }
 };
 
-enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_CACHE = 3, KERNEL_COUNT = 4 };
+enum KernelId { KERNEL_EXTEND = 0, KERNEL_SHADE = 1, KERNEL_SHADOW = 2, KERNEL_CACHE = 3, KERNEL_REFILL = 4, KERNEL_COUNT = 5 };
 
 // Optional per-launch timing of the main stages with events on the context's stream.
 struct KernelTimer {
@@ -226,7 +226,7 @@ struct Ctx {
     bool use_graphs = true;
     // launch configuration (PT_OPT_BLOCKS_PER_CU / POOL_SLOTS / LDS_STACK / LEAF_SIZE)
     uint32_t cus = 1;
-    uint32_t blocks_per_cu = 4;
+    uint32_t blocks_per_cu = 6;
     uint32_t pool_slots = 0;
     uint32_t lds_stack = LDS_STACK;
     uint32_t leaf_size = 4;
