@@ -76,11 +76,10 @@ struct CacheState {
     float* records = nullptr;    // training records (MLP_REC_FLOATS each)
     uint32_t record_capacity = 0;
     float* queries = nullptr;    // query records of one iteration
-    uint32_t* query_slot = nullptr;
+    uint32_t* query_slot = nullptr;  // radiance target of each query
     uint32_t query_capacity = 0;
     uint32_t* counters = nullptr;  // [0] training records, [1] queries
     float* loss_acc = nullptr;
-    uint32_t* train_rec = nullptr;  // per pool slot: training record index or ~0
 };
 
 }  // namespace ptk

@@ -290,11 +290,13 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uin
 }
 
 // Inference. Query mode (features == nullptr): each query record's prediction, clamped at
-// zero and weighted by the record's throughput and weight, is added to the radiance of its slot.
-// Test mode: predictions of explicit feature rows are written to out (n x 3).
+// zero and weighted by the record's throughput and weight, is added to the radiance of its
+// path: next_rad[target] for a path that goes on, rad_out[path] for one that has ended
+// (target = TARGET_ENDED | path). Test mode: predictions of explicit feature rows are
+// written to out (n x 3).
 __global__ __launch_bounds__(256) void k_mlp_infer(CacheArgs a, const float* queries, const uint32_t* query_slot,
-                                                   const uint32_t* count_ptr, float4* rad, const float* features,
-                                                   uint32_t n_explicit, float* out)
+                                                   const uint32_t* count_ptr, float4* next_rad, float4* rad_out,
+                                                   const float* features, uint32_t n_explicit, float* out)
 {
     __shared__ __attribute__((aligned(16))) uint16_t lds[4 * 2 * WS * ST];
     const int w = threadIdx.x >> 6;
@@ -330,7 +332,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(CacheArgs a, const float* que
                     out[size_t(q) * 3 + r] = y[mt][i];
                 } else {
                     const float* rec = queries + size_t(q) * MLP_REC_FLOATS;
-                    float* dst = reinterpret_cast<float*>(&rad[query_slot[q]]) + r;
+                    const uint32_t tgt = query_slot[q];
+                    float4* px = (tgt & TARGET_ENDED) ? &rad_out[tgt & ~TARGET_ENDED] : &next_rad[tgt];
+                    float* dst = reinterpret_cast<float*>(px) + r;
                     *dst += rec[19] * rec[13 + r] * fmaxf(y[mt][i], 0.0f);
                 }
             }
@@ -405,7 +409,7 @@ int mlp_param_count() { return LAYOUT.count; }
 void cache_free(CacheState& c)
 {
     void* ptrs[] = {c.params, c.grads, c.adam_m, c.adam_v, c.wb, c.wt, c.records, c.queries,
-                    c.query_slot, c.counters, c.loss_acc, c.train_rec};
+                    c.query_slot, c.counters, c.loss_acc};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -452,14 +456,10 @@ hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capaci
             (void)hipFree(c.queries);
         if (c.query_slot)
             (void)hipFree(c.query_slot);
-        if (c.train_rec)
-            (void)hipFree(c.train_rec);
         c.queries = nullptr;
         c.query_slot = nullptr;
-        c.train_rec = nullptr;
         A(c.queries, size_t(pool_slots) * MLP_REC_FLOATS * 4);
         A(c.query_slot, size_t(pool_slots) * 4);
-        A(c.train_rec, size_t(pool_slots) * 4);
         c.query_capacity = pool_slots;
     }
     if (e != hipSuccess)
@@ -512,7 +512,7 @@ hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, flo
 {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + 4 * WS - 1) / (4 * WS) + 1, 4096));
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), nullptr, nullptr, nullptr, nullptr,
-                       d_features, n, d_out);
+                       nullptr, d_features, n, d_out);
     return hipGetLastError();
 }
 
@@ -523,10 +523,11 @@ hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* 
     return hipGetLastError();
 }
 
-hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* rad, unsigned blocks, hipStream_t st)
+hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad, float4* rad_out, unsigned blocks,
+                       hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), c.queries, c.query_slot, d_count, rad,
-                       nullptr, 0u, nullptr);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), c.queries, c.query_slot, d_count,
+                       next_rad, rad_out, nullptr, 0u, nullptr);
     return hipGetLastError();
 }
 

@@ -40,6 +40,11 @@ constexpr uint32_t CHILD_LEAF = 0x80000000u;
 constexpr uint32_t LEAF_FIRST_MASK = (1u << 27) - 1u;
 constexpr uint32_t LEAF_MAX_TRIS = 16;
 
+// A shadow ray's or cache query's radiance target: an entry of the next path queue (the
+// path goes on) or, with this bit, the path's place in the job's radiance output (the path
+// has ended).
+constexpr uint32_t TARGET_ENDED = 0x80000000u;
+
 // Triangle record, 48 B: object-space corners; v0.w holds the mesh-relative primitive
 // index (as raw bits).
 struct TriRec {

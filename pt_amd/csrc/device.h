@@ -17,37 +17,38 @@ constexpr uint32_t TRACE_BLOCK = 256;     // threads per traversal block (4 wave
 constexpr uint32_t LDS_STACK = 16;        // stack entries per lane held in LDS
 constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to global memory
 
-// A pool of path slots with their ray queues (struct-of-arrays). Paths of a job (every
-// pixel x every sample of a pt_render chunk) stream through the pool: a slot whose path
-// ends is refilled with the next path of the job, so every extend/shadow launch stays full
-// until the job's last paths drain.
+// The path pool as two queues (ping-pong, struct-of-arrays): entry i of a queue is a ray
+// together with the state of its path, so every stage reads and writes its entries in queue
+// order (coalesced), and a path's state moves with it from queue to queue. Paths of a job
+// (every pixel x every sample of a pt_render chunk) stream through: after each iteration the
+// continuing paths are compacted to the front of the next queue and the job's next paths fill
+// the rest, so every extend/shadow launch stays full until the job's last paths drain.
 // The wavefront counters sit 4 KB apart: the atomics of the queue appends then go to
 // different memory channels instead of queueing on one line.
 constexpr uint32_t CTR_STRIDE = 1024;
 __host__ __device__ constexpr uint32_t CTR(uint32_t k) { return k * CTR_STRIDE; }
 
 struct Wavefront {
-    uint32_t capacity = 0;         // slots
-    // per slot: job path index, py << 16 | px of its pixel, sample index, view << 16 | segment
-    uint4* pstate = nullptr;
-    float4* thr_pdf = nullptr;     // throughput rgb, brdf pdf
-    float4* rad = nullptr;         // radiance rgb
-    uint32_t* q_slot[2] = {nullptr, nullptr};  // extension queues (ping-pong)
+    uint32_t capacity = 0;         // entries per queue
     float4* q_o[2] = {nullptr, nullptr};       // origin, tmin
     float4* q_d[2] = {nullptr, nullptr};       // direction, tmax
+    // job path index, py << 16 | px of its pixel, sample index, view << 16 | segment
+    uint4* q_ps[2] = {nullptr, nullptr};
+    float4* q_thr[2] = {nullptr, nullptr};     // throughput rgb, brdf pdf
+    float4* q_rad[2] = {nullptr, nullptr};     // radiance rgb, bits of its training record (~0: none)
     uint32_t* q_scene[2] = {nullptr, nullptr}; // scene of each queued ray (scene batches)
     float4* hit = nullptr;         // t, u, v, instance bits (by queue position)
     uint32_t* hit_prim = nullptr;
-    uint32_t* s_slot = nullptr;    // shadow rays
+    uint32_t* s_target = nullptr;  // shadow rays: radiance target (TARGET_ENDED | path, or next entry)
     float4* s_o = nullptr;         // origin, tmax
     float4* s_d = nullptr;         // direction
     float4* s_c = nullptr;         // unoccluded contribution
     uint32_t* s_scene = nullptr;   // scene of each shadow ray (scene batches)
-    uint32_t* retire = nullptr;    // slots whose path ended this iteration
+    uint2* train_end = nullptr;    // (path, training record) of recorded paths that ended this iteration
     float4* rad_out = nullptr;     // radiance of every path of the job, by path id
     uint32_t rad_out_capacity = 0; // paths
-    // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] retire
-    // queue, [5] next path to start
+    // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] ended
+    // recorded paths, [5] next path to start
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
@@ -69,8 +70,7 @@ struct CacheDev {
     uint32_t* counters;       // [0] training records of the job, [1] queries of the iteration
     uint32_t record_capacity;
     float* queries;
-    uint32_t* query_slot;
-    uint32_t* train_rec;      // per slot
+    uint32_t* query_slot;     // radiance target of each query (TARGET_ENDED | path, or next entry)
     uint32_t train_stride;
     uint32_t collect;         // record training paths at the 2nd bounce
     uint32_t query;           // end other paths there with the cache's prediction
@@ -296,7 +296,8 @@ hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st);
 hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st);
 hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, float* d_out, hipStream_t st);
 hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* d_out, hipStream_t st);
-hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* rad, unsigned blocks, hipStream_t st);
+hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad, float4* rad_out, unsigned blocks,
+                       hipStream_t st);
 hipError_t cache_backward(CacheState& c, const float* d_features, const float* d_targets, uint32_t n, uint32_t n_rec,
                           uint32_t start, uint32_t stride, hipStream_t st);
 hipError_t cache_adam(CacheState& c, hipStream_t st);

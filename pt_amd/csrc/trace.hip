@@ -404,14 +404,11 @@ __device__ __forceinline__ uint32_t find_segment(const JobDev* job, uint32_t p)
     return lo;
 }
 
-// Starts job path p in slot s: sampler, camera ray of its view (pathtrace.rg.hlsl:18-33),
-// fresh payload. Returns the ray's [tmin, tmax] and scene.
-__device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, const Wavefront& wf,
-                                           const CacheDev& cd, uint32_t s, uint32_t p, float3& o, float3& d,
-                                           float& tmin, float& tmax, uint32_t& scene)
+// Starts job path p as entry pos of queue par: camera ray of its view (pathtrace.rg.hlsl:18-33)
+// and a fresh payload.
+__device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, const Wavefront& wf, int par,
+                                           uint32_t pos, uint32_t p)
 {
-    if (cd.collect)
-        cd.train_rec[s] = NONE;
     const SegDev sg = job->seg[find_segment(job, p)];
     const ViewDev& vw = views[sg.view];
     const uint32_t local = p - sg.path_offset;
@@ -421,13 +418,14 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     smp.start(px, py, sample, 0);
     const float2 up = smp.s2();
     const float2 ul = smp.s2();
+    float3 o, d;
     camera_ray(vw.cam, px, py, vw.width, vw.height, up, ul, o, d);
-    tmin = vw.cam.near_clip;
-    tmax = vw.cam.far_clip;
-    scene = vw.scene;
-    wf.pstate[s] = make_uint4(p, (py << 16) | px, sample, sg.view << 16);  // segment 0
-    wf.thr_pdf[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    wf.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    wf.q_o[par][pos] = make_float4(o.x, o.y, o.z, vw.cam.near_clip);
+    wf.q_d[par][pos] = make_float4(d.x, d.y, d.z, vw.cam.far_clip);
+    wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, sg.view << 16);  // segment 0
+    wf.q_thr[par][pos] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+    wf.q_rad[par][pos] = make_float4(0.0f, 0.0f, 0.0f, u32_as_float(NONE));  // no training record
+    wf.q_scene[par][pos] = vw.scene;
 }
 
 // Fills the pool with the job's first paths; their camera rays form queue 0.
@@ -438,16 +436,8 @@ __global__ __launch_bounds__(256) void k_start_job(const ViewDev* views, const J
         cd.counters[0] = 0;
         cd.counters[1] = 0;
     }
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        float3 o, d;
-        float tmin, tmax;
-        uint32_t scene;
-        start_path(views, job, wf, cd, s, s, o, d, tmin, tmax, scene);
-        wf.q_slot[0][s] = s;
-        wf.q_o[0][s] = make_float4(o.x, o.y, o.z, tmin);
-        wf.q_d[0][s] = make_float4(d.x, d.y, d.z, tmax);
-        wf.q_scene[0][s] = scene;
-    }
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+        start_path(views, job, wf, 0, s, s);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         wf.counters[CTR(0)] = n;
         wf.counters[CTR(1)] = 0;
@@ -489,7 +479,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 }
 
 template <bool COUNT, bool MULTI, uint32_t D>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(DeviceScene s0, const DeviceScene* scenes, Wavefront wf)
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
+    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[D * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(2)];
@@ -504,13 +495,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         float trans;
         const bool occluded = traverse<true, COUNT>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
         if (!occluded && trans >= 1e-5f) {
-            const uint32_t slot = wf.s_slot[i];
+            const uint32_t tgt = wf.s_target[i];
+            float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
             const float4 c = wf.s_c[i];
-            float4 r = wf.rad[slot];
+            float4 r = *dst;
             r.x += c.x * trans;
             r.y += c.y * trans;
             r.z += c.z * trans;
-            wf.rad[slot] = r;
+            *dst = r;
         }
         if (stk.overflow)
             atomicOr(&wf.counters[CTR(3)], 1u);
@@ -523,8 +515,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         atomicAdd(&wf.totals[1], (unsigned long long)count);
 }
 
-// Miss / closest-hit / Russian roulette of one segment for every ray of queue q. Continuing
-// paths go to queue 1-q, ended paths to the retire queue, next-event rays to the shadow queue.
+// Miss / closest-hit / Russian roulette of one segment for every entry of queue q. Continuing
+// paths move, with their state, to the front of queue 1-q; an ended path's radiance goes to
+// rad_out; next-event rays go to the shadow queue with the place their contribution is added.
 // WPE: waves per SIMD the register allocation must allow (3 = no limit beyond the block).
 template <bool MULTI, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_shade(
@@ -541,7 +534,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         float qf = 1.0f;  // weight of the cache's prediction
         float cr[13];
         float3 cT = f3(0, 0, 0), cL = f3(0, 0, 0);
-        uint32_t slot = 0;
+        uint4 ps = make_uint4(0, 0, 0, 0);  // path state of the entry
+        float4 thr_v = make_float4(0, 0, 0, 0), rad_v = make_float4(0, 0, 0, 0);
+        uint32_t rec = NONE;  // training record of the path
         float3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
         float3 sh_o = f3(0, 0, 0), sh_d = f3(0, 0, 0), sh_c = f3(0, 0, 0);
         float sh_tmax = 0.0f;
@@ -551,11 +546,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             const uint32_t qi = order ? order[i] : i;  // material-sorted order when given
             // Loads are grouped by the round trip that resolves their addresses, and every
             // group is issued before the data of the previous one are used: (1) the queue
-            // entry; (2) path state and the hit instance; (3) view, material, triangle indices
-            // and the light table's first level; (4) vertices and the chosen light bin; then,
-            // for next-event estimation, (5-7) an emitting object's triangle table, emitter
-            // instance, vertices and emission.
-            slot = wf.q_slot[q][qi];
+            // entry (ray, path state, hit); (2) the hit instance; (3) view, material, triangle
+            // indices and the light table's first level; (4) vertices and the chosen light bin;
+            // then, for next-event estimation, (5-7) an emitting object's triangle table,
+            // emitter instance, vertices and emission.
+            ps = wf.q_ps[q][qi];  // job path, py << 16 | px, sample, view << 16 | segment
+            const float4 tp = wf.q_thr[q][qi];
+            const float4 rr = wf.q_rad[q][qi];
             const float4 hh = wf.hit[qi];
             const uint32_t prim = wf.hit_prim[qi];
             const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
@@ -563,9 +560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 sid = wf.q_scene[q][qi];
             const DeviceScene& s = MULTI ? scenes[sid] : s0;
             const uint32_t hinst = float_as_u32(hh.w);
-            const uint4 ps = wf.pstate[slot];  // job path, py << 16 | px, sample, view << 16 | segment
-            const float4 tp = wf.thr_pdf[slot];
-            const float4 rr = wf.rad[slot];
+            rec = float_as_u32(rr.w);
             const ViewDev& vw = views[ps.w >> 16];
             float3 T = xyz(tp);
             float brdf_pdf = tp.w;
@@ -829,35 +824,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 }
             }
             cont = !terminated;
-            wf.thr_pdf[slot] = make_float4(T.x, T.y, T.z, brdf_pdf);
-            wf.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            if (cont)
-                reinterpret_cast<uint32_t*>(wf.pstate + slot)[3] += 1u;  // next segment
+            thr_v = make_float4(T.x, T.y, T.z, brdf_pdf);
+            rad_v = make_float4(L.x, L.y, L.z, 0.0f);
         }
+        // a recorded path that ends here: its training record is completed after the shadow
+        // rays of this iteration have added their share
+        const bool ended_rec = valid && !cont && (rec != NONE || want_train);
         uint32_t apos[5] = {0, 0, 0, 0, 0};
-        if (cd.collect)  // uniform: training records and cache queries are appended too
-            block_append<5>({cont, valid && !cont, shadow, want_train, want_query},
-                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(4)], &wf.counters[CTR(2)], &cd.counters[0],
-                             &cd.counters[1]},
+        if (cd.collect) {  // uniform: training records, cache queries and ended records are appended too
+            block_append<5>({cont, shadow, want_train, want_query, ended_rec},
+                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)], &cd.counters[0], &cd.counters[1],
+                             &wf.counters[CTR(4)]},
                             apos);
-        else {
-            uint32_t p3[3];
-            block_append<3>({cont, valid && !cont, shadow},
-                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(4)], &wf.counters[CTR(2)]}, p3);
-            apos[0] = p3[0], apos[1] = p3[1], apos[2] = p3[2];
+        } else {
+            uint32_t p2[2];
+            block_append<2>({cont, shadow}, {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)]}, p2);
+            apos[0] = p2[0], apos[1] = p2[1];
         }
-        const uint32_t pos = apos[0], rpos = apos[1], spos = apos[2];
+        const uint32_t pos = apos[0], spos = apos[1], tpos = apos[2], qpos = apos[3];
+        if (want_train)
+            rec = tpos < cd.record_capacity ? tpos : NONE;
         if (cont) {
-            wf.q_slot[1 - q][pos] = slot;
             wf.q_o[1 - q][pos] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
             wf.q_d[1 - q][pos] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
+            wf.q_ps[1 - q][pos] = make_uint4(ps.x, ps.y, ps.z, ps.w + 1u);  // next segment
+            wf.q_thr[1 - q][pos] = thr_v;
+            wf.q_rad[1 - q][pos] = make_float4(rad_v.x, rad_v.y, rad_v.z, u32_as_float(rec));
             if (MULTI)
                 wf.q_scene[1 - q][pos] = sid;
+        } else if (valid) {
+            wf.rad_out[ps.x] = rad_v;
         }
-        if (valid && !cont)
-            wf.retire[rpos] = slot;
+        const uint32_t target = cont ? pos : (TARGET_ENDED | ps.x);
+        if (ended_rec)
+            wf.train_end[apos[4]] = make_uint2(ps.x, rec);
         if (shadow) {
-            wf.s_slot[spos] = slot;
+            wf.s_target[spos] = target;
             wf.s_o[spos] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
             wf.s_d[spos] = make_float4(sh_d.x, sh_d.y, sh_d.z, 0.0f);
             wf.s_c[spos] = make_float4(sh_c.x, sh_c.y, sh_c.z, 0.0f);
@@ -866,24 +868,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         }
         if (cd.collect) {
             // training record: inputs, throughput and radiance so far (target filled at the end)
-            const uint32_t tpos = apos[3];
             if (want_train && tpos < cd.record_capacity) {
-                float* rec = cd.records + size_t(tpos) * MLP_REC_FLOATS;
+                float* r = cd.records + size_t(tpos) * MLP_REC_FLOATS;
                 for (int k = 0; k < 13; k++)
-                    rec[k] = cr[k];
-                rec[13] = cT.x, rec[14] = cT.y, rec[15] = cT.z;
-                rec[16] = cL.x, rec[17] = cL.y, rec[18] = cL.z;
-                rec[19] = 0.0f;
-                cd.train_rec[slot] = tpos;
+                    r[k] = cr[k];
+                r[13] = cT.x, r[14] = cT.y, r[15] = cT.z;
+                r[16] = cL.x, r[17] = cL.y, r[18] = cL.z;
+                r[19] = 0.0f;
             }
-            const uint32_t qpos = apos[4];
             if (want_query) {
-                float* rec = cd.queries + size_t(qpos) * MLP_REC_FLOATS;
+                float* r = cd.queries + size_t(qpos) * MLP_REC_FLOATS;
                 for (int k = 0; k < 13; k++)
-                    rec[k] = cr[k];
-                rec[13] = cT.x, rec[14] = cT.y, rec[15] = cT.z;
-                rec[19] = qf;
-                cd.query_slot[qpos] = slot;
+                    r[k] = cr[k];
+                r[13] = cT.x, r[14] = cT.y, r[15] = cT.z;
+                r[19] = qf;
+                cd.query_slot[qpos] = target;
             }
         }
     }
@@ -915,51 +914,45 @@ __global__ __launch_bounds__(256) void k_hit_keys(DeviceScene s0, const DeviceSc
     }
 }
 
-// Stores the radiance of every ended path and refills its slot with the job's next path,
-// whose camera ray joins queue 1-q.
-__global__ __launch_bounds__(256) void k_regenerate(const ViewDev* views, const JobDev* job, Wavefront wf, int q,
-                                                    CacheDev cd)
+// Ends an iteration: completes the training records of recorded paths that ended in it
+// (their radiance is final once the shadow rays and cache queries have added theirs), then
+// appends the job's next paths behind the continuing ones in queue 1-q: consecutive path ids
+// into consecutive entries, so no atomics; k_refill_commit then advances the counters.
+__global__ __launch_bounds__(256) void k_refill(const ViewDev* views, const JobDev* job, Wavefront wf, int q,
+                                                CacheDev cd)
 {
-    const uint32_t paths = job->paths;
-    const uint32_t count = wf.counters[CTR(4)];
-    const uint32_t rounded = (count + 255u) & ~255u;  // whole blocks take part in the appends
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
-        const bool valid = i < count;
-        uint32_t slot = 0;
-        if (valid) {
-            slot = wf.retire[i];
-            const float4 lf = wf.rad[slot];
-            wf.rad_out[wf.pstate[slot].x] = lf;
-            if (cd.collect) {
-                // training target: radiance gathered after the 2nd-bounce vertex, divided by
-                // the throughput that reached it
-                const uint32_t tr = cd.train_rec[slot];
-                if (tr != NONE) {
-                    float* rec = cd.records + size_t(tr) * MLP_REC_FLOATS;
-                    const float lv[3] = {lf.x, lf.y, lf.z};
-                    for (int c = 0; c < 3; c++) {
-                        const float tc = rec[13 + c];
-                        rec[13 + c] = tc > 0.0f ? (lv[c] - rec[16 + c]) / tc : 0.0f;
-                    }
-                    rec[19] = 1.0f;  // complete
-                }
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (cd.collect) {
+        const uint32_t ended = wf.counters[CTR(4)];
+        for (uint32_t i = tid; i < ended; i += stride) {
+            const uint2 e = wf.train_end[i];
+            if (e.y == NONE)
+                continue;
+            // training target: radiance gathered after the 2nd-bounce vertex, divided by the
+            // throughput that reached it
+            const float4 lf = wf.rad_out[e.x];
+            float* rec = cd.records + size_t(e.y) * MLP_REC_FLOATS;
+            const float lv[3] = {lf.x, lf.y, lf.z};
+            for (int c = 0; c < 3; c++) {
+                const float tc = rec[13 + c];
+                rec[13 + c] = tc > 0.0f ? (lv[c] - rec[16 + c]) / tc : 0.0f;
             }
+            rec[19] = 1.0f;  // complete
         }
-        // one atomic per block hands out the next path ids
-        const uint32_t first = block_append1(valid, &wf.counters[CTR(5)]);
-        const bool start = valid && first < paths;
-        float3 o = f3(0, 0, 0), d = f3(0, 0, 0);
-        float tmin = 0.0f, tmax = 0.0f;
-        uint32_t scene = 0;
-        if (start)
-            start_path(views, job, wf, cd, slot, first, o, d, tmin, tmax, scene);
-        const uint32_t pos = block_append1(start, &wf.counters[CTR(1 - q)]);
-        if (start) {
-            wf.q_slot[1 - q][pos] = slot;
-            wf.q_o[1 - q][pos] = make_float4(o.x, o.y, o.z, tmin);
-            wf.q_d[1 - q][pos] = make_float4(d.x, d.y, d.z, tmax);
-            wf.q_scene[1 - q][pos] = scene;
-        }
+    }
+    const uint32_t n_cont = wf.counters[CTR(1 - q)], next = wf.counters[CTR(5)];
+    const uint32_t n_new = min(wf.capacity - n_cont, job->paths - next);
+    for (uint32_t k = tid; k < n_new; k += stride)
+        start_path(views, job, wf, 1 - q, n_cont + k, next + k);
+}
+
+__global__ void k_refill_commit(const JobDev* job, Wavefront wf, int q)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t n_cont = wf.counters[CTR(1 - q)], next = wf.counters[CTR(5)];
+        const uint32_t n_new = min(wf.capacity - n_cont, job->paths - next);
+        wf.counters[CTR(1 - q)] = n_cont + n_new;
+        wf.counters[CTR(5)] = next + n_new;
     }
 }
 
@@ -1087,23 +1080,22 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
         if (e == hipSuccess)
             e = hipMalloc((void**)&p, bytes);
     };
-    A(wf.pstate, c * 16);
     A(wf.s_scene, c * 4);
-    A(wf.thr_pdf, c * 16);
-    A(wf.rad, c * 16);
     for (int k = 0; k < 2; k++) {
-        A(wf.q_slot[k], c * 4);
         A(wf.q_o[k], c * 16);
         A(wf.q_d[k], c * 16);
+        A(wf.q_ps[k], c * 16);
+        A(wf.q_thr[k], c * 16);
+        A(wf.q_rad[k], c * 16);
         A(wf.q_scene[k], c * 4);
     }
     A(wf.hit, c * 16);
     A(wf.hit_prim, c * 4);
-    A(wf.s_slot, c * 4);
+    A(wf.s_target, c * 4);
     A(wf.s_o, c * 16);
     A(wf.s_d, c * 16);
     A(wf.s_c, c * 16);
-    A(wf.retire, c * 4);
+    A(wf.train_end, c * 8);
     A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 16);
     A(wf.counters, size_t(CTR(16)) * 4);
     A(wf.totals, 8 * 8);
@@ -1121,11 +1113,12 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
 
 void wavefront_free(Wavefront& wf)
 {
-    void* ptrs[] = {wf.thr_pdf, wf.rad,      wf.q_slot[0], wf.q_slot[1], wf.q_o[0],
-                    wf.q_o[1],  wf.q_d[0], wf.q_d[1], wf.hit,      wf.hit_prim,  wf.s_slot,    wf.s_o,
-                    wf.s_d,     wf.s_c,   wf.retire,  wf.rad_out,  wf.counters,  wf.totals,    wf.spill,
-                    wf.pstate, wf.s_scene, wf.q_scene[0], wf.q_scene[1],
-                    wf.sort_keys[0], wf.sort_keys[1], wf.sort_vals[0], wf.sort_vals[1], wf.sort_temp};
+    void* ptrs[] = {wf.q_o[0],      wf.q_o[1],      wf.q_d[0],      wf.q_d[1],      wf.q_ps[0],
+                    wf.q_ps[1],     wf.q_thr[0],    wf.q_thr[1],    wf.q_rad[0],    wf.q_rad[1],
+                    wf.q_scene[0],  wf.q_scene[1],  wf.hit,         wf.hit_prim,    wf.s_target,
+                    wf.s_o,         wf.s_d,         wf.s_c,         wf.s_scene,     wf.train_end,
+                    wf.rad_out,     wf.counters,    wf.totals,      wf.spill,       wf.sort_keys[0],
+                    wf.sort_keys[1], wf.sort_vals[0], wf.sort_vals[1], wf.sort_temp};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1140,7 +1133,7 @@ void wavefront_free(Wavefront& wf)
 namespace {
 
 typedef void (*ExtendFn)(DeviceScene, const DeviceScene*, Wavefront, int);
-typedef void (*ShadowFn)(DeviceScene, const DeviceScene*, Wavefront);
+typedef void (*ShadowFn)(DeviceScene, const DeviceScene*, Wavefront, int);
 
 template <uint32_t D>
 ExtendFn pick_extend_d(bool count, bool multi)
@@ -1220,7 +1213,7 @@ unsigned shade_grid(bool multi)
 }
 
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
-// cache queries, retire and refill, and copy the count of the next queue to pinned memory.
+// cache queries, refill, and copy the count of the next queue to pinned memory.
 hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
 {
     Wavefront& wf = *io.wf;
@@ -1260,20 +1253,22 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
         return e;
     if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf);
+    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
     if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
         return e;
     if (cd.query) {
         if ((e = timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
             return e;
-        if ((e = cache_query(*io.cache, &io.cache->counters[1], wf.rad, io.grid_blocks, st)) != hipSuccess)
+        if ((e = cache_query(*io.cache, &io.cache->counters[1], wf.q_rad[1 - q], wf.rad_out, io.grid_blocks, st)) !=
+            hipSuccess)
             return e;
         if ((e = timer.mark(st, KERNEL_CACHE, false)) != hipSuccess)
             return e;
     }
     if ((e = timer.mark(st, KERNEL_REFILL, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_regenerate, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, wf, q, cd);
+    hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, wf, q, cd);
+    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, st, io.d_job, wf, q);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_REFILL, false)) != hipSuccess)
@@ -1293,7 +1288,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
     mix(reinterpret_cast<uintptr_t>(io.d_views));
     mix(reinterpret_cast<uintptr_t>(io.d_job));
-    mix(reinterpret_cast<uintptr_t>(io.wf->pstate));
+    mix(reinterpret_cast<uintptr_t>(io.wf->q_ps[0]));
     mix(io.wf->capacity);
     mix(io.grid_blocks);
     mix(io.cache ? reinterpret_cast<uintptr_t>(io.cache->params) : 0);
@@ -1441,7 +1436,6 @@ hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
             cd.record_capacity = cs->record_capacity;
             cd.queries = cs->queries;
             cd.query_slot = cs->query_slot;
-            cd.train_rec = cs->train_rec;
             cd.train_stride = std::max<uint32_t>(1, cs->train_stride);
             cd.collect = 1;
             cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
