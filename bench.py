@@ -135,15 +135,17 @@ def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
             "plain_trajectory": plain_traj[-4:], "cached_trajectory": cached_traj[-4:]}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a stage from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "k_extend_instanced_1080p_pmc_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-            return float(f[2])
+def profile_figure(name, kernel):
+    """A per-launch figure of a stage from the committed rocprofv3 PMC summaries, if any:
+    traffic_bytes_per_launch (HBM bytes, scaled), l2_hit_pct, valu_busy_pct."""
+    for fn in ("k_extend_instanced_1080p_pmc_traffic.txt", "k_extend_instanced_1080p_pmc_l2_valu.txt"):
+        path = os.path.join(ROOT, "profiles", fn)
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == name and f[1] == kernel:
+                return float(f[2])
     return None
 
 
@@ -235,11 +237,17 @@ def run_instanced(a):
     dom = max(stages, key=lambda k: stages[k]["ms"])
     d = stages[dom]
     achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
+    traffic = profile_figure("traffic_bytes_per_launch", "k_" + dom)
+    avg_ms = d["ms"] / max(1, d["launches"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_" + dom),
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_" + dom, "launches": d["launches"],
-                "bytes_per_launch": d["bytes"] / max(1, d["launches"]),
-                "avg_launch_ms": d["ms"] / max(1, d["launches"])}
+                "bytes_per_launch": d["bytes"] / max(1, d["launches"]), "avg_launch_ms": avg_ms,
+                # measured HBM traffic over the same launch time (the algorithmic node bytes are
+                # mostly served by L2 and the Infinity Cache, so they can exceed the HBM peak)
+                "traffic_gbs": round(traffic / (avg_ms / 1e3) / 1e9, 1) if traffic and avg_ms > 0 else None,
+                "l2_hit_pct": profile_figure("l2_hit_pct", "k_" + dom),
+                "valu_busy_pct": profile_figure("valu_busy_pct", "k_" + dom)}
 
     conv = None
     if not a.no_convergence:
