@@ -201,6 +201,14 @@ int pt_render(pt_ctx* ctx, uint32_t spp);
  * width*height RGBA32F rows of row_pitch bytes into caller memory. Synchronises. */
 int pt_read_accumulation(pt_ctx* ctx, float* rgba, size_t row_pitch);
 
+/* The PNG output path of the headless dispatcher: PathTraceDispatcher::put_result's ACES tone
+ * mapper over the accumulation (shaders/compute/tonemap.cs.hlsl with src_is_srgb = false,
+ * dst_is_srgb = true, src/pathtrace/ptdispatch.cpp:29,155-180) and write_png's 8-bit
+ * conversion (uchar)clamp(x * 255, 0, 255) (src/io/imagewrite.cpp:39-62), into width x height
+ * RGBA8 pixels, rows row_pitch bytes apart. The accumulation is not modified (the reference
+ * tone-maps it in place before writing the file). Synchronises. */
+int pt_read_tonemapped(pt_ctx* ctx, uint8_t* rgba8, size_t row_pitch);
+
 /* Primary-ray hits of one sample index: per pixel, the first TraceRay of the raygen shader
  * (pathtrace.rg.hlsl:16-35) with its camera ray and [near, far] interval. hits has
  * width*height entries, row-major. Synchronises. */
@@ -246,6 +254,8 @@ int pt_batch_set_view(pt_batch* batch, uint32_t view, const pt_view* v);
 /* Asynchronous on the batch's stream. */
 int pt_batch_render(pt_batch* batch);
 int pt_batch_read(pt_batch* batch, uint32_t view, float* rgba, size_t row_pitch);
+/* pt_read_tonemapped for one view of the batch. */
+int pt_batch_read_tonemapped(pt_batch* batch, uint32_t view, uint8_t* rgba8, size_t row_pitch);
 /* Counters summed over all views (and BVH figures over all scenes); width/height are 0. */
 int pt_batch_get_stats(pt_batch* batch, pt_stats* out);
 int pt_batch_set_option(pt_batch* batch, uint32_t option, int64_t value);

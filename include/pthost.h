@@ -75,6 +75,11 @@ int pth_camera_block(const pth_camera_params* params, pt_camera* out);
 int pth_write_hdr(const char* path, const float* rgba, uint32_t width, uint32_t height,
                   size_t row_pitch);
 
+/* write_png (src/io/imagewrite.cpp:44-67): 8-bit RGBA rows (row_pitch bytes apart, e.g. from
+ * pt_read_tonemapped) -> PNG file. The reference encodes with stb_image_write; this is another
+ * valid PNG of the same pixels (per-row filters, fixed-code DEFLATE). */
+int pth_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height, size_t row_pitch);
+
 /* Reads a Radiance .hdr (RGBE, flat or RLE) into width*height*4 floats (alpha = 1), with the
  * RGBE decode stb_image 2.30 uses; *out is malloc'd, release with pth_free. */
 int pth_read_hdr(const char* path, float** out, uint32_t* width, uint32_t* height);

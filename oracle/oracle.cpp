@@ -1420,4 +1420,44 @@ int oracle_sample_light(const oracle_scene* s, const float* point, const float* 
     return PT_OK;
 }
 
+// The PNG path's ACES tone mapping (shaders/compute/tonemap.cs.hlsl:30-66 with
+// src_is_srgb = false and dst_is_srgb = true, as PathTraceDispatcher constructs its
+// ToneMapper, src/pathtrace/ptdispatch.cpp:29) and write_png's per-channel
+// (uchar)clamp(x * 255, 0, 255) (src/io/imagewrite.cpp:39-42); alpha passes through.
+// n RGBA float pixels -> n RGBA8 pixels.
+int oracle_tonemap(const float* rgba, size_t n, uint8_t* out)
+{
+    auto to_srgb = [](float x) {  // tonemap.cs.hlsl:30-32, threshold as the shader writes it
+        return x <= 0.00031308f ? 12.92f * x : 1.055f * std::pow(x, 1.0f / 2.4f) - 0.055f;
+    };
+    auto fit = [](float v) {  // rrt_to_odt_fit, tonemap.cs.hlsl:48-53
+        const float a = v * (v + 0.0245786f) - 0.000090537f;
+        const float b = v * (0.983729f * v + 0.4329510f) + 0.238081f;
+        return a / b;
+    };
+    auto byte = [](float x) {
+        const float y = std::min(std::max(x * 255.0f, 0.0f), 255.0f);
+        return uint8_t(y);
+    };
+    static const float rrt[3][3] = {{0.59719f, 0.35458f, 0.04823f},
+                                    {0.07600f, 0.90834f, 0.01566f},
+                                    {0.02840f, 0.13383f, 0.83777f}};
+    static const float odt[3][3] = {{1.60475f, -0.53108f, -0.07367f},
+                                    {-0.10208f, 1.10813f, -0.00605f},
+                                    {-0.00327f, -0.07276f, 1.07602f}};
+    for (size_t i = 0; i < n; i++) {
+        const float* c = rgba + i * 4;
+        const float s[3] = {to_srgb(c[0]), to_srgb(c[1]), to_srgb(c[2])};
+        float f[3];
+        for (int r = 0; r < 3; r++)
+            f[r] = fit(rrt[r][0] * s[0] + rrt[r][1] * s[1] + rrt[r][2] * s[2]);
+        for (int r = 0; r < 3; r++) {
+            const float o = odt[r][0] * f[0] + odt[r][1] * f[1] + odt[r][2] * f[2];
+            out[i * 4 + r] = byte(std::min(std::max(o, 0.0f), 1.0f));  // saturate (NaN -> 0)
+        }
+        out[i * 4 + 3] = byte(c[3]);
+    }
+    return 0;
+}
+
 }  // extern "C"

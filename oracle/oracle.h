@@ -71,6 +71,10 @@ int oracle_camera_ray(const pt_camera* cam, uint32_t px, uint32_t py, uint32_t w
 int oracle_sample_light(const oracle_scene* s, const float* point, const float* u3, float* emission,
                         float* light_dir, float* light_dist, float* pdf);
 
+/* The PNG path's ACES tone mapping (tonemap.cs.hlsl:30-66, ptdispatch.cpp:29) and write_png's
+ * 8-bit conversion (imagewrite.cpp:39-42): n RGBA float pixels -> n RGBA8 pixels. */
+int oracle_tonemap(const float* rgba, size_t n, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,6 +149,7 @@ def host_lib():
         lib.pth_scene_free.argtypes = [C.c_void_p]
         lib.pth_camera_block.argtypes = [C.POINTER(CameraParams), C.POINTER(Camera)]
         lib.pth_write_hdr.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_size_t]
+        lib.pth_write_png.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_size_t]
         lib.pth_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint32)]
         lib.pth_meansq.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
@@ -180,6 +181,7 @@ def hip_lib():
         lib.pt_set_max_bounces.argtypes = [P, C.c_uint32]
         lib.pt_render.argtypes = [P, C.c_uint32]
         lib.pt_read_accumulation.argtypes = [P, C.c_void_p, C.c_size_t]
+        lib.pt_read_tonemapped.argtypes = [P, C.c_void_p, C.c_size_t]
         lib.pt_trace_primary.argtypes = [P, C.c_uint32, C.c_void_p]
         lib.pt_trace_rays.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
         lib.pt_get_stats.argtypes = [P, C.POINTER(Stats)]
@@ -196,6 +198,7 @@ def hip_lib():
         lib.pt_batch_set_view.argtypes = [P, C.c_uint32, C.POINTER(View)]
         lib.pt_batch_render.argtypes = [P]
         lib.pt_batch_read.argtypes = [P, C.c_uint32, C.c_void_p, C.c_size_t]
+        lib.pt_batch_read_tonemapped.argtypes = [P, C.c_uint32, C.c_void_p, C.c_size_t]
         lib.pt_batch_get_stats.argtypes = [P, C.POINTER(Stats)]
         lib.pt_batch_set_option.argtypes = [P, C.c_uint32, C.c_int64]
         lib.pt_batch_reset_stats.argtypes = [P]
@@ -318,6 +321,13 @@ def write_hdr(path, rgba):
     _check_host(host_lib().pth_write_hdr(os.fsencode(path), rgba.ctypes.data, w, h, w * 16))
 
 
+def write_png(path, rgba8):
+    """write_png (src/io/imagewrite.cpp:44-67) of H x W x 4 uint8 pixels."""
+    rgba8 = np.ascontiguousarray(rgba8, dtype=np.uint8)
+    h, w = rgba8.shape[:2]
+    _check_host(host_lib().pth_write_png(os.fsencode(path), rgba8.ctypes.data, w, h, w * 4))
+
+
 def read_hdr(path):
     lib = host_lib()
     ptr = C.POINTER(C.c_float)()
@@ -394,6 +404,12 @@ class Tracer:
     def read(self):
         out = np.zeros((self.height, self.width, 4), np.float32)
         self._check(self._lib.pt_read_accumulation(self._h, out.ctypes.data, self.width * 16))
+        return out
+
+    def read_tonemapped(self):
+        """The PNG output path: ACES tone mapping and 8-bit conversion, H x W x 4 uint8."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        self._check(self._lib.pt_read_tonemapped(self._h, out.ctypes.data, self.width * 4))
         return out
 
     def trace_primary(self, sample_index):
@@ -517,6 +533,12 @@ class Batch:
         _, w, h, _, _, _ = self.views[i]
         out = np.zeros((h, w, 4), np.float32)
         self._check(self._lib.pt_batch_read(self._h, i, out.ctypes.data, w * 16))
+        return out
+
+    def read_tonemapped(self, i):
+        _, w, h, _, _, _ = self.views[i]
+        out = np.zeros((h, w, 4), np.uint8)
+        self._check(self._lib.pt_batch_read_tonemapped(self._h, i, out.ctypes.data, w * 4))
         return out
 
     def stats(self):

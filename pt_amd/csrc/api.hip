@@ -658,6 +658,38 @@ int pt_read_accumulation(pt_ctx* h, float* rgba, size_t row_pitch)
     return PT_OK;
 }
 
+namespace {
+
+// ACES-tone-mapped RGBA8 of an accumulation image into host rows of row_pitch bytes.
+hipError_t read_tonemapped(const float4* accum, uint32_t w, uint32_t h, uint8_t* out, size_t row_pitch,
+                           hipStream_t st)
+{
+    uint32_t* d = nullptr;
+    hipError_t e = hipMalloc(&d, std::max<size_t>(size_t(w) * h * 4, 4));
+    if (e != hipSuccess)
+        return e;
+    e = tonemap_rgba8(accum, w, h, d, st);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(out, row_pitch, d, size_t(w) * 4, size_t(w) * 4, h, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    return e;
+}
+
+}  // namespace
+
+int pt_read_tonemapped(pt_ctx* h, uint8_t* rgba8, size_t row_pitch)
+{
+    if (!h)
+        return PT_EINVAL;
+    Ctx& c = h->c;
+    if (!rgba8 || row_pitch < size_t(c.width) * 4)
+        return fail(&c, PT_EINVAL, "bad output buffer");
+    HIP_TRY(&c, read_tonemapped(c.accum, c.width, c.height, rgba8, row_pitch, c.stream), "tone-mapped readback");
+    return PT_OK;
+}
+
 int pt_trace_primary(pt_ctx* h, uint32_t sample_index, pt_hit* hits)
 {
     if (!h)
@@ -1171,6 +1203,20 @@ int pt_batch_read(pt_batch* b, uint32_t view, float* rgba, size_t row_pitch)
                               size_t(v.width) * 16, v.height, hipMemcpyDeviceToHost, b->stream),
              "readback");
     BHIP_TRY(b, hipStreamSynchronize(b->stream), "readback");
+    return PT_OK;
+}
+
+int pt_batch_read_tonemapped(pt_batch* b, uint32_t view, uint8_t* rgba8, size_t row_pitch)
+{
+    if (!b)
+        return PT_EINVAL;
+    if (view >= b->views.size())
+        return bfail(b, PT_EINVAL, "no such view");
+    const pt_view& v = b->views[view];
+    if (!rgba8 || row_pitch < size_t(v.width) * 4)
+        return bfail(b, PT_EINVAL, "bad output buffer");
+    BHIP_TRY(b, read_tonemapped(b->accum + b->accum_offset[view], v.width, v.height, rgba8, row_pitch, b->stream),
+             "tone-mapped readback");
     return PT_OK;
 }
 

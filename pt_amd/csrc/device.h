@@ -296,6 +296,10 @@ hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st);
 hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st);
 hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, float* d_out, hipStream_t st);
 hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* d_out, hipStream_t st);
+// ACES tone mapping + 8-bit conversion of the PNG output path (post.hip): width x height
+// RGBA8 words (R in the low byte) into d_out.
+hipError_t tonemap_rgba8(const float4* accum, uint32_t width, uint32_t height, uint32_t* d_out, hipStream_t st);
+
 hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad, float4* rad_out, unsigned blocks,
                        hipStream_t st);
 hipError_t cache_backward(CacheState& c, const float* d_features, const float* d_targets, uint32_t n, uint32_t n_rec,

@@ -1,8 +1,11 @@
 // C ABI of the host scene surface (include/pthost.h).
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "hdrio.h"
 #include "png.h"
@@ -136,6 +139,26 @@ int pth_write_hdr(const char* path, const float* rgba, uint32_t width, uint32_t 
         return PT_EINVAL;
     }
     return guarded([&] { pth::write_hdr_file(path, rgba, width, height, row_pitch); });
+}
+
+int pth_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height, size_t row_pitch)
+{
+    if (!path || !rgba8) {
+        g_error = "null argument";
+        return PT_EINVAL;
+    }
+    return guarded([&] {
+        if (width == 0 || height == 0 || row_pitch < size_t(width) * 4)
+            throw std::runtime_error("invalid image size");
+        const std::vector<uint8_t> png = pth::png_encode_rgba8(rgba8, width, height, row_pitch);
+        FILE* f = std::fopen(path, "wb");
+        if (!f)
+            throw std::runtime_error("Could not write PNG file.");
+        const size_t wrote = std::fwrite(png.data(), 1, png.size(), f);
+        const bool ok = std::fclose(f) == 0 && wrote == png.size();
+        if (!ok)
+            throw std::runtime_error("Could not write PNG file.");
+    });
 }
 
 int pth_read_hdr(const char* path, float** out, uint32_t* width, uint32_t* height)

@@ -2,13 +2,14 @@
 // with PathTraceDispatcher::start (src/pathtrace/ptdispatch.cpp:42-95) driving the MI355X
 // back end through its C ABI.
 //
-//   pt --headless -w W -h H -s SPP -b BOUNCES -o out.hdr -f hdr scene.scn
+//   pt --headless -w W -h H -s SPP -b BOUNCES -o out.png [-f png|hdr] scene.scn
 //
 // Options, defaults and error messages follow src/main.cpp and src/argparse.cpp: long and
 // short names, attached short values (-w1024), `--` ends options; spp 32, bounces 2,
-// 400x400, png. Only the hdr format is produced (the png path needs the tone mapper, which
-// is not part of this back end). Without --headless the reference opens a window; this build
-// has no interactive front end and reports that.
+// 400x400, png. png: the accumulation through the ACES tone mapper to 8-bit RGBA
+// (pt_read_tonemapped, as PathTraceDispatcher::put_result + write_png do); hdr: the
+// accumulation as Radiance RGBE (write_hdr). Without --headless the reference opens a
+// window; this build has no interactive front end and reports that.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -193,10 +194,7 @@ int main(int argc, char** argv)
         std::cerr << "Error: this build has no interactive front end; use --headless." << std::endl;
         return 1;
     }
-    if (format == "png" || format == "PNG") {
-        std::cerr << "Error: only the hdr output format is supported (-f hdr)." << std::endl;
-        return 1;
-    }
+    const bool png = format == "png" || format == "PNG";
 
     pth_scene* scene = nullptr;
     if (pth_scene_load(args[0].c_str(), &scene) != PT_OK) {
@@ -232,13 +230,16 @@ int main(int argc, char** argv)
     int rc = pt_set_camera(ctx, &cam);
     if (rc == PT_OK)
         rc = pt_set_max_bounces(ctx, uint32_t(bounces));
-    // Timing window of ptdispatch.cpp:54-65: from the first submit to the readback.
-    std::vector<float> img(size_t(width) * height * 4);
+    // Timing window of ptdispatch.cpp:54-65: from the first submit to the readback (the tone
+    // mapping of the png path included, as put_result runs it inside the window).
+    std::vector<float> img(png ? 0 : size_t(width) * height * 4);
+    std::vector<uint8_t> img8(png ? size_t(width) * height * 4 : 0);
     const auto t0 = std::chrono::high_resolution_clock::now();
     if (rc == PT_OK)
         rc = pt_render(ctx, uint32_t(samples));
     if (rc == PT_OK)
-        rc = pt_read_accumulation(ctx, img.data(), size_t(width) * 16);
+        rc = png ? pt_read_tonemapped(ctx, img8.data(), size_t(width) * 4)
+                 : pt_read_accumulation(ctx, img.data(), size_t(width) * 16);
     const float secs = std::chrono::duration<float>(std::chrono::high_resolution_clock::now() - t0).count();
     if (rc != PT_OK) {
         std::cerr << "Error: " << pt_last_error(ctx) << std::endl;
@@ -253,7 +254,9 @@ int main(int argc, char** argv)
         std::cout << (st.extension_rays + st.shadow_rays) / secs * 1e-6 << " Mrays per second ("
                   << st.extension_rays << " extension + " << st.shadow_rays << " shadow rays).\n";
     std::cout << "Writing image to " << *out << std::endl;
-    if (pth_write_hdr(out->c_str(), img.data(), uint32_t(width), uint32_t(height), size_t(width) * 16) != PT_OK) {
+    const int wrc = png ? pth_write_png(out->c_str(), img8.data(), uint32_t(width), uint32_t(height), size_t(width) * 4)
+                        : pth_write_hdr(out->c_str(), img.data(), uint32_t(width), uint32_t(height), size_t(width) * 16);
+    if (wrc != PT_OK) {
         std::cerr << "Error: " << pth_last_error() << std::endl;
         rc = 1;
     }

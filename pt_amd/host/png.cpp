@@ -7,6 +7,8 @@
 // (77, 150, 29) / 256.
 #include "png.h"
 
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -478,6 +480,223 @@ Image load_texture_image(const uint8_t* data, size_t size)
     if (!png_info(data, size, w, h, c))
         bad("unsupported image (only PNG textures are decoded)");
     return png_decode(data, size, c == 3 ? 4 : c);
+}
+
+// --- encoding (the PNG path of write_png, src/io/imagewrite.cpp:44-67) -----------------------
+//
+// The reference writes with stb_image_write; the file here is another valid PNG of the same
+// pixels (8-bit RGBA, per-row filter chosen by the smallest sum of absolute filtered bytes,
+// one DEFLATE block with the fixed Huffman codes and LZ77 matches from hash chains).
+
+namespace {
+
+struct BitWriter {
+    std::vector<uint8_t>& out;
+    uint64_t buf = 0;
+    int cnt = 0;
+    void put(uint32_t v, int k)  // k bits of v, least significant first
+    {
+        buf |= uint64_t(v) << cnt;
+        cnt += k;
+        while (cnt >= 8) {
+            out.push_back(uint8_t(buf));
+            buf >>= 8;
+            cnt -= 8;
+        }
+    }
+    void put_code(uint32_t code, int len)  // a Huffman code, most significant bit first
+    {
+        uint32_t r = 0;
+        for (int i = 0; i < len; i++)
+            r |= ((code >> i) & 1u) << (len - 1 - i);
+        put(r, len);
+    }
+    void flush()
+    {
+        if (cnt > 0)
+            out.push_back(uint8_t(buf));
+        buf = 0;
+        cnt = 0;
+    }
+};
+
+// Fixed literal/length codes (RFC 1951 3.2.6).
+void put_symbol(BitWriter& w, int s)
+{
+    if (s < 144)
+        w.put_code(uint32_t(0x30 + s), 8);
+    else if (s < 256)
+        w.put_code(uint32_t(0x190 + (s - 144)), 9);
+    else if (s < 280)
+        w.put_code(uint32_t(s - 256), 7);
+    else
+        w.put_code(uint32_t(0xC0 + (s - 280)), 8);
+}
+
+void put_match(BitWriter& w, int len, int dist)
+{
+    int li = 28;
+    while (LEN_BASE[li] > len)
+        li--;
+    put_symbol(w, 257 + li);
+    if (LEN_EXTRA[li])
+        w.put(uint32_t(len - LEN_BASE[li]), LEN_EXTRA[li]);
+    int di = 29;
+    while (DIST_BASE[di] > dist)
+        di--;
+    w.put_code(uint32_t(di), 5);
+    if (DIST_EXTRA[di])
+        w.put(uint32_t(dist - DIST_BASE[di]), DIST_EXTRA[di]);
+}
+
+uint32_t crc32_of(const uint8_t* p, size_t n, uint32_t crc = 0)
+{
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++)
+                c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        ready = true;
+    }
+    crc = ~crc;
+    for (size_t i = 0; i < n; i++)
+        crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return ~crc;
+}
+
+void put_be32(std::vector<uint8_t>& v, uint32_t x)
+{
+    v.push_back(uint8_t(x >> 24));
+    v.push_back(uint8_t(x >> 16));
+    v.push_back(uint8_t(x >> 8));
+    v.push_back(uint8_t(x));
+}
+
+void put_chunk(std::vector<uint8_t>& png, const char* type, const std::vector<uint8_t>& data)
+{
+    put_be32(png, uint32_t(data.size()));
+    const size_t start = png.size();
+    png.insert(png.end(), type, type + 4);
+    png.insert(png.end(), data.begin(), data.end());
+    put_be32(png, crc32_of(png.data() + start, png.size() - start));
+}
+
+uint8_t paeth_pred(int a, int b, int c)
+{
+    const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    if (pa <= pb && pa <= pc)
+        return uint8_t(a);
+    return uint8_t(pb <= pc ? b : c);
+}
+
+}  // namespace
+
+std::vector<uint8_t> zlib_deflate(const uint8_t* data, size_t n)
+{
+    std::vector<uint8_t> out = {0x78, 0x01};
+    BitWriter w{out};
+    w.put(1, 1);  // BFINAL
+    w.put(1, 2);  // BTYPE 01: fixed Huffman codes
+    constexpr int HBITS = 15;
+    constexpr size_t WINDOW = 32768, WMASK = WINDOW - 1;
+    constexpr int CHAIN = 32;
+    std::vector<int64_t> head(size_t(1) << HBITS, -1), prev(WINDOW, -1);
+    auto hash3 = [&](size_t i) {
+        const uint32_t v = (uint32_t(data[i]) << 16) | (uint32_t(data[i + 1]) << 8) | data[i + 2];
+        return (v * 2654435761u) >> (32 - HBITS);
+    };
+    auto insert = [&](size_t i) {
+        if (i + 3 > n)
+            return;
+        const uint32_t h = hash3(i);
+        prev[i & WMASK] = head[h];
+        head[h] = int64_t(i);
+    };
+    size_t i = 0;
+    while (i < n) {
+        size_t best_len = 0, best_dist = 0;
+        if (i + 3 <= n) {
+            const size_t max_len = std::min<size_t>(258, n - i);
+            int64_t cand = head[hash3(i)];
+            for (int chain = 0; cand >= 0 && i - size_t(cand) <= WINDOW && chain < CHAIN; chain++) {
+                size_t l = 0;
+                while (l < max_len && data[size_t(cand) + l] == data[i + l])
+                    l++;
+                if (l > best_len) {
+                    best_len = l;
+                    best_dist = i - size_t(cand);
+                    if (l == max_len)
+                        break;
+                }
+                const int64_t next = prev[size_t(cand) & WMASK];
+                if (next >= cand)  // the ring slot was reused by a newer position
+                    break;
+                cand = next;
+            }
+        }
+        if (best_len >= 3) {
+            put_match(w, int(best_len), int(best_dist));
+            for (size_t k = 0; k < best_len; k++)
+                insert(i + k);
+            i += best_len;
+        } else {
+            put_symbol(w, data[i]);
+            insert(i);
+            i++;
+        }
+    }
+    put_symbol(w, 256);
+    w.flush();
+    uint32_t a = 1, b = 0;  // Adler-32
+    for (size_t k = 0; k < n; k++) {
+        a = (a + data[k]) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put_be32(out, (b << 16) | a);
+    return out;
+}
+
+std::vector<uint8_t> png_encode_rgba8(const uint8_t* px, uint32_t width, uint32_t height, size_t row_pitch)
+{
+    const size_t rb = size_t(width) * 4;
+    std::vector<uint8_t> raw;
+    raw.reserve((rb + 1) * height);
+    std::vector<uint8_t> cand(rb), best(rb);
+    for (uint32_t y = 0; y < height; y++) {
+        const uint8_t* row = px + size_t(y) * row_pitch;
+        const uint8_t* up = y ? px + size_t(y - 1) * row_pitch : nullptr;
+        uint64_t best_sum = ~0ull;
+        int best_f = 0;
+        for (int f = 0; f < 5; f++) {
+            uint64_t sum = 0;
+            for (size_t i = 0; i < rb; i++) {
+                const int a = i >= 4 ? row[i - 4] : 0, b = up ? up[i] : 0, c = (up && i >= 4) ? up[i - 4] : 0;
+                const int pred = f == 0 ? 0 : f == 1 ? a : f == 2 ? b : f == 3 ? (a + b) >> 1 : paeth_pred(a, b, c);
+                cand[i] = uint8_t(row[i] - pred);
+                sum += uint64_t(std::abs(int(int8_t(cand[i]))));
+            }
+            if (sum < best_sum) {
+                best_sum = sum;
+                best_f = f;
+                best.swap(cand);
+            }
+        }
+        raw.push_back(uint8_t(best_f));
+        raw.insert(raw.end(), best.begin(), best.end());
+    }
+    std::vector<uint8_t> png = {137, 80, 78, 71, 13, 10, 26, 10};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, width);
+    put_be32(ihdr, height);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});  // 8 bits, RGBA, deflate, filter method 0, no interlace
+    put_chunk(png, "IHDR", ihdr);
+    put_chunk(png, "IDAT", zlib_deflate(raw.data(), raw.size()));
+    put_chunk(png, "IEND", {});
+    return png;
 }
 
 }  // namespace pth

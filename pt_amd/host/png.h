@@ -28,4 +28,10 @@ Image load_texture_image(const uint8_t* data, size_t size);
 // zlib stream (RFC 1950/1951) to bytes; exposed for tests.
 std::vector<uint8_t> zlib_inflate(const uint8_t* data, size_t size);
 
+// Bytes to a zlib stream (one DEFLATE block, fixed Huffman codes, LZ77 matches).
+std::vector<uint8_t> zlib_deflate(const uint8_t* data, size_t size);
+
+// An 8-bit RGBA PNG file of width x height pixels whose rows are row_pitch bytes apart.
+std::vector<uint8_t> png_encode_rgba8(const uint8_t* px, uint32_t width, uint32_t height, size_t row_pitch);
+
 }  // namespace pth

@@ -37,6 +37,7 @@ def lib():
         L.oracle_camera_ray.argtypes = [C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_sample_light.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_tonemap.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
         _lib = L
     return _lib
 
@@ -122,3 +123,11 @@ def camera_ray(cam, px, py, w, h, u_pixel, u_lens):
     up, ul = _f32(u_pixel), _f32(u_lens)  # held for the duration of the call
     lib().oracle_camera_ray(C.byref(cam), px, py, w, h, up.ctypes.data, ul.ctypes.data, o.ctypes.data, d.ctypes.data)
     return o, d
+
+
+def tonemap(rgba):
+    """The PNG path's ACES tone mapping + 8-bit conversion of an H x W x 4 float image."""
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    out = np.zeros(a.shape, np.uint8)
+    lib().oracle_tonemap(a.ctypes.data, a.size // 4, out.ctypes.data)
+    return out

@@ -125,3 +125,44 @@ def test_cli_renders_hdr(tmp_path, box_scn):
     pt_amd.write_hdr(ref_file, ref)
     # same seeds -> the files agree up to rare fp32 path flips (RGBE-quantised)
     assert pt_amd.meansq(img, pt_amd.read_hdr(ref_file)) <= 1e-3
+
+
+def test_cli_renders_png_by_default(tmp_path, box_scn):
+    # the reference's default format is png: the accumulation through the ACES tone mapper and
+    # write_png's 8-bit conversion (src/pathtrace/ptdispatch.cpp:155-180, imagewrite.cpp:44-67)
+    pt = os.path.join(ROOT, "pt_amd", "bin", "pt")
+    out = str(tmp_path / "box.png")
+    r = subprocess.run([pt, "--headless", "-w64", "-h48", "-s8", "-b4", "-o", out, box_scn],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    from test_png import decode
+    w, h, c, px = decode(open(out, "rb").read())
+    assert (w, h, c) == (64, 48, 4)
+    img = np.frombuffer(px, np.uint8).reshape(48, 64, 4)
+    # the same render through the library: identical bytes
+    scene = pt_amd.Scene(box_scn)
+    tr = pt_amd.Tracer(scene, 64, 48, max_bounces=4)
+    tr.set_camera(scene.camera(64, 48))
+    tr.render(8)
+    np.testing.assert_array_equal(img, tr.read_tonemapped())
+    # and the tone mapping itself is the oracle's, up to one step where fp32 power functions
+    # round differently
+    ref = ob.tonemap(tr.read())
+    diff = np.abs(img.astype(int) - ref.astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert np.all(img[..., 3] == 255)
+    tr.close()
+
+
+def test_tonemapped_batch_view_equals_single_render(box_scn):
+    scene = pt_amd.Scene(box_scn)
+    cam = scene.camera(40, 30)
+    tr = pt_amd.Tracer(scene, 40, 30, max_bounces=4)
+    tr.set_camera(cam)
+    tr.render(4)
+    single = tr.read_tonemapped()
+    batch = pt_amd.Batch([tr], [(0, 40, 30, 4, 4, cam)])
+    batch.render()
+    np.testing.assert_array_equal(batch.read_tonemapped(0), single)
+    batch.close()
+    tr.close()
