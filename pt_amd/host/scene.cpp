@@ -556,6 +556,28 @@ Scene load_gltf(const std::string& path)
         walk_nodes(g.doc, r, M4::identity(), mesh_instances, mesh_prims, 0);
     for (size_t i = 0; i < scene.variants.size(); i++)
         scene.variants[i].instances = std::move(mesh_instances[i]);
+
+    // Punctual lights (process_lights, src/scene/scene.cpp:449-474, reading Assimp's import of
+    // KHR_lights_punctual): point and directional lights in the extension's order, colour =
+    // color x intensity (defaults 1), position (0, 0, 0) and direction (0, 0, -1). Assimp keeps
+    // a light in the space of the node that references it, and process_lights takes those
+    // members without the node's transform, so a glTF light sits at the object's origin. Spot
+    // lights are dropped, as process_lights drops them.
+    const Json& lights = g.doc["extensions"]["KHR_lights_punctual"]["lights"];
+    for (size_t i = 0; i < lights.size(); i++) {
+        const Json& L = lights[i];
+        const std::string type = L["type"].kind == Json::STR ? L["type"].str : "";
+        const float intensity = float(L["intensity"].number(1.0));
+        const Json& c = L["color"];
+        const V3 col{float(c[size_t(0)].number(1.0)) * intensity, float(c[size_t(1)].number(1.0)) * intensity,
+                     float(c[size_t(2)].number(1.0)) * intensity};
+        if (type == "point")
+            scene.point_lights.push_back({V3{0, 0, 0}, col});
+        else if (type == "directional")
+            scene.dir_lights.push_back({V3{0, 0, -1}, col});
+        else if (type != "spot")
+            throw std::runtime_error("glTF: unknown light type " + type);
+    }
     return scene;
 }
 
@@ -790,6 +812,17 @@ Scene load_scn(const std::string& path)
         Scene loaded = load_gltf(scene.resource_directory + sub.path);
         for (Mesh& m : loaded.meshes)
             scene.meshes.push_back(std::move(m));
+        // the object's lights, once per instance, in the instance's frame (scenebuild.cpp:323-335)
+        for (const SceneInstance& si : sub.instances) {
+            for (const PointLight& pl : loaded.point_lights) {
+                const V4 p = si.transform * V4(pl.pos.x, pl.pos.y, pl.pos.z, 1.0f);
+                scene.point_lights.push_back({V3{p.x, p.y, p.z}, pl.col});
+            }
+            for (const DirLight& dl : loaded.dir_lights) {
+                const V4 d = si.transform * V4(dl.dir.x, dl.dir.y, dl.dir.z, 0.0f);
+                scene.dir_lights.push_back({V3{d.x, d.y, d.z}, dl.col});
+            }
+        }
         for (const Variant& lv : loaded.variants) {
             Variant v;
             v.mesh = lv.mesh + mesh_offset;

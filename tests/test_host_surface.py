@@ -91,6 +91,39 @@ def test_point_and_directional_lights_in_light_table(tmp_path):
     np.testing.assert_allclose(p, [3 / (3 + f), f / (3 + f)], rtol=1e-6)
 
 
+def test_gltf_punctual_lights(tmp_path):
+    """KHR_lights_punctual as process_lights (src/scene/scene.cpp:449-474) receives it from
+    Assimp: colour = color x intensity, not sRGB-decoded; position (0, 0, 0) / direction
+    (0, 0, -1) in the referencing node's space, without the node transform; spot lights
+    dropped; then one copy per .scn instance in the instance's frame (scenebuild.cpp:323-335).
+    Assimp itself is absent here: parity unpinned beyond these rules."""
+    import json
+    _scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n tri\n")  # writes tri.gltf
+    doc = json.load(open(os.path.join(tmp_path, "tri.gltf")))
+    doc.setdefault("extensions", {})["KHR_lights_punctual"] = {"lights": [
+        {"type": "point", "color": [1.0, 0.5, 0.25], "intensity": 4.0},
+        {"type": "spot", "intensity": 9.0, "spot": {}},
+        {"type": "directional", "intensity": 2.0},
+    ]}
+    doc["nodes"].append({"extensions": {"KHR_lights_punctual": {"light": 0}}, "translation": [5, 5, 5]})
+    json.dump(doc, open(os.path.join(tmp_path, "lit.gltf"), "w"))
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n lit lit.gltf\n\n lit\n  t 1 2 3\n lit\n"
+                                    "  r 0 1 0 1.5707964\n  t -1 0 0\n", gltf=False))
+    info = s.info()
+    assert info["point_lights"] == 2 and info["directional_lights"] == 2
+    a = s.arrays()
+    assert a["light_count"] == 4
+    words = a["light_words"].reshape(-1, 10)[:4]
+    assert list(words[:, 3]) == [1, 1, 2, 2]
+    pos = words[:2, 4:7].view(np.float32)
+    np.testing.assert_array_equal(pos, [[1, 2, 3], [-1, 0, 0]])  # the instances' origins
+    np.testing.assert_array_equal(words[0, 7:10].view(np.float32), [4.0, 2.0, 1.0])
+    np.testing.assert_array_equal(words[2, 7:10].view(np.float32), [2.0, 2.0, 2.0])
+    d0, d1 = words[2, 4:7].view(np.float32), words[3, 4:7].view(np.float32)
+    np.testing.assert_array_equal(d0, [0, 0, -1])
+    np.testing.assert_allclose(d1, [-1, 0, 0], atol=1e-6)  # (0, 0, -1) turned pi/2 about y
+
+
 # --- the generated correctness scene ------------------------------------------------------
 
 def test_box_scene_packing(box_scn):
