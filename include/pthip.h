@@ -194,7 +194,12 @@ int pt_set_max_bounces(pt_ctx* ctx, uint32_t max_bounces);
  * samples per pixel into the accumulation image, one sample index after another exactly as
  * the reference's 1-spp submits do (running mean of correct_radiance(L*exposure)), and
  * advances sample_index by spp (PathTracer::update_uniforms, pathtrace.cpp:511-516).
- * spp == 0 -> PT_EINVAL (pathtrace.cpp:400-401). Asynchronous on the context's stream. */
+ * spp == 0 -> PT_EINVAL (pathtrace.cpp:400-401). Asynchronous on the context's stream, and
+ * returns once every path of the call has started: the last paths may still be in flight
+ * while the next pt_render's paths enter the pool (two job slots), so a job's tail overlaps
+ * the next job's start. Every call that reads results or changes what the paths use
+ * (pt_read_*, pt_get_stats, pt_set_*, pt_trace_*, pt_cache_*, pt_autotune, pt_synchronize,
+ * pt_destroy) first finishes them; the image equals one pt_render of all the samples. */
 int pt_render(pt_ctx* ctx, uint32_t spp);
 
 /* Readback of the accumulation image (ptdispatch.cpp:77-94 copy_image + vkMapMemory):
@@ -251,7 +256,8 @@ int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* 
                     pt_batch** out);
 /* Replaces a view's camera, scene, spp or depth (not its size) and restarts its accumulation. */
 int pt_batch_set_view(pt_batch* batch, uint32_t view, const pt_view* v);
-/* Asynchronous on the batch's stream. */
+/* Asynchronous on the batch's stream; like pt_render, returns once all its paths have started,
+ * and the other pt_batch_* calls finish them first. */
 int pt_batch_render(pt_batch* batch);
 int pt_batch_read(pt_batch* batch, uint32_t view, float* rgba, size_t row_pitch);
 /* pt_read_tonemapped for one view of the batch. */

@@ -378,8 +378,9 @@ struct pt_batch {
     DeviceScene* d_scenes = nullptr;   // scene table (several scenes only)
     ViewDev* d_views = nullptr;
     ViewDev* h_views = nullptr;        // pinned staging
-    JobDev* d_job = nullptr;
+    JobDev* d_job = nullptr;   // the two job slots
     JobDev* h_job = nullptr;
+    StreamState stream_state;
     Wavefront wf;
     uint32_t grid_blocks = 0;
     bool count_fetches = false;
@@ -472,6 +473,22 @@ void view_dev(const pt_batch& b, uint32_t i, ViewDev& d)
 }  // namespace
 
 extern "C" {
+
+namespace {
+
+// Renders return once all their paths have started; the last ones may still be in flight,
+// overlapping the next render. Every call that reads the results or changes what those paths
+// use finishes them first (and stamps the end of the render for the statistics).
+int settle(Ctx& c)
+{
+    if (c.stream_state.idle())
+        return PT_OK;
+    HIP_TRY(&c, drain_stream(c), "render");
+    HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
+    return PT_OK;
+}
+
+}  // namespace
 
 int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out)
 {
@@ -579,9 +596,10 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              c.job_paths = npix * job_spp;
              HIP_TRY(&c, apply_launch(c), "wavefront buffers");
              HIP_TRY(&c, hipMalloc(&c.d_views, sizeof(ViewDev)), "view table");
-             HIP_TRY(&c, hipMalloc(&c.d_job, sizeof(JobDev)), "job table");
+             HIP_TRY(&c, hipMalloc(&c.d_job, 2 * sizeof(JobDev)), "job table");
+             HIP_TRY(&c, hipMemsetAsync(c.d_job, 0, 2 * sizeof(JobDev), st), "job table");
              HIP_TRY(&c, hipHostMalloc((void**)&c.h_view, sizeof(ViewDev), hipHostMallocDefault), "view table");
-             HIP_TRY(&c, hipHostMalloc((void**)&c.h_job, sizeof(JobDev), hipHostMallocDefault), "job table");
+             HIP_TRY(&c, hipHostMalloc((void**)&c.h_job, 2 * sizeof(JobDev), hipHostMallocDefault), "job table");
              if (const char* env = getenv("PT_GRAPHS"))
                  c.use_graphs = atoi(env) != 0;
              HIP_TRY(&c, hipStreamSynchronize(st), "upload");
@@ -596,6 +614,8 @@ int pt_set_camera(pt_ctx* h, const pt_camera* cam)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     if (!cam)
         return fail(&h->c, PT_EINVAL, "camera is null");
     static_assert(sizeof(CameraBlock) == sizeof(pt_camera), "camera block layout");
@@ -610,6 +630,8 @@ int pt_set_max_bounces(pt_ctx* h, uint32_t max_bounces)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     if (max_bounces > 1024)
         return fail(&h->c, PT_EINVAL, "max_bounces too large");
     h->c.max_bounces = max_bounces;
@@ -636,7 +658,7 @@ int pt_render(pt_ctx* h, uint32_t spp)
         c.last_paths = 0;
         c.stats_fresh = false;
     }
-    HIP_TRY(&c, render_samples(c, spp), "render");
+    HIP_TRY(&c, render_samples(c, spp, false), "render");
     HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
     c.sample_index += spp;
     c.last_paths += uint64_t(spp) * c.width * c.height;
@@ -647,6 +669,8 @@ int pt_read_accumulation(pt_ctx* h, float* rgba, size_t row_pitch)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!rgba || row_pitch < size_t(c.width) * 16)
         return fail(&c, PT_EINVAL, "bad output buffer");
@@ -683,6 +707,8 @@ int pt_read_tonemapped(pt_ctx* h, uint8_t* rgba8, size_t row_pitch)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!rgba8 || row_pitch < size_t(c.width) * 4)
         return fail(&c, PT_EINVAL, "bad output buffer");
@@ -694,6 +720,8 @@ int pt_trace_primary(pt_ctx* h, uint32_t sample_index, pt_hit* hits)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!hits)
         return fail(&c, PT_EINVAL, "hits is null");
@@ -717,6 +745,8 @@ int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, i
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (count == 0)
         return PT_OK;
@@ -748,6 +778,8 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!out)
         return fail(&c, PT_EINVAL, "out is null");
@@ -799,6 +831,8 @@ int pt_reset_stats(pt_ctx* h)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     h->c.stats_fresh = true;
     return PT_OK;
 }
@@ -807,6 +841,8 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     switch (option) {
     case PT_OPT_PROFILE_KERNELS:
@@ -872,6 +908,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!cfg)
         return fail(&c, PT_EINVAL, "config is null");
@@ -910,6 +948,8 @@ int pt_cache_get_stats(pt_ctx* h, pt_cache_stats* out)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!out)
         return fail(&c, PT_EINVAL, "out is null");
@@ -945,6 +985,8 @@ int pt_cache_get_params(pt_ctx* h, float* params, size_t count)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (int v = need_cache(c))
         return v;
@@ -958,6 +1000,8 @@ int pt_cache_set_params(pt_ctx* h, const float* params, size_t count)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (int v = need_cache(c))
         return v;
@@ -972,6 +1016,8 @@ int pt_cache_forward(pt_ctx* h, const float* features, uint32_t n, float* out)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (int v = need_cache(c))
         return v;
@@ -995,6 +1041,8 @@ int pt_cache_encode(pt_ctx* h, const float* records, uint32_t n, float* features
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (int v = need_cache(c))
         return v;
@@ -1019,6 +1067,8 @@ int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets,
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (int v = need_cache(c))
         return v;
@@ -1046,6 +1096,28 @@ int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets,
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache train");
     return PT_OK;
 }
+
+namespace {
+
+RenderIO batch_io(pt_batch& b)
+{
+    return RenderIO{b.stream,   &b.wf,   b.grid_blocks, b.count_fetches,
+                    &b.timer,   b.scenes[0]->c.scene,   b.d_scenes,     b.d_views,
+                    b.d_job,    b.h_job, &b.stream_state,
+                    b.use_graphs ? &b.graphs : nullptr, nullptr, b.lds_stack, b.sort_hits};
+}
+
+// As settle() for a context: finishes the paths a batch render left in flight.
+int bsettle(pt_batch* b)
+{
+    if (b->stream_state.idle())
+        return PT_OK;
+    BHIP_TRY(b, render_views(batch_io(*b), {}, true), "render");
+    BHIP_TRY(b, hipEventRecord(b->ev_stop, b->stream), "event");
+    return PT_OK;
+}
+
+}  // namespace
 
 int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* views, uint32_t view_count,
                     pt_batch** out)
@@ -1100,8 +1172,9 @@ int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* 
         BHIP_TRY(b, hipMalloc(&b->d_views, view_count * sizeof(ViewDev)), "view table");
         BHIP_TRY(b, hipHostMalloc((void**)&b->h_views, view_count * sizeof(ViewDev), hipHostMallocDefault),
                  "view table");
-        BHIP_TRY(b, hipMalloc(&b->d_job, sizeof(JobDev)), "job table");
-        BHIP_TRY(b, hipHostMalloc((void**)&b->h_job, sizeof(JobDev), hipHostMallocDefault), "job table");
+        BHIP_TRY(b, hipMalloc(&b->d_job, 2 * sizeof(JobDev)), "job table");
+        BHIP_TRY(b, hipMemset(b->d_job, 0, 2 * sizeof(JobDev)), "job table");
+        BHIP_TRY(b, hipHostMalloc((void**)&b->h_job, 2 * sizeof(JobDev), hipHostMallocDefault), "job table");
         for (uint32_t i = 0; i < view_count; i++)
             view_dev(*b, i, b->h_views[i]);
         BHIP_TRY(b,
@@ -1138,6 +1211,8 @@ int pt_batch_set_view(pt_batch* b, uint32_t view, const pt_view* v)
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     if (!v || view >= b->views.size())
         return bfail(b, PT_EINVAL, "no such view");
     const pt_view& old = b->views[view];
@@ -1177,11 +1252,7 @@ int pt_batch_render(pt_batch* b)
         b->last_paths = 0;
         b->stats_fresh = false;
     }
-    RenderIO io{b->stream,   &b->wf,   b->grid_blocks, b->count_fetches,
-                &b->timer,   b->scenes[0]->c.scene,    b->d_scenes,    b->d_views,
-                b->d_job,    b->h_job, b->use_graphs ? &b->graphs : nullptr,
-                nullptr,     b->lds_stack, b->sort_hits};
-    BHIP_TRY(b, render_views(io, work), "render");
+    BHIP_TRY(b, render_views(batch_io(*b), work, false), "render");
     BHIP_TRY(b, hipEventRecord(b->ev_stop, b->stream), "event");
     for (uint32_t i = 0; i < b->views.size(); i++)
         b->sample_index[i] += b->views[i].spp;
@@ -1193,6 +1264,8 @@ int pt_batch_read(pt_batch* b, uint32_t view, float* rgba, size_t row_pitch)
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     if (view >= b->views.size())
         return bfail(b, PT_EINVAL, "no such view");
     const pt_view& v = b->views[view];
@@ -1210,6 +1283,8 @@ int pt_batch_read_tonemapped(pt_batch* b, uint32_t view, uint8_t* rgba8, size_t 
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     if (view >= b->views.size())
         return bfail(b, PT_EINVAL, "no such view");
     const pt_view& v = b->views[view];
@@ -1224,6 +1299,8 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     if (!out)
         return bfail(b, PT_EINVAL, "out is null");
     BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
@@ -1272,6 +1349,8 @@ int pt_batch_set_option(pt_batch* b, uint32_t option, int64_t value)
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     switch (option) {
     case PT_OPT_PROFILE_KERNELS:
         BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
@@ -1304,6 +1383,8 @@ int pt_batch_reset_stats(pt_batch* b)
 {
     if (!b)
         return PT_EINVAL;
+    if (int v = bsettle(b))
+        return v;
     b->stats_fresh = true;
     return PT_OK;
 }
@@ -1314,6 +1395,7 @@ void pt_batch_destroy(pt_batch* b)
 {
     if (!b)
         return;
+    (void)bsettle(b);
     batch_release(*b);
     delete b;
 }
@@ -1322,6 +1404,8 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (spp == 0)
         return fail(&c, PT_EINVAL, "Sample count was zero.");
@@ -1364,7 +1448,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
             HIP_TRY(&c, hipMemcpy(t0, c.wf.totals, sizeof(t0), hipMemcpyDeviceToHost), "stats");
             c.sample_index = 0;
             HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
-            HIP_TRY(&c, render_samples(c, spp), "render");
+            HIP_TRY(&c, render_samples(c, spp, true), "render");
             HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
             HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
             float ms = 0.0f;
@@ -1435,6 +1519,8 @@ int pt_build_light_tables(pt_ctx* h)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!c.light_layout_known)
         return fail(&c, PT_ESTATE, "the light-sampler words do not have the packed layout");
@@ -1449,6 +1535,8 @@ int pt_read_light_words(pt_ctx* h, uint32_t* words, size_t count)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     Ctx& c = h->c;
     if (!words || count != c.light_word_count)
         return fail(&c, PT_EINVAL, "the buffer must hold the scene's light_word_count words");
@@ -1462,6 +1550,8 @@ int pt_synchronize(pt_ctx* h)
 {
     if (!h)
         return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
     HIP_TRY(&h->c, hipStreamSynchronize(h->c.stream), "sync");
     return PT_OK;
 }
@@ -1472,6 +1562,7 @@ void pt_destroy(pt_ctx* h)
 {
     if (!h)
         return;
+    (void)settle(h->c);
     (void)hipStreamSynchronize(h->c.stream);
     release(h->c);
     delete h;

@@ -48,7 +48,8 @@ struct Wavefront {
     float4* rad_out = nullptr;     // radiance of every path of the job, by path id
     uint32_t rad_out_capacity = 0; // paths
     // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] ended
-    // recorded paths, [5] next path to start
+    // recorded paths, [5],[6] next path to start of job slot 0, 1, [7],[8] ended paths of job
+    // slot 0, 1
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
@@ -60,7 +61,8 @@ struct Wavefront {
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     uint32_t spill_threads = 0;
-    uint32_t* host_counts = nullptr;  // pinned: queue counts read back with a one-iteration lag
+    uint32_t* snapshot = nullptr;     // per parity 8 words: next queue, ended[2], issued[2]
+    uint32_t* host_counts = nullptr;  // pinned copy of snapshot, read back with a one-iteration lag
     hipEvent_t count_ready[2] = {nullptr, nullptr};
 };
 
@@ -99,12 +101,32 @@ struct SegDev {
     uint32_t pad[3];
 };
 struct JobDev {
-    uint32_t paths;            // paths of the job
+    uint32_t paths;            // paths of the job (0: the slot holds no job)
     uint32_t nseg;
     uint32_t signed_radiance;  // keep negative per-sample sums (control-variate estimates)
     uint32_t pix_total;        // sum of the segments' pixel counts
-    uint32_t pad[4];
+    uint32_t seq;              // order of the jobs in the two slots: the older one issues first
+    uint32_t pad[3];
     SegDev seg[MAX_SEGS];
+};
+
+// Host side of a wavefront's path stream, kept between renders: a job is loaded into one of
+// two slots (device tables d_job[0], d_job[1]); its paths are issued into the pool, the older
+// slot's first, and once every path of a job has ended its radiance is accumulated and the
+// slot is free again. The next job's paths thus fill the pool while the previous job's last
+// paths drain, within a render and from one render call to the next.
+struct StreamState {
+    int q = 0;               // parity of the queue the next iteration traces
+    uint64_t launched = 0;   // iterations queued so far
+    uint32_t seq = 0;
+    struct Slot {
+        bool busy = false;
+        uint32_t paths = 0, seq = 0;
+        uint64_t first = 0;   // the first iteration queued after the job was loaded
+        uint32_t issued = 0;  // paths started, as last read back
+        uint32_t ended = 0;   // paths ended, as last read back
+    } slot[2];
+    bool idle() const { return !slot[0].busy && !slot[1].busy; }
 };
 
 // Executable graphs of one wavefront iteration (queue parity 0 and 1), captured on first use
@@ -220,8 +242,9 @@ struct Ctx {
     CacheState cache;
     ViewDev* d_views = nullptr;  // the context's one view
     ViewDev* h_view = nullptr;   // pinned staging of it
-    JobDev* d_job = nullptr;
-    JobDev* h_job = nullptr;     // pinned staging of the job table
+    JobDev* d_job = nullptr;     // the two job slots
+    JobDev* h_job = nullptr;     // pinned staging of the job slots
+    StreamState stream_state;
     IterGraphs graphs;
     bool use_graphs = true;
     // launch configuration (PT_OPT_BLOCKS_PER_CU / POOL_SLOTS / LDS_STACK / LEAF_SIZE)
@@ -258,8 +281,9 @@ struct RenderIO {
     DeviceScene scene;              // the scene when there is one
     const DeviceScene* d_scenes;    // scene table when there are several, else nullptr
     const ViewDev* d_views;
-    JobDev* d_job;
-    JobDev* h_job;
+    JobDev* d_job;                  // the two job slots (device)
+    JobDev* h_job;                  // their pinned staging
+    StreamState* stream;            // the path stream of this wavefront
     IterGraphs* graphs;             // nullptr: launch every kernel directly
     CacheState* cache;              // nullptr: no radiance cache
     uint32_t lds_stack;             // LDS stack entries per lane of the traversal kernels
@@ -272,10 +296,17 @@ struct ViewWork {
 };
 
 // Renders the given samples of every view, all views' paths streaming through one pool.
-hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work);
+// Returns once every path of them has been started (the last ones may still be in flight,
+// overlapping the next render), or, with drain, once every started path has ended and been
+// accumulated.
+hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, bool drain);
 
-// Renders samples [ctx.sample_index, ctx.sample_index + spp) into ctx.accum.
-hipError_t render_samples(Ctx& ctx, uint32_t spp);
+// Renders samples [ctx.sample_index, ctx.sample_index + spp) into ctx.accum (see
+// render_views for drain).
+hipError_t render_samples(Ctx& ctx, uint32_t spp, bool drain);
+
+// Finishes every path in flight of the context's stream (a no-op when it is idle).
+hipError_t drain_stream(Ctx& ctx);
 
 // Primary hits of one sample index, into a device array of width*height pt_hit.
 hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits);

@@ -404,10 +404,10 @@ __device__ __forceinline__ uint32_t find_segment(const JobDev* job, uint32_t p)
     return lo;
 }
 
-// Starts job path p as entry pos of queue par: camera ray of its view (pathtrace.rg.hlsl:18-33)
-// and a fresh payload.
-__device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, const Wavefront& wf, int par,
-                                           uint32_t pos, uint32_t p)
+// Starts path p of the job in slot `slot` as entry pos of queue par: camera ray of its view
+// (pathtrace.rg.hlsl:18-33) and a fresh payload.
+__device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, uint32_t slot, const Wavefront& wf,
+                                           int par, uint32_t pos, uint32_t p)
 {
     const SegDev sg = job->seg[find_segment(job, p)];
     const ViewDev& vw = views[sg.view];
@@ -422,29 +422,10 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     camera_ray(vw.cam, px, py, vw.width, vw.height, up, ul, o, d);
     wf.q_o[par][pos] = make_float4(o.x, o.y, o.z, vw.cam.near_clip);
     wf.q_d[par][pos] = make_float4(d.x, d.y, d.z, vw.cam.far_clip);
-    wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, sg.view << 16);  // segment 0
+    wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, (sg.view << 16) | (slot << 15));  // segment 0
     wf.q_thr[par][pos] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     wf.q_rad[par][pos] = make_float4(0.0f, 0.0f, 0.0f, u32_as_float(NONE));  // no training record
     wf.q_scene[par][pos] = vw.scene;
-}
-
-// Fills the pool with the job's first paths; their camera rays form queue 0.
-__global__ __launch_bounds__(256) void k_start_job(const ViewDev* views, const JobDev* job, Wavefront wf, CacheDev cd)
-{
-    const uint32_t n = min(job->paths, wf.capacity);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && cd.collect) {
-        cd.counters[0] = 0;
-        cd.counters[1] = 0;
-    }
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
-        start_path(views, job, wf, 0, s, s);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        wf.counters[CTR(0)] = n;
-        wf.counters[CTR(1)] = 0;
-        wf.counters[CTR(2)] = 0;
-        wf.counters[CTR(4)] = 0;
-        wf.counters[CTR(5)] = n;
-    }
 }
 
 // Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
@@ -567,7 +548,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             float3 L = xyz(rr);
             const float3 ro = xyz(ro4), rd = xyz(rd4);
             const uint32_t px = ps.y & 0xFFFFu, py = ps.y >> 16, sample = ps.z;
-            segment = ps.w & 0xFFFFu;
+            segment = ps.w & 0x7FFFu;
             bool terminated = false;
 
             if (hinst == NONE) {
@@ -829,18 +810,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         }
         // a recorded path that ends here: its training record is completed after the shadow
         // rays of this iteration have added their share
-        const bool ended_rec = valid && !cont && (rec != NONE || want_train);
-        uint32_t apos[5] = {0, 0, 0, 0, 0};
+        const bool ended = valid && !cont;
+        const bool ended_rec = ended && (rec != NONE || want_train);
+        const uint32_t slot = (ps.w >> 15) & 1u;  // job slot of the path
+        // ended paths are counted per job slot: a job is complete when all its paths ended
+        uint32_t apos[7] = {0, 0, 0, 0, 0, 0, 0};
         if (cd.collect) {  // uniform: training records, cache queries and ended records are appended too
-            block_append<5>({cont, shadow, want_train, want_query, ended_rec},
+            block_append<7>({cont, shadow, want_train, want_query, ended_rec, ended && slot == 0u, ended && slot == 1u},
                             {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)], &cd.counters[0], &cd.counters[1],
-                             &wf.counters[CTR(4)]},
+                             &wf.counters[CTR(4)], &wf.counters[CTR(7)], &wf.counters[CTR(8)]},
                             apos);
         } else {
-            uint32_t p2[2];
-            block_append<2>({cont, shadow}, {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)]}, p2);
-            apos[0] = p2[0], apos[1] = p2[1];
+            uint32_t p4[4];
+            block_append<4>({cont, shadow, ended && slot == 0u, ended && slot == 1u},
+                            {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)], &wf.counters[CTR(7)],
+                             &wf.counters[CTR(8)]},
+                            p4);
+            apos[0] = p4[0], apos[1] = p4[1];
         }
+        // the path's radiance slot: its job slot's half of rad_out
+        const uint32_t ring = slot * wf.rad_out_capacity + ps.x;
         const uint32_t pos = apos[0], spos = apos[1], tpos = apos[2], qpos = apos[3];
         if (want_train)
             rec = tpos < cd.record_capacity ? tpos : NONE;
@@ -853,11 +842,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             if (MULTI)
                 wf.q_scene[1 - q][pos] = sid;
         } else if (valid) {
-            wf.rad_out[ps.x] = rad_v;
+            wf.rad_out[ring] = rad_v;
         }
-        const uint32_t target = cont ? pos : (TARGET_ENDED | ps.x);
+        const uint32_t target = cont ? pos : (TARGET_ENDED | ring);
         if (ended_rec)
-            wf.train_end[apos[4]] = make_uint2(ps.x, rec);
+            wf.train_end[apos[4]] = make_uint2(ring, rec);
         if (shadow) {
             wf.s_target[spos] = target;
             wf.s_o[spos] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
@@ -916,10 +905,33 @@ __global__ __launch_bounds__(256) void k_hit_keys(DeviceScene s0, const DeviceSc
 
 // Ends an iteration: completes the training records of recorded paths that ended in it
 // (their radiance is final once the shadow rays and cache queries have added theirs), then
-// appends the job's next paths behind the continuing ones in queue 1-q: consecutive path ids
-// into consecutive entries, so no atomics; k_refill_commit then advances the counters.
-__global__ __launch_bounds__(256) void k_refill(const ViewDev* views, const JobDev* job, Wavefront wf, int q,
-                                                CacheDev cd)
+// appends new paths behind the continuing ones in queue 1-q: the remaining paths of the older
+// job slot first, then those of the newer one, consecutive path ids into consecutive entries,
+// so no atomics. k_refill_commit then advances the counters.
+struct RefillPlan {
+    uint32_t n_cont, first;   // continuing entries, the older slot
+    uint32_t next[2], n[2];   // per slot: first new path id, new paths
+};
+
+__device__ __forceinline__ RefillPlan refill_plan(const JobDev* j0, const JobDev* j1, const Wavefront& wf, int q)
+{
+    RefillPlan r;
+    r.n_cont = wf.counters[CTR(1 - q)];
+    r.first = int(j1->seq - j0->seq) < 0 ? 1u : 0u;
+    const JobDev* jb[2] = {j0, j1};
+    uint32_t room = wf.capacity - r.n_cont;
+    for (int k = 0; k < 2; k++) {
+        const uint32_t s = k == 0 ? r.first : 1u - r.first;
+        r.next[s] = wf.counters[CTR(5 + s)];
+        const uint32_t paths = jb[s]->paths;  // a free slot has none
+        r.n[s] = paths > r.next[s] ? min(room, paths - r.next[s]) : 0u;
+        room -= r.n[s];
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_refill(const ViewDev* views, const JobDev* j0, const JobDev* j1, Wavefront wf,
+                                                int q, CacheDev cd)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
     if (cd.collect) {
@@ -940,19 +952,50 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev* views, const JobD
             rec[19] = 1.0f;  // complete
         }
     }
-    const uint32_t n_cont = wf.counters[CTR(1 - q)], next = wf.counters[CTR(5)];
-    const uint32_t n_new = min(wf.capacity - n_cont, job->paths - next);
-    for (uint32_t k = tid; k < n_new; k += stride)
-        start_path(views, job, wf, 1 - q, n_cont + k, next + k);
+    const RefillPlan r = refill_plan(j0, j1, wf, q);
+    const uint32_t s0 = r.first, s1 = 1u - r.first;
+    for (uint32_t k = tid; k < r.n[s0]; k += stride)
+        start_path(views, s0 ? j1 : j0, s0, wf, 1 - q, r.n_cont + k, r.next[s0] + k);
+    for (uint32_t k = tid; k < r.n[s1]; k += stride)
+        start_path(views, s1 ? j1 : j0, s1, wf, 1 - q, r.n_cont + r.n[s0] + k, r.next[s1] + k);
 }
 
-__global__ void k_refill_commit(const JobDev* job, Wavefront wf, int q)
+// Advances the counters by what k_refill appended and leaves the counts the host steers by
+// (next queue, paths ended and paths started per slot) in wf.snapshot[q].
+__global__ void k_refill_commit(const JobDev* j0, const JobDev* j1, Wavefront wf, int q)
 {
     if (threadIdx.x == 0) {
-        const uint32_t n_cont = wf.counters[CTR(1 - q)], next = wf.counters[CTR(5)];
-        const uint32_t n_new = min(wf.capacity - n_cont, job->paths - next);
-        wf.counters[CTR(1 - q)] = n_cont + n_new;
-        wf.counters[CTR(5)] = next + n_new;
+        const RefillPlan r = refill_plan(j0, j1, wf, q);
+        const uint32_t count = r.n_cont + r.n[0] + r.n[1];
+        wf.counters[CTR(1 - q)] = count;
+        wf.counters[CTR(5)] = r.next[0] + r.n[0];
+        wf.counters[CTR(6)] = r.next[1] + r.n[1];
+        uint32_t* snap = wf.snapshot + 8 * q;
+        snap[0] = count;
+        snap[1] = wf.counters[CTR(7)];
+        snap[2] = wf.counters[CTR(8)];
+        snap[3] = r.next[0] + r.n[0];
+        snap[4] = r.next[1] + r.n[1];
+    }
+}
+
+// The job in slot s has been accumulated: the slot issues nothing until it is loaded again,
+// and its counts restart from zero.
+__global__ void k_slot_free(JobDev* job, Wavefront wf, uint32_t s)
+{
+    if (threadIdx.x == 0) {
+        job->paths = 0;
+        wf.counters[CTR(5 + s)] = 0;
+        wf.counters[CTR(7 + s)] = 0;
+    }
+}
+
+// A job was loaded into slot s: no paths of it started or ended yet.
+__global__ void k_slot_reset(Wavefront wf, uint32_t s)
+{
+    if (threadIdx.x == 0) {
+        wf.counters[CTR(5 + s)] = 0;
+        wf.counters[CTR(7 + s)] = 0;
     }
 }
 
@@ -1096,12 +1139,13 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
     A(wf.s_d, c * 16);
     A(wf.s_c, c * 16);
     A(wf.train_end, c * 8);
-    A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 16);
+    A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 2 * 16);  // one half per job slot
     A(wf.counters, size_t(CTR(16)) * 4);
+    A(wf.snapshot, 16 * 4);
     A(wf.totals, 8 * 8);
     A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
     if (e == hipSuccess)
-        e = hipHostMalloc((void**)&wf.host_counts, 4 * sizeof(uint32_t), hipHostMallocDefault);
+        e = hipHostMalloc((void**)&wf.host_counts, 16 * sizeof(uint32_t), hipHostMallocDefault);
     for (int k = 0; k < 2 && e == hipSuccess; k++)
         e = hipEventCreateWithFlags(&wf.count_ready[k], hipEventDisableTiming);
     if (e == hipSuccess)
@@ -1117,7 +1161,7 @@ void wavefront_free(Wavefront& wf)
                     wf.q_ps[1],     wf.q_thr[0],    wf.q_thr[1],    wf.q_rad[0],    wf.q_rad[1],
                     wf.q_scene[0],  wf.q_scene[1],  wf.hit,         wf.hit_prim,    wf.s_target,
                     wf.s_o,         wf.s_d,         wf.s_c,         wf.s_scene,     wf.train_end,
-                    wf.rad_out,     wf.counters,    wf.totals,      wf.spill,       wf.sort_keys[0],
+                    wf.rad_out,     wf.counters,    wf.totals,      wf.spill,       wf.snapshot,    wf.sort_keys[0],
                     wf.sort_keys[1], wf.sort_vals[0], wf.sort_vals[1], wf.sort_temp};
     for (void* p : ptrs)
         if (p)
@@ -1213,8 +1257,8 @@ unsigned shade_grid(bool multi)
 }
 
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
-// cache queries, refill, and copy the count of the next queue to pinned memory.
-hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
+// cache queries, refill, and copy the counts the host steers by to pinned memory.
+hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
 {
     Wavefront& wf = *io.wf;
     hipStream_t st = io.st;
@@ -1267,13 +1311,13 @@ hipError_t launch_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned sh
     }
     if ((e = timer.mark(st, KERNEL_REFILL, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, wf, q, cd);
-    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, st, io.d_job, wf, q);
+    hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, io.d_job + 1, wf, q, cd);
+    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, st, io.d_job, io.d_job + 1, wf, q);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_REFILL, false)) != hipSuccess)
         return e;
-    return hipMemcpyAsync(&wf.host_counts[q], &wf.counters[CTR(1 - q)], 4, hipMemcpyDeviceToHost, st);
+    return hipMemcpyAsync(wf.host_counts + 8 * q, wf.snapshot + 8 * q, 8 * 4, hipMemcpyDeviceToHost, st);
 }
 
 // Launch configuration an iteration graph depends on.
@@ -1304,7 +1348,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
 }
 
 // Runs one iteration through its captured graph (captured on first use).
-hipError_t graph_iteration(RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
+hipError_t graph_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
 {
     IterGraphs& g = *io.graphs;
     const uint64_t sig = graph_signature(io, cd);
@@ -1405,9 +1449,10 @@ hipError_t wavefront_sort_buffers(Wavefront& wf)
     return hipMalloc(&wf.sort_temp, std::max<size_t>(bytes, 16));
 }
 
-hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
+hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, bool drain)
 {
     Wavefront& wf = *io.wf;
+    StreamState& S = *io.stream;
     hipStream_t st = io.st;
     hipError_t e;
     if (io.sort_hits && (e = wavefront_sort_buffers(wf)) != hipSuccess)
@@ -1418,87 +1463,140 @@ hipError_t render_views(RenderIO& io, const std::vector<ViewWork>& work)
     for (const ViewWork& w : work)
         if (w.npix == 0 || w.npix > wf.rad_out_capacity)
             return hipErrorInvalidValue;
-    const auto jobs = plan_jobs(work, wf.rad_out_capacity, train_between ? cs->frame_spp : 0);
+    auto jobs = plan_jobs(work, wf.rad_out_capacity, train_between ? cs->frame_spp : 0);
+    // The radiance cache records, trains and counts per job: its jobs run one at a time.
+    const int slots = cs ? 1 : 2;
+    if (cs)
+        drain = true;
     // grid-stride stages: the blocks that fill the chip in one round, not one per pool slot
     const unsigned shade_blocks = blocks_for(wf.capacity, 256, shade_grid(io.d_scenes != nullptr));
-    for (const JobPlan& plan : jobs) {
-        const auto& segs = plan.segs;
-        JobDev& jh = *io.h_job;
-        jh.nseg = uint32_t(segs.size());
-        for (size_t k = 0; k < segs.size(); k++)
-            jh.seg[k] = segs[k];
+    CacheDev cd{};
+    if (cs) {
+        cd.records = cs->records;
+        cd.counters = cs->counters;
+        cd.record_capacity = cs->record_capacity;
+        cd.queries = cs->queries;
+        cd.query_slot = cs->query_slot;
+        cd.train_stride = std::max<uint32_t>(1, cs->train_stride);
+        cd.collect = 1;
+        cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
+        cd.cont_prob = cs->cont_prob;
+    }
+    // (the library radix sort is launched directly: it may record memsets)
+    const bool graphs = io.graphs && !io.timer->enabled && !train_inflight && !io.sort_hits;
+    size_t next_job = 0;
+    auto load = [&](int s) -> hipError_t {
+        const JobPlan& plan = jobs[next_job++];
+        JobDev& jh = io.h_job[s];
+        jh.nseg = uint32_t(plan.segs.size());
+        for (size_t k = 0; k < plan.segs.size(); k++)
+            jh.seg[k] = plan.segs[k];
         jh.paths = uint32_t(plan.paths);
         jh.pix_total = uint32_t(plan.pix);
-        CacheDev cd{};
-        if (cs) {
-            cd.records = cs->records;
-            cd.counters = cs->counters;
-            cd.record_capacity = cs->record_capacity;
-            cd.queries = cs->queries;
-            cd.query_slot = cs->query_slot;
-            cd.train_stride = std::max<uint32_t>(1, cs->train_stride);
-            cd.collect = 1;
-            cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
-            cd.cont_prob = cs->cont_prob;
-        }
+        jh.seq = ++S.seq;
         jh.signed_radiance = (cd.query && cd.cont_prob > 0.0f) ? 1u : 0u;
-        const size_t job_bytes = offsetof(JobDev, seg) + segs.size() * sizeof(SegDev);
-        if ((e = hipMemcpyAsync(io.d_job, io.h_job, job_bytes, hipMemcpyHostToDevice, st)) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(k_start_job, dim3(blocks_for(std::min<uint32_t>(jh.paths, wf.capacity), 256, 65535u * 4)),
-                           dim3(256), 0, st, io.d_views, io.d_job, wf, cd);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        // (the library radix sort is launched directly: it may record memsets)
-        const bool graphs = io.graphs && !io.timer->enabled && !train_inflight && !io.sort_hits;
-        int q = 0;
-        // Iterate until the queue an iteration leaves behind is empty. The count of iteration
-        // i is read back (pinned memory, event) while iteration i+1 is already queued.
-        for (uint32_t it = 0; it < 1000000; it++) {
-            e = graphs ? graph_iteration(io, cd, q, shade_blocks) : launch_iteration(io, cd, q, shade_blocks);
-            if (e != hipSuccess)
-                return e;
-            if (train_inflight) {
-                if ((e = io.timer->mark(st, KERNEL_CACHE, true)) != hipSuccess)
-                    return e;
-                if ((e = cache_train_inflight(*cs, st)) != hipSuccess)
-                    return e;
-                if ((e = io.timer->mark(st, KERNEL_CACHE, false)) != hipSuccess)
-                    return e;
-            }
-            if ((e = hipEventRecord(wf.count_ready[q], st)) != hipSuccess)
-                return e;
-            q = 1 - q;
-            if (it >= 1) {
-                // q now names the parity of iteration it - 1
-                if ((e = hipEventSynchronize(wf.count_ready[q])) != hipSuccess)
-                    return e;
-                if (wf.host_counts[q] == 0)
-                    break;  // iteration it-1 left nothing to trace; iteration it was empty
-            }
+        const size_t bytes = offsetof(JobDev, seg) + plan.segs.size() * sizeof(SegDev);
+        hipError_t le = hipMemcpyAsync(io.d_job + s, &jh, bytes, hipMemcpyHostToDevice, st);
+        if (le != hipSuccess)
+            return le;
+        hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(64), 0, st, wf, uint32_t(s));
+        if (cs && cd.collect) {  // a job's training records start afresh
+            const uint32_t zero[2] = {0, 0};
+            if ((le = hipMemcpyAsync(cd.counters, zero, 8, hipMemcpyHostToDevice, st)) != hipSuccess)
+                return le;
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(jh.pix_total, 256, 65535u * 4)), dim3(256), 0, st, io.d_views,
-                           io.d_job, wf.rad_out);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
+        S.slot[s] = StreamState::Slot{true, jh.paths, jh.seq, S.launched, 0, 0};
+        return hipGetLastError();
+    };
+    // The job in slot s has ended: accumulate it, free the slot (its paths are read before a
+    // later job's paths can be written there: stream order), train between frames.
+    auto finish = [&](int s) -> hipError_t {
+        const uint32_t pix_total = io.h_job[s].pix_total;
+        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(pix_total, 256, 65535u * 4)), dim3(256), 0, st, io.d_views,
+                           io.d_job + s, wf.rad_out + size_t(s) * wf.rad_out_capacity);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess)
+            return le;
+        hipLaunchKernelGGL(k_slot_free, dim3(1), dim3(64), 0, st, io.d_job + s, wf, uint32_t(s));
+        if ((le = hipGetLastError()) != hipSuccess)
+            return le;
+        S.slot[s].busy = false;
         if (train_between) {
             // online training on the path suffixes this job recorded
             uint32_t n_rec = 0;
-            if ((e = hipMemcpyAsync(&n_rec, &cs->counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
-                return e;
-            if ((e = hipStreamSynchronize(st)) != hipSuccess)
-                return e;
+            if ((le = hipMemcpyAsync(&n_rec, &cs->counters[0], 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+                return le;
+            if ((le = hipStreamSynchronize(st)) != hipSuccess)
+                return le;
             n_rec = std::min(n_rec, cs->record_capacity);
             cs->records_total += n_rec;
             cs->last_records = n_rec;
-            if ((e = cache_train_records(*cs, n_rec, st)) != hipSuccess)
+            if ((le = cache_train_records(*cs, n_rec, st)) != hipSuccess)
+                return le;
+        }
+        return hipSuccess;
+    };
+    while (true) {
+        // jobs into free slots (one slot with the cache)
+        for (int s = 0; s < slots && next_job < jobs.size(); s++)
+            if (!S.slot[s].busy)
+                if ((e = load(s)) != hipSuccess)
+                    return e;
+        if (S.idle())
+            break;  // nothing in flight and nothing left to do
+        if (!drain && next_job == jobs.size()) {
+            // every job of this call is loaded; return once all its paths have started
+            bool started = true;
+            for (int s = 0; s < 2; s++)
+                started &= !S.slot[s].busy || S.slot[s].issued == S.slot[s].paths;
+            if (started)
+                break;
+        }
+        // One iteration. With an empty pool it only refills, so a stream starts by itself.
+        e = graphs ? graph_iteration(io, cd, S.q, shade_blocks) : launch_iteration(io, cd, S.q, shade_blocks);
+        if (e != hipSuccess)
+            return e;
+        if (train_inflight) {
+            if ((e = io.timer->mark(st, KERNEL_CACHE, true)) != hipSuccess)
+                return e;
+            if ((e = cache_train_inflight(*cs, st)) != hipSuccess)
+                return e;
+            if ((e = io.timer->mark(st, KERNEL_CACHE, false)) != hipSuccess)
+                return e;
+        }
+        if ((e = hipEventRecord(wf.count_ready[S.q], st)) != hipSuccess)
+            return e;
+        S.q = 1 - S.q;
+        if (++S.launched < 2)
+            continue;
+        // the counts of the iteration before the one just queued (q now names its parity)
+        if ((e = hipEventSynchronize(wf.count_ready[S.q])) != hipSuccess)
+            return e;
+        const uint64_t counted = S.launched - 2;
+        const uint32_t* snap = wf.host_counts + 8 * S.q;
+        for (int s = 0; s < 2; s++) {
+            // a snapshot taken before a job was loaded still holds its slot's previous job
+            if (!S.slot[s].busy || counted < S.slot[s].first)
+                continue;
+            S.slot[s].ended = snap[1 + s];
+            S.slot[s].issued = snap[3 + s];
+        }
+        // finished jobs in the order they were loaded
+        for (int round = 0; round < 2; round++) {
+            int oldest = -1;
+            for (int s = 0; s < 2; s++)
+                if (S.slot[s].busy && (oldest < 0 || int(S.slot[s].seq - S.slot[oldest].seq) < 0))
+                    oldest = s;
+            if (oldest < 0 || S.slot[oldest].ended != S.slot[oldest].paths)
+                break;
+            if ((e = finish(oldest)) != hipSuccess)
                 return e;
         }
     }
     return hipSuccess;
 }
 
-hipError_t render_samples(Ctx& ctx, uint32_t spp)
+hipError_t render_samples(Ctx& ctx, uint32_t spp, bool drain)
 {
     hipError_t e;
     ViewDev& v = *ctx.h_view;
@@ -1513,9 +1611,22 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp)
         return e;
     RenderIO io{ctx.stream,       &ctx.wf,     ctx.grid_blocks, ctx.count_fetches,
                 &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
-                ctx.d_job,        ctx.h_job,   ctx.use_graphs ? &ctx.graphs : nullptr,
+                ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
                 &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
-    return render_views(io, {ViewWork{0, v.npix, ctx.sample_index, spp}});
+    return render_views(io, spp ? std::vector<ViewWork>{ViewWork{0, v.npix, ctx.sample_index, spp}}
+                                : std::vector<ViewWork>{},
+                        drain);
+}
+
+hipError_t drain_stream(Ctx& ctx)
+{
+    if (ctx.stream_state.idle())
+        return hipSuccess;
+    RenderIO io{ctx.stream,       &ctx.wf,     ctx.grid_blocks, ctx.count_fetches,
+                &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
+                ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
+                &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
+    return render_views(io, {}, true);
 }
 
 hipError_t primary_hits(Ctx& ctx, uint32_t sample_index, pt_hit* d_hits)
