@@ -76,6 +76,12 @@ struct RayPre {
 
 __device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+// Hardware reciprocal (v_rcp_f32, within 1 ulp) instead of IEEE division (about ten
+// instructions each). A ray's constants below only have to be the same for every box and
+// triangle it meets (the watertight test shears all vertices with the same sx, sy, sz), not
+// correctly rounded; the slab test's conservative factor covers the extra ulp of 1/d.
+__device__ __forceinline__ float rcp1(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
 {
     r.o = o;
@@ -83,7 +89,7 @@ __device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
     const float ex = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
     const float ey = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
     const float ez = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
-    r.inv = f3(1.0f / ex, 1.0f / ey, 1.0f / ez);
+    r.inv = f3(rcp1(ex), rcp1(ey), rcp1(ez));
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = 0;
     if (ay > ax)
@@ -100,10 +106,9 @@ __device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
     r.kx = kx;
     r.ky = ky;
     r.kz = kz;
-    const float dz = comp(d, kz);
-    r.sx = comp(d, kx) / dz;
-    r.sy = comp(d, ky) / dz;
-    r.sz = 1.0f / dz;
+    r.sz = rcp1(comp(d, kz));
+    r.sx = comp(d, kx) * r.sz;
+    r.sy = comp(d, ky) * r.sz;
 }
 
 // Watertight test; on a hit in [tmin, tmax] returns t and the barycentric weights of v1, v2.
@@ -129,7 +134,7 @@ __device__ __forceinline__ bool intersect_tri(const RayPre& r, float3 v0, float3
     if (det == 0.0f)
         return false;
     const float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
-    const float rcp = 1.0f / det;
+    const float rcp = rcp1(det);
     t = T * rcp;
     if (!(t >= tmin && t < tmax))
         return false;
@@ -197,7 +202,9 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
     float opacity = 1.0f;
     uint32_t node = s.tlas_root;
     stk.sp = 0;
-    const float cons = 1.0000004f;  // conservative slab far distance
+    // Conservative slab far distance: each slab distance fl(fl(p - o) * inv), inv within 1 ulp,
+    // is within 4u of exact (u = 2^-24), so a near/far pair within 8u; 1 + 2^-20 = 1 + 16u.
+    const float cons = 1.00000095367431640625f;
 
     while (true) {
         // ---- visit an internal node
