@@ -1,6 +1,8 @@
 // C ABI of the device back end (include/pthip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -546,6 +548,7 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
                      "textures");
              HIP_TRY(&c, upload(c.d_texels, scene->texels, scene->texel_bytes, st), "textures");
              HIP_TRY(&c, upload(c.d_inst_opacity, opacity.data(), opacity.size() * 4, st), "opacity");
+             c.transparent = std::any_of(opacity.begin(), opacity.end(), [](float o) { return o != 1.0f; });
              c.instance_count = scene->instance_count;
 
              c.meshes.assign(scene->meshes, scene->meshes + scene->mesh_count);
@@ -1101,10 +1104,12 @@ namespace {
 
 RenderIO batch_io(pt_batch& b)
 {
-    return RenderIO{b.stream,   &b.wf,   b.grid_blocks, b.count_fetches,
-                    &b.timer,   b.scenes[0]->c.scene,   b.d_scenes,     b.d_views,
-                    b.d_job,    b.h_job, &b.stream_state,
-                    b.use_graphs ? &b.graphs : nullptr, nullptr, b.lds_stack, b.sort_hits};
+    RenderIO io{b.stream,   &b.wf,   b.grid_blocks, b.count_fetches,
+                &b.timer,   b.scenes[0]->c.scene,   b.d_scenes,     b.d_views,
+                b.d_job,    b.h_job, &b.stream_state,
+                b.use_graphs ? &b.graphs : nullptr, nullptr, b.lds_stack, b.sort_hits};
+    io.alpha_shadows = std::any_of(b.scenes.begin(), b.scenes.end(), [](const pt_ctx* c) { return c->c.transparent; });
+    return io;
 }
 
 // As settle() for a context: finishes the paths a batch render left in flight.

@@ -252,6 +252,7 @@ struct Ctx {
     uint32_t blocks_per_cu = 6;
     uint32_t pool_slots = 0;
     uint32_t lds_stack = LDS_STACK;
+    bool transparent = false;  // some instance's material is transparent
     uint32_t leaf_size = 4;
     uint32_t node_width = 4;
     uint32_t bvh_builder = 0;  // BVH_BUILDER_*
@@ -288,6 +289,7 @@ struct RenderIO {
     CacheState* cache;              // nullptr: no radiance cache
     uint32_t lds_stack;             // LDS stack entries per lane of the traversal kernels
     bool sort_hits;                 // shade the extension hits in (scene, material) order
+    bool alpha_shadows = true;      // some instance is transparent: shadow rays weigh alpha
 };
 
 // Samples [sample_index, sample_index + spp) of one view.

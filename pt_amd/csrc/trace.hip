@@ -185,8 +185,9 @@ struct HitOut {
 };
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
-// transmittance through transparent instances is returned in trans (shadowray.ah.hlsl).
-template <bool ANY, bool COUNT = false, typename Stk>
+// transmittance through transparent instances is returned in trans (shadowray.ah.hlsl);
+// ALPHA = false compiles that out for scenes without transparent instances (every hit occludes).
+template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true>
 __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
                          float& trans, uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr)
 {
@@ -232,22 +233,23 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
             if (!h)
                 cc[k] = NONE;
         }
-        // sort the (up to) four hits near to far
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3 - a; b++)
-                if (tn[b + 1] < tn[b]) {
-                    const float tt = tn[b];
-                    tn[b] = tn[b + 1];
-                    tn[b + 1] = tt;
-                    const uint32_t cu = cc[b];
-                    cc[b] = cc[b + 1];
-                    cc[b + 1] = cu;
-                }
-        uint32_t next = NONE;
-        uint32_t deferred[3];
-        int nd = 0;
+        // sort the (up to) four hits near to far (optimal 5-comparator network)
+        auto cmpx = [&](int a, int b) {
+            if (tn[b] < tn[a]) {
+                const float tt = tn[a];
+                tn[a] = tn[b];
+                tn[b] = tt;
+                const uint32_t cu = cc[a];
+                cc[a] = cc[b];
+                cc[b] = cu;
+            }
+        };
+        cmpx(0, 1);
+        cmpx(2, 3);
+        cmpx(0, 2);
+        cmpx(1, 3);
+        cmpx(1, 2);
+        // triangle leaves first, near to far (they shorten `best` for the boxes after them)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint32_t c = cc[k];
@@ -265,7 +267,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                     if (!intersect_tri(r, xyz(tr.v0), xyz(tr.v1), xyz(tr.v2), tmin, best, t, b1, b2))
                         continue;
                     if (ANY) {
-                        if (opacity < 1.0f) {
+                        if (ALPHA && opacity < 1.0f) {
                             float alpha = opacity;
                             if (alpha < 0.0f) {
                                 // base-colour map alpha at the hit (shadowray.ah.hlsl:21-33)
@@ -292,14 +294,20 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                     best = t;
                     hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
                 }
-            } else if (next == NONE) {
-                next = c;
-            } else {
-                deferred[nd++] = c;
             }
         }
-        for (int k = nd - 1; k >= 0; k--)
-            stk.push(deferred[k]);
+        // then the nodes (or TLAS leaves) still in reach, far to near: every one but the
+        // nearest goes on the stack, so they are popped near to far
+        uint32_t next = NONE;
+#pragma unroll
+        for (int k = 3; k >= 0; k--) {
+            const uint32_t c = cc[k];
+            if (c == NONE || tn[k] > best || ((c & CHILD_LEAF) && inst != NONE))
+                continue;
+            if (next != NONE)
+                stk.push(next);
+            next = c;
+        }
 
         // ---- choose the next internal node (entering instances as needed)
         while (true) {
@@ -329,7 +337,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 precompute(r, oo, od);
                 inst = id;
                 blas_sp = stk.sp;
-                if (ANY)
+                if (ANY && ALPHA)
                     opacity = s.inst_opacity[id];
                 // Roots are single nodes; continue with the BLAS root as an internal node.
                 next = root;
@@ -466,7 +474,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         atomicAdd(&wf.totals[0], (unsigned long long)count);
 }
 
-template <bool COUNT, bool MULTI, uint32_t D>
+template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
@@ -481,7 +489,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         const DeviceScene& s = MULTI ? scenes[wf.s_scene[i]] : s0;
         HitOut h;
         float trans;
-        const bool occluded = traverse<true, COUNT>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
+        const bool occluded =
+            traverse<true, COUNT, StackT<D>, ALPHA>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
         if (!occluded && trans >= 1e-5f) {
             const uint32_t tgt = wf.s_target[i];
             float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
@@ -1192,11 +1201,16 @@ ExtendFn pick_extend_d(bool count, bool multi)
     return count ? (multi ? k_extend<true, true, D> : k_extend<true, false, D>)
                  : (multi ? k_extend<false, true, D> : k_extend<false, false, D>);
 }
-template <uint32_t D>
-ShadowFn pick_shadow_d(bool count, bool multi)
+template <uint32_t D, bool ALPHA>
+ShadowFn pick_shadow_da(bool count, bool multi)
 {
-    return count ? (multi ? k_shadow<true, true, D> : k_shadow<true, false, D>)
-                 : (multi ? k_shadow<false, true, D> : k_shadow<false, false, D>);
+    return count ? (multi ? k_shadow<true, true, D, ALPHA> : k_shadow<true, false, D, ALPHA>)
+                 : (multi ? k_shadow<false, true, D, ALPHA> : k_shadow<false, false, D, ALPHA>);
+}
+template <uint32_t D>
+ShadowFn pick_shadow_d(bool count, bool multi, bool alpha)
+{
+    return alpha ? pick_shadow_da<D, true>(count, multi) : pick_shadow_da<D, false>(count, multi);
 }
 // Traversal kernel variants by LDS stack depth (8, 16 or 32 entries per lane).
 ExtendFn pick_extend(bool count, bool multi, uint32_t depth)
@@ -1204,10 +1218,10 @@ ExtendFn pick_extend(bool count, bool multi, uint32_t depth)
     return depth == 8 ? pick_extend_d<8>(count, multi)
                       : (depth == 32 ? pick_extend_d<32>(count, multi) : pick_extend_d<16>(count, multi));
 }
-ShadowFn pick_shadow(bool count, bool multi, uint32_t depth)
+ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha)
 {
-    return depth == 8 ? pick_shadow_d<8>(count, multi)
-                      : (depth == 32 ? pick_shadow_d<32>(count, multi) : pick_shadow_d<16>(count, multi));
+    return depth == 8 ? pick_shadow_d<8>(count, multi, alpha)
+                      : (depth == 32 ? pick_shadow_d<32>(count, multi, alpha) : pick_shadow_d<16>(count, multi, alpha));
 }
 
 using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
@@ -1304,7 +1318,8 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
         return e;
     if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
+    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene,
+                       tab, wf, q);
     if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
         return e;
     if (cd.query) {
@@ -1334,6 +1349,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
     mix(io.count_fetches);
     mix(io.lds_stack);
+    mix(io.alpha_shadows);
     mix(io.sort_hits);
     mix(reinterpret_cast<uintptr_t>(io.scene.nodes));
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
@@ -1620,6 +1636,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp, bool drain)
                 &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
                 ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
                 &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
+    io.alpha_shadows = ctx.transparent;
     return render_views(io, spp ? std::vector<ViewWork>{ViewWork{0, v.npix, ctx.sample_index, spp}}
                                 : std::vector<ViewWork>{},
                         drain);
@@ -1633,6 +1650,7 @@ hipError_t drain_stream(Ctx& ctx)
                 &ctx.timer,       ctx.scene,   nullptr,         ctx.d_views,
                 ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
                 &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
+    io.alpha_shadows = ctx.transparent;
     return render_views(io, {}, true);
 }
 
