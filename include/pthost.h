@@ -94,8 +94,9 @@ int pth_alias_table(const float* weights, uint32_t n, const uint32_t* values, ui
                     uint32_t* out_words);
 
 /* Texture image decoding as TextureData does it (src/scene/texturedata.cpp:30-46 with
- * stb_image 2.30): PNG only; 1, 2 or 4 channels of 8 bits (three widened to four with alpha
- * 255). *px is malloc'd (width*height*channels bytes), release with pth_free. */
+ * stb_image 2.30): PNG or JPEG (baseline, progressive); 1, 2 or 4 channels of 8 bits (three
+ * widened to four with alpha 255; colour JPEGs are RGBA, gray ones one channel). *px is
+ * malloc'd (width*height*channels bytes), release with pth_free. */
 int pth_decode_image(const uint8_t* data, size_t size, uint32_t* width, uint32_t* height, uint32_t* channels,
                      uint8_t** px);
 

@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "lib")
 BIN = os.path.join(PKG, "bin")
 INC = os.path.join(ROOT, "include")
 
-HOST_SRCS = ["mathx.cpp", "json.cpp", "scene.cpp", "hdrio.cpp", "png.cpp", "api.cpp"]
+HOST_SRCS = ["mathx.cpp", "json.cpp", "scene.cpp", "hdrio.cpp", "png.cpp", "jpeg.cpp", "api.cpp"]
 HIP_SRCS = ["bvh.hip", "trace.hip", "cache.hip", "lights.hip", "post.hip", "api.hip"]
 HIP_HDRS = ["common.h", "device.h", "bvh.h", "shading.h", "cache.h"]
 

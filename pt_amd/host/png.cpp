@@ -7,6 +7,8 @@
 // (77, 150, 29) / 256.
 #include "png.h"
 
+#include "jpeg.h"
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -477,9 +479,11 @@ Image png_decode(const uint8_t* data, size_t size, uint32_t want)
 Image load_texture_image(const uint8_t* data, size_t size)
 {
     uint32_t w, h, c;
-    if (!png_info(data, size, w, h, c))
-        bad("unsupported image (only PNG textures are decoded)");
-    return png_decode(data, size, c == 3 ? 4 : c);
+    if (png_info(data, size, w, h, c))
+        return png_decode(data, size, c == 3 ? 4 : c);
+    if (jpeg_info(data, size, w, h, c))
+        return jpeg_decode(data, size);
+    throw std::runtime_error("texture: unsupported image (PNG and JPEG are decoded)");
 }
 
 // --- encoding (the PNG path of write_png, src/io/imagewrite.cpp:44-67) -----------------------

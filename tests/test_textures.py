@@ -136,3 +136,25 @@ def test_textured_emitter_intensity(tex_dir):
     inv = 1.0 / (np.float64(np.float32(w_e)) + np.float64(np.float32(w_p)))
     assert pvals[0] == np.float32(np.float64(np.float32(w_e)) * inv)
     assert pvals[1] == np.float32(np.float64(np.float32(w_p)) * inv)
+
+
+def test_jpeg_texture_loads_like_png(tex_dir):
+    """A glTF image may be a JPEG (the reference's stb_image reads both): the wall's base-colour
+    map replaced by a JPEG file becomes a texture of the JPEG's decoded pixels."""
+    import json
+    import shutil
+    jpg = os.path.join(os.path.dirname(__file__), "golden", "jpeg", "rgb420_q75.jpg")
+    shutil.copy(jpg, os.path.join(tex_dir, "wall.jpg"))
+    doc = json.load(open(os.path.join(tex_dir, "textured.gltf")))
+    src = doc["textures"][doc["materials"][1]["pbrMetallicRoughness"]["baseColorTexture"]["index"]]["source"]
+    doc["images"][src] = {"uri": "wall.jpg", "mimeType": "image/jpeg"}
+    json.dump(doc, open(os.path.join(tex_dir, "jpeg.gltf"), "w"))
+    with open(os.path.join(tex_dir, "jpeg.scn"), "w") as fh:
+        fh.write("camera:\nobjects:\n j jpeg.gltf\n\n j\n")
+    s = pt_amd.Scene(os.path.join(tex_dir, "jpeg.scn"))
+    f, u = _materials(s)
+    descs, texels = _textures(s)
+    off, w, h, c, srgb = descs[u[1, 0]]
+    dw, dh, dc, px = decode(open(jpg, "rb").read())
+    assert (w, h, c, srgb) == (dw, dh, dc, 1) == (33, 21, 4, 1)
+    assert texels[off:off + w * h * c] == px
