@@ -166,3 +166,22 @@ def test_tonemapped_batch_view_equals_single_render(box_scn):
     np.testing.assert_array_equal(batch.read_tonemapped(0), single)
     batch.close()
     tr.close()
+
+
+def test_jobs_streamed_through_two_slots_equal_separate_renders(box_scn):
+    """A 16x16 image takes 1024 samples per job, so 2500 samples stream as three jobs through
+    the two job slots (the next job's paths enter while the previous one drains, one slot
+    frees while the other still runs); the running mean must equal three separate calls."""
+    scene = pt_amd.Scene(box_scn)
+    cam = scene.camera(16, 16)
+    a = pt_amd.Tracer(scene, 16, 16, max_bounces=4)
+    a.set_camera(cam)
+    a.render(2500)
+    b = pt_amd.Tracer(scene, 16, 16, max_bounces=4)
+    b.set_camera(cam)
+    for spp in (1000, 1000, 500):
+        b.render(spp)
+    np.testing.assert_array_equal(a.read(), b.read())
+    assert a.stats()["paths"] == b.stats()["paths"] == 2500 * 256
+    a.close()
+    b.close()
