@@ -169,6 +169,10 @@ int validate_blob(const pt_scene_blob* b, std::string& why)
 int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& why)
 {
     const BvhBuffers& bv = c.bvh;
+    // traversal addresses nodes by 32-bit byte offsets
+    if (uint64_t(bv.node_count) * sizeof(Node4) > 0xFFFFFFFFull)
+        return (why = "scene too large: " + std::to_string(bv.node_count) + " BVH nodes (at most 33554431)",
+                PT_EINVAL);
     std::vector<Node4> nodes(bv.node_count);
     if (bv.node_count &&
         hipMemcpy(nodes.data(), bv.nodes, size_t(bv.node_count) * sizeof(Node4), hipMemcpyDeviceToHost) != hipSuccess)

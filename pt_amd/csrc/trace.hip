@@ -211,12 +211,16 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
         // ---- visit an internal node
         if (COUNT)
             ++*n_nodes;
-        const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
-        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-        const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
-        const float4 nx = r.inv.x >= 0.0f ? lox : hix, fx = r.inv.x >= 0.0f ? hix : lox;
-        const float4 ny = r.inv.y >= 0.0f ? loy : hiy, fy = r.inv.y >= 0.0f ? hiy : loy;
-        const float4 nz = r.inv.z >= 0.0f ? loz : hiz, fz = r.inv.z >= 0.0f ? hiz : loz;
+        // The near and far plane rows of each axis are picked by the load address (the ray's
+        // direction sign: row hi first when the ray runs towards -axis), from a 32-bit offset off
+        // the scene's node array.
+        const char* nb = reinterpret_cast<const char*>(s.nodes);
+        const uint32_t nbase = node * uint32_t(sizeof(Node4));
+        const uint32_t ox = (float_as_u32(r.inv.x) >> 31) << 4, oy = 32u + ((float_as_u32(r.inv.y) >> 31) << 4),
+                       oz = 64u + ((float_as_u32(r.inv.z) >> 31) << 4);
+        auto row = [&](uint32_t off) { return *reinterpret_cast<const float4*>(nb + (nbase + off)); };
+        const float4 nx = row(ox), fx = row(ox ^ 16u), ny = row(oy), fy = row(oy ^ 16u), nz = row(oz), fz = row(oz ^ 16u);
+        const uint4 ch = *reinterpret_cast<const uint4*>(nb + (nbase + 96u));
         float tn[4];
         uint32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
         const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
