@@ -173,6 +173,9 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     if (uint64_t(bv.node_count) * sizeof(Node4) > 0xFFFFFFFFull)
         return (why = "scene too large: " + std::to_string(bv.node_count) + " BVH nodes (at most 33554431)",
                 PT_EINVAL);
+    if (uint64_t(bv.tri_count) * sizeof(TriRec) > 0xFFFFFFFFull)
+        return (why = "scene too large: " + std::to_string(bv.tri_count) + " triangles (at most 89478485)",
+                PT_EINVAL);
     std::vector<Node4> nodes(bv.node_count);
     if (bv.node_count &&
         hipMemcpy(nodes.data(), bv.nodes, size_t(bv.node_count) * sizeof(Node4), hipMemcpyDeviceToHost) != hipSuccess)

@@ -111,6 +111,15 @@ __device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
     r.sy = comp(d, ky) * r.sz;
 }
 
+// Element i of a scene array by a 32-bit byte offset from its (uniform) base: the loads take
+// the scalar-base form, with no 64-bit address arithmetic per lane. The scene validation keeps
+// the node and triangle-record arrays under 4 GiB.
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, uint32_t i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+}
+
 // Watertight test; on a hit in [tmin, tmax] returns t and the barycentric weights of v1, v2.
 __device__ __forceinline__ bool intersect_tri(const RayPre& r, float3 v0, float3 v1, float3 v2, float tmin, float tmax,
                                               float& t, float& b1, float& b2)
@@ -266,7 +275,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 for (uint32_t q = 0; q < count; q++) {
                     if (COUNT)
                         ++*n_tris;
-                    const TriRec tr = s.tris[first + q];
+                    const TriRec tr = at32(s.tris, first + q);
                     float t, b1, b2;
                     if (!intersect_tri(r, xyz(tr.v0), xyz(tr.v1), xyz(tr.v2), tmin, best, t, b1, b2))
                         continue;
@@ -327,11 +336,11 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
             }
             if ((next & CHILD_LEAF) && inst == NONE) {  // TLAS leaf: enter the instance
                 const uint32_t id = next & ~CHILD_LEAF;
-                const uint32_t root = s.inst_blas_root[id];
+                const uint32_t root = at32(s.inst_blas_root, id);
                 next = NONE;
                 if (root == CHILD_EMPTY)
                     continue;
-                const InstXform x = s.inst_inv[id];
+                const InstXform x = at32(s.inst_inv, id);
                 const float3 oo = f3(x.r0.x * wo.x + x.r0.y * wo.y + x.r0.z * wo.z + x.r0.w,
                                      x.r1.x * wo.x + x.r1.y * wo.y + x.r1.z * wo.z + x.r1.w,
                                      x.r2.x * wo.x + x.r2.y * wo.y + x.r2.z * wo.z + x.r2.w);
