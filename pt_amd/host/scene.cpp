@@ -242,6 +242,40 @@ Gltf open_gltf(const std::string& path)
     return g;
 }
 
+// aiProcess_GenNormals for a primitive without normals (Assimp's GenFaceNormals, which the
+// reference's import requests, src/scene/scene.cpp:537-550; glTF 2.0 also prescribes flat
+// normals then): every face gets its own three vertices carrying the face normal
+// (v1 - v0) x (v2 - v0), divided by its length when that is not zero. Faces left as zero
+// triangles keep indices 0, 0, 0.
+void flat_normals(Mesh& mesh)
+{
+    std::vector<Vertex> out;
+    out.reserve(mesh.tris.size() * 3);
+    for (Tri& t : mesh.tris) {
+        if (t.i[0] == 0 && t.i[1] == 0 && t.i[2] == 0)
+            continue;
+        const Vertex& a = mesh.vertices[t.i[0]];
+        const Vertex& b = mesh.vertices[t.i[1]];
+        const Vertex& c = mesh.vertices[t.i[2]];
+        const float e1[3] = {b.pos[0] - a.pos[0], b.pos[1] - a.pos[1], b.pos[2] - a.pos[2]};
+        const float e2[3] = {c.pos[0] - a.pos[0], c.pos[1] - a.pos[1], c.pos[2] - a.pos[2]};
+        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len > 0.0f)
+            n[0] /= len, n[1] /= len, n[2] /= len;
+        const uint32_t base = uint32_t(out.size());
+        for (const Vertex* v : {&a, &b, &c}) {
+            Vertex w = *v;
+            memcpy(w.normal, n, sizeof(n));
+            out.push_back(w);
+        }
+        t = Tri{{base, base + 1, base + 2}};
+    }
+    if (out.empty())
+        out.push_back(mesh.vertices.empty() ? Vertex{} : mesh.vertices[0]);
+    mesh.vertices = std::move(out);
+}
+
 M4 node_matrix(const Json& node)
 {
     if (node.has("matrix")) {
@@ -470,19 +504,22 @@ Scene load_gltf(const std::string& path)
             if (int(prim["mode"].number(4)) != 4)
                 throw std::runtime_error("glTF: only triangle primitives are supported");
             const Json& attr = prim["attributes"];
-            if (!attr.has("POSITION") || !attr.has("NORMAL"))
-                throw std::runtime_error("glTF: primitives need POSITION and NORMAL");
+            if (!attr.has("POSITION"))
+                throw std::runtime_error("glTF: primitives need POSITION");
+            const bool has_normals = attr.has("NORMAL");
             if (!prim.has("material"))
                 throw std::runtime_error("glTF: primitives need a material");
             std::vector<float> pos = g.floats(int(attr["POSITION"].number(-1)), 3);
-            std::vector<float> nrm = g.floats(int(attr["NORMAL"].number(-1)), 3);
+            std::vector<float> nrm;
+            if (has_normals)
+                nrm = g.floats(int(attr["NORMAL"].number(-1)), 3);
             std::vector<float> uv;
             if (attr.has("TEXCOORD_0"))
                 uv = g.floats(int(attr["TEXCOORD_0"].number(-1)), 2);
             else
                 std::cerr << "Warning: mesh \"" << meshes[mi]["name"].str << "\" has no UVs." << std::endl;
             const size_t nv = pos.size() / 3;
-            if (nrm.size() / 3 != nv || (!uv.empty() && uv.size() / 2 != nv))
+            if ((has_normals && nrm.size() / 3 != nv) || (!uv.empty() && uv.size() / 2 != nv))
                 throw std::runtime_error("glTF: attribute counts differ");
             Mesh mesh;
             mesh.vertices.resize(nv);
@@ -490,7 +527,7 @@ Scene load_gltf(const std::string& path)
                 Vertex& vx = mesh.vertices[v];
                 for (int k = 0; k < 3; k++) {
                     vx.pos[k] = pos[v * 3 + k];
-                    vx.normal[k] = nrm[v * 3 + k];
+                    vx.normal[k] = has_normals ? nrm[v * 3 + k] : 0.0f;
                 }
                 // Assimp's glTF importer flips v and aiProcess_FlipUVs flips it back.
                 vx.uv[0] = uv.empty() ? 0.0f : uv[v * 2 + 0];
@@ -521,6 +558,8 @@ Scene load_gltf(const std::string& path)
                 else
                     mesh.tris[t] = Tri{{a, b, c}};
             }
+            if (!has_normals)
+                flat_normals(mesh);
             const uint32_t mat = uint32_t(prim["material"].number(-1));
             if (mat >= scene.materials.size())
                 throw std::runtime_error("glTF: bad material index");

@@ -221,3 +221,36 @@ def test_pt_create_argument_checks():
     assert lib.pt_create(C.byref(blob), 16, 16, C.byref(h)) == -1
     assert b"index out of range" in lib.pt_last_error(None)
     assert lib.pt_render(None, 1) == -1
+
+
+def test_gltf_without_normals_gets_flat_normals(tmp_path):
+    """A primitive without NORMAL loads with Assimp's generated face normals (the reference's
+    import requests aiProcess_GenNormals, src/scene/scene.cpp:537-550; glTF 2.0 prescribes flat
+    normals): each face gets its own vertices with normal (v1 - v0) x (v2 - v0) / |...|, and the
+    triangles' corners stay where they were."""
+    import json
+    from pt_amd.scenes import generate
+    generate.box_scene(str(tmp_path))
+    doc = json.load(open(tmp_path / "box.gltf"))
+    for m in doc["meshes"]:
+        for p in m["primitives"]:
+            p["attributes"].pop("NORMAL", None)
+    json.dump(doc, open(tmp_path / "flat.gltf", "w"))
+    (tmp_path / "flat.scn").write_text((tmp_path / "box.scn").read_text().replace("box.gltf", "flat.gltf"))
+    ref = pt_amd.Scene(str(tmp_path / "box.scn")).arrays()
+    got = pt_amd.Scene(str(tmp_path / "flat.scn")).arrays()
+    assert len(ref["meshes"]) == len(got["meshes"])
+    for (rv0, ri0, rn, _), (gv0, gi0, gn, gvn) in zip(ref["meshes"], got["meshes"]):
+        assert rn == gn and gvn == 3 * gn
+        rv, gv = ref["vertices"], got["vertices"]
+        rt = ref["indices"][ri0:ri0 + 3 * rn].reshape(-1, 3)
+        gt = got["indices"][gi0:gi0 + 3 * gn].reshape(-1, 3)
+        np.testing.assert_array_equal(gv[gv0 + gt, :3], rv[rv0 + rt, :3])  # same corners
+        p = gv[gv0 + gt, :3]
+        n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]).astype(np.float32)
+        n /= np.linalg.norm(n, axis=1, keepdims=True).astype(np.float32)
+        for k in range(3):
+            np.testing.assert_allclose(gv[gv0 + gt[:, k], 3:6], n, atol=1e-6)
+        if gn <= 2:  # the box's quads: their given normals are the face normals (up to side)
+            np.testing.assert_allclose(np.abs((gv[gv0 + gt[:, 0], 3:6] * rv[rv0 + rt[:, 0], 3:6]).sum(1)), 1,
+                                       atol=1e-5)
