@@ -117,35 +117,79 @@ struct Gltf {
     std::vector<std::vector<uint8_t>> buffers;
     std::string dir;
 
-    // Reads accessor `index` as `comps` floats per element (float components only).
+    // Reads accessor `index` as `comps` floats per element: float components as they are,
+    // integer ones as numbers or, when `normalized`, mapped to [0, 1] / [-1, 1] (glTF 2.0
+    // 3.11, and KHR_mesh_quantization's integer positions and normals), with sparse
+    // substitutions applied.
     std::vector<float> floats(int index, int comps) const
     {
         const Json& acc = doc["accessors"][size_t(index)];
         if (acc.kind != Json::OBJ)
             throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
-        if (acc.has("sparse"))
-            throw std::runtime_error("glTF: sparse accessors are not supported");
-        if (int(acc["componentType"].number(0)) != 5126)
-            throw std::runtime_error("glTF: expected a float accessor");
+        const int ct = int(acc["componentType"].number(0));
+        const bool normalized = acc["normalized"].kind == Json::BOOL && acc["normalized"].b;
+        const size_t csize = ct == 5126 || ct == 5125 ? 4 : (ct == 5122 || ct == 5123 ? 2 : (ct == 5120 || ct == 5121 ? 1 : 0));
+        if (!csize)
+            throw std::runtime_error("glTF: unsupported accessor component type");
+        auto component = [&](const uint8_t* p) -> float {
+            switch (ct) {
+            case 5126: {
+                float v;
+                memcpy(&v, p, 4);
+                return v;
+            }
+            case 5125: {
+                uint32_t v;
+                memcpy(&v, p, 4);
+                return float(v);
+            }
+            case 5123: {
+                uint16_t v;
+                memcpy(&v, p, 2);
+                return normalized ? float(v) / 65535.0f : float(v);
+            }
+            case 5122: {
+                int16_t v;
+                memcpy(&v, p, 2);
+                return normalized ? std::max(float(v) / 32767.0f, -1.0f) : float(v);
+            }
+            case 5121:
+                return normalized ? float(*p) / 255.0f : float(*p);
+            default:
+                return normalized ? std::max(float(int8_t(*p)) / 127.0f, -1.0f) : float(int8_t(*p));
+            }
+        };
         const size_t count = size_t(acc["count"].number(0));
         std::vector<float> out(count * comps, 0.0f);
-        if (!acc.has("bufferView"))
-            return out;
-        const uint8_t* base;
-        size_t stride;
-        view(acc, comps * 4, base, stride, count);
-        for (size_t i = 0; i < count; i++)
-            memcpy(&out[i * comps], base + i * stride, size_t(comps) * 4);
+        if (acc.has("bufferView")) {
+            const uint8_t* base;
+            size_t stride;
+            view(acc, comps * csize, base, stride, count);
+            for (size_t i = 0; i < count; i++)
+                for (int k = 0; k < comps; k++)
+                    out[i * comps + k] = component(base + i * stride + k * csize);
+        }
+        if (acc.has("sparse")) {  // elements given by index replace the base values
+            const Json& sp = acc["sparse"];
+            const size_t n = size_t(sp["count"].number(0));
+            const std::vector<uint32_t> at = uints_from(sp["indices"], n);
+            const uint8_t* vals;
+            size_t vstride;
+            view(sp["values"], comps * csize, vals, vstride, n);
+            for (size_t j = 0; j < n; j++) {
+                if (at[j] >= count)
+                    throw std::runtime_error("glTF: sparse index out of range");
+                for (int k = 0; k < comps; k++)
+                    out[at[j] * comps + k] = component(vals + j * (comps * csize) + k * csize);
+            }
+        }
         return out;
     }
 
-    std::vector<uint32_t> uints(int index) const
+    // n unsigned integers (componentType 5121, 5123 or 5125) from an accessor-like object.
+    std::vector<uint32_t> uints_from(const Json& acc, size_t count) const
     {
-        const Json& acc = doc["accessors"][size_t(index)];
-        if (acc.kind != Json::OBJ)
-            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
         const int ct = int(acc["componentType"].number(0));
-        const size_t count = size_t(acc["count"].number(0));
         const int csize = ct == 5125 ? 4 : ct == 5123 ? 2 : ct == 5121 ? 1 : 0;
         if (!csize)
             throw std::runtime_error("glTF: unsupported index component type");
@@ -154,7 +198,7 @@ struct Gltf {
             return out;
         const uint8_t* base;
         size_t stride;
-        view(acc, csize, base, stride, count);
+        view(acc, size_t(csize), base, stride, count);
         for (size_t i = 0; i < count; i++) {
             const uint8_t* p = base + i * stride;
             if (csize == 4) {
@@ -170,6 +214,16 @@ struct Gltf {
             }
         }
         return out;
+    }
+
+    std::vector<uint32_t> uints(int index) const
+    {
+        const Json& acc = doc["accessors"][size_t(index)];
+        if (acc.kind != Json::OBJ)
+            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
+        if (acc.has("sparse"))
+            throw std::runtime_error("glTF: sparse index accessors are not supported");
+        return uints_from(acc, size_t(acc["count"].number(0)));
     }
 
     void view(const Json& acc, size_t elem_bytes, const uint8_t*& base, size_t& stride, size_t count) const
@@ -501,8 +555,9 @@ Scene load_gltf(const std::string& path)
         const Json& prims = meshes[mi]["primitives"];
         for (size_t pi = 0; pi < prims.size(); pi++) {
             const Json& prim = prims[pi];
-            if (int(prim["mode"].number(4)) != 4)
-                throw std::runtime_error("glTF: only triangle primitives are supported");
+            const int mode = int(prim["mode"].number(4));
+            if (mode < 0 || mode > 6)
+                throw std::runtime_error("glTF: unknown primitive mode");
             const Json& attr = prim["attributes"];
             if (!attr.has("POSITION"))
                 throw std::runtime_error("glTF: primitives need POSITION");
@@ -540,6 +595,24 @@ Scene load_gltf(const std::string& path)
                 idx.resize(nv);
                 std::iota(idx.begin(), idx.end(), 0u);
             }
+            // Triangle lists as they are; strips and fans as glTF 2.0 (3.7.2.1) unrolls them;
+            // points and lines are faces of fewer than three corners, which process_meshes
+            // leaves as all-zero triangles (scene.cpp:433-437).
+            size_t zero_faces = 0;
+            if (mode == 5 || mode == 6) {
+                std::vector<uint32_t> list;
+                for (size_t i = 0; i + 2 < idx.size(); i++) {
+                    const uint32_t t[3] = {mode == 5 ? idx[i] : idx[i + 1],
+                                           mode == 5 ? idx[i + 1 + i % 2] : idx[i + 2],
+                                           mode == 5 ? idx[i + 2 - i % 2] : idx[0]};
+                    list.insert(list.end(), t, t + 3);
+                }
+                idx.swap(list);
+            } else if (mode != 4) {
+                const size_t n = idx.size();
+                zero_faces = mode == 0 ? n : mode == 1 ? n / 2 : mode == 2 ? (n >= 2 ? n : 0) : (n >= 2 ? n - 1 : 0);
+                idx.clear();
+            }
             if (idx.size() % 3 != 0)
                 throw std::runtime_error("glTF: index count is not a multiple of 3");
             mesh.tris.resize(idx.size() / 3);
@@ -558,6 +631,7 @@ Scene load_gltf(const std::string& path)
                 else
                     mesh.tris[t] = Tri{{a, b, c}};
             }
+            mesh.tris.resize(mesh.tris.size() + zero_faces, Tri{{0, 0, 0}});
             if (!has_normals)
                 flat_normals(mesh);
             const uint32_t mat = uint32_t(prim["material"].number(-1));

@@ -254,3 +254,60 @@ def test_gltf_without_normals_gets_flat_normals(tmp_path):
         if gn <= 2:  # the box's quads: their given normals are the face normals (up to side)
             np.testing.assert_allclose(np.abs((gv[gv0 + gt[:, 0], 3:6] * rv[rv0 + rt[:, 0], 3:6]).sum(1)), 1,
                                        atol=1e-5)
+
+
+def _gltf_doc(buffers_bytes, accessors, views, prims):
+    import base64
+    return {
+        "asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+        "meshes": [{"primitives": prims}],
+        "materials": [{"pbrMetallicRoughness": {"baseColorFactor": [0.5, 0.5, 0.5, 1]}}],
+        "buffers": [{"byteLength": len(buffers_bytes),
+                     "uri": "data:application/octet-stream;base64," + base64.b64encode(buffers_bytes).decode()}],
+        "bufferViews": views, "accessors": accessors,
+    }
+
+
+def test_gltf_modes_quantised_and_sparse_accessors(tmp_path):
+    """glTF primitive modes (glTF 2.0 3.7.2.1: strips and fans unrolled, points and lines left
+    as all-zero triangles as process_meshes leaves faces of fewer than three corners,
+    scene.cpp:433-437), normalised integer texture coordinates and a sparse accessor."""
+    import json
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (4, 1))
+    uv8 = np.array([[0, 0], [255, 0], [0, 255], [255, 128]], np.uint8)  # normalised UNSIGNED_BYTE
+    strip = np.array([0, 1, 2, 3], np.uint16)
+    fan = np.array([0, 1, 3, 2], np.uint16)
+    sp_idx = np.array([3], np.uint8)
+    sp_val = np.array([[2, 2, 0]], np.float32)
+    parts = [pos.tobytes(), nrm.tobytes(), uv8.tobytes(), strip.tobytes(), fan.tobytes(), sp_idx.tobytes() + b"\0\0\0",
+             sp_val.tobytes()]
+    offs = np.cumsum([0] + [len(p) for p in parts])
+    views = [{"buffer": 0, "byteOffset": int(o), "byteLength": len(p)} for o, p in zip(offs, parts)]
+    acc = [
+        {"bufferView": 0, "componentType": 5126, "count": 4, "type": "VEC3"},
+        {"bufferView": 1, "componentType": 5126, "count": 4, "type": "VEC3"},
+        {"bufferView": 2, "componentType": 5121, "normalized": True, "count": 4, "type": "VEC2"},
+        {"bufferView": 3, "componentType": 5123, "count": 4, "type": "SCALAR"},
+        {"bufferView": 4, "componentType": 5123, "count": 4, "type": "SCALAR"},
+        {"bufferView": 0, "componentType": 5126, "count": 4, "type": "VEC3",
+         "sparse": {"count": 1, "indices": {"bufferView": 5, "componentType": 5121},
+                    "values": {"bufferView": 6}}},
+    ]
+    attrs = {"POSITION": 0, "NORMAL": 1, "TEXCOORD_0": 2}
+    prims = [dict(attributes=attrs, indices=3, mode=5, material=0),
+             dict(attributes=attrs, indices=4, mode=6, material=0),
+             dict(attributes=attrs, mode=1, material=0),
+             dict(attributes=dict(attrs, POSITION=5), indices=3, mode=5, material=0)]
+    json.dump(_gltf_doc(b"".join(parts), acc, views, prims), open(tmp_path / "m.gltf", "w"))
+    (tmp_path / "m.scn").write_text("camera:\nobjects:\n m m.gltf\n\n m\n")
+    a = pt_amd.Scene(str(tmp_path / "m.scn")).arrays()
+    tris = [a["indices"][i0:i0 + 3 * n].reshape(-1, 3).tolist() for _, i0, n, _ in a["meshes"]]
+    assert tris[0] == [[0, 1, 2], [1, 3, 2]]  # strip: v_i, v_(i+1+i%2), v_(i+2-i%2)
+    assert tris[1] == [[1, 3, 0], [3, 2, 0]]  # fan: v_(i+1), v_(i+2), v_0
+    assert tris[2] == [[0, 0, 0], [0, 0, 0]]  # two lines
+    v = a["vertices"]
+    v0 = a["meshes"][0][0]
+    np.testing.assert_array_equal(v[v0:v0 + 4, 6:8], uv8 / np.float32(255))
+    v3 = a["meshes"][3][0]
+    np.testing.assert_array_equal(v[v3:v3 + 4, :3], np.vstack([pos[:3], sp_val]))
