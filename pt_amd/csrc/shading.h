@@ -33,7 +33,9 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x)
 
 // Direction numbers of Sobol dimensions 0-3 (primitive polynomials 1, x+1, x^2+x+1,
 // x^3+x+1 with initial numbers 1 | 1,3 | 1,3,1), filled in by the host at library load.
-extern __constant__ uint32_t c_sobol[4][32];
+// Sobol dimensions 2 and 3: the XOR of the direction numbers that each byte value selects, per
+// byte position of the index (c_sobol_bytes[dim - 2][byte position][byte value]).
+extern __constant__ uint32_t c_sobol_bytes[2][4][256];
 
 __device__ __forceinline__ uint32_t lk_permutation(uint32_t x, uint32_t seed)
 {
@@ -50,15 +52,26 @@ __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint32_t seed)
     return __builtin_bitreverse32(lk_permutation(__builtin_bitreverse32(x), seed));
 }
 
+// Sobol point `index` in dimension dim (0..3): the XOR of the direction numbers of the index's
+// set bits, evaluated without a loop over the bits.
 __device__ __forceinline__ uint32_t sobol(uint32_t index, uint32_t dim)
 {
-    if (dim == 0)
+    if (dim == 0)  // van der Corput
         return __builtin_bitreverse32(index);
-    uint32_t x = 0;
-#pragma unroll 8
-    for (int b = 0; b < 32; b++)
-        x ^= ((index >> b) & 1u) ? c_sobol[dim][b] : 0u;
-    return x;
+    if (dim == 1) {
+        // Polynomial x + 1: the generator matrix is Pascal's triangle mod 2, so by Lucas' theorem
+        // output bit 31 - u is the parity of the index bits at the positions b that contain u
+        // (b & u == u): a superset sum over the five bits of the position.
+        uint32_t z = index;
+        z ^= (z >> 1) & 0x55555555u;
+        z ^= (z >> 2) & 0x33333333u;
+        z ^= (z >> 4) & 0x0F0F0F0Fu;
+        z ^= (z >> 8) & 0x00FF00FFu;
+        z ^= z >> 16;
+        return __builtin_bitreverse32(z);
+    }
+    const uint32_t* t = c_sobol_bytes[dim - 2][0];
+    return t[index & 255u] ^ t[256u + ((index >> 8) & 255u)] ^ t[512u + ((index >> 16) & 255u)] ^ t[768u + (index >> 24)];
 }
 
 struct Sampler {

@@ -23,7 +23,7 @@
 
 namespace ptk {
 
-__constant__ uint32_t c_sobol[4][32];
+__constant__ uint32_t c_sobol_bytes[2][4][256];
 __constant__ float c_srgb[256];
 
 hipError_t upload_sobol_table()
@@ -50,7 +50,18 @@ hipError_t upload_sobol_table()
         for (int k = 0; k < 32; k++)
             v[d][k] = m[k] << (31 - k);
     }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol), v, sizeof(v));
+    // dimensions 2 and 3 as byte tables (the device evaluates dimension 1 in closed form)
+    static uint32_t bytes[2][4][256];
+    for (int d = 2; d < 4; d++)
+        for (int c = 0; c < 4; c++)
+            for (int x = 0; x < 256; x++) {
+                uint32_t y = 0;
+                for (int b = 0; b < 8; b++)
+                    if ((x >> b) & 1)
+                        y ^= v[d][8 * c + b];
+                bytes[d - 2][c][x] = y;
+            }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sobol_bytes), bytes, sizeof(bytes));
     if (e != hipSuccess)
         return e;
     // sRGB transfer function of 8-bit texels (the decode VK_FORMAT_R8G8B8A8_SRGB applies)
