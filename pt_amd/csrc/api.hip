@@ -481,8 +481,6 @@ void view_dev(const pt_batch& b, uint32_t i, ViewDev& d)
 
 }  // namespace
 
-extern "C" {
-
 namespace {
 
 // Renders return once all their paths have started; the last ones may still be in flight,
@@ -498,6 +496,8 @@ int settle(Ctx& c)
 }
 
 }  // namespace
+
+extern "C" {
 
 int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out)
 {
@@ -1107,6 +1107,8 @@ int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets,
     return PT_OK;
 }
 
+}  // extern "C"
+
 namespace {
 
 RenderIO batch_io(pt_batch& b)
@@ -1130,6 +1132,8 @@ int bsettle(pt_batch* b)
 }
 
 }  // namespace
+
+extern "C" {
 
 int pt_batch_create(pt_ctx* const* scenes, uint32_t scene_count, const pt_view* views, uint32_t view_count,
                     pt_batch** out)
