@@ -609,6 +609,26 @@ struct oracle_scene {
         }
         return mk(m.emission.x, m.emission.y, m.emission.z) * m.emission.w;
     }
+    f3 specular_at(const Material& m, f2 uv) const  // get_specular (material.hlsli:70-78)
+    {
+        if (bits_of(m.rmnb.w) & (1u << 5)) {
+            const f4 t = sample_tex(bits_of(m.specular.x), uv);
+            return mk(t.x, t.y, t.z);
+        }
+        return mk(m.specular.x, m.specular.y, m.specular.z);
+    }
+    f3 sheen_at(const Material& m, f2 uv) const  // get_sheen (material.hlsli:80-88)
+    {
+        if (bits_of(m.rmnb.w) & (1u << 7)) {
+            const f4 t = sample_tex(bits_of(m.sheen.x), uv);
+            return mk(t.x, t.y, t.z);
+        }
+        return mk(m.sheen.x, m.sheen.y, m.sheen.z);
+    }
+    float clearcoat_at(const Material& m, f2 uv) const  // get_clearcoat_anisotropy (material.hlsli:90-102)
+    {
+        return (bits_of(m.rmnb.w) & (1u << 9)) ? sample_tex(bits_of(m.cc.x), uv).x : m.cc.x;
+    }
     // get_roughness_metalness_normal
     void rmn_at(const Material& m, f2 uv, float& roughness, float& metalness, f3& normal) const
     {
@@ -1035,12 +1055,12 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     float roughness;
     f3 mapped_n;
     s.rmn_at(mat, uv, roughness, bi.metalness, mapped_n);
-    bi.specular = mk(mat.specular.x, mat.specular.y, mat.specular.z);
-    bi.clearcoat = mat.cc.x;
+    bi.specular = s.specular_at(mat, uv);
+    bi.clearcoat = s.clearcoat_at(mat, uv);
     const float cc_rough = mat.cc.y;
     const float anisotropy = mat.cc.z;
     bi.clearcoat_alpha = lerpf(0.001f, 0.1f, cc_rough * cc_rough);
-    bi.sheen = mk(mat.sheen.x, mat.sheen.y, mat.sheen.z);
+    bi.sheen = s.sheen_at(mat, uv);
     roughness = std::max(0.00001f, roughness * roughness);
     const float aspect = std::sqrt(1.0f - 0.9f * std::fabs(anisotropy));
     bi.alpha = {roughness / aspect, roughness * aspect};

@@ -375,14 +375,6 @@ Material convert_material(const Json& m, const std::function<uint32_t(const Json
     // emission, metallic-roughness, normal.
     const Json& pbr = m["pbrMetallicRoughness"];
     const Json& ext = m["extensions"];
-    for (const char* e : {"KHR_materials_specular", "KHR_materials_sheen", "KHR_materials_clearcoat",
-                          "KHR_materials_anisotropy"})
-        for (const char* t : {"specularTexture", "specularColorTexture", "sheenColorTexture",
-                              "sheenRoughnessTexture", "clearcoatTexture", "clearcoatRoughnessTexture",
-                              "clearcoatNormalTexture", "anisotropyTexture"})
-            if (has_texture(ext[e][t]))
-                throw std::runtime_error(std::string("glTF: texture slot ") + e + "." + t +
-                                         " is not supported by this loader");
     Material out;
     uint32_t bits = 0;
     if (has_texture(pbr["baseColorTexture"])) {  // process_base_colour (scene.cpp:74-111)
@@ -430,19 +422,61 @@ Material convert_material(const Json& m, const std::function<uint32_t(const Json
         out.rough_metal_normal_bits[2] = 0.0f;
     }
 
-    const Json& spec = ext["KHR_materials_specular"];
-    if (spec.kind == Json::OBJ && spec.has("specularColorFactor")) {
+    // Specular, sheen and clearcoat as Assimp 5.4.3's glTF importer hands them over and
+    // process_specular / process_sheen / process_clearcoat_anisotropy (scene.cpp:208-290) read
+    // them. The importer's defaults for these extensions are zero, and it keeps an extension
+    // (its factors and textures) only when they are not all zero: specular factor or colour,
+    // sheen colour, clearcoat factor. The reference then looks up texture slot 0 of each
+    // (specularTexture, sheenColorTexture, clearcoatTexture, linear, after the normal map); a map
+    // replaces the factor. Slot-1/2 textures are never looked up, but one present without a
+    // slot-0 texture gives the reference an empty texture path, so it is refused here.
+    auto slot0 = [&](const Json& e, const char* slot, std::initializer_list<const char*> others,
+                     uint32_t bit, float& dst) {
+        if (has_texture(e[slot])) {
+            bits |= bit;
+            dst = index_bits(tex(e[slot], false));
+            return true;
+        }
+        for (const char* o : others)
+            if (has_texture(e[o]))
+                throw std::runtime_error(std::string("glTF: material has ") + o + " but no " + slot +
+                                         " (the reference's importer gives that texture an empty path)");
+        return false;
+    };
+    auto factor3 = [](const Json& e, const char* key, float* v) {
         for (int i = 0; i < 3; i++)
-            out.specular[i] = float(spec["specularColorFactor"][size_t(i)].number(1));
+            v[i] = float(e[key][size_t(i)].number(0));
+    };
+    const Json& spec = ext["KHR_materials_specular"];
+    float spec_colour[3];
+    factor3(spec, "specularColorFactor", spec_colour);
+    const float spec_factor = float(spec["specularFactor"].number(0));
+    if (spec.kind == Json::OBJ &&
+        (spec_colour[0] != 0.0f || spec_colour[1] != 0.0f || spec_colour[2] != 0.0f || spec_factor != 0.0f)) {
+        if (!slot0(spec, "specularTexture", {"specularColorTexture"}, SPECULAR_MAP_BIT, out.specular[0])) {
+            // AI_MATKEY_SPECULAR_FACTOR is one float read into an aiColor3D: only red is set
+            out.specular[0] = spec_factor;
+            out.specular[1] = out.specular[2] = 0.0f;
+        }
     } else {
         out.specular[0] = out.specular[1] = out.specular[2] = 0.5f;
     }
     const Json& sheen = ext["KHR_materials_sheen"];
-    for (int i = 0; i < 3; i++)
-        out.sheen[i] = float(sheen["sheenColorFactor"][size_t(i)].number(0));
+    factor3(sheen, "sheenColorFactor", out.sheen);
+    if (sheen.kind == Json::OBJ && (out.sheen[0] != 0.0f || out.sheen[1] != 0.0f || out.sheen[2] != 0.0f) &&
+        slot0(sheen, "sheenColorTexture", {"sheenRoughnessTexture"}, SHEEN_MAP_BIT, out.sheen[0]))
+        out.sheen[1] = out.sheen[2] = 0.0f;
     const Json& cc = ext["KHR_materials_clearcoat"];
-    out.cc_ccrgh_aniso[0] = float(cc["clearcoatFactor"].number(0));
-    out.cc_ccrgh_aniso[1] = float(cc["clearcoatRoughnessFactor"].number(0.5));
+    const float cc_factor = float(cc["clearcoatFactor"].number(0));
+    if (cc.kind == Json::OBJ && cc_factor != 0.0f) {
+        if (!slot0(cc, "clearcoatTexture", {"clearcoatRoughnessTexture", "clearcoatNormalTexture"},
+                   CLEARCOAT_MAP_BIT, out.cc_ccrgh_aniso[0]))
+            out.cc_ccrgh_aniso[0] = cc_factor;
+        out.cc_ccrgh_aniso[1] = float(cc["clearcoatRoughnessFactor"].number(0));
+    } else {
+        out.cc_ccrgh_aniso[0] = 0.0f;
+        out.cc_ccrgh_aniso[1] = 0.5f;
+    }
     out.cc_ccrgh_aniso[2] = float(ext["KHR_materials_anisotropy"]["anisotropyStrength"].number(0));
 
     out.set_map_bits(bits);

@@ -158,3 +158,71 @@ def test_jpeg_texture_loads_like_png(tex_dir):
     dw, dh, dc, px = decode(open(jpg, "rb").read())
     assert (w, h, c, srgb) == (dw, dh, dc, 1) == (33, 21, 4, 1)
     assert texels[off:off + w * h * c] == px
+
+
+SPECULAR, SHEEN, CLEARCOAT = 1 << 5, 1 << 7, 1 << 9
+
+
+def _edited(tex_dir, name, edit):
+    import json
+    doc = json.load(open(os.path.join(tex_dir, "textured.gltf")))
+    edit(doc)
+    json.dump(doc, open(os.path.join(tex_dir, name + ".gltf"), "w"))
+    with open(os.path.join(tex_dir, name + ".scn"), "w") as fh:
+        fh.write("camera:\nobjects:\n e %s.gltf\n\n e\n" % name)
+    return pt_amd.Scene(os.path.join(tex_dir, name + ".scn"))
+
+
+def test_specular_sheen_clearcoat_slots(tex_dir):
+    """KHR_materials_specular / _sheen / _clearcoat as Assimp 5.4.3 passes them to
+    process_specular / process_sheen / process_clearcoat_anisotropy (src/scene/scene.cpp:208-290):
+    slot-0 maps after the normal map, linear; factors otherwise; all-zero extensions dropped with
+    their textures (Assimp's defaults are zero); the specular factor read into a colour sets red."""
+    base_bits = _materials(pt_amd.Scene(os.path.join(tex_dir, "textured.scn")))[1][:, 11].copy()
+
+    def edit(doc):
+        m = doc["materials"]
+        m[2]["extensions"] = {  # ceiling
+            "KHR_materials_specular": {"specularFactor": 1.0, "specularTexture": {"index": 1}},
+            "KHR_materials_sheen": {"sheenColorFactor": [0.2, 0.3, 0.4], "sheenColorTexture": {"index": 3}},
+            "KHR_materials_clearcoat": {"clearcoatFactor": 0.7, "clearcoatTexture": {"index": 4},
+                                        "clearcoatRoughnessTexture": {"index": 0}}}
+        m[3]["extensions"] = {  # left wall: every factor zero -> nothing kept, no texture loaded
+            "KHR_materials_specular": {"specularTexture": {"index": 5}},
+            "KHR_materials_sheen": {"sheenColorTexture": {"index": 5}},
+            "KHR_materials_clearcoat": {"clearcoatTexture": {"index": 5}, "clearcoatRoughnessFactor": 0.9}}
+        m[6]["extensions"] = {  # sphere: factors only
+            "KHR_materials_specular": {"specularFactor": 0.6, "specularColorFactor": [1, 1, 1]},
+            "KHR_materials_sheen": {"sheenColorFactor": [0.1, 0.2, 0.3]},
+            "KHR_materials_clearcoat": {"clearcoatFactor": 0.25}}
+
+    s = _edited(tex_dir, "slots", edit)
+    f, u = _materials(s)
+    assert u[2, 11] == base_bits[2] | SPECULAR | SHEEN | CLEARCOAT
+    # the ceiling's maps: the floor's metallic-roughness image (texture 1), the back wall's
+    # base-colour image (3), and the veil image, first used here, so texture 4 and linear
+    assert (u[2, 12], u[2, 16], u[2, 20]) == (1, 3, 4)
+    assert f[2, 21] == 0.0  # clearcoatRoughnessFactor's default in the importer
+    assert u[3, 11] == base_bits[3]
+    np.testing.assert_array_equal(f[3, 12:15], [0.5, 0.5, 0.5])
+    np.testing.assert_array_equal(f[3, 16:19], [0, 0, 0])
+    np.testing.assert_array_equal(f[3, 20:22], [0.0, 0.5])
+    assert u[6, 11] == base_bits[6]
+    np.testing.assert_allclose(f[6, 12:15], [0.6, 0, 0])
+    np.testing.assert_allclose(f[6, 16:19], [0.1, 0.2, 0.3])
+    np.testing.assert_allclose(f[6, 20:22], [0.25, 0.0])
+    # the veil's base colour and the light's emission reuse / follow those textures
+    assert u[5, 0] == 4 and u[7, 4] == 5
+    descs, _ = _textures(s)
+    assert [d[4] for d in descs] == [1, 0, 0, 1, 0, 1]
+
+
+def test_texture_without_slot0_texture_is_refused(tex_dir):
+    import re
+
+    def edit(doc):
+        doc["materials"][2]["extensions"] = {
+            "KHR_materials_specular": {"specularFactor": 1.0, "specularColorTexture": {"index": 1}}}
+
+    with pytest.raises(pt_amd.PtError, match=re.escape("specularColorTexture but no specularTexture")):
+        _edited(tex_dir, "noslot0", edit)
