@@ -500,6 +500,10 @@ private:
 
     bool get_bit() { return get_bits(1) != 0; }
 
+    // Coefficient arithmetic is modulo 2^32 and stored as its low 16 bits (the C conversion to
+    // short): a corrupt stream's out-of-range DC sums and products wrap instead of overflowing.
+    static int16_t wrap16(uint32_t v) { return int16_t(uint16_t(v)); }
+
     void decode_block(int16_t data[64], Component& c)
     {
         const uint16_t* q = dequant_[c.tq];
@@ -507,9 +511,9 @@ private:
         if (t > 15)
             bad("bad DC category");
         memset(data, 0, 64 * sizeof(int16_t));
-        const int dc = c.dc_pred + (t ? receive_extend(t) : 0);
-        c.dc_pred = dc;
-        data[0] = int16_t(dc * q[0]);
+        const uint32_t dc = uint32_t(c.dc_pred) + uint32_t(t ? receive_extend(t) : 0);
+        c.dc_pred = int32_t(dc);
+        data[0] = wrap16(dc * q[0]);
         int k = 1;
         do {
             const int rs = decode_symbol(huff_ac_[c.ha]);
@@ -521,7 +525,7 @@ private:
             } else {
                 k += r;
                 const int z = ZIGZAG[k++];
-                data[z] = int16_t(receive_extend(s) * q[z]);
+                data[z] = wrap16(uint32_t(receive_extend(s)) * q[z]);
             }
         } while (k < 64);
     }
@@ -535,11 +539,11 @@ private:
             const int t = decode_symbol(huff_dc_[c.hd]);
             if (t > 15)
                 bad("bad DC category");
-            const int dc = c.dc_pred + (t ? receive_extend(t) : 0);
-            c.dc_pred = dc;
-            data[0] = int16_t(dc * (1 << succ_low_));
+            const uint32_t dc = uint32_t(c.dc_pred) + uint32_t(t ? receive_extend(t) : 0);
+            c.dc_pred = int32_t(dc);
+            data[0] = wrap16(dc << succ_low_);
         } else if (get_bit()) {
-            data[0] = int16_t(data[0] + (1 << succ_low_));
+            data[0] = wrap16(uint32_t(data[0]) + (1u << succ_low_));
         }
     }
 
@@ -572,7 +576,7 @@ private:
                     k += 16;
                 } else {
                     k += r;
-                    data[ZIGZAG[k++]] = int16_t(receive_extend(s) * (1 << succ_low_));
+                    data[ZIGZAG[k++]] = wrap16(uint32_t(receive_extend(s)) << succ_low_);
                 }
             } while (k <= spec_end_);
             return;
@@ -695,36 +699,43 @@ private:
     static constexpr int fix12(float c) { return int(c * 4096 + 0.5); }
 
     // 1-D islow IDCT of s[0..7] into the even part x0..x3 and the odd part t0..t3 (outputs:
-    // x0+t3, x1+t2, x2+t1, x3+t0, x3-t0, x2-t1, x1-t2, x0-t3), constants scaled by 2^12.
+    // x0+t3, x1+t2, x2+t1, x3+t0, x3-t0, x2-t1, x1-t2, x0-t3), constants scaled by 2^12. The
+    // arithmetic is modulo 2^32: the bits of 32-bit int arithmetic, whose overflow on the
+    // out-of-range coefficients of a corrupt stream is thereby defined.
+    static constexpr uint32_t k12(float c) { return uint32_t(fix12(c)); }
     struct Idct1 {
-        int x0, x1, x2, x3, t0, t1, t2, t3;
-        Idct1(int s0, int s1, int s2, int s3, int s4, int s5, int s6, int s7)
+        uint32_t x0, x1, x2, x3, t0, t1, t2, t3;
+        Idct1(int i0, int i1, int i2, int i3, int i4, int i5, int i6, int i7)
         {
-            const int e1 = (s2 + s6) * fix12(0.5411961f);
-            const int e2 = e1 + s6 * fix12(-1.847759065f);
-            const int e3 = e1 + s2 * fix12(0.765366865f);
-            const int e0 = (s0 + s4) * 4096, e4 = (s0 - s4) * 4096;
+            const uint32_t s0 = uint32_t(i0), s1 = uint32_t(i1), s2 = uint32_t(i2), s3 = uint32_t(i3),
+                           s4 = uint32_t(i4), s5 = uint32_t(i5), s6 = uint32_t(i6), s7 = uint32_t(i7);
+            const uint32_t e1 = (s2 + s6) * k12(0.5411961f);
+            const uint32_t e2 = e1 + s6 * k12(-1.847759065f);
+            const uint32_t e3 = e1 + s2 * k12(0.765366865f);
+            const uint32_t e0 = (s0 + s4) * 4096u, e4 = (s0 - s4) * 4096u;
             x0 = e0 + e3;
             x3 = e0 - e3;
             x1 = e4 + e2;
             x2 = e4 - e2;
-            int a0 = s7, a1 = s5, a2 = s3, a3 = s1;
-            const int z3 = a0 + a2, z4 = a1 + a3, z1 = a0 + a3, z2 = a1 + a2;
-            const int z5 = (z3 + z4) * fix12(1.175875602f);
-            a0 *= fix12(0.298631336f);
-            a1 *= fix12(2.053119869f);
-            a2 *= fix12(3.072711026f);
-            a3 *= fix12(1.501321110f);
-            const int y1 = z5 + z1 * fix12(-0.899976223f);
-            const int y2 = z5 + z2 * fix12(-2.562915447f);
-            const int y3 = z3 * fix12(-1.961570560f);
-            const int y4 = z4 * fix12(-0.390180644f);
+            uint32_t a0 = s7, a1 = s5, a2 = s3, a3 = s1;
+            const uint32_t z3 = a0 + a2, z4 = a1 + a3, z1 = a0 + a3, z2 = a1 + a2;
+            const uint32_t z5 = (z3 + z4) * k12(1.175875602f);
+            a0 *= k12(0.298631336f);
+            a1 *= k12(2.053119869f);
+            a2 *= k12(3.072711026f);
+            a3 *= k12(1.501321110f);
+            const uint32_t y1 = z5 + z1 * k12(-0.899976223f);
+            const uint32_t y2 = z5 + z2 * k12(-2.562915447f);
+            const uint32_t y3 = z3 * k12(-1.961570560f);
+            const uint32_t y4 = z4 * k12(-0.390180644f);
             t3 = a3 + y1 + y4;
             t2 = a2 + y2 + y3;
             t1 = a1 + y2 + y4;
             t0 = a0 + y1 + y3;
         }
     };
+    // arithmetic right shift of a modulo-2^32 value read as a signed 32-bit int
+    static int sar(uint32_t v, int n) { return int32_t(v) >> n; }
 
     static uint8_t clamp255(int x) { return uint8_t(x < 0 ? 0 : (x > 255 ? 255 : x)); }
 
@@ -739,29 +750,29 @@ private:
                 continue;
             }
             Idct1 f(c[0], c[8], c[16], c[24], c[32], c[40], c[48], c[56]);
-            f.x0 += 512, f.x1 += 512, f.x2 += 512, f.x3 += 512;
-            v[i + 0] = (f.x0 + f.t3) >> 10;
-            v[i + 56] = (f.x0 - f.t3) >> 10;
-            v[i + 8] = (f.x1 + f.t2) >> 10;
-            v[i + 48] = (f.x1 - f.t2) >> 10;
-            v[i + 16] = (f.x2 + f.t1) >> 10;
-            v[i + 40] = (f.x2 - f.t1) >> 10;
-            v[i + 24] = (f.x3 + f.t0) >> 10;
-            v[i + 32] = (f.x3 - f.t0) >> 10;
+            f.x0 += 512u, f.x1 += 512u, f.x2 += 512u, f.x3 += 512u;
+            v[i + 0] = sar(f.x0 + f.t3, 10);
+            v[i + 56] = sar(f.x0 - f.t3, 10);
+            v[i + 8] = sar(f.x1 + f.t2, 10);
+            v[i + 48] = sar(f.x1 - f.t2, 10);
+            v[i + 16] = sar(f.x2 + f.t1, 10);
+            v[i + 40] = sar(f.x2 - f.t1, 10);
+            v[i + 24] = sar(f.x3 + f.t0, 10);
+            v[i + 32] = sar(f.x3 - f.t0, 10);
         }
         for (int r = 0; r < 8; r++, out += stride) {  // rows
             const int* s = v + 8 * r;
             Idct1 f(s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]);
-            const int bias = 65536 + (128 << 17);
+            const uint32_t bias = 65536u + (128u << 17);
             f.x0 += bias, f.x1 += bias, f.x2 += bias, f.x3 += bias;
-            out[0] = clamp255((f.x0 + f.t3) >> 17);
-            out[7] = clamp255((f.x0 - f.t3) >> 17);
-            out[1] = clamp255((f.x1 + f.t2) >> 17);
-            out[6] = clamp255((f.x1 - f.t2) >> 17);
-            out[2] = clamp255((f.x2 + f.t1) >> 17);
-            out[5] = clamp255((f.x2 - f.t1) >> 17);
-            out[3] = clamp255((f.x3 + f.t0) >> 17);
-            out[4] = clamp255((f.x3 - f.t0) >> 17);
+            out[0] = clamp255(sar(f.x0 + f.t3, 17));
+            out[7] = clamp255(sar(f.x0 - f.t3, 17));
+            out[1] = clamp255(sar(f.x1 + f.t2, 17));
+            out[6] = clamp255(sar(f.x1 - f.t2, 17));
+            out[2] = clamp255(sar(f.x2 + f.t1, 17));
+            out[5] = clamp255(sar(f.x2 - f.t1, 17));
+            out[3] = clamp255(sar(f.x3 + f.t0, 17));
+            out[4] = clamp255(sar(f.x3 - f.t0, 17));
         }
     }
 
