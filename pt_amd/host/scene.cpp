@@ -84,6 +84,24 @@ static std::string directory_of(const std::string& file)
 
 namespace {
 
+// JSON numbers used as array indices, counts and byte offsets. A double outside size_t's range
+// has no defined conversion, so both check first: an index that is not an integer in
+// [0, 2^32) becomes SIZE_MAX, which no array holds (the lookup then reports the bad reference);
+// a count or offset must be an integer in [0, 2^40).
+size_t json_index(const Json& v, double dflt = -1.0)
+{
+    const double d = v.number(dflt);
+    return (d >= 0.0 && d < 4294967296.0 && d == std::floor(d)) ? size_t(d) : SIZE_MAX;
+}
+
+size_t json_size(const Json& v, double dflt, const char* what)
+{
+    const double d = v.number(dflt);
+    if (!(d >= 0.0 && d < 1099511627776.0 && d == std::floor(d)))
+        throw std::runtime_error(std::string("glTF: bad ") + what);
+    return size_t(d);
+}
+
 std::vector<uint8_t> decode_base64(const std::string& s)
 {
     auto val = [](char c) -> int {
@@ -121,12 +139,12 @@ struct Gltf {
     // integer ones as numbers or, when `normalized`, mapped to [0, 1] / [-1, 1] (glTF 2.0
     // 3.11, and KHR_mesh_quantization's integer positions and normals), with sparse
     // substitutions applied.
-    std::vector<float> floats(int index, int comps) const
+    std::vector<float> floats(const Json& ref, int comps) const
     {
-        const Json& acc = doc["accessors"][size_t(index)];
+        const Json& acc = doc["accessors"][json_index(ref)];
         if (acc.kind != Json::OBJ)
-            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
-        const int ct = int(acc["componentType"].number(0));
+            throw std::runtime_error("glTF: missing accessor");
+        const size_t ct = json_index(acc["componentType"], 0);
         const bool normalized = acc["normalized"].kind == Json::BOOL && acc["normalized"].b;
         const size_t csize = ct == 5126 || ct == 5125 ? 4 : (ct == 5122 || ct == 5123 ? 2 : (ct == 5120 || ct == 5121 ? 1 : 0));
         if (!csize)
@@ -159,19 +177,20 @@ struct Gltf {
                 return normalized ? std::max(float(int8_t(*p)) / 127.0f, -1.0f) : float(int8_t(*p));
             }
         };
-        const size_t count = size_t(acc["count"].number(0));
-        std::vector<float> out(count * comps, 0.0f);
-        if (acc.has("bufferView")) {
-            const uint8_t* base;
-            size_t stride;
+        const size_t count = json_size(acc["count"], 0, "accessor count");
+        const uint8_t* base = nullptr;
+        size_t stride = 0;
+        if (acc.has("bufferView"))
             view(acc, comps * csize, base, stride, count);
+        std::vector<float> out(count * comps, 0.0f);
+        if (base) {
             for (size_t i = 0; i < count; i++)
                 for (int k = 0; k < comps; k++)
                     out[i * comps + k] = component(base + i * stride + k * csize);
         }
         if (acc.has("sparse")) {  // elements given by index replace the base values
             const Json& sp = acc["sparse"];
-            const size_t n = size_t(sp["count"].number(0));
+            const size_t n = json_size(sp["count"], 0, "sparse count");
             const std::vector<uint32_t> at = uints_from(sp["indices"], n);
             const uint8_t* vals;
             size_t vstride;
@@ -180,7 +199,7 @@ struct Gltf {
                 if (at[j] >= count)
                     throw std::runtime_error("glTF: sparse index out of range");
                 for (int k = 0; k < comps; k++)
-                    out[at[j] * comps + k] = component(vals + j * (comps * csize) + k * csize);
+                    out[at[j] * comps + k] = component(vals + j * vstride + k * csize);
             }
         }
         return out;
@@ -189,7 +208,7 @@ struct Gltf {
     // n unsigned integers (componentType 5121, 5123 or 5125) from an accessor-like object.
     std::vector<uint32_t> uints_from(const Json& acc, size_t count) const
     {
-        const int ct = int(acc["componentType"].number(0));
+        const size_t ct = json_index(acc["componentType"], 0);
         const int csize = ct == 5125 ? 4 : ct == 5123 ? 2 : ct == 5121 ? 1 : 0;
         if (!csize)
             throw std::runtime_error("glTF: unsupported index component type");
@@ -216,27 +235,30 @@ struct Gltf {
         return out;
     }
 
-    std::vector<uint32_t> uints(int index) const
+    std::vector<uint32_t> uints(const Json& ref) const
     {
-        const Json& acc = doc["accessors"][size_t(index)];
+        const Json& acc = doc["accessors"][json_index(ref)];
         if (acc.kind != Json::OBJ)
-            throw std::runtime_error("glTF: missing accessor " + std::to_string(index));
+            throw std::runtime_error("glTF: missing accessor");
         if (acc.has("sparse"))
             throw std::runtime_error("glTF: sparse index accessors are not supported");
-        return uints_from(acc, size_t(acc["count"].number(0)));
+        return uints_from(acc, json_size(acc["count"], 0, "accessor count"));
     }
 
     void view(const Json& acc, size_t elem_bytes, const uint8_t*& base, size_t& stride, size_t count) const
     {
-        const Json& bv = doc["bufferViews"][size_t(acc["bufferView"].number(-1))];
+        const Json& bv = doc["bufferViews"][json_index(acc["bufferView"])];
         if (bv.kind != Json::OBJ)
             throw std::runtime_error("glTF: bad bufferView");
-        const size_t b = size_t(bv["buffer"].number(-1));
+        const size_t b = json_index(bv["buffer"]);
         if (b >= buffers.size())
             throw std::runtime_error("glTF: bad buffer index");
-        const size_t off = size_t(bv["byteOffset"].number(0)) + size_t(acc["byteOffset"].number(0));
-        stride = bv.has("byteStride") ? size_t(bv["byteStride"].number(0)) : elem_bytes;
-        if (count > 0 && off + (count - 1) * stride + elem_bytes > buffers[b].size())
+        const size_t off = json_size(bv["byteOffset"], 0, "byteOffset") + json_size(acc["byteOffset"], 0, "byteOffset");
+        stride = bv.has("byteStride") ? json_size(bv["byteStride"], 0, "byteStride") : elem_bytes;
+        // off + (count - 1) * stride + elem_bytes <= size, without wrapping
+        const size_t size = buffers[b].size();
+        if (count > 0 && (elem_bytes > size || off > size - elem_bytes ||
+                          (stride > 0 && count - 1 > (size - elem_bytes - off) / stride)))
             throw std::runtime_error("glTF: accessor exceeds its buffer");
         base = buffers[b].data() + off;
     }
@@ -510,7 +532,7 @@ void walk_nodes(const Json& doc, size_t node_index, const M4& parent, std::vecto
         throw std::runtime_error("glTF: bad node index");
     const M4 m = parent * node_matrix(node);
     if (node.has("mesh")) {
-        size_t mi = size_t(node["mesh"].number(-1));
+        size_t mi = json_index(node["mesh"]);
         if (mi >= mesh_prims.size())
             throw std::runtime_error("glTF: bad mesh index");
         for (uint32_t prim_mesh : mesh_prims[mi])
@@ -518,7 +540,7 @@ void walk_nodes(const Json& doc, size_t node_index, const M4& parent, std::vecto
     }
     const Json& ch = node["children"];
     for (size_t i = 0; i < ch.size(); i++)
-        walk_nodes(doc, size_t(ch[i].number(0)), m, mesh_instances, mesh_prims, depth + 1);
+        walk_nodes(doc, json_index(ch[i]), m, mesh_instances, mesh_prims, depth + 1);
 }
 
 }  // namespace
@@ -533,10 +555,10 @@ Scene load_gltf(const std::string& path)
     // embedded image): one entry per distinct image, in order of first use.
     std::map<std::string, uint32_t> tex_index;
     auto tex = [&](const Json& ref, bool srgb) -> uint32_t {
-        const Json& t = g.doc["textures"][size_t(ref["index"].number(-1))];
+        const Json& t = g.doc["textures"][json_index(ref["index"])];
         if (t.kind != Json::OBJ || !t.has("source"))
             throw std::runtime_error("glTF: texture without an image source");
-        const size_t ii = size_t(t["source"].number(-1));
+        const size_t ii = json_index(t["source"]);
         const Json& img = g.doc["images"][ii];
         if (img.kind != Json::OBJ)
             throw std::runtime_error("glTF: bad image index");
@@ -556,9 +578,9 @@ Scene load_gltf(const std::string& path)
                 throw std::runtime_error("glTF: unsupported image data URI");
             bytes = decode_base64(uri.substr(comma + 1));
         } else if (img.has("bufferView")) {
-            const Json& bv = g.doc["bufferViews"][size_t(img["bufferView"].number(-1))];
-            const size_t b = size_t(bv["buffer"].number(-1));
-            const size_t off = size_t(bv["byteOffset"].number(0)), len = size_t(bv["byteLength"].number(0));
+            const Json& bv = g.doc["bufferViews"][json_index(img["bufferView"])];
+            const size_t b = json_index(bv["buffer"]);
+            const size_t off = json_size(bv["byteOffset"], 0, "byteOffset"), len = json_size(bv["byteLength"], 0, "byteLength");
             if (b >= g.buffers.size() || off + len > g.buffers[b].size())
                 throw std::runtime_error("glTF: image bufferView exceeds its buffer");
             bytes.assign(g.buffers[b].begin() + long(off), g.buffers[b].begin() + long(off + len));
@@ -589,8 +611,8 @@ Scene load_gltf(const std::string& path)
         const Json& prims = meshes[mi]["primitives"];
         for (size_t pi = 0; pi < prims.size(); pi++) {
             const Json& prim = prims[pi];
-            const int mode = int(prim["mode"].number(4));
-            if (mode < 0 || mode > 6)
+            const size_t mode = json_index(prim["mode"], 4);
+            if (mode > 6)
                 throw std::runtime_error("glTF: unknown primitive mode");
             const Json& attr = prim["attributes"];
             if (!attr.has("POSITION"))
@@ -598,13 +620,13 @@ Scene load_gltf(const std::string& path)
             const bool has_normals = attr.has("NORMAL");
             if (!prim.has("material"))
                 throw std::runtime_error("glTF: primitives need a material");
-            std::vector<float> pos = g.floats(int(attr["POSITION"].number(-1)), 3);
+            std::vector<float> pos = g.floats(attr["POSITION"], 3);
             std::vector<float> nrm;
             if (has_normals)
-                nrm = g.floats(int(attr["NORMAL"].number(-1)), 3);
+                nrm = g.floats(attr["NORMAL"], 3);
             std::vector<float> uv;
             if (attr.has("TEXCOORD_0"))
-                uv = g.floats(int(attr["TEXCOORD_0"].number(-1)), 2);
+                uv = g.floats(attr["TEXCOORD_0"], 2);
             else
                 std::cerr << "Warning: mesh \"" << meshes[mi]["name"].str << "\" has no UVs." << std::endl;
             const size_t nv = pos.size() / 3;
@@ -624,7 +646,7 @@ Scene load_gltf(const std::string& path)
             }
             std::vector<uint32_t> idx;
             if (prim.has("indices")) {
-                idx = g.uints(int(prim["indices"].number(-1)));
+                idx = g.uints(prim["indices"]);
             } else {
                 idx.resize(nv);
                 std::iota(idx.begin(), idx.end(), 0u);
@@ -668,13 +690,13 @@ Scene load_gltf(const std::string& path)
             mesh.tris.resize(mesh.tris.size() + zero_faces, Tri{{0, 0, 0}});
             if (!has_normals)
                 flat_normals(mesh);
-            const uint32_t mat = uint32_t(prim["material"].number(-1));
+            const size_t mat = json_index(prim["material"]);
             if (mat >= scene.materials.size())
                 throw std::runtime_error("glTF: bad material index");
             mesh_prims[mi].push_back(uint32_t(scene.meshes.size()));
             Variant var;
             var.mesh = uint32_t(scene.meshes.size());
-            var.material = mat;
+            var.material = uint32_t(mat);
             scene.meshes.push_back(std::move(mesh));
             scene.variants.push_back(var);
         }
@@ -683,15 +705,15 @@ Scene load_gltf(const std::string& path)
     std::vector<std::vector<M4>> mesh_instances(scene.meshes.size());
     std::vector<size_t> roots;
     if (g.doc.has("scenes") && g.doc["scenes"].size() > 0) {
-        const size_t si = size_t(g.doc["scene"].number(0));
+        const size_t si = json_index(g.doc["scene"], 0);
         const Json& sc = g.doc["scenes"][si];
         for (size_t i = 0; i < sc["nodes"].size(); i++)
-            roots.push_back(size_t(sc["nodes"][i].number(0)));
+            roots.push_back(json_index(sc["nodes"][i]));
     } else {
         std::vector<bool> is_child(g.doc["nodes"].size(), false);
         for (size_t n = 0; n < g.doc["nodes"].size(); n++)
             for (size_t c = 0; c < g.doc["nodes"][n]["children"].size(); c++) {
-                size_t ci = size_t(g.doc["nodes"][n]["children"][c].number(0));
+                size_t ci = json_index(g.doc["nodes"][n]["children"][c]);
                 if (ci < is_child.size())
                     is_child[ci] = true;
             }

@@ -311,3 +311,29 @@ def test_gltf_modes_quantised_and_sparse_accessors(tmp_path):
     np.testing.assert_array_equal(v[v0:v0 + 4, 6:8], uv8 / np.float32(255))
     v3 = a["meshes"][3][0]
     np.testing.assert_array_equal(v[v3:v3 + 4, :3], np.vstack([pos[:3], sp_val]))
+
+
+@pytest.mark.parametrize("edit, msg", [
+    (lambda d: d["bufferViews"][0].update(byteStride=1e30), "glTF: bad byteStride"),
+    (lambda d: d["bufferViews"][0].update(byteStride=2 ** 39), "glTF: accessor exceeds its buffer"),
+    (lambda d: d["accessors"][0].update(count=2 ** 33), "glTF: accessor exceeds its buffer"),
+    (lambda d: d["accessors"][0].update(byteOffset=-4), "glTF: bad byteOffset"),
+    (lambda d: d["accessors"][0].update(count=2.5), "glTF: bad accessor count"),
+    (lambda d: d["meshes"][0]["primitives"][0]["attributes"].update(POSITION=1e20), "glTF: missing accessor"),
+    (lambda d: d["meshes"][0]["primitives"][0].update(material=-1), "glTF: bad material index"),
+])
+def test_gltf_sizes_and_indices_out_of_range_are_errors(tmp_path, edit, msg):
+    """Counts, strides, offsets and indices that do not fit (huge, negative, fractional) are
+    refused before any read: a stride near 2^63 once wrapped the bounds check of an accessor
+    (found by a sanitizer fuzz run of the loader)."""
+    import json
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    views = [{"buffer": 0, "byteOffset": 0, "byteLength": pos.nbytes}]
+    acc = [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}]
+    prims = [dict(attributes={"POSITION": 0}, material=0)]
+    doc = _gltf_doc(pos.tobytes(), acc, views, prims)
+    edit(doc)
+    json.dump(doc, open(tmp_path / "m.gltf", "w"))
+    (tmp_path / "m.scn").write_text("camera:\nobjects:\n m m.gltf\n\n m\n")
+    with pytest.raises(pt_amd.PtError, match=re.escape(msg)):
+        pt_amd.Scene(str(tmp_path / "m.scn"))
