@@ -116,11 +116,13 @@ def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
     log("convergence: %d-spp reference in %.1f s" % (ref_spp, ref_s))
     plain_s, plain_spp, plain_e, plain_traj = time_to_threshold(tr, cam, ref, 16, 4096, 60.0, threshold)
     log("convergence: plain %.3f s, %d spp, rel-MSE %.3e" % (plain_s, plain_spp, plain_e))
-    # in-flight training (two Adam steps of 8192 records after every wavefront iteration), one
+    # in-flight training (three Adam steps of 8192 records after every wavefront iteration), one
     # path in 16 trains; the others end at their second vertex with the prediction. Learning
     # rate 3e-3: the best of a sweep over 1e-3..1e-2, 2..4 steps, batches of 8K..16K records,
-    # strides 4..16 and continuation probabilities 0..0.75 on this scene.
-    cfg = dict(query=True, train_stride=16, train_steps=2, batch=8192, learning_rate=3e-3, frame_spp=0,
+    # strides 4..16 and continuation probabilities 0..0.75 on this scene. Three steps rather
+    # than two since the autotuner's 16M-slot pool halves the iterations per sample (at 16M
+    # slots two steps gave 0.98x the plain tracer's time, three and four 0.83x).
+    cfg = dict(query=True, train_stride=16, train_steps=3, batch=8192, learning_rate=3e-3, frame_spp=0,
                continue_prob=0.0, seed=1)
     tr.cache_configure(**cfg)
     warm = 16
@@ -169,11 +171,15 @@ def run_instanced(a):
         tr.set_option(pt_amd.PT_OPT_LEAF_SIZE, a.leaf_size)
     if a.blocks_per_cu:
         tr.set_option(pt_amd.PT_OPT_BLOCKS_PER_CU, a.blocks_per_cu)
+    if a.pool_slots:
+        tr.set_option(pt_amd.PT_OPT_POOL_SLOTS, a.pool_slots)
     st0 = tr.stats()
     tune = None
     if not a.no_autotune:
         tr.render(1)
-        tune = tr.autotune(4)
+        # trials of 16 spp (four pools of paths at 1080p): long enough to time the streaming
+        # steady state the timed 64-spp steps run in, not one pool's fill and drain
+        tune = tr.autotune(16)
         log("autotune: %d trials, blocks/CU %d, LDS stack %d, leaf %d, pool %d: %.3f -> %.3f ns/ray"
             % (tune["trials"], tune["blocks_per_cu"], tune["lds_stack"], tune["leaf_size"], tune["pool_slots"],
                tune["ns_per_ray_start"], tune["ns_per_ray_best"]))
@@ -435,6 +441,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--leaf-size", type=int, default=0, help="BLAS leaf size to start from (0: the library's)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="traversal blocks per CU to start from")
+    ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))

@@ -1486,7 +1486,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
     const std::vector<int64_t> bpcs = {2, 3, 4, 5, 6, 8};
     const std::vector<int64_t> ldss = {8, 16, 32};
     std::vector<int64_t> pools;
-    for (int64_t p : {1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20})
+    for (int64_t p : {1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20})
         if (uint64_t(p) <= std::max<uint64_t>(npix * spp, 1u << 20))
             pools.push_back(p);
     const std::vector<int64_t> leaves = {2, 4, 8};
