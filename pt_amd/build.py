@@ -95,14 +95,6 @@ def build_cli(with_hip=True):
     return out
 
 
-def build_copy_calib():
-    """A plain copy kernel of known traffic, for rocprofv3 counter calibration."""
-    os.makedirs(BIN, exist_ok=True)
-    out = os.path.join(BIN, "copy_calib")
-    run([HIPCC, "--offload-arch=gfx950", "-O3", "-o", out, os.path.join(PKG, "csrc", "copy_calib.cpp")])
-    return out
-
-
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference/src"):
@@ -120,8 +112,6 @@ def main(argv=None):
     if not a.no_hip:
         build_hip(a.j, force=not a.incremental)
     build_cli(with_hip=not a.no_hip)
-    if not a.no_hip:
-        build_copy_calib()
     if not a.no_oracle:
         build_oracle()
 
