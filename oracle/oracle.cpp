@@ -878,7 +878,8 @@ f3 xform_normal(const Instance& in, f3 v)
 
 inline f3 emission_of(const Material& m) { return mk(m.emission.x, m.emission.y, m.emission.z) * m.emission.w; }
 
-f3 sample_light(const oracle_scene& s, f3 u, f3 pt, f3& light_dir, float& light_dist, float& pdf, bool& is_delta)
+f3 sample_light(const oracle_scene& s, f3 u, f3 pt, f3& light_dir, float& light_dist, float& pdf, bool& is_delta,
+               uint32_t flags = 0)
 {
     uint32_t bo = sample_alias_table(s.light, s.light_count, PT_LIGHT_BIN_WORDS, 0, u.x, pdf);
     const uint32_t type = s.light[bo + 0];
@@ -904,7 +905,7 @@ f3 sample_light(const oracle_scene& s, f3 u, f3 pt, f3& light_dir, float& light_
         const float area = std::sqrt(dot(tn, tn)) * 0.5f;
         const float cosine = std::fabs(dot(normalize(light_dir), normalize(tn)));
         const float dist2 = dot(light_dir, light_dir);
-        pdf *= dist2 / (area * cosine);
+        pdf *= ((flags & ORACLE_PLANT_NO_DIST2) ? 1.0f : dist2) / (area * cosine);
         is_delta = false;
         if (!std::isfinite(pdf))
             return mk(0, 0, 0);
@@ -939,6 +940,7 @@ struct Payload {
     float brdf_pdf;  // -1 before the first bounce
     Sampler smp;
     f3 next_origin, next_dir;
+    uint32_t flags;  // ORACLE_NO_NEE / ORACLE_NO_RR / ORACLE_PLANT_* (oracle.h)
 };
 
 struct Counters {
@@ -971,7 +973,7 @@ void camera_ray(const pt_camera& c, uint32_t px, uint32_t py, uint32_t w, uint32
 }
 
 f3 light_contribution(const oracle_scene& s, const BrdfInput& in, f3 o, f3 x, const f3 fw[3], Sampler& smp,
-                      Counters& cnt)
+                      Counters& cnt, uint32_t flags)
 {
     if (s.light_count == 0)
         return mk(0, 0, 0);
@@ -980,7 +982,7 @@ f3 light_contribution(const oracle_scene& s, const BrdfInput& in, f3 o, f3 x, co
     f3 light_dir;
     float light_dist, pdf;
     bool is_delta;
-    f3 emission = sample_light(s, mk(u0, u12.x, u12.y), x, light_dir, light_dist, pdf, is_delta);
+    f3 emission = sample_light(s, mk(u0, u12.x, u12.y), x, light_dir, light_dist, pdf, is_delta, flags);
     if (maxc(emission) <= 0.0f)
         return mk(0, 0, 0);
     f3 i = {dot(fw[0], light_dir), dot(fw[1], light_dir), dot(fw[2], light_dir)};
@@ -1024,7 +1026,7 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
     const f3 obj_dir = {float(iv[0] * rd.x + iv[1] * rd.y + iv[2] * rd.z), float(iv[4] * rd.x + iv[5] * rd.y + iv[6] * rd.z),
                         float(iv[8] * rd.x + iv[9] * rd.y + iv[10] * rd.z)};
     if (maxc(emission) > 0.0f && dot(obj_dir, obj_n) < 0.0f) {
-        if (pl.brdf_pdf < 0.0f) {
+        if (pl.brdf_pdf < 0.0f || (pl.flags & ORACLE_NO_NEE)) {
             pl.radiance = pl.radiance + pl.throughput * emission;
         } else {
             const f3 wab = xform_vec3(in, vb.pos - va.pos);
@@ -1036,7 +1038,7 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
             float pdf_l = light_pdf(s, in.emitter_index, h.prim);
             pdf_l *= dist2 / (area * cosine);
             if (pdf_l > 0.0f && dist2 > 0.0f) {
-                const float w = pl.brdf_pdf / (pl.brdf_pdf + pdf_l);
+                const float w = ((pl.flags & ORACLE_PLANT_MIS_SWAP) ? pdf_l : pl.brdf_pdf) / (pl.brdf_pdf + pdf_l);
                 pl.radiance = pl.radiance + pl.throughput * (w * emission);
             }
         }
@@ -1116,7 +1118,8 @@ void closest_hit(const oracle_scene& s, Payload& pl, f3 ro, f3 rd, const HitRec&
         return;
     }
     const f3 x = ro + rd * t + twn * ORIGIN_OFFSET;
-    pl.radiance = pl.radiance + pl.throughput * light_contribution(s, bi, o, x, fw, pl.smp, cnt);
+    if (!(pl.flags & ORACLE_NO_NEE))  // SAMPLE_LIGHTS (pathtrace.ch.hlsl:8,338-340)
+        pl.radiance = pl.radiance + pl.throughput * light_contribution(s, bi, o, x, fw, pl.smp, cnt, pl.flags);
 
     f3 m;
     float w, pdf;
@@ -1156,7 +1159,7 @@ bool russian_roulette(Payload& pl)
         const float q = std::max(0.0f, 1.0f - mt);
         if (u < q)
             return true;
-        pl.throughput = pl.throughput * (1.0f / (1.0f - q));
+        pl.throughput = pl.throughput * (1.0f / ((pl.flags & ORACLE_PLANT_RR_Q) ? q : 1.0f - q));
     }
     return false;
 }
@@ -1164,9 +1167,10 @@ bool russian_roulette(Payload& pl)
 inline float correct(float x) { return (x < 0.0f || !std::isfinite(x)) ? 0.0f : x; }
 
 f3 trace_path(const oracle_scene& s, const pt_camera& c, uint32_t px, uint32_t py, uint32_t w, uint32_t h, uint32_t si,
-              uint32_t max_bounces, Counters& cnt)
+              uint32_t max_bounces, Counters& cnt, uint32_t flags)
 {
     Payload pl;
+    pl.flags = flags;
     pl.radiance = {0, 0, 0};
     pl.smp.start(px, py, si, 0);
     const f2 up = pl.smp.s2();
@@ -1185,7 +1189,7 @@ f3 trace_path(const oracle_scene& s, const pt_camera& c, uint32_t px, uint32_t p
             rd = pl.next_dir;
             pl.final_segment = seg >= max_bounces;
             trace_segment(s, pl, ro, rd, 0.0f, INFINITY, cnt);
-            if (pl.final_segment || russian_roulette(pl))
+            if (pl.final_segment || (!(pl.flags & ORACLE_NO_RR) && russian_roulette(pl)))
                 break;
         }
     }
@@ -1247,6 +1251,14 @@ int oracle_render(const oracle_scene* s, const pt_camera* cam, uint32_t width, u
                   uint32_t spp, uint32_t max_bounces, float* accum, uint32_t rows_step, int threads,
                   oracle_counts* counts)
 {
+    return oracle_render_ex(s, cam, width, height, sample_index, spp, max_bounces, accum, rows_step, threads, counts,
+                            0u);
+}
+
+int oracle_render_ex(const oracle_scene* s, const pt_camera* cam, uint32_t width, uint32_t height, uint32_t sample_index,
+                     uint32_t spp, uint32_t max_bounces, float* accum, uint32_t rows_step, int threads,
+                     oracle_counts* counts, uint32_t flags)
+{
     if (!s || !cam || !accum || spp == 0 || width == 0 || height == 0)
         return PT_EINVAL;
     if (rows_step == 0)
@@ -1263,7 +1275,7 @@ int oracle_render(const oracle_scene* s, const pt_camera* cam, uint32_t width, u
             float* px = accum + (size_t(y) * width + x) * 4;
             for (uint32_t k = 0; k < spp; k++) {
                 const uint32_t si = sample_index + k;
-                const f3 r = trace_path(*s, *cam, x, uint32_t(y), width, height, si, max_bounces, c);
+                const f3 r = trace_path(*s, *cam, x, uint32_t(y), width, height, si, max_bounces, c, flags);
                 np++;
                 // pathtrace.rg.hlsl:61-67 with samples == 1 per dispatch
                 if (si == 0) {
@@ -1353,6 +1365,33 @@ int oracle_sampler_draw(uint32_t px, uint32_t py, uint32_t sample_index, uint32_
         }
     }
     return PT_OK;
+}
+
+int oracle_eval_brdf_many(const float* p, size_t n, const float* i, const float* o, float* rgb_out)
+{
+    BrdfInput in;
+    in.base_colour = mk(p[0], p[1], p[2]);
+    in.alpha = {p[3], p[4]};
+    in.metalness = p[5];
+    in.specular = mk(p[6], p[7], p[8]);
+    in.sheen = mk(p[9], p[10], p[11]);
+    in.clearcoat = p[12];
+    in.clearcoat_alpha = p[13];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < int64_t(n); k++) {
+        const f3 ii = mk(i[3 * k], i[3 * k + 1], i[3 * k + 2]), oo = mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+        float pdf;
+        const f3 r = evaluate_brdf(in, ii, oo, normalize(ii + oo), pdf);
+        rgb_out[3 * k] = r.x;
+        rgb_out[3 * k + 1] = r.y;
+        rgb_out[3 * k + 2] = r.z;
+    }
+    return PT_OK;
+}
+
+float oracle_light_pdf(const oracle_scene* s, uint32_t emitter_index, uint32_t prim)
+{
+    return s ? light_pdf(*s, emitter_index, prim) : 0.0f;
 }
 
 int oracle_eval_brdf(const float* p, const float* i, const float* o, const float* m, float* rgb_out, float* pdf_out)

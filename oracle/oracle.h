@@ -42,6 +42,31 @@ int oracle_render(const oracle_scene* s, const pt_camera* cam, uint32_t width, u
                   uint32_t sample_index, uint32_t spp, uint32_t max_bounces, float* accum,
                   uint32_t rows_step, int threads, oracle_counts* counts);
 
+/* Estimator variants of oracle_render_ex, for the oracle's own checks against closed forms and
+ * equivalent estimators (the reference build always samples lights and plays Russian roulette):
+ *   ORACLE_NO_NEE   SAMPLE_LIGHTS undefined (pathtrace.ch.hlsl:8): no light sampling, emission
+ *                   added unweighted at every hit (pathtrace.ch.hlsl:266-268);
+ *   ORACLE_NO_RR    no Russian roulette (pathtrace.rg.hlsl:53).
+ * Planted mistakes, which the checks must detect:
+ *   ORACLE_PLANT_NO_DIST2  light sampling's area-to-solid-angle conversion without d^2;
+ *   ORACLE_PLANT_MIS_SWAP  the emitter-hit balance weight with the light pdf on top;
+ *   ORACLE_PLANT_RR_Q      surviving paths divided by q instead of 1 - q. */
+#define ORACLE_NO_NEE 1u
+#define ORACLE_NO_RR 2u
+#define ORACLE_PLANT_NO_DIST2 0x100u
+#define ORACLE_PLANT_MIS_SWAP 0x200u
+#define ORACLE_PLANT_RR_Q 0x400u
+int oracle_render_ex(const oracle_scene* s, const pt_camera* cam, uint32_t width, uint32_t height,
+                     uint32_t sample_index, uint32_t spp, uint32_t max_bounces, float* accum,
+                     uint32_t rows_step, int threads, oracle_counts* counts, uint32_t flags);
+
+/* light_pdf (lightsample.hlsli:36-50): area probability of the emitter triangle prim of the
+ * light bin emitter_index (light table p times triangle table p). */
+float oracle_light_pdf(const oracle_scene* s, uint32_t emitter_index, uint32_t prim);
+
+/* evaluate_brdf of n (i, o) pairs with m = normalize(i + o) (params as oracle_eval_brdf). */
+int oracle_eval_brdf_many(const float* params, size_t n, const float* i, const float* o, float* rgb_out);
+
 /* Primary hits of one sample index (pathtrace.rg.hlsl:18-35). */
 int oracle_primary(const oracle_scene* s, const pt_camera* cam, uint32_t width, uint32_t height,
                    uint32_t sample_index, pt_hit* hits, pt_ray* rays_out);

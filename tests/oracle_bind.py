@@ -26,6 +26,10 @@ def lib():
         L.oracle_scene_free.argtypes = [P]
         L.oracle_render.argtypes = [P, C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_void_p, C.c_uint32, C.c_int, C.POINTER(Counts)]
+        L.oracle_render_ex.argtypes = L.oracle_render.argtypes + [C.c_uint32]
+        L.oracle_light_pdf.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.oracle_light_pdf.restype = C.c_float
+        L.oracle_eval_brdf_many.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_primary.argtypes = [P, C.POINTER(pt_amd.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                      C.c_void_p]
         L.oracle_trace_rays.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
@@ -40,6 +44,11 @@ def lib():
         L.oracle_tonemap.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
         _lib = L
     return _lib
+
+
+# estimator variants and planted mistakes of oracle_render_ex (oracle/oracle.h)
+NO_NEE, NO_RR = 1, 2
+PLANT_NO_DIST2, PLANT_MIS_SWAP, PLANT_RR_Q = 0x100, 0x200, 0x400
 
 
 def _f32(a):
@@ -60,12 +69,13 @@ class Oracle:
             lib().oracle_scene_free(self._h)
             self._h = None
 
-    def render(self, cam, width, height, spp, max_bounces, sample_index=0, accum=None, rows_step=1, threads=0):
+    def render(self, cam, width, height, spp, max_bounces, sample_index=0, accum=None, rows_step=1, threads=0,
+               flags=0):
         if accum is None:
             accum = np.zeros((height, width, 4), np.float32)
         cnt = Counts()
-        rc = lib().oracle_render(self._h, C.byref(cam), width, height, sample_index, spp, max_bounces,
-                                 accum.ctypes.data, rows_step, threads, C.byref(cnt))
+        rc = lib().oracle_render_ex(self._h, C.byref(cam), width, height, sample_index, spp, max_bounces,
+                                    accum.ctypes.data, rows_step, threads, C.byref(cnt), flags)
         assert rc == 0, rc
         return accum, {k: getattr(cnt, k) for k, _ in Counts._fields_}
 
@@ -83,6 +93,9 @@ class Oracle:
         out = np.zeros(len(rays), pt_amd.HIT_DTYPE)
         assert lib().oracle_trace_rays(self._h, rays.ctypes.data, len(rays), out.ctypes.data, int(any_hit)) == 0
         return out
+
+    def light_pdf(self, emitter_index, prim):
+        return float(lib().oracle_light_pdf(self._h, emitter_index, prim))
 
     def sample_light(self, point, u3):
         e, d, dist, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), C.c_float(), C.c_float()
@@ -116,6 +129,14 @@ def sample_direction(o, alpha, metalness, u3):
     lib().oracle_sample_direction(a[0].ctypes.data, a[1].ctypes.data, metalness, a[2].ctypes.data,
                                   i.ctypes.data, m.ctypes.data, w.ctypes.data, pdf.ctypes.data)
     return i, m, float(w[0]), float(pdf[0])
+
+
+def eval_brdf_many(params, i, o):
+    """evaluate_brdf of rows of directions i, o (n x 3) with m = normalize(i + o): n x 3."""
+    p, ii, oo = _f32(params), _f32(i), _f32(o)
+    out = np.zeros((len(ii), 3), np.float32)
+    lib().oracle_eval_brdf_many(p.ctypes.data, len(ii), ii.ctypes.data, oo.ctypes.data, out.ctypes.data)
+    return out
 
 
 def camera_ray(cam, px, py, w, h, u_pixel, u_lens):
