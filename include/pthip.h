@@ -28,7 +28,11 @@ extern "C" {
 #define PT_OK 0
 #define PT_EINVAL (-1)  /* bad argument (e.g. spp == 0, mirrors pathtrace.cpp:400-401) */
 #define PT_ENOMEM (-2)  /* device allocation failed */
-#define PT_EDEVICE (-3) /* a HIP runtime call or kernel failed */
+#define PT_EDEVICE (-3) /* a HIP runtime call or kernel failed, or a traversal stack overflowed (the
+                         * rays lost BVH nodes: pt_render, pt_read_*, pt_synchronize and pt_get_stats
+                         * report it until the accumulation restarts with pt_set_camera,
+                         * pt_set_max_bounces, pt_cache_configure or pt_build_light_tables; the
+                         * pt_trace_* call that overflowed reports its own) */
 #define PT_ESTATE (-4)  /* call not valid in the current state (e.g. no camera set) */
 #define PT_ENODEV (-5)  /* no HIP device present */
 
@@ -176,13 +180,17 @@ typedef struct pt_stats {
                                    * every node is one 128-byte line): rebuilds the BVH */
 #define PT_OPT_BVH_BUILDER 10u    /* binary tree under the SAH collapse: 0 PLOC clustering (default),
                                    * 1 Karras radix tree over the Morton codes: rebuilds the BVH */
+#define PT_OPT_SPILL_DEPTH 11u    /* traversal stack entries per lane spilled to global memory beyond the
+                                   * LDS ones (0..112, default 112); a ray whose stack would grow deeper
+                                   * loses nodes, which the calls below report as PT_EDEVICE */
 
 typedef struct pt_ctx pt_ctx;
 
 /* PathTracer::PathTracer + SceneBuffer upload + AccelerationStructure build
  * (src/pathtrace/pathtrace.cpp:81-116, accstruct.cpp:288-296). Builds one BVH per mesh and
- * a TLAS over the instances on the device. width/height (1..65535 each) size the
- * accumulation image (PathTraceDispatcher ctor, src/pathtrace/ptdispatch.cpp:26-27). */
+ * a TLAS over the instances on the device, and the light sampler's alias tables
+ * (pt_build_light_tables) when light_words has the packed layout. width/height (1..65535
+ * each) size the accumulation image (PathTraceDispatcher ctor, src/pathtrace/ptdispatch.cpp:26-27). */
 int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ctx** out);
 
 /* PathTracer::set_camera (src/pathtrace/pathtrace.cpp:381-396). Resets sample_index to 0. */
@@ -341,10 +349,12 @@ int pt_cache_train_batch(pt_ctx* ctx, const float* features, const float* target
 
 /* Device build of the light sampler's alias tables: TableLightSampler's light table and the
  * per-emissive-mesh triangle tables (src/scene/lightsample.cpp:5-178, src/aliastable.h:21-67),
- * recomputed on the device from the scene on the GPU (material emission, point/directional
- * light colours, triangle areas) and written over the words pt_create uploaded; the result is
- * bit-identical to the host packing. Resets sample_index. PT_ESTATE when the words passed to
- * pt_create do not follow that packing. */
+ * recomputed on the device from the scene on the GPU (material emission - for an emission map
+ * the Monte Carlo intensity of calculate_object_intensity, lightsample.cpp:41-87 - point and
+ * directional light colours, triangle areas) and written over the words pt_create uploaded;
+ * the result is bit-identical to the host packing. pt_create runs it; this call rebuilds.
+ * Resets sample_index. PT_ESTATE when the words passed to pt_create do not follow that
+ * packing. */
 int pt_build_light_tables(pt_ctx* ctx);
 /* Copies the light-sampler words (light_word_count of the scene) back to host memory. */
 int pt_read_light_words(pt_ctx* ctx, uint32_t* words, size_t count);

@@ -100,6 +100,7 @@ PT_OPT_LEAF_SIZE = 7
 PT_OPT_SORT_HITS = 8
 PT_OPT_NODE_WIDTH = 9
 PT_OPT_BVH_BUILDER = 10
+PT_OPT_SPILL_DEPTH = 11
 
 
 class TuneResult(C.Structure):

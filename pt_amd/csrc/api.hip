@@ -275,6 +275,18 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     return PT_OK;
 }
 
+// A ray query (pt_trace_*) whose traversal stack overflowed: its hits are not valid.
+int query_overflow(Ctx& c)
+{
+    uint32_t flags = 0;
+    HIP_TRY(&c, hipMemcpy(&flags, c.wf.counters + CTR(9), 4, hipMemcpyDeviceToHost), "error flags");
+    if (flags)
+        return fail(&c, PT_EDEVICE,
+                    "traversal stack overflow: some rays missed BVH nodes (the hierarchy is deeper than the traversal "
+                    "stack of " + std::to_string(c.lds_stack) + " + " + std::to_string(c.spill_depth) + " entries)");
+    return PT_OK;
+}
+
 int need_cache(Ctx& c)
 {
     if (!c.cache.configured)
@@ -369,7 +381,32 @@ hipError_t apply_launch(Ctx& c)
 {
     c.grid_blocks = ((c.cus * c.blocks_per_cu + 7) / 8) * 8;
     c.graphs.destroy();
-    return wavefront_alloc(c.wf, c.pool_slots, c.job_paths, c.grid_blocks * TRACE_BLOCK);
+    hipError_t e = wavefront_alloc(c.wf, c.pool_slots, c.job_paths, c.grid_blocks * TRACE_BLOCK);
+    c.wf.spill_depth = c.spill_depth;
+    return e;
+}
+
+// The error flags of the path stages (traversal stack overflow): a render whose paths lost
+// BVH nodes is reported until the accumulation restarts (a new camera, depth, cache or light
+// tables), as the reference reports its failures by exception (src/pathtrace/ptdispatch.cpp:44-47).
+bool overflowed(Ctx& c, std::string& why)
+{
+    uint32_t flags = 0;
+    if (!c.stream_state.overflow &&
+        hipMemcpy(&flags, c.wf.counters + CTR(3), 4, hipMemcpyDeviceToHost) == hipSuccess && flags)
+        c.stream_state.overflow = true;
+    if (c.stream_state.overflow)
+        why = "traversal stack overflow: the render missed BVH nodes (the hierarchy is deeper than the "
+              "traversal stack of " + std::to_string(c.lds_stack) + " + " + std::to_string(c.spill_depth) +
+              " entries); the image is invalid until the accumulation restarts";
+    return c.stream_state.overflow;
+}
+
+// Clears the error flags when the accumulation restarts.
+hipError_t clear_overflow(Ctx& c)
+{
+    c.stream_state.overflow = false;
+    return hipMemsetAsync(c.wf.counters + CTR(3), 0, 4, c.stream);
 }
 
 }  // namespace
@@ -495,6 +532,19 @@ int settle(Ctx& c)
     return PT_OK;
 }
 
+// settle() for the calls that hand out results: also reports a traversal stack overflow of
+// the renders since the accumulation last restarted.
+int settle_checked(Ctx& c)
+{
+    if (int v = settle(c))
+        return v;
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+    std::string why;
+    if (overflowed(c, why))
+        return fail(&c, PT_EDEVICE, why);
+    return PT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -612,6 +662,11 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              HIP_TRY(&c, hipHostMalloc((void**)&c.h_job, 2 * sizeof(JobDev), hipHostMallocDefault), "job table");
              if (const char* env = getenv("PT_GRAPHS"))
                  c.use_graphs = atoi(env) != 0;
+             // The light sampler's alias tables are built on the device from the uploaded scene
+             // (TableLightSampler, src/scene/lightsample.cpp:5-178), over the words the caller
+             // packed; words of another layout are used as given.
+             if (c.light_layout_known && c.light_count)
+                 HIP_TRY(&c, build_light_tables(c), "light tables");
              HIP_TRY(&c, hipStreamSynchronize(st), "upload");
              return PT_OK;
          }()) != PT_OK)
@@ -633,6 +688,7 @@ int pt_set_camera(pt_ctx* h, const pt_camera* cam)
     h->c.have_camera = true;
     h->c.sample_index = 0;
     h->c.stats_fresh = true;
+    HIP_TRY(&h->c, clear_overflow(h->c), "error flags");
     return PT_OK;
 }
 
@@ -647,6 +703,7 @@ int pt_set_max_bounces(pt_ctx* h, uint32_t max_bounces)
     h->c.max_bounces = max_bounces;
     h->c.sample_index = 0;
     h->c.stats_fresh = true;
+    HIP_TRY(&h->c, clear_overflow(h->c), "error flags");
     return PT_OK;
 }
 
@@ -672,6 +729,11 @@ int pt_render(pt_ctx* h, uint32_t spp)
     HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
     c.sample_index += spp;
     c.last_paths += uint64_t(spp) * c.width * c.height;
+    if (c.stream_state.overflow) {  // seen in the counts the render steered by
+        std::string why;
+        overflowed(c, why);
+        return fail(&c, PT_EDEVICE, why);
+    }
     return PT_OK;
 }
 
@@ -679,7 +741,7 @@ int pt_read_accumulation(pt_ctx* h, float* rgba, size_t row_pitch)
 {
     if (!h)
         return PT_EINVAL;
-    if (int v = settle(h->c))
+    if (int v = settle_checked(h->c))
         return v;
     Ctx& c = h->c;
     if (!rgba || row_pitch < size_t(c.width) * 16)
@@ -717,7 +779,7 @@ int pt_read_tonemapped(pt_ctx* h, uint8_t* rgba8, size_t row_pitch)
 {
     if (!h)
         return PT_EINVAL;
-    if (int v = settle(h->c))
+    if (int v = settle_checked(h->c))
         return v;
     Ctx& c = h->c;
     if (!rgba8 || row_pitch < size_t(c.width) * 4)
@@ -739,6 +801,7 @@ int pt_trace_primary(pt_ctx* h, uint32_t sample_index, pt_hit* hits)
         return fail(&c, PT_ESTATE, "no camera set");
     const size_t n = size_t(c.width) * c.height;
     pt_hit* d = nullptr;
+    HIP_TRY(&c, hipMemsetAsync(c.wf.counters + CTR(9), 0, 4, c.stream), "error flags");
     HIP_TRY(&c, hipMalloc(&d, n * sizeof(pt_hit)), "hits");
     hipError_t e = primary_hits(c, sample_index, d);
     if (e == hipSuccess)
@@ -748,7 +811,7 @@ int pt_trace_primary(pt_ctx* h, uint32_t sample_index, pt_hit* hits)
     (void)hipFree(d);
     if (e != hipSuccess)
         return hip_fail(&c, e, "primary hits");
-    return PT_OK;
+    return query_overflow(c);
 }
 
 int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit)
@@ -764,6 +827,7 @@ int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, i
         return fail(&c, PT_EINVAL, "null ray or hit array");
     pt_ray* dr = nullptr;
     pt_hit* dh = nullptr;
+    HIP_TRY(&c, hipMemsetAsync(c.wf.counters + CTR(9), 0, 4, c.stream), "error flags");
     hipError_t e = hipMalloc(&dr, size_t(count) * sizeof(pt_ray));
     if (e == hipSuccess)
         e = hipMalloc(&dh, size_t(count) * sizeof(pt_hit));
@@ -781,7 +845,7 @@ int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, i
         (void)hipFree(dh);
     if (e != hipSuccess)
         return hip_fail(&c, e, "trace rays");
-    return PT_OK;
+    return query_overflow(c);
 }
 
 int pt_get_stats(pt_ctx* h, pt_stats* out)
@@ -795,9 +859,7 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
         return fail(&c, PT_EINVAL, "out is null");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
-    uint32_t error_flags = 0;
     HIP_TRY(&c, hipMemcpy(totals, c.wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
-    HIP_TRY(&c, hipMemcpy(&error_flags, c.wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "stats");
     float ms = 0.0f;
     if (c.last_paths)
         HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
@@ -832,8 +894,9 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
         s.refill_launches = kl[KERNEL_REFILL];
     }
     *out = s;
-    if (error_flags)
-        return fail(&c, PT_EDEVICE, "traversal stack overflow");
+    std::string why;
+    if (overflowed(c, why))
+        return fail(&c, PT_EDEVICE, why);
     return PT_OK;
 }
 
@@ -890,6 +953,13 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
             return fail(&c, PT_EINVAL, "LDS stack depth must be 8, 16 or 32");
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
         c.lds_stack = uint32_t(value);
+        return PT_OK;
+    case PT_OPT_SPILL_DEPTH:
+        if (value < 0 || value > int64_t(SPILL_STACK))
+            return fail(&c, PT_EINVAL, "spill depth must be 0.." + std::to_string(SPILL_STACK));
+        HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
+        c.spill_depth = c.wf.spill_depth = uint32_t(value);
+        c.graphs.destroy();  // the captured launches carry the Wavefront by value
         return PT_OK;
     case PT_OPT_LEAF_SIZE:
     case PT_OPT_NODE_WIDTH:
@@ -948,6 +1018,7 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
     const uint64_t job_paths = uint64_t(c.wf.rad_out_capacity);
     const uint32_t rec_cap = uint32_t(std::min<uint64_t>(job_paths / cfg->train_stride + 1024, 1u << 24));
     HIP_TRY(&c, cache_init(cs, c.wf.capacity, rec_cap, c.stream), "radiance cache");
+    HIP_TRY(&c, clear_overflow(c), "error flags");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "radiance cache");
     c.sample_index = 0;
     c.stats_fresh = true;
@@ -1131,6 +1202,24 @@ int bsettle(pt_batch* b)
     return PT_OK;
 }
 
+// As settle_checked() for a context: a traversal stack overflow of the batch's renders since a
+// view last restarted is an error of the calls that hand out results.
+int bsettle_checked(pt_batch* b)
+{
+    if (int v = bsettle(b))
+        return v;
+    BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
+    uint32_t flags = 0;
+    BHIP_TRY(b, hipMemcpy(&flags, b->wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "error flags");
+    if (flags || b->stream_state.overflow) {
+        b->stream_state.overflow = true;
+        return bfail(b, PT_EDEVICE,
+                     "traversal stack overflow: the batch render missed BVH nodes; its images are invalid until "
+                     "the views restart (pt_batch_set_view)");
+    }
+    return PT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1242,6 +1331,8 @@ int pt_batch_set_view(pt_batch* b, uint32_t view, const pt_view* v)
     BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
     b->views[view] = *v;
     b->sample_index[view] = 0;
+    b->stream_state.overflow = false;
+    BHIP_TRY(b, hipMemsetAsync(b->wf.counters + CTR(3), 0, 4, b->stream), "error flags");
     view_dev(*b, view, b->h_views[view]);
     BHIP_TRY(b,
              hipMemcpyAsync(b->d_views + view, b->h_views + view, sizeof(ViewDev), hipMemcpyHostToDevice, b->stream),
@@ -1275,6 +1366,8 @@ int pt_batch_render(pt_batch* b)
     for (uint32_t i = 0; i < b->views.size(); i++)
         b->sample_index[i] += b->views[i].spp;
     b->last_paths += paths;
+    if (b->stream_state.overflow)
+        return bfail(b, PT_EDEVICE, "traversal stack overflow: the batch render missed BVH nodes");
     return PT_OK;
 }
 
@@ -1282,7 +1375,7 @@ int pt_batch_read(pt_batch* b, uint32_t view, float* rgba, size_t row_pitch)
 {
     if (!b)
         return PT_EINVAL;
-    if (int v = bsettle(b))
+    if (int v = bsettle_checked(b))
         return v;
     if (view >= b->views.size())
         return bfail(b, PT_EINVAL, "no such view");
@@ -1301,7 +1394,7 @@ int pt_batch_read_tonemapped(pt_batch* b, uint32_t view, uint8_t* rgba8, size_t 
 {
     if (!b)
         return PT_EINVAL;
-    if (int v = bsettle(b))
+    if (int v = bsettle_checked(b))
         return v;
     if (view >= b->views.size())
         return bfail(b, PT_EINVAL, "no such view");
@@ -1326,6 +1419,7 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
     uint32_t error_flags = 0;
     BHIP_TRY(b, hipMemcpy(totals, b->wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
     BHIP_TRY(b, hipMemcpy(&error_flags, b->wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "stats");
+    error_flags |= b->stream_state.overflow ? 1u : 0u;
     float ms = 0.0f;
     if (b->last_paths)
         BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev_start, b->ev_stop), "event");
@@ -1517,6 +1611,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
     c.sample_index = 0;
     c.stats_fresh = true;
     HIP_TRY(&c, hipMemsetAsync(c.accum, 0, npix * 16, c.stream), "accumulation image");
+    HIP_TRY(&c, clear_overflow(c), "error flags");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     if (out) {
         pt_tune_result r{};
@@ -1544,6 +1639,7 @@ int pt_build_light_tables(pt_ctx* h)
         return fail(&c, PT_ESTATE, "the light-sampler words do not have the packed layout");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     HIP_TRY(&c, build_light_tables(c), "light tables");
+    HIP_TRY(&c, clear_overflow(c), "error flags");
     c.sample_index = 0;
     c.stats_fresh = true;
     return PT_OK;
@@ -1568,7 +1664,7 @@ int pt_synchronize(pt_ctx* h)
 {
     if (!h)
         return PT_EINVAL;
-    if (int v = settle(h->c))
+    if (int v = settle_checked(h->c))
         return v;
     HIP_TRY(&h->c, hipStreamSynchronize(h->c.stream), "sync");
     return PT_OK;

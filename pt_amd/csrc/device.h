@@ -47,9 +47,9 @@ struct Wavefront {
     uint2* train_end = nullptr;    // (path, training record) of recorded paths that ended this iteration
     float4* rad_out = nullptr;     // radiance of every path of the job, by path id
     uint32_t rad_out_capacity = 0; // paths
-    // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags, [4] ended
-    // recorded paths, [5],[6] next path to start of job slot 0, 1, [7],[8] ended paths of job
-    // slot 0, 1
+    // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags of the path
+    // stages, [4] ended recorded paths, [5],[6] next path to start of job slot 0, 1, [7],[8]
+    // ended paths of job slot 0, 1, [9] error flags of the ray queries (pt_trace_*)
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
@@ -61,7 +61,8 @@ struct Wavefront {
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     uint32_t spill_threads = 0;
-    uint32_t* snapshot = nullptr;     // per parity 8 words: next queue, ended[2], issued[2]
+    uint32_t spill_depth = SPILL_STACK;  // spill entries per lane in use (PT_OPT_SPILL_DEPTH)
+    uint32_t* snapshot = nullptr;     // per parity 8 words: next queue, ended[2], issued[2], error flags
     uint32_t* host_counts = nullptr;  // pinned copy of snapshot, read back with a one-iteration lag
     hipEvent_t count_ready[2] = {nullptr, nullptr};
 };
@@ -119,6 +120,7 @@ struct StreamState {
     int q = 0;               // parity of the queue the next iteration traces
     uint64_t launched = 0;   // iterations queued so far
     uint32_t seq = 0;
+    bool overflow = false;   // a traversal stack overflowed (seen in a snapshot)
     struct Slot {
         bool busy = false;
         uint32_t paths = 0, seq = 0;
@@ -252,6 +254,7 @@ struct Ctx {
     uint32_t blocks_per_cu = 6;
     uint32_t pool_slots = 0;
     uint32_t lds_stack = LDS_STACK;
+    uint32_t spill_depth = SPILL_STACK;  // PT_OPT_SPILL_DEPTH
     bool transparent = false;  // some instance's material is transparent
     uint32_t leaf_size = 4;
     uint32_t node_width = 4;

@@ -1,10 +1,11 @@
 // Device build of the light sampler's alias tables (src/aliastable.h:21-67,
 // src/scene/lightsample.cpp:5-178): the table over the lights (emitting objects weighted by
-// (r + g + b) * strength of their emission, point and directional lights by r + g + b) and one
-// table per emissive mesh over its triangles (weighted by the object-space cross-product
-// length). Weights are computed in parallel; each table is then built by one thread with the
-// same arithmetic and the same LIFO under/overfull lists as the host build, so the words are
-// bit-identical to the ones the host packs (the bins' payload words are kept).
+// (r + g + b) * strength of their emission - for an emission map the Monte Carlo mean of
+// calculate_object_intensity, lightsample.cpp:41-87 - point and directional lights by
+// r + g + b) and one table per emissive mesh over its triangles (weighted by the object-space
+// cross-product length). Weights are computed in parallel; each table is then built by one
+// thread with the same arithmetic and the same LIFO under/overfull lists as the host build, so
+// the words are bit-identical to the ones the host packs (the bins' payload words are kept).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,8 +26,81 @@ struct TableJob {
     uint32_t weight_offset;  // first weight in the weight buffer
 };
 
+// The random stream calculate_object_intensity draws from: Xshiro128 (src/rng.cpp:37-60)
+// seeded by four Splitmix32(0xbd7a0d48) outputs, which the reference's g++ build passes in
+// reverse order (its argument list is evaluated right to left), uniform floats result / 2^32.
+struct IntensityRng {
+    uint32_t st[4];
+    __device__ IntensityRng()
+    {
+        uint32_t sm = 0xbd7a0d48u, d[4];
+        for (int k = 0; k < 4; k++) {
+            uint32_t z = (sm += 0x9e3779b9u);
+            z = (z ^ (z >> 16)) * 0x85ebca6bu;
+            z = (z ^ (z >> 13)) * 0xc2b2ae35u;
+            d[k] = z ^ (z >> 16);
+        }
+        st[0] = d[3], st[1] = d[2], st[2] = d[1], st[3] = d[0];
+    }
+    __device__ static uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+    __device__ float next()
+    {
+        const uint32_t result = rotl(st[0] + st[3], 7) + st[0];
+        const uint32_t t = st[1] << 9;
+        st[2] ^= st[0];
+        st[3] ^= st[1];
+        st[1] ^= st[2];
+        st[0] ^= st[3];
+        st[2] ^= t;
+        st[3] = rotl(st[3], 11);
+        return float(result) / 4294967296.0f;
+    }
+};
+
+// calculate_object_intensity of a mesh with an emission map (lightsample.cpp:41-87): the mean
+// of r + g + b of the texel nearest to the interpolated UV (raw 8-bit channels / 255, the
+// fraction of the UV rounded to a texel, negative fractions clamped to the last texel) over 20
+// uniform points per triangle, in triangle order, times the strength.
+__device__ float textured_intensity(const float4* vertices, const uint32_t* indices, const pt_mesh_desc& m,
+                                    const pt_texture_desc& tx, const uint8_t* texels, float strength)
+{
+#pragma clang fp contract(off)
+    IntensityRng rng;
+    const uint32_t spt = 20;
+    double total = 0.0;
+    for (uint32_t t = 0; t < m.tri_count; t++) {
+        const uint32_t* ix = indices + m.index_start + 3 * size_t(t);
+        const float4 a = vertices[size_t(m.vertex_start + ix[0]) * 2 + 1];
+        const float4 b = vertices[size_t(m.vertex_start + ix[1]) * 2 + 1];
+        const float4 c = vertices[size_t(m.vertex_start + ix[2]) * 2 + 1];
+        double sum = 0.0;
+        for (uint32_t j = 0; j < spt; j++) {
+            const float beta = 1.0f - sqrtf(rng.next());
+            const float gamma = (1.0f - beta) * rng.next();
+            const float alpha = 1.0f - beta - gamma;
+            const float u = alpha * a.z + beta * b.z + gamma * c.z;
+            const float v = alpha * a.w + beta * b.w + gamma * c.w;
+            float ip;
+            const float fx = roundf(modff(u, &ip) * float(tx.width));
+            const float fy = roundf(modff(v, &ip) * float(tx.height));
+            const uint32_t x = fx < 0.0f ? tx.width - 1 : min(uint32_t(fx), tx.width - 1);
+            const uint32_t y = fy < 0.0f ? tx.height - 1 : min(uint32_t(fy), tx.height - 1);
+            const uint8_t* p = texels + tx.offset + (size_t(y) * tx.width + x) * tx.channels;
+            const float c0 = p[0] / 255.0f;
+            const float c1 = tx.channels >= 2 ? p[1] / 255.0f : 0.0f;
+            const float c2 = tx.channels == 4 ? p[2] / 255.0f : 0.0f;
+            sum += double(c0 + c1 + c2);
+        }
+        total += sum;
+    }
+    return float(total * double(strength) / double(size_t(m.tri_count) * spt));
+}
+
 __global__ void k_light_weights(const uint32_t* words, uint32_t light_count, const InstanceData* instances,
-                                const MaterialRec* materials, float* w)
+                                const MaterialRec* materials, const uint32_t* instance_mesh,
+                                const pt_mesh_desc* meshes, const float4* vertices, const uint32_t* indices,
+                                const pt_texture_desc* textures, uint32_t texture_count, const uint8_t* texels,
+                                float* w)
 {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -34,8 +108,13 @@ __global__ void k_light_weights(const uint32_t* words, uint32_t light_count, con
         return;
     const uint32_t* b = words + size_t(i) * PT_LIGHT_BIN_WORDS;
     if (b[3] == LIGHT_TYPE_EMITTER) {
-        const float4 e = materials[instances[b[6]].material_index].emission;
-        w[i] = (e.x + e.y + e.z) * e.w;
+        const MaterialRec& mat = materials[instances[b[6]].material_index];
+        const float4 e = mat.emission;
+        const uint32_t tex = __float_as_uint(e.x);
+        if ((__float_as_uint(mat.rmnb.w) & MAP_EMISSION) && tex < texture_count)
+            w[i] = textured_intensity(vertices, indices, meshes[instance_mesh[b[6]]], textures[tex], texels, e.w);
+        else
+            w[i] = (e.x + e.y + e.z) * e.w;
     } else {
         w[i] = __uint_as_float(b[7]) + __uint_as_float(b[8]) + __uint_as_float(b[9]);
     }
@@ -176,8 +255,9 @@ hipError_t build_light_tables(Ctx& c)
         e = hipMemcpyAsync(d_table_mesh, table_mesh.data(), table_mesh.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_clear_bins, dim3(64, njobs), dim3(256), 0, st, d_jobs, njobs, c.d_light);
-        hipLaunchKernelGGL(k_light_weights, dim3((c.light_count + 255) / 256), dim3(256), 0, st, c.d_light,
-                           c.light_count, c.d_instances, c.d_materials, d_w);
+        hipLaunchKernelGGL(k_light_weights, dim3((c.light_count + 63) / 64), dim3(64), 0, st, c.d_light,
+                           c.light_count, c.d_instances, c.d_materials, c.d_instance_mesh, c.d_meshes,
+                           c.d_vertices, c.d_indices, c.d_tex, c.scene.tex_count, c.d_texels, d_w);
         if (tri_total)
             hipLaunchKernelGGL(k_tri_weights, dim3((tri_total + 255) / 256), dim3(256), 0, st, c.d_vertices,
                                c.d_indices, c.d_meshes, d_table_mesh, d_jobs, njobs - 1, tri_total, c.d_light, d_w);

@@ -171,6 +171,7 @@ struct StackT {
     uint32_t* lds;    // this lane's column: entry k at lds[k * TRACE_BLOCK]
     uint32_t* spill;  // this thread's spill area: entry k at spill[k * spill_stride]
     uint32_t spill_stride;
+    uint32_t spill_max;  // spill entries in use (at most SPILL_STACK)
     uint32_t sp;
     bool overflow;
 
@@ -178,7 +179,7 @@ struct StackT {
     {
         if (sp >= D) {
             const uint32_t deep = sp - D;
-            if (deep >= SPILL_STACK) {
+            if (deep >= spill_max) {
                 overflow = true;
                 return;
             }
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     const uint32_t count = wf.counters[CTR(q)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     const uint32_t count = wf.counters[CTR(2)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     uint32_t n_nodes = 0, n_tris = 0;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
@@ -1016,6 +1017,7 @@ __global__ void k_refill_commit(const JobDev* j0, const JobDev* j1, Wavefront wf
         snap[2] = wf.counters[CTR(8)];
         snap[3] = r.next[0] + r.n[0];
         snap[4] = r.next[1] + r.n[1];
+        snap[5] = wf.counters[CTR(3)];  // error flags (traversal stack overflow)
     }
 }
 
@@ -1103,7 +1105,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, Cam
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
     const uint32_t n = width * height;
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     for (uint32_t p = gtid; p < n; p += gridDim.x * TRACE_BLOCK) {
         const uint32_t px = p % width, py = p / width;
         Sampler smp;
@@ -1117,7 +1119,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, Cam
         traverse<false>(s, o, d, cam.near_clip, cam.far_clip, stk, h, trans);
         hits[p] = h.inst == NONE ? pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT} : pt_hit{h.t, h.u, h.v, h.inst, h.prim};
         if (stk.overflow)
-            atomicOr(&wf.counters[CTR(3)], 1u);
+            atomicOr(&wf.counters[CTR(9)], 1u);
     }
 }
 
@@ -1126,7 +1128,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene s, const
 {
     __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, 0, false};
+    Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     for (uint32_t p = gtid; p < n; p += gridDim.x * TRACE_BLOCK) {
         const pt_ray r = rays[p];
         HitOut h;
@@ -1139,7 +1141,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene s, const
             found = traverse<false>(s, o, d, r.tmin, r.tmax, stk, h, trans);
         hits[p] = found ? pt_hit{h.t, h.u, h.v, h.inst, h.prim} : pt_hit{0.0f, 0.0f, 0.0f, PT_NO_HIT, PT_NO_HIT};
         if (stk.overflow)
-            atomicOr(&wf.counters[CTR(3)], 1u);
+            atomicOr(&wf.counters[CTR(9)], 1u);
     }
 }
 
@@ -1628,6 +1630,8 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
             S.slot[s].ended = snap[1 + s];
             S.slot[s].issued = snap[3 + s];
         }
+        if (snap[5])
+            S.overflow = true;
         // finished jobs in the order they were loaded
         for (int round = 0; round < 2; round++) {
             int oldest = -1;

@@ -16,7 +16,7 @@
 #include <iostream>
 #include <map>
 #include <optional>
-#include <regex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -25,169 +25,208 @@
 
 namespace {
 
-enum class OptType { STRING, SWITCH, INTEGER, DECIMAL };
+// The option table: every option the reference's CLI accepts, in its help order
+// (src/main.cpp:18-26, with --help first as CommandLineParser's constructor adds it).
+enum class Kind { FLAG, TEXT, INT, REAL };
 
-struct Option {
-    std::string name, short_name;
-    OptType type;
-    std::string description;
-    std::optional<std::string> value;
+struct OptSpec {
+    const char* name;  // "--width"
+    char letter;       // 'w' for -w, 0: none
+    Kind kind;
+    const char* help;
 };
 
-struct Parser {
-    std::vector<Option> opts;
-    std::map<std::string, size_t> by_name, by_short;
+constexpr OptSpec kOptions[] = {
+    {"--help", 0, Kind::FLAG, "Displays help."},
+    {"--width", 'w', Kind::INT, "The width of the rendered image."},
+    {"--height", 'h', Kind::INT, "The height of the rendered image."},
+    {"--headless", 0, Kind::FLAG, "Run without window and user interface."},
+    {"--samples", 's', Kind::INT, "Number of samples to take."},
+    {"--bounces", 'b', Kind::INT, "Maximum number of ray bounces."},
+    {"--out", 'o', Kind::TEXT, "Specifies in which file to place the output."},
+    {"--format", 'f', Kind::TEXT, "The image format of the output file."},
+    {"--time", 't', Kind::REAL, "The maximum rendering time."},
+};
 
-    void add(const std::string& n, const std::string& s, OptType t, const std::string& d)
+const OptSpec* by_name(const std::string& tok)
+{
+    for (const OptSpec& o : kOptions)
+        if (tok == o.name)
+            return &o;
+    return nullptr;
+}
+
+const OptSpec* by_letter(char c)
+{
+    for (const OptSpec& o : kOptions)
+        if (o.letter != 0 && o.letter == c)
+            return &o;
+    return nullptr;
+}
+
+// The command line as scanned: raw option values by long name, then the operands.
+struct CommandLine {
+    std::map<std::string, std::string> values;
+    std::vector<std::string> operands;
+
+    bool has(const char* name) const { return values.count(name) != 0; }
+    std::optional<std::string> text(const char* name) const
     {
-        by_name[n] = opts.size();
-        if (!s.empty())
-            by_short[s] = opts.size();
-        opts.push_back({n, s, t, d, std::nullopt});
+        auto it = values.find(name);
+        return it == values.end() ? std::nullopt : std::optional<std::string>(it->second);
     }
+};
 
-    void parse(int argc, char** argv, std::vector<std::string>& rest)
-    {
-        bool ended = false;
-        for (int i = 1; i < argc; i++) {
-            const std::string tok = argv[i];
-            if (!ended && tok.size() >= 3 && tok[0] == '-' && tok[1] == '-') {
-                if (!by_name.count(tok))
-                    throw std::runtime_error("Unknown option " + tok + ".");
-                Option& o = opts[by_name[tok]];
-                if (o.type == OptType::SWITCH) {
-                    o.value = "true";
-                } else {
-                    if (++i >= argc)
-                        throw std::runtime_error("Option " + tok + " has no value.");
-                    o.value = std::string(argv[i]);
-                }
-            } else if (!ended && tok.size() >= 2 && tok[0] == '-' && tok[1] != '-') {
-                const std::string s = tok.substr(0, 2);
-                if (!by_short.count(s))
-                    throw std::runtime_error("Unknown option " + s + ".");
-                Option& o = opts[by_short[s]];
-                if (o.type == OptType::SWITCH) {
-                    if (tok.size() != 2)
-                        throw std::runtime_error("Switch option " + s + " has a value.");
-                    o.value = "true";
-                } else if (tok.size() == 2) {
-                    if (++i >= argc)
-                        throw std::runtime_error("Option " + tok + " has no value.");
-                    o.value = std::string(argv[i]);
-                } else {
-                    o.value = tok.substr(2);
-                }
-            } else if (tok == "--") {
-                ended = true;
+// Scans argv once, left to right. A token is one of: "--" (every later token is an operand),
+// "--name" (a long option, its value the next token unless it is a flag), "-x..." (a short
+// option: the value attached, else the next token; a flag takes nothing), or an operand. The
+// failures are the reference's messages (src/argparse.cpp:61-105).
+CommandLine scan(int argc, char** argv)
+{
+    CommandLine cl;
+    bool operands_only = false;
+    int i = 1;
+    auto next_value = [&](const std::string& shown) -> std::string {
+        if (i + 1 >= argc)
+            throw std::runtime_error("Option " + shown + " has no value.");
+        return argv[++i];
+    };
+    for (; i < argc; i++) {
+        const std::string tok = argv[i];
+        const bool dashed = !operands_only && tok.size() >= 2 && tok[0] == '-';
+        if (tok == "--") {
+            operands_only = true;
+        } else if (dashed && tok[1] == '-') {
+            const OptSpec* o = by_name(tok);
+            if (!o)
+                throw std::runtime_error("Unknown option " + tok + ".");
+            cl.values[o->name] = o->kind == Kind::FLAG ? std::string("true") : next_value(tok);
+        } else if (dashed) {
+            const std::string head = tok.substr(0, 2);
+            const OptSpec* o = by_letter(tok[1]);
+            if (!o)
+                throw std::runtime_error("Unknown option " + head + ".");
+            const std::string attached = tok.substr(2);
+            if (o->kind == Kind::FLAG) {
+                if (!attached.empty())
+                    throw std::runtime_error("Switch option " + head + " has a value.");
+                cl.values[o->name] = "true";
             } else {
-                rest.push_back(tok);
+                cl.values[o->name] = attached.empty() ? next_value(tok) : attached;
             }
+        } else {
+            cl.operands.push_back(tok);
         }
     }
+    return cl;
+}
 
-    const Option& get(const std::string& n) const { return opts.at(by_name.at(n)); }
-    bool flag(const std::string& n) const { return get(n).value == std::string("true"); }
-    std::optional<std::string> str(const std::string& n) const { return get(n).value; }
-    std::optional<int> integer(const std::string& n) const
-    {
-        const Option& o = get(n);
-        if (!o.value)
-            return std::nullopt;
-        if (!std::regex_match(*o.value, std::regex("[\\-\\+]?[0-9]+")))
-            throw std::runtime_error("The value of option " + n + " was not an integer: " + *o.value);
-        return std::stoi(*o.value);
-    }
-    std::optional<float> decimal(const std::string& n) const
-    {
-        const Option& o = get(n);
-        if (!o.value)
-            return std::nullopt;
-        if (!std::regex_match(*o.value, std::regex("[\\-\\+]?[0-9]+(\\.[0-9]*)?")))
-            throw std::runtime_error("The value of option " + n + " was not a decimal number: " + *o.value);
-        return std::stof(*o.value);
-    }
-    std::string help() const
-    {
-        std::string h = "Options:";
-        for (const Option& o : opts)
-            h += "\n  " + o.name + (o.short_name.empty() ? "" : ", " + o.short_name) + "\t" + o.description;
-        return h;
-    }
-};
-
-void print_help(const char* prog, const Parser& p)
+// Number syntax the reference accepts: an optional sign, digits, and for decimals an optional
+// '.' with further digits (src/argparse.cpp:22-32).
+bool number_syntax(const std::string& v, bool fraction)
 {
-    std::cout << "Usage: " << prog << " [options] path-to-scene\n" << p.help() << std::endl;
+    size_t k = (!v.empty() && (v[0] == '+' || v[0] == '-')) ? 1 : 0;
+    const size_t digits_at = k;
+    while (k < v.size() && v[k] >= '0' && v[k] <= '9')
+        k++;
+    if (k == digits_at)
+        return false;
+    if (fraction && k < v.size() && v[k] == '.') {
+        k++;
+        while (k < v.size() && v[k] >= '0' && v[k] <= '9')
+            k++;
+    }
+    return k == v.size();
+}
+
+std::optional<int> int_value(const CommandLine& cl, const char* name)
+{
+    const auto v = cl.text(name);
+    if (!v)
+        return std::nullopt;
+    if (!number_syntax(*v, false))
+        throw std::runtime_error(std::string("The value of option ") + name + " was not an integer: " + *v);
+    return std::stoi(*v);
+}
+
+std::optional<float> real_value(const CommandLine& cl, const char* name)
+{
+    const auto v = cl.text(name);
+    if (!v)
+        return std::nullopt;
+    if (!number_syntax(*v, true))
+        throw std::runtime_error(std::string("The value of option ") + name + " was not a decimal number: " + *v);
+    return std::stof(*v);
+}
+
+void print_help(const char* prog)
+{
+    std::cout << "Usage: " << prog << " [options] path-to-scene\nOptions:";
+    for (const OptSpec& o : kOptions) {
+        std::cout << "\n  " << o.name;
+        if (o.letter)
+            std::cout << ", -" << o.letter;
+        std::cout << "\t" << o.help;
+    }
+    std::cout << std::endl;
 }
 
 }  // namespace
 
 int main(int argc, char** argv)
 {
-    Parser parser;
-    parser.add("--help", "", OptType::SWITCH, "Displays help.");
-    parser.add("--width", "-w", OptType::INTEGER, "The width of the rendered image.");
-    parser.add("--height", "-h", OptType::INTEGER, "The height of the rendered image.");
-    parser.add("--headless", "", OptType::SWITCH, "Run without window and user interface.");
-    parser.add("--samples", "-s", OptType::INTEGER, "Number of samples to take.");
-    parser.add("--bounces", "-b", OptType::INTEGER, "Maximum number of ray bounces.");
-    parser.add("--out", "-o", OptType::STRING, "Specifies in which file to place the output.");
-    parser.add("--format", "-f", OptType::STRING, "The image format of the output file.");
-    parser.add("--time", "-t", OptType::DECIMAL, "The maximum rendering time.");
-
-    std::vector<std::string> args;
+    CommandLine cl;
     int samples, bounces, width, height;
     std::optional<std::string> out;
     std::string format;
     bool headless;
     try {
-        parser.parse(argc, argv, args);
-        if (parser.flag("--help")) {
-            print_help(argv[0], parser);
+        cl = scan(argc, argv);
+        if (cl.has("--help")) {
+            print_help(argv[0]);
             return 0;
         }
-        if (args.empty()) {
+        if (cl.operands.empty()) {
             std::cerr << "Error: Expected path to scene." << std::endl;
-            print_help(argv[0], parser);
+            print_help(argv[0]);
             return 1;
         }
-        if (args.size() > 1) {
+        if (cl.operands.size() > 1) {
             std::cerr << "Error: Unexpected number of arguments." << std::endl;
-            print_help(argv[0], parser);
+            print_help(argv[0]);
             return 1;
         }
-        headless = parser.flag("--headless");
-        samples = parser.integer("--samples").value_or(32);
+        headless = cl.has("--headless");
+        samples = int_value(cl, "--samples").value_or(32);
         if (samples < 1) {
             std::cerr << "Error: Invalid sample count: " << samples << "." << std::endl;
             return 1;
         }
-        bounces = parser.integer("--bounces").value_or(2);
+        bounces = int_value(cl, "--bounces").value_or(2);
         if (bounces < 0) {
             std::cerr << "Error: Invalid number of ray bounces: " << bounces << "." << std::endl;
             return 1;
         }
-        out = parser.str("--out");
+        out = cl.text("--out");
         if (headless && !out) {
             std::cerr << "Error: No output file was given (required for headless execution)." << std::endl;
             return 1;
         }
-        width = parser.integer("--width").value_or(400);
-        height = parser.integer("--height").value_or(400);
+        width = int_value(cl, "--width").value_or(400);
+        height = int_value(cl, "--height").value_or(400);
         if (width < 1 || height < 1) {
             std::cerr << "Error: Invalid dimensions: (" << width << ", " << height << ")." << std::endl;
             return 1;
         }
-        format = parser.str("--format").value_or("png");
+        format = cl.text("--format").value_or("png");
         if (format != "png" && format != "PNG" && format != "hdr" && format != "HDR") {
             std::cerr << "Error: Output format is invalid: " << format << std::endl;
             return 1;
         }
-        (void)parser.decimal("--time");  // parsed, unused (as in the reference)
+        (void)real_value(cl, "--time");  // checked, unused (as in the reference)
     } catch (const std::exception& e) {
         std::cerr << "Error: " << e.what() << std::endl;
-        print_help(argv[0], parser);
+        print_help(argv[0]);
         return 1;
     }
     if (!headless) {
@@ -197,7 +236,7 @@ int main(int argc, char** argv)
     const bool png = format == "png" || format == "PNG";
 
     pth_scene* scene = nullptr;
-    if (pth_scene_load(args[0].c_str(), &scene) != PT_OK) {
+    if (pth_scene_load(cl.operands[0].c_str(), &scene) != PT_OK) {
         std::cerr << "Error: " << pth_last_error() << std::endl;
         return 1;
     }

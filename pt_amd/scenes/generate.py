@@ -368,8 +368,9 @@ MESH_TRIS = {"grid": 2 * 512 * 512, "soup": 200000, "ico6": 81920, "ico4": 5120,
 
 def counts_for_triangles(total):
     """Instance counts of the five mesh kinds whose triangles (with the 20 emissive
-    level-2 spheres) add up to about `total`: the default scene's mix of kinds, scaled, and
-    the remainder filled with the largest spheres that fit."""
+    level-2 spheres) add up to at least `total` and less than 320 more: the default scene's mix
+    of kinds, scaled, the remainder filled with the largest spheres that fit and one more
+    level-2 sphere for what is left."""
     default_total = sum(INSTANCED_COUNTS[k] * MESH_TRIS[k] for k in INSTANCED_COUNTS)
     left = max(0, int(total) - INSTANCED_EMITTERS * MESH_TRIS["ico2"])
     counts = {}
@@ -382,6 +383,8 @@ def counts_for_triangles(total):
         n = rem // MESH_TRIS[kind]
         counts[kind] += n
         rem -= n * MESH_TRIS[kind]
+    if rem > 0:
+        counts["ico2"] += 1
     counts["ico2"] = max(1, counts["ico2"])
     return counts
 

@@ -1,4 +1,6 @@
 """Launch-configuration options and the autotuner: the image must not depend on them."""
+import os
+
 import numpy as np
 import pytest
 
@@ -96,12 +98,39 @@ def test_autotune_picks_a_configuration(inst):
     t.close()
 
 
-@pytest.mark.parametrize("kind", ["box", "inst"])
-def test_device_light_tables_equal_host_packing(kind, box_scn, scene_dir):
+def _mixed_scene(tmp, box_scn, scene_dir):
+    """The box and the textured scene in one .scn, with a point and a directional light: plain and
+    textured emitters of different meshes and all three light types in one table."""
+    import shutil
     from pt_amd.scenes import generate
-    scn = box_scn if kind == "box" else generate.cached_scene("instanced", root=scene_dir)
+    tex_scn = generate.cached_scene("textured", root=scene_dir)
+    for src in (os.path.dirname(box_scn), os.path.dirname(tex_scn)):
+        for f in os.listdir(src):
+            if not f.endswith(".scn") and not f.startswith("."):
+                shutil.copy(os.path.join(src, f), tmp)
+    path = os.path.join(tmp, "mixed.scn")
+    with open(path, "w") as f:
+        f.write("camera:\n position 0 1 -3\n lookdir 0 0 1\n fov 60\n envcol 0.375 0.45 0.9\n\n"
+                "objects:\n box box.gltf\n tex textured.gltf\n plight 0.3 -0.6 0.2 , 1 0.8 0.6\n"
+                " dlight 0.4 -1 0.3 , 0.3 0.3 0.35\n\n box\n tex\n  t 3 0 0\n")
+    return path
+
+
+@pytest.mark.parametrize("kind", ["box", "inst", "textured", "mixed"])
+def test_device_light_tables_equal_host_packing(kind, box_scn, scene_dir, tmp_path):
+    """pt_create builds the light sampler's alias tables on the device (textured emitters
+    weighted by the Monte Carlo intensity of their emission map); the words equal the host
+    packing bit for bit, and so does an explicit rebuild."""
+    from pt_amd.scenes import generate
+    scn = {"box": lambda: box_scn, "inst": lambda: generate.cached_scene("instanced", root=scene_dir),
+           "textured": lambda: generate.cached_scene("textured", root=scene_dir),
+           "mixed": lambda: _mixed_scene(str(tmp_path), box_scn, scene_dir)}[kind]()
     scene = pt_amd.Scene(scn)
-    host = scene.arrays()["light_words"].view(np.uint32).copy()
+    arr = scene.arrays()
+    host = arr["light_words"].view(np.uint32).copy()
+    if kind == "mixed":
+        types = set(host[:arr["light_count"] * 10].reshape(-1, 10)[:, 3].tolist())
+        assert types == {1, 2, 3} and arr["light_count"] == 4
     t = pt_amd.Tracer(scene, 32, 32, max_bounces=4)
     np.testing.assert_array_equal(t.light_words(), host)
     t.build_light_tables()  # clears p, q, alias and the triangle payloads, then rebuilds them

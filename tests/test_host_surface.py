@@ -172,6 +172,15 @@ PT = os.path.join(ROOT, "pt_amd", "bin", "pt")
     (["--nope", "a.scn"], 1, "Unknown option --nope."),
     (["-s", "x", "a.scn"], 1, "was not an integer: x"),
     (["--help"], 0, "Usage:"),
+    (["a.scn", "--width"], 1, "Option --width has no value."),
+    (["a.scn", "-o"], 1, "Option -o has no value."),
+    (["-x", "a.scn"], 1, "Unknown option -x."),
+    (["-qfoo", "a.scn"], 1, "Unknown option -q."),
+    (["-t", "1.5x", "a.scn"], 1, "was not a decimal number: 1.5x"),
+    (["-s", "+", "a.scn"], 1, "was not an integer: +"),
+    (["-w64", "-h0", "a.scn"], 1, "Invalid dimensions: (64, 0)"),
+    (["--", "-w", "a.scn"], 1, "Unexpected number of arguments"),
+    (["-", "a.scn"], 1, "Unexpected number of arguments"),
 ])
 def test_cli_argument_errors(args, code, msg):
     if not os.path.exists(PT):
@@ -179,6 +188,17 @@ def test_cli_argument_errors(args, code, msg):
     r = subprocess.run([PT] + args, capture_output=True, text=True, timeout=60)
     assert r.returncode == code
     assert msg in r.stdout + r.stderr
+
+
+def test_cli_help_lists_the_options_in_order():
+    if not os.path.exists(PT):
+        pytest.skip("pt CLI not built")
+    r = subprocess.run([PT, "--help"], capture_output=True, text=True, timeout=60)
+    lines = r.stdout.splitlines()
+    assert lines[0].endswith(" [options] path-to-scene") and lines[1] == "Options:"
+    assert [ln.split("\t")[0].strip() for ln in lines[2:]] == [
+        "--help", "--width, -w", "--height, -h", "--headless", "--samples, -s", "--bounces, -b", "--out, -o",
+        "--format, -f", "--time, -t"]
 
 
 # --- C ABI ---------------------------------------------------------------------------------
