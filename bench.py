@@ -11,20 +11,26 @@ reference's headless `-s 64` run renders it). Inputs are resident in HBM before 
 region (scene, BVH, camera); the timed region is K back-to-back pt_render(64) calls
 bracketed by stream synchronisation.
 
-Other workloads (--workload views | scenes): 64 views of the box scene, or 16 instanced
-scenes of 10k-20M triangles with 4 views each, traced in one persistent wavefront run
-(pt_batch_*); a step is one render of every view.
+Other workloads (--workload views | scenes | cache): 64 views of the instanced scene (or, with
+--scene box, the box scene) in one persistent wavefront run; 16 instanced scenes of 10k-20M
+triangles with 4 views each in one run (pt_batch_*; a step is one render of every view); the
+learned radiance cache on a 64-frame camera path at 1-4 spp per frame (a step is the path).
 
 The JSON line of the default workload also carries:
-  roofline      the dominant stage's algorithmic bytes per launch / its average HIP-event
-                duration, against the 8 TB/s HBM peak. Bytes per ray: 32 (ray) + 20 (hit)
-                + 128 x node fetches + 48 x triangle-record fetches (extension) and
-                32 + 4 + 128 x nodes + 48 x tris (shadow); the fetch counts come from an
-                instrumented replay of the same sample indices.
-  convergence   seconds of rendering until the relative MSE against a 4096-spp plain
-                reference is <= 1e-3, for the plain estimator (16-spp render calls) and for the
-                learned-cache estimator (16-spp render calls, the network trained in flight
-                between wavefront iterations, paths ended at their second vertex with its
+  roofline      the dominant stage (k_extend, closest-hit traversal) against the bound that
+                holds for it, VALU instruction issue: its VALU wave-instructions per launch
+                (the per-ray count of the committed rocprofv3 summary x the launch's rays) over
+                the launch's HIP-event mean time, against 1024 SIMDs x 2.4 GHz / 4 cycles.
+                Beside it: the measured HBM traffic per launch (rocprofv3 FETCH_SIZE +
+                WRITE_SIZE, calibrated on a copy of known size) against 8 TB/s, and the demand
+                bytes of the per-ray model (32 ray + 20 hit + 128 per node line + 48 per
+                triangle record, fetch counts from an instrumented replay of the same sample
+                indices), which the caches serve and which therefore exceed the HBM peak.
+  convergence   per scene (instanced and box, 1920x1080, depth 8): seconds of rendering until
+                the relative MSE against a 4096-spp plain reference of disjoint sample indices
+                (samples 4096..8191) is <= 1e-3, for the plain estimator (16-spp render calls)
+                and the learned-cache estimator (16-spp render calls, the network trained in
+                flight between wavefront iterations, paths ended at their second vertex with its
                 prediction; a 16-spp warm-up counts towards its time). Error evaluation is
                 outside the clock.
   cpu_baseline  the CPU oracle (this project's C++ restatement of the reference shaders,
@@ -43,6 +49,7 @@ sys.path.insert(0, ROOT)
 
 import pt_amd  # noqa: E402
 from pt_amd.scenes import generate  # noqa: E402
+from pt_amd.scenes import workloads as wl  # noqa: E402
 
 W, H, DEPTH = 1920, 1080, 8
 HBM_PEAK_GBS = 8000.0
@@ -77,6 +84,23 @@ def cpu_baseline(scene, cam, threads, target_s=12.0):
                       % (frames, paths, rays, dt)}
 
 
+def reference_image(tr, cam, spp=4096, chunk=512):
+    """The plain estimator's mean over sample indices [spp, 2 spp) of every pixel: 2 A(2 spp) -
+    A(spp) of the running means after spp and 2 spp samples. The timed runs use indices from 0,
+    so the reference shares no sample with them."""
+    tr.set_camera(cam)
+    for _ in range(spp // chunk):
+        tr.render(chunk)
+    a1 = tr.read()[..., :3].astype(np.float64)
+    for _ in range(spp // chunk):
+        tr.render(chunk)
+    a2 = tr.read()[..., :3].astype(np.float64)
+    out = np.zeros(a1.shape[:2] + (4,), np.float32)
+    out[..., :3] = 2.0 * a2 - a1
+    out[..., 3] = 1.0
+    return out
+
+
 def time_to_threshold(tr, cam, ref, chunk, limit_spp, limit_s, threshold, before=None):
     """Renders chunk-spp batches until the relative MSE against ref is <= threshold; returns
     (seconds of rendering incl. before(), spp, final rel-MSE, trajectory). Readback and the
@@ -104,26 +128,25 @@ def time_to_threshold(tr, cam, ref, chunk, limit_spp, limit_s, threshold, before
     return el, done, e, traj
 
 
+# The learned-cache estimator of the convergence runs: in-flight training (three Adam steps of
+# 8192 records after every wavefront iteration), one path in 16 trains; the others end at their
+# second vertex with the prediction. Learning rate 3e-3: the best of a sweep over 1e-3..1e-2,
+# 2..4 steps, batches of 8K..16K records, strides 4..16 and continuation probabilities 0..0.75
+# on the instanced scene.
+CACHE_CONVERGENCE = dict(query=True, train_stride=16, train_steps=3, batch=8192, learning_rate=3e-3, frame_spp=0,
+                         continue_prob=0.0, seed=1)
+
+
 def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
-    """Seconds to rel-MSE <= threshold against a ref_spp plain reference, plain vs cached."""
+    """Seconds to rel-MSE <= threshold against a ref_spp plain reference of disjoint sample
+    indices, plain vs cached."""
     t = time.perf_counter()
-    tr.set_camera(cam)
-    for _ in range(ref_spp // 512):
-        tr.render(512)
-    tr.synchronize()
+    ref = reference_image(tr, cam, ref_spp)
     ref_s = time.perf_counter() - t
-    ref = tr.read()
-    log("convergence: %d-spp reference in %.1f s" % (ref_spp, ref_s))
+    log("convergence: %d-spp reference (samples %d..%d) in %.1f s" % (ref_spp, ref_spp, 2 * ref_spp - 1, ref_s))
     plain_s, plain_spp, plain_e, plain_traj = time_to_threshold(tr, cam, ref, 16, 4096, 60.0, threshold)
     log("convergence: plain %.3f s, %d spp, rel-MSE %.3e" % (plain_s, plain_spp, plain_e))
-    # in-flight training (three Adam steps of 8192 records after every wavefront iteration), one
-    # path in 16 trains; the others end at their second vertex with the prediction. Learning
-    # rate 3e-3: the best of a sweep over 1e-3..1e-2, 2..4 steps, batches of 8K..16K records,
-    # strides 4..16 and continuation probabilities 0..0.75 on this scene. Three steps rather
-    # than two since the autotuner's 16M-slot pool halves the iterations per sample (at 16M
-    # slots two steps gave 0.98x the plain tracer's time, three and four 0.83x).
-    cfg = dict(query=True, train_stride=16, train_steps=3, batch=8192, learning_rate=3e-3, frame_spp=0,
-               continue_prob=0.0, seed=1)
+    cfg = dict(CACHE_CONVERGENCE)
     tr.cache_configure(**cfg)
     warm = 16
     cached_s, cached_spp, cached_e, cached_traj = time_to_threshold(
@@ -131,7 +154,8 @@ def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
     cs = tr.cache_stats()
     tr.cache_configure(enable=False)
     log("convergence: cached %.3f s, %d spp, rel-MSE %.3e" % (cached_s, cached_spp, cached_e))
-    return {"threshold": threshold, "reference_spp": ref_spp, "reference_s": round(ref_s, 3),
+    return {"threshold": threshold, "reference_spp": ref_spp, "reference_samples": [ref_spp, 2 * ref_spp - 1],
+            "reference_s": round(ref_s, 3),
             "plain_s": round(plain_s, 4), "plain_spp": plain_spp, "plain_relmse": plain_e,
             "cached_s": round(cached_s, 4), "cached_spp": cached_spp, "cached_relmse": cached_e,
             "cached_over_plain_time": round(cached_s / plain_s, 3) if plain_s > 0 else None,
@@ -140,17 +164,50 @@ def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
 
 
 def profile_figure(name, kernel):
-    """A per-launch figure of a stage from the committed rocprofv3 PMC summaries, if any:
-    traffic_bytes_per_launch (HBM bytes, scaled), l2_hit_pct, valu_busy_pct."""
-    for fn in ("k_extend_instanced_1080p_pmc_traffic.txt", "k_extend_instanced_1080p_pmc_l2_valu.txt"):
-        path = os.path.join(ROOT, "profiles", fn)
-        if not os.path.exists(path):
+    """A figure of a kernel from the committed rocprofv3 summaries (machine-readable lines
+    `name kernel value` in profiles/*.txt), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith(".txt"):
             continue
-        for line in open(path):
+        for line in open(os.path.join(d, fn)):
             f = line.split()
             if len(f) == 3 and f[0] == name and f[1] == kernel:
                 return float(f[2])
     return None
+
+
+PEAK_VALU_GINST = 1024 * 2.4 / 4.0  # 1024 SIMDs x 2.4 GHz, a wave64 VALU instruction per 4 cycles
+
+
+def stage_roofline(kernel, d):
+    """The roofline of a traversal stage by the bound that holds. Its rays visit BVH node lines
+    that the caches serve (L1/L2/MALL), so its demand bytes can exceed the HBM peak while the
+    measured HBM traffic is a few percent of it; what limits it is VALU instruction issue along
+    each wave's dependent node chain. achieved = VALU wave-instructions per launch (per-ray count
+    of the committed rocprofv3 summary x the launch's rays) / the launch's HIP-event mean time,
+    against 1024 SIMDs x 2.4 GHz / 4 cycles; the HBM figures and the demand bytes are beside it."""
+    launches = max(1, d["launches"])
+    avg_s = d["ms"] / 1e3 / launches
+    rays = d["rays"] / launches
+    inst_per_ray = profile_figure("valu_inst_per_ray", kernel)
+    hbm_per_ray = profile_figure("hbm_bytes_per_ray", kernel)
+    achieved = inst_per_ray * rays / avg_s / 1e9 if inst_per_ray and avg_s > 0 else None
+    traffic = hbm_per_ray * rays if hbm_per_ray else None
+    demand = d["bytes"] / launches
+    return {"bound": "valu", "achieved": round(achieved, 2) if achieved else None, "peak": PEAK_VALU_GINST,
+            "unit": "G wave-instructions/s", "frac": round(achieved / PEAK_VALU_GINST, 4) if achieved else None,
+            "traffic": round(traffic) if traffic else None,
+            "kernel": kernel, "launches": d["launches"], "avg_launch_ms": round(avg_s * 1e3, 4),
+            "rays_per_launch": round(rays), "valu_inst_per_ray": inst_per_ray,
+            "lane_utilisation_pct": profile_figure("valu_lane_util_pct", kernel),
+            "hbm": {"achieved": round(traffic / avg_s / 1e9, 1) if traffic else None, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
+            "demand_bytes_per_launch": round(demand),
+            "demand_gbs_cache_served": round(demand / avg_s / 1e9, 1) if avg_s > 0 else None,
+            "l2_hit_pct": profile_figure("l2_hit_pct", kernel)}
 
 
 def run_instanced(a):
@@ -231,35 +288,30 @@ def run_instanced(a):
     log("profiled pass: stages extend %.1f ms, shade %.1f ms, shadow %.1f ms, refill %.1f ms (render %.1f ms)"
         % (sp["extend_ms"], sp["shade_ms"], sp["shadow_ms"], sp["refill_ms"], sp["render_ms"]))
 
-    # the 8-bounce shade stage reads per hit: hit 20 + path state 32 + instance 128 + material
-    # 96 + indices 12 + 3 vertices 96 + light bins 64 + emitter vertices 36, writes next ray
-    # 36 + path state 32 + shadow record 64.
+    # Demand bytes (the cache-independent per-ray model): extension ray 32 in + 20 out + 128 per
+    # node line + 48 per triangle record; shadow ray 32 + 4 + the same. The 8-bounce shade stage
+    # reads per hit: hit 20 + path state 32 + instance 128 + material 96 + indices 12 + 3 vertices
+    # 96 + light bins 64 + emitter vertices 36, writes next ray 36 + path state 32 + shadow
+    # record 64.
     stages = {
-        "extend": {"ms": sp["extend_ms"], "launches": sp["extend_launches"],
+        "extend": {"ms": sp["extend_ms"], "launches": sp["extend_launches"], "rays": sp["extension_rays"],
                    "bytes": sp["extension_rays"] * bytes_ext},
-        "shadow": {"ms": sp["shadow_ms"], "launches": sp["shadow_launches"],
+        "shadow": {"ms": sp["shadow_ms"], "launches": sp["shadow_launches"], "rays": sp["shadow_rays"],
                    "bytes": sp["shadow_rays"] * bytes_sh},
-        "shade": {"ms": sp["shade_ms"], "launches": sp["shade_launches"],
+        "shade": {"ms": sp["shade_ms"], "launches": sp["shade_launches"], "rays": sp["extension_rays"],
                   "bytes": sp["extension_rays"] * 616},
     }
     dom = max(stages, key=lambda k: stages[k]["ms"])
-    d = stages[dom]
-    achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
-    traffic = profile_figure("traffic_bytes_per_launch", "k_" + dom)
-    avg_ms = d["ms"] / max(1, d["launches"])
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_" + dom, "launches": d["launches"],
-                "bytes_per_launch": d["bytes"] / max(1, d["launches"]), "avg_launch_ms": avg_ms,
-                # measured HBM traffic over the same launch time (the algorithmic node bytes are
-                # mostly served by L2 and the Infinity Cache, so they can exceed the HBM peak)
-                "traffic_gbs": round(traffic / (avg_ms / 1e3) / 1e9, 1) if traffic and avg_ms > 0 else None,
-                "l2_hit_pct": profile_figure("l2_hit_pct", "k_" + dom),
-                "valu_busy_pct": profile_figure("valu_busy_pct", "k_" + dom)}
+    roofline = stage_roofline("k_" + dom, stages[dom])
 
     conv = None
     if not a.no_convergence:
-        conv = convergence(tr, cam)
+        conv = {"instanced": convergence(tr, cam)}
+        tr.close()
+        box = pt_amd.Scene(generate.cached_scene("box", root=a.scene_dir, seed=1))
+        tb = pt_amd.Tracer(box, W, H, max_bounces=DEPTH)
+        conv["box"] = convergence(tb, box.camera(W, H))
+        tb.close()
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -288,13 +340,6 @@ def run_instanced(a):
     }
 
 
-def _random_dir(rng):
-    z = rng.uniform(-1.0, 1.0)
-    phi = rng.uniform(0.0, 2 * np.pi)
-    r = (1 - z * z) ** 0.5
-    return (r * np.cos(phi), r * np.sin(phi), z)
-
-
 def _timed_batch(batch, steps, warmup):
     for _ in range(warmup):
         batch.render()
@@ -308,36 +353,28 @@ def _timed_batch(batch, steps, warmup):
 
 
 def run_views(a):
-    """64 views of the box scene in one persistent wavefront run: resolution log-uniform in
-    128^2..1024^2 pixels, aspect U(0.5, 2), spp log-uniform in 4..256, eye uniform in the box,
-    direction uniform on the sphere, depth 8. A step is one pt_batch_render of all views."""
-    from pt_amd.scenes.rng import Xshiro128
-    scene = pt_amd.Scene(generate.cached_scene("box", root=a.scene_dir, seed=1))
+    """64 views of one scene in one persistent wavefront run (pt_amd/scenes/workloads.py
+    views_batch): resolution log-uniform in 128^2..1024^2 pixels, aspect U(0.5, 2), spp
+    log-uniform in 4..256, eye uniform in the scene box, direction uniform on the sphere,
+    depth 8. The instanced scene by default (its box U(-50, 50)^3), the box scene with
+    --scene box. A step is one pt_batch_render of all views."""
+    if a.scene == "box":
+        scene = pt_amd.Scene(generate.cached_scene("box", root=a.scene_dir, seed=1))
+        views = wl.views_batch(box=((-1.0, -2.0, -1.0), (1.0, 0.0, 1.0)), focus_dist=1.0, lens_radius=0.01)
+    else:
+        scene = pt_amd.Scene(generate.cached_scene("instanced", root=a.scene_dir))
+        views = wl.views_batch()
     tr = pt_amd.Tracer(scene, 64, 64)
-    rng = Xshiro128.seeded(3)
-    views = []
-    for _ in range(64):
-        area = rng.log_uniform(128.0 ** 2, 1024.0 ** 2)
-        aspect = rng.uniform(0.5, 2.0)
-        w = max(1, int(round((area * aspect) ** 0.5)))
-        h = max(1, int(round(area / w)))
-        spp = int(round(rng.log_uniform(4.0, 256.0)))
-        eye = (rng.uniform(-1, 1), rng.uniform(-2, 0), rng.uniform(-1, 1))
-        fwd = _random_dir(rng)
-        # the reference's camera looks along -lookdir from -position (src/scene/camera.h:16)
-        cam = pt_amd.camera_block([-e for e in eye], [-f for f in fwd], fov_deg=60.0, aspect=w / h,
-                                  focus_dist=1.0, lens_radius=0.01)
-        views.append((0, w, h, spp, DEPTH, cam))
-    paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
-    log("views: 64 views, %d pixel-samples per step" % paths)
-    batch = pt_amd.Batch([tr], views)
+    paths = sum(w * h * spp for w, h, spp, _, _ in views)
+    log("views: 64 views of the %s scene, %d pixel-samples per step" % (a.scene, paths))
+    batch = pt_amd.Batch([tr], [(0,) + v for v in views])
     el, st = _timed_batch(batch, a.steps, a.warmup)
     rays = st["extension_rays"] + st["shadow_rays"]
     batch.close()
     # the same views one after another, each its own one-view batch
     seq = 0.0
     for v in views:
-        b = pt_amd.Batch([tr], [v])
+        b = pt_amd.Batch([tr], [(0,) + v])
         b.render()
         b.stats()
         t = time.perf_counter()
@@ -350,8 +387,8 @@ def run_views(a):
         "metric": "Mrays/s", "value": round(rays / el / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": "64 views of the box scene in one batch (res 128^2-1024^2, spp 4-256, depth 8)",
-                   "pixel_samples_per_step": paths},
+        "config": {"workload": "64 views of the %s scene in one batch (res 128^2-1024^2, spp 4-256, depth 8)"
+                               % a.scene, "pixel_samples_per_step": paths},
         "pixel_samples_per_s": round(paths * a.steps / el, 1),
         "sequential_views_s": round(seq, 4), "batched_s": round(el, 4),
         "batched_speedup": round(seq / el, 3),
@@ -359,17 +396,22 @@ def run_views(a):
 
 
 def run_scenes(a):
-    """16 seeded instanced scenes with triangle totals log-uniform in 10k..20M, each with its
-    own device-built BVH and 4 views (log-uniform 256^2..1920x1080 pixels, 16-128 spp, depth
-    8), all traced in one persistent wavefront run."""
-    from pt_amd.scenes.rng import Xshiro128
-    rng = Xshiro128.seeded(4)
+    """16 seeded instanced scenes with triangle totals log-uniformly spaced over 10k..20M
+    (workloads.scene_sizes), each with its own device-built BVH and 4 views (log-uniform
+    256^2..1920x1080 pixels, 16-128 spp, depth 8), all traced in one persistent wavefront run;
+    then each scene's views alone. Per scene: BVH build time, Mrays/s, and the L2 hit rate of
+    its traversal from the committed rocprofv3 summary (l2_hit_pct k_extend/scene<k>)."""
+    sizes = wl.scene_sizes()
+    only = a.only_scene
     tracers, scenes_info, views = [], [], []
-    for k in range(16):
-        target = rng.log_uniform(1e4, 2e7)
-        counts = generate.counts_for_triangles(target)
+    for k, target in enumerate(sizes):
+        if only >= 0 and k != only:
+            tracers.append(None)
+            scenes_info.append(None)
+            continue
         t = time.time()
-        scn = generate.cached_scene("instanced", root=a.scene_dir, seed=100 + k, counts=counts)
+        scn = generate.cached_scene("instanced", root=a.scene_dir, seed=100 + k,
+                                    counts=generate.counts_for_triangles(target))
         scene = pt_amd.Scene(scn)
         tr = pt_amd.Tracer(scene, 64, 64)
         s0 = tr.stats()
@@ -379,31 +421,21 @@ def run_scenes(a):
         log("scene %d: %d triangles, %d instances, BVH %.1f ms (%.1f s incl. load)"
             % (k, info["total_triangles"], info["instances"], s0["bvh_build_ms"], time.time() - t))
         tracers.append(tr)
-        for _ in range(4):
-            area = rng.log_uniform(256.0 ** 2, 1920.0 * 1080.0)
-            aspect = rng.uniform(0.5, 2.0)
-            w = max(1, int(round((area * aspect) ** 0.5)))
-            h = max(1, int(round(area / w)))
-            spp = int(round(rng.log_uniform(16.0, 128.0)))
-            eye = [rng.uniform(-20, 20) for _ in range(3)]
-            fwd = _random_dir(rng)
-            cam = pt_amd.camera_block([-e for e in eye], [-f for f in fwd], fov_deg=60.0, aspect=w / h,
-                                      focus_dist=30.0, lens_radius=0.03)
-            views.append((k, w, h, spp, DEPTH, cam))
-    if a.only_scene >= 0:
+        views += [(k,) + v for v in wl.scene_views(100 + k)]
+    if only >= 0:
         # one scene's four views alone (per-scene profiling)
-        k = a.only_scene
-        vk = [(0,) + v[1:] for v in views if v[0] == k]
-        b = pt_amd.Batch([tracers[k]], vk)
+        vk = [(0,) + v[1:] for v in views]
+        b = pt_amd.Batch([tracers[only]], vk)
         ek, sk = _timed_batch(b, a.steps, a.warmup)
         rays = sk["extension_rays"] + sk["shadow_rays"]
         b.close()
+        print("scene %d alone: %.1f Mrays/s" % (only, rays / ek / 1e6), file=sys.stderr, flush=True)
         return {"metric": "Mrays/s", "value": round(rays / ek / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
                 "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ek / a.steps * 1e3, 3),
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                 "data": "synthetic (seeded procedural scenes)",
-                "config": {"workload": "scene %d of the 16-scene batch alone (4 views, depth 8)" % k},
-                "scene": scenes_info[k]}
+                "config": {"workload": "scene %d of the 16-scene batch alone (4 views, depth 8)" % only},
+                "scene": scenes_info[only]}
     paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
     log("scenes: 16 scenes, 64 views, %d pixel-samples per step" % paths)
     batch = pt_amd.Batch(tracers, views)
@@ -416,6 +448,7 @@ def run_scenes(a):
         b = pt_amd.Batch([tracers[k]], vk)
         ek, sk = _timed_batch(b, 1, 1)
         scenes_info[k]["mrays_s"] = round((sk["extension_rays"] + sk["shadow_rays"]) / ek / 1e6, 2)
+        scenes_info[k]["l2_hit_pct"] = profile_figure("l2_hit_pct", "k_extend/scene%d" % k)
         b.close()
     return {
         "metric": "Mrays/s", "value": round(rays / el / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
@@ -428,13 +461,85 @@ def run_scenes(a):
     }
 
 
+# The learned cache of the camera-path workload: Adam at learning rate 1e-3 on 4 minibatches of
+# 16384 records after every frame (65,536 record samples per frame), one path in 4 recording
+# its suffix from the second vertex, the others ending there with the prediction.
+CACHE_PATH = dict(query=True, train_stride=4, train_steps=4, batch=16384, learning_rate=1e-3, frame_spp=4,
+                  continue_prob=0.0, seed=1)
+
+
+def run_cache(a):
+    """The learned radiance cache on the instanced scene: 64 frames of 1920x1080 at 1-4 spp
+    along a camera path through 8 random keyframes (workloads.camera_path), depth 8, the cache
+    trained online after every frame on that frame's path suffixes and queried at the second
+    vertex. The same frames with the plain estimator for comparison, and at 4 frames the
+    relative MSE of both against a 4096-spp plain reference of disjoint sample indices. A step
+    is the 64-frame path."""
+    scene = pt_amd.Scene(generate.cached_scene("instanced", root=a.scene_dir))
+    tr = pt_amd.Tracer(scene, W, H, max_bounces=DEPTH)
+    path = wl.camera_path()
+    check = [15, 31, 47, 63]
+    refs = {}
+    t = time.perf_counter()
+    for f in check:
+        refs[f] = reference_image(tr, path[f][1], 4096)
+    log("cache: %d references in %.1f s" % (len(check), time.perf_counter() - t))
+
+    def run(cached):
+        frames, errs, losses, records = [], {}, [], []
+        if cached:
+            tr.cache_configure(**CACHE_PATH)
+        rays = 0
+        for f, (spp, cam) in enumerate(path):
+            tr.set_camera(cam)
+            tr.synchronize()
+            t0 = time.perf_counter()
+            tr.render(spp)
+            tr.synchronize()
+            frames.append(time.perf_counter() - t0)
+            st = tr.stats()
+            rays += st["extension_rays"] + st["shadow_rays"]
+            if cached:
+                cs = tr.cache_stats()
+                losses.append(round(cs["last_loss"], 5))
+                records.append(cs["last_records"])
+            if f in refs:
+                errs[f] = float("%.4e" % pt_amd.rel_mse(tr.read(), refs[f]))
+        if cached:
+            tr.cache_configure(enable=False)
+        return frames, errs, losses, records, rays
+
+    for _ in range(max(0, a.warmup)):
+        run(False)
+    pf, perr, _, _, prays = run(False)
+    cf, cerr, loss, recs, crays = run(True)
+    log("cache: plain %.1f ms/frame, cached %.1f ms/frame; rel-MSE plain %s cached %s"
+        % (1e3 * np.mean(pf), 1e3 * np.mean(cf), perr, cerr))
+    total = sum(cf)
+    return {
+        "metric": "Mrays/s", "value": round(crays / total / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": 1,
+        "warmup": a.warmup, "ms_per_step": round(total * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (cache network bf16 weights, fp32 accumulation)",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "instanced scene, 64 frames of 1920x1080 at 1-4 spp on a keyframed camera path, "
+                               "depth 8, learned radiance cache trained online", "cache": CACHE_PATH},
+        "frame_ms": {"plain_mean": round(1e3 * np.mean(pf), 3), "cached_mean": round(1e3 * np.mean(cf), 3)},
+        "frame_spp": [spp for spp, _ in path],
+        "relmse_vs_4096spp": {"frames": check, "plain": [perr[f] for f in check], "cached": [cerr[f] for f in check]},
+        "cache_loss_first_last": [loss[:4], loss[-4:]],
+        "records_per_frame": int(np.median(recs)) if recs else 0,
+        "plain_mrays_s": round(prays / sum(pf) / 1e6, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--spp-per-step", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["instanced", "views", "scenes"], default="instanced")
+    ap.add_argument("--workload", choices=["instanced", "views", "scenes", "cache"], default="instanced")
+    ap.add_argument("--scene", choices=["instanced", "box"], default="instanced", help="views workload: the scene")
     ap.add_argument("--scene-dir", default=os.environ.get("PT_SCENE_DIR", "/tmp/pt_scenes"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
@@ -450,7 +555,7 @@ def main():
         print("bench.py: this build measures one GPU; every box here has one, so --gpus %d is not run"
               % a.gpus, file=sys.stderr)
         return 2
-    out = {"instanced": run_instanced, "views": run_views, "scenes": run_scenes}[a.workload](a)
+    out = {"instanced": run_instanced, "views": run_views, "scenes": run_scenes, "cache": run_cache}[a.workload](a)
     print(json.dumps(out), flush=True)
     return 0
 

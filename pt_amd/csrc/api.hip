@@ -406,6 +406,7 @@ bool overflowed(Ctx& c, std::string& why)
 hipError_t clear_overflow(Ctx& c)
 {
     c.stream_state.overflow = false;
+    c.stream_state.flags_from = c.stream_state.launched;
     return hipMemsetAsync(c.wf.counters + CTR(3), 0, 4, c.stream);
 }
 
@@ -1332,6 +1333,7 @@ int pt_batch_set_view(pt_batch* b, uint32_t view, const pt_view* v)
     b->views[view] = *v;
     b->sample_index[view] = 0;
     b->stream_state.overflow = false;
+    b->stream_state.flags_from = b->stream_state.launched;
     BHIP_TRY(b, hipMemsetAsync(b->wf.counters + CTR(3), 0, 4, b->stream), "error flags");
     view_dev(*b, view, b->h_views[view]);
     BHIP_TRY(b,

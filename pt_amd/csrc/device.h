@@ -121,6 +121,7 @@ struct StreamState {
     uint64_t launched = 0;   // iterations queued so far
     uint32_t seq = 0;
     bool overflow = false;   // a traversal stack overflowed (seen in a snapshot)
+    uint64_t flags_from = 0; // the first iteration queued after the error flags were last cleared
     struct Slot {
         bool busy = false;
         uint32_t paths = 0, seq = 0;

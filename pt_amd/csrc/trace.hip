@@ -1630,7 +1630,7 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
             S.slot[s].ended = snap[1 + s];
             S.slot[s].issued = snap[3 + s];
         }
-        if (snap[5])
+        if (snap[5] && counted >= S.flags_from)  // earlier snapshots predate the last clear
             S.overflow = true;
         // finished jobs in the order they were loaded
         for (int round = 0; round < 2; round++) {

@@ -113,3 +113,31 @@ def test_batch_of_16_scenes_up_to_20m_triangles(scene_dir):
         t2.close()
     for _, t in tracers:
         t.close()
+
+
+@pytest.mark.timeout(600)
+def test_learned_cache_on_a_camera_path(inst):
+    """The camera-path workload at reduced resolution: 64 frames at 1-4 spp through 8 keyframes,
+    the cache trained online after every frame (Adam, learning rate 1e-3, 4 x 16384 record
+    samples per frame) and queried at the second vertex. Every frame's image is finite, records
+    arrive every frame, and the training loss falls from the first frames to the last."""
+    scene, _ = inst
+    w, h = 320, 180
+    path = wl.camera_path(w=w, h=h)
+    assert len(path) == 64 and all(1 <= spp <= 4 for spp, _ in path)
+    tr = pt_amd.Tracer(scene, w, h, max_bounces=8)
+    tr.cache_configure(query=True, train_stride=4, train_steps=4, batch=16384, learning_rate=1e-3, frame_spp=4,
+                       seed=1)
+    losses, records = [], []
+    for spp, cam in path:
+        tr.set_camera(cam)
+        tr.render(spp)
+        assert np.isfinite(tr.read()).all()
+        cs = tr.cache_stats()
+        losses.append(cs["last_loss"])
+        records.append(cs["last_records"])
+    assert min(records) > 0
+    assert tr.cache_stats()["steps"] == 4 * 64
+    print("cache losses", np.round(losses, 4).tolist())
+    assert np.mean(losses[-16:]) < 0.8 * np.mean(losses[:4]), (losses[:4], losses[-16:])
+    tr.close()
