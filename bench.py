@@ -259,6 +259,12 @@ def run_instanced(a):
     n_tri_ext = cs["ext_tri_fetches"] / max(1, cs["extension_rays"])
     n_node_sh = cs["shadow_node_fetches"] / max(1, cs["shadow_rays"])
     n_tri_sh = cs["shadow_tri_fetches"] / max(1, cs["shadow_rays"])
+    per_ray = lambda k, n: round(cs[k] / max(1, cs[n]), 3)  # noqa: E731
+    topo = {"ext_tlas_nodes": per_ray("ext_tlas_node_fetches", "extension_rays"),
+            "ext_instances_entered": per_ray("ext_instance_entries", "extension_rays"),
+            "shadow_tlas_nodes": per_ray("shadow_tlas_node_fetches", "shadow_rays"),
+            "shadow_instances_entered": per_ray("shadow_instance_entries", "shadow_rays")}
+    log("per ray: %s" % topo)
     bytes_ext = 32 + 20 + 128 * n_node_ext + 48 * n_tri_ext
     bytes_sh = 32 + 4 + 128 * n_node_sh + 48 * n_tri_sh
     log("fetches/ray: extension %.1f nodes %.1f tris (%.0f B), shadow %.1f nodes %.1f tris (%.0f B)"
@@ -327,8 +333,8 @@ def run_instanced(a):
         "samples_per_s": round(a.steps * a.spp_per_step / elapsed, 3),
         "pixel_samples_per_s": round(a.steps * a.spp_per_step * W * H / elapsed, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
-        "fetches_per_ray": {"ext_nodes": round(n_node_ext, 3), "ext_tris": round(n_tri_ext, 3),
-                            "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)},
+        "fetches_per_ray": dict({"ext_nodes": round(n_node_ext, 3), "ext_tris": round(n_tri_ext, 3),
+                                 "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)}, **topo),
         "stages_ms": dict({k: round(v["ms"], 3) for k, v in stages.items()}, refill=round(sp["refill_ms"], 3)),
         "stage_launches": dict({k: v["launches"] for k, v in stages.items()}, refill=sp["refill_launches"]),
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
@@ -478,7 +484,7 @@ def run_cache(a):
     scene = pt_amd.Scene(generate.cached_scene("instanced", root=a.scene_dir))
     tr = pt_amd.Tracer(scene, W, H, max_bounces=DEPTH)
     path = wl.camera_path()
-    check = [15, 31, 47, 63]
+    check = [] if a.no_reference else [15, 31, 47, 63]
     refs = {}
     t = time.perf_counter()
     for f in check:
@@ -544,6 +550,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-reference", action="store_true", help="cache workload: no 4096-spp references")
     ap.add_argument("--leaf-size", type=int, default=0, help="BLAS leaf size to start from (0: the library's)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="traversal blocks per CU to start from")
     ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")

@@ -156,6 +156,12 @@ typedef struct pt_stats {
     uint32_t shadow_launches;
     uint32_t refill_launches;
     double refill_ms;
+    /* With PT_OPT_COUNT_FETCHES: the node fetches of the top-level hierarchy among the node
+     * fetches above, and the instances (BLAS) entered. */
+    uint64_t ext_tlas_node_fetches;
+    uint64_t ext_instance_entries;
+    uint64_t shadow_tlas_node_fetches;
+    uint64_t shadow_instance_entries;
 } pt_stats;
 
 /* Options for pt_set_option (instrumentation; no counterpart in the reference, whose only

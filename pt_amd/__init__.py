@@ -88,7 +88,9 @@ class Stats(C.Structure):
                 ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64),
                 ("extend_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("extend_launches", C.c_uint32), ("shade_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
-                ("refill_launches", C.c_uint32), ("refill_ms", C.c_double)]
+                ("refill_launches", C.c_uint32), ("refill_ms", C.c_double),
+                ("ext_tlas_node_fetches", C.c_uint64), ("ext_instance_entries", C.c_uint64),
+                ("shadow_tlas_node_fetches", C.c_uint64), ("shadow_instance_entries", C.c_uint64)]
 
 PT_OPT_PROFILE_KERNELS = 1
 PT_OPT_COUNT_FETCHES = 2

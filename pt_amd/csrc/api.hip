@@ -721,7 +721,7 @@ int pt_render(pt_ctx* h, uint32_t spp)
         return fail(&c, PT_EINVAL, "sample index overflow");
     if (c.stats_fresh) {
         c.timer.reset();
-        HIP_TRY(&c, hipMemsetAsync(c.wf.totals, 0, 8 * 8, c.stream), "stats");
+        HIP_TRY(&c, hipMemsetAsync(c.wf.totals, 0, TOTALS * 8, c.stream), "stats");
         HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
         c.last_paths = 0;
         c.stats_fresh = false;
@@ -859,7 +859,7 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
     if (!out)
         return fail(&c, PT_EINVAL, "out is null");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
-    unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long totals[10] = {};
     HIP_TRY(&c, hipMemcpy(totals, c.wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
     float ms = 0.0f;
     if (c.last_paths)
@@ -881,6 +881,10 @@ int pt_get_stats(pt_ctx* h, pt_stats* out)
     s.ext_tri_fetches = totals[3];
     s.shadow_node_fetches = totals[4];
     s.shadow_tri_fetches = totals[5];
+    s.ext_tlas_node_fetches = totals[6];
+    s.ext_instance_entries = totals[7];
+    s.shadow_tlas_node_fetches = totals[8];
+    s.shadow_instance_entries = totals[9];
     if (c.timer.enabled) {
         double kms[KERNEL_COUNT];
         uint32_t kl[KERNEL_COUNT];
@@ -1358,7 +1362,7 @@ int pt_batch_render(pt_batch* b)
     }
     if (b->stats_fresh) {
         b->timer.reset();
-        BHIP_TRY(b, hipMemsetAsync(b->wf.totals, 0, 8 * 8, b->stream), "stats");
+        BHIP_TRY(b, hipMemsetAsync(b->wf.totals, 0, TOTALS * 8, b->stream), "stats");
         BHIP_TRY(b, hipEventRecord(b->ev_start, b->stream), "event");
         b->last_paths = 0;
         b->stats_fresh = false;
@@ -1417,7 +1421,7 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
     if (!out)
         return bfail(b, PT_EINVAL, "out is null");
     BHIP_TRY(b, hipStreamSynchronize(b->stream), "sync");
-    unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long totals[10] = {};
     uint32_t error_flags = 0;
     BHIP_TRY(b, hipMemcpy(totals, b->wf.totals, sizeof(totals), hipMemcpyDeviceToHost), "stats");
     BHIP_TRY(b, hipMemcpy(&error_flags, b->wf.counters + CTR(3), 4, hipMemcpyDeviceToHost), "stats");
@@ -1440,6 +1444,10 @@ int pt_batch_get_stats(pt_batch* b, pt_stats* out)
     s.ext_tri_fetches = totals[3];
     s.shadow_node_fetches = totals[4];
     s.shadow_tri_fetches = totals[5];
+    s.ext_tlas_node_fetches = totals[6];
+    s.ext_instance_entries = totals[7];
+    s.shadow_tlas_node_fetches = totals[8];
+    s.shadow_instance_entries = totals[9];
     if (b->timer.enabled) {
         double kms[KERNEL_COUNT];
         uint32_t kl[KERNEL_COUNT];

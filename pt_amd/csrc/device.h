@@ -26,6 +26,7 @@ constexpr uint32_t SPILL_STACK = 112;     // further entries per lane spilled to
 // The wavefront counters sit 4 KB apart: the atomics of the queue appends then go to
 // different memory channels instead of queueing on one line.
 constexpr uint32_t CTR_STRIDE = 1024;
+constexpr uint32_t TOTALS = 16;  // words of Wavefront::totals
 __host__ __device__ constexpr uint32_t CTR(uint32_t k) { return k * CTR_STRIDE; }
 
 struct Wavefront {
@@ -52,7 +53,8 @@ struct Wavefront {
     // ended paths of job slot 0, 1, [9] error flags of the ray queries (pt_trace_*)
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
-    // rays, [4]/[5] of shadow rays (counted only when fetch counting is on)
+    // rays, [4]/[5] of shadow rays, [6]/[7] TLAS node fetches / instances entered by extension
+    // rays, [8]/[9] by shadow rays (fetches counted only when fetch counting is on)
     unsigned long long* totals = nullptr;
     uint32_t* spill = nullptr;     // traversal stack spill, SPILL_STACK words per grid thread
     // material sort of the extension hits (allocated on first use)

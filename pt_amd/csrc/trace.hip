@@ -205,12 +205,18 @@ struct HitOut {
     uint32_t inst, prim;
 };
 
+// Fetch counts of one lane's rays (COUNT = true): node lines, triangle records, node lines of
+// the TLAS, instances entered.
+struct FetchCount {
+    uint32_t nodes = 0, tris = 0, tlas_nodes = 0, entries = 0;
+};
+
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
 // transmittance through transparent instances is returned in trans (shadowray.ah.hlsl);
 // ALPHA = false compiles that out for scenes without transparent instances (every hit occludes).
 template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true>
 __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
-                         float& trans, uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr)
+                         float& trans, FetchCount* fc = nullptr)
 {
     hit.inst = NONE;
     trans = 1.0f;
@@ -230,8 +236,10 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
 
     while (true) {
         // ---- visit an internal node
-        if (COUNT)
-            ++*n_nodes;
+        if (COUNT) {
+            fc->nodes++;
+            fc->tlas_nodes += inst == NONE ? 1u : 0u;
+        }
         // The near and far plane rows of each axis are picked by the load address (the ray's
         // direction sign: row hi first when the ray runs towards -axis), from a 32-bit offset off
         // the scene's node array.
@@ -286,7 +294,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 const uint32_t count = ((c >> 27) & 0xFu) + 1u;
                 for (uint32_t q = 0; q < count; q++) {
                     if (COUNT)
-                        ++*n_tris;
+                        fc->tris++;
                     const TriRec tr = at32(s.tris, first + q);
                     float t, b1, b2;
                     if (!intersect_tri(r, xyz(tr.v0), xyz(tr.v1), xyz(tr.v2), tmin, best, t, b1, b2))
@@ -362,6 +370,8 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 precompute(r, oo, od);
                 inst = id;
                 blas_sp = stk.sp;
+                if (COUNT)
+                    fc->entries++;
                 if (ANY && ALPHA)
                     opacity = s.inst_opacity[id];
                 // Roots are single nodes; continue with the BLAS root as an internal node.
@@ -479,21 +489,23 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
-    uint32_t n_nodes = 0, n_tris = 0;
+    FetchCount fc;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
         const DeviceScene& s = MULTI ? scenes[wf.q_scene[q][i]] : s0;
         HitOut h;
         float trans;
-        traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &n_nodes, &n_tris);
+        traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
         if (stk.overflow)
             atomicOr(&wf.counters[CTR(3)], 1u);
     }
-    if (COUNT && (n_nodes | n_tris)) {
-        atomicAdd(&wf.totals[2], (unsigned long long)n_nodes);
-        atomicAdd(&wf.totals[3], (unsigned long long)n_tris);
+    if (COUNT && (fc.nodes | fc.tris)) {
+        atomicAdd(&wf.totals[2], (unsigned long long)fc.nodes);
+        atomicAdd(&wf.totals[3], (unsigned long long)fc.tris);
+        atomicAdd(&wf.totals[6], (unsigned long long)fc.tlas_nodes);
+        atomicAdd(&wf.totals[7], (unsigned long long)fc.entries);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&wf.totals[0], (unsigned long long)count);
@@ -508,14 +520,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
-    uint32_t n_nodes = 0, n_tris = 0;
+    FetchCount fc;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
         const DeviceScene& s = MULTI ? scenes[wf.s_scene[i]] : s0;
         HitOut h;
         float trans;
         const bool occluded =
-            traverse<true, COUNT, StackT<D>, ALPHA>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &n_nodes, &n_tris);
+            traverse<true, COUNT, StackT<D>, ALPHA>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &fc);
         if (!occluded && trans >= 1e-5f) {
             const uint32_t tgt = wf.s_target[i];
             float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
@@ -529,9 +541,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (stk.overflow)
             atomicOr(&wf.counters[CTR(3)], 1u);
     }
-    if (COUNT && (n_nodes | n_tris)) {
-        atomicAdd(&wf.totals[4], (unsigned long long)n_nodes);
-        atomicAdd(&wf.totals[5], (unsigned long long)n_tris);
+    if (COUNT && (fc.nodes | fc.tris)) {
+        atomicAdd(&wf.totals[4], (unsigned long long)fc.nodes);
+        atomicAdd(&wf.totals[5], (unsigned long long)fc.tris);
+        atomicAdd(&wf.totals[8], (unsigned long long)fc.tlas_nodes);
+        atomicAdd(&wf.totals[9], (unsigned long long)fc.entries);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&wf.totals[1], (unsigned long long)count);
@@ -1184,7 +1198,7 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
     A(wf.rad_out, size_t(std::max<uint32_t>(rad_out_paths, 1)) * 2 * 16);  // one half per job slot
     A(wf.counters, size_t(CTR(16)) * 4);
     A(wf.snapshot, 16 * 4);
-    A(wf.totals, 8 * 8);
+    A(wf.totals, TOTALS * 8);
     A(wf.spill, size_t(spill_threads) * SPILL_STACK * 4);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&wf.host_counts, 16 * sizeof(uint32_t), hipHostMallocDefault);
@@ -1193,7 +1207,7 @@ hipError_t wavefront_alloc(Wavefront& wf, uint32_t capacity, uint32_t rad_out_pa
     if (e == hipSuccess)
         e = hipMemset(wf.counters, 0, size_t(CTR(16)) * 4);
     if (e == hipSuccess)
-        e = hipMemset(wf.totals, 0, 8 * 8);
+        e = hipMemset(wf.totals, 0, TOTALS * 8);
     return e;
 }
 
