@@ -535,6 +535,11 @@ def run_cache(a):
         "cache_loss_first_last": [loss[:4], loss[-4:]],
         "records_per_frame": int(np.median(recs)) if recs else 0,
         "plain_mrays_s": round(prays / sum(pf) / 1e6, 2),
+        # the network's kernels against the dense BF16 MFMA peak (committed rocprofv3 summary)
+        "mfma": {k: {"tflops": profile_figure("mfma_tflops", k), "busy_pct": profile_figure("mfma_busy_pct", k),
+                     "frac_of_2500_tflops": (round(profile_figure("mfma_tflops", k) / 2500.0, 5)
+                                             if profile_figure("mfma_tflops", k) else None)}
+                 for k in ("k_mlp_train", "k_mlp_infer")},
     }
 
 
