@@ -230,6 +230,8 @@ def run_instanced(a):
         tr.set_option(pt_amd.PT_OPT_BLOCKS_PER_CU, a.blocks_per_cu)
     if a.pool_slots:
         tr.set_option(pt_amd.PT_OPT_POOL_SLOTS, a.pool_slots)
+    if a.tlas_open >= 0:
+        tr.set_option(pt_amd.PT_OPT_TLAS_OPEN, a.tlas_open)
     st0 = tr.stats()
     tune = None
     if not a.no_autotune:
@@ -237,9 +239,9 @@ def run_instanced(a):
         # trials of 16 spp (four pools of paths at 1080p): long enough to time the streaming
         # steady state the timed 64-spp steps run in, not one pool's fill and drain
         tune = tr.autotune(16)
-        log("autotune: %d trials, blocks/CU %d, LDS stack %d, leaf %d, pool %d: %.3f -> %.3f ns/ray"
+        log("autotune: %d trials, blocks/CU %d, LDS stack %d, leaf %d, pool %d, TLAS opening %d: %.3f -> %.3f ns/ray"
             % (tune["trials"], tune["blocks_per_cu"], tune["lds_stack"], tune["leaf_size"], tune["pool_slots"],
-               tune["ns_per_ray_start"], tune["ns_per_ray_best"]))
+               tune["tlas_open"], tune["ns_per_ray_start"], tune["ns_per_ray_best"]))
         st0 = tr.stats()
 
     tr.set_option(pt_amd.PT_OPT_SORT_HITS, a.sort_hits)
@@ -559,6 +561,7 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=0, help="BLAS leaf size to start from (0: the library's)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="traversal blocks per CU to start from")
     ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")
+    ap.add_argument("--tlas-open", type=int, default=-1, help="TLAS opening depth to start from (-1: the library's)")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))

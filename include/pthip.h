@@ -186,6 +186,10 @@ typedef struct pt_stats {
                                    * every node is one 128-byte line): rebuilds the BVH */
 #define PT_OPT_BVH_BUILDER 10u    /* binary tree under the SAH collapse: 0 PLOC clustering (default),
                                    * 1 Karras radix tree over the Morton codes: rebuilds the BVH */
+#define PT_OPT_TLAS_OPEN 12u      /* levels of each instance's BLAS the TLAS may open (0..2, default 0):
+                                   * an instance whose BLAS children have much tighter world boxes than
+                                   * its own is entered through those subtrees (partial re-braiding);
+                                   * rebuilds the BVH */
 #define PT_OPT_SPILL_DEPTH 11u    /* traversal stack entries per lane spilled to global memory beyond the
                                    * LDS ones (0..112, default 112); a ray whose stack would grow deeper
                                    * loses nodes, which the calls below report as PT_EDEVICE */
@@ -367,8 +371,8 @@ int pt_read_light_words(pt_ctx* ctx, uint32_t* words, size_t count);
 
 /* Autotuner (no counterpart in the reference): times spp samples of the current view under
  * candidate launch configurations (coordinate descent over blocks per CU, LDS stack depth,
- * pool slots, BVH node width and BLAS leaf size), keeps the fastest (device time per traced
- * ray) and clears the accumulation (sample_index 0).
+ * pool slots, BVH node width, BLAS leaf size and TLAS opening depth), keeps the fastest (device
+ * time per traced ray) and clears the accumulation (sample_index 0).
  * Not with the radiance cache or instrumentation options on. Synchronises. */
 typedef struct pt_tune_result {
     uint32_t blocks_per_cu;
@@ -379,6 +383,8 @@ typedef struct pt_tune_result {
     uint32_t trials;
     double ns_per_ray_start; /* the configuration the tuner started from */
     double ns_per_ray_best;
+    uint32_t tlas_open;      /* PT_OPT_TLAS_OPEN */
+    uint32_t reserved;
 } pt_tune_result;
 int pt_autotune(pt_ctx* ctx, uint32_t spp, pt_tune_result* out);
 

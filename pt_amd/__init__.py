@@ -103,12 +103,14 @@ PT_OPT_SORT_HITS = 8
 PT_OPT_NODE_WIDTH = 9
 PT_OPT_BVH_BUILDER = 10
 PT_OPT_SPILL_DEPTH = 11
+PT_OPT_TLAS_OPEN = 12
 
 
 class TuneResult(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("pool_slots", C.c_uint32), ("lds_stack", C.c_uint32),
                 ("leaf_size", C.c_uint32), ("node_width", C.c_uint32), ("trials", C.c_uint32),
-                ("ns_per_ray_start", C.c_double), ("ns_per_ray_best", C.c_double)]
+                ("ns_per_ray_start", C.c_double), ("ns_per_ray_best", C.c_double),
+                ("tlas_open", C.c_uint32), ("reserved", C.c_uint32)]
 PT_CACHE_FEATURES = 96
 
 
@@ -171,10 +173,11 @@ def host_lib():
 
 
 def hip_lib():
-    """Loads libpthip.so; raises if it is not built (there is no CPU fallback)."""
+    """Loads libpthip.so (or the build named by $PT_LIBPTHIP, for comparing builds); raises if it
+    is not built (there is no CPU fallback)."""
     global _hip
     if _hip is None:
-        path = os.path.join(LIB_DIR, "libpthip.so")
+        path = os.environ.get("PT_LIBPTHIP") or os.path.join(LIB_DIR, "libpthip.so")
         if not os.path.exists(path):
             raise ImportError("libpthip.so is not built: run python -m pt_amd.build")
         lib = C.CDLL(path)

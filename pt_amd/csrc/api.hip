@@ -184,7 +184,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     if (bv.tri_count &&
         hipMemcpy(tris.data(), bv.tris, size_t(bv.tri_count) * sizeof(TriRec), hipMemcpyDeviceToHost) != hipSuccess)
         return (why = "BVH readback failed", PT_EDEVICE);
-    std::vector<uint8_t> tri_seen(bv.tri_count, 0), prim_seen(bv.tri_count, 0), inst_seen(c.instance_count, 0);
+    std::vector<uint8_t> tri_seen(bv.tri_count, 0), prim_seen(bv.tri_count, 0);
     std::vector<uint32_t> stack;
     uint32_t tri_base = 0;
     for (uint32_t m = 0; m < meshes.size(); m++) {
@@ -231,6 +231,11 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     if (c.instance_count) {
         if (bv.tlas_root >= bv.node_count)
             return (why = "TLAS root out of range", PT_EDEVICE);
+        std::vector<uint2> entries(bv.entry_count);
+        if (bv.entry_count && hipMemcpy(entries.data(), bv.tlas_entries, size_t(bv.entry_count) * sizeof(uint2),
+                                        hipMemcpyDeviceToHost) != hipSuccess)
+            return (why = "BVH readback failed", PT_EDEVICE);
+        std::vector<uint8_t> entry_seen(bv.entry_count, 0);
         stack.assign(1, bv.tlas_root);
         size_t visited = 0;
         while (!stack.empty()) {
@@ -244,7 +249,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
                     continue;
                 if (ch & CHILD_LEAF) {
                     const uint32_t id = ch & ~CHILD_LEAF;
-                    if (id >= c.instance_count || inst_seen[id]++)
+                    if (id >= bv.entry_count || entry_seen[id]++)
                         return (why = "TLAS leaf out of range or repeated", PT_EDEVICE);
                 } else {
                     if (ch >= bv.node_count || ch < bv.tlas_root)
@@ -253,9 +258,59 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
                 }
             }
         }
-        for (uint32_t i = 0; i < c.instance_count; i++)
-            if (!inst_seen[i])
-                return (why = "instance not reachable from the TLAS", PT_EDEVICE);
+        // every entry reached; each instance's entries are an exact cut of its BLAS: walking down
+        // from the root, every path meets one of them before any triangle leaf, and each is met once
+        std::vector<std::vector<uint32_t>> cut(c.instance_count);
+        for (uint32_t e = 0; e < bv.entry_count; e++) {
+            if (!entry_seen[e])
+                return (why = "TLAS entry not reachable", PT_EDEVICE);
+            if (entries[e].x >= c.instance_count)
+                return (why = "TLAS entry names a missing instance", PT_EDEVICE);
+            cut[entries[e].x].push_back(entries[e].y);
+        }
+        std::vector<uint32_t> inst_mesh(c.instance_count);
+        if (hipMemcpy(inst_mesh.data(), c.d_instance_mesh, size_t(c.instance_count) * 4, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return (why = "BVH readback failed", PT_EDEVICE);
+        for (uint32_t i = 0; i < c.instance_count; i++) {
+            std::vector<uint32_t>& cu = cut[i];
+            std::sort(cu.begin(), cu.end());
+            if (cu.empty() || std::adjacent_find(cu.begin(), cu.end()) != cu.end())
+                return (why = "instance " + std::to_string(i) + " has no TLAS entry or a repeated one", PT_EDEVICE);
+            const uint32_t m = inst_mesh[i];
+            if (meshes[m].tri_count == 0) {
+                if (cu.size() != 1 || cu[0] != CHILD_EMPTY)
+                    return (why = "instance of an empty mesh enters a node", PT_EDEVICE);
+                continue;
+            }
+            size_t met = 0;
+            stack.assign(1, m);
+            while (!stack.empty()) {
+                const uint32_t n = stack.back();
+                stack.pop_back();
+                if (std::binary_search(cu.begin(), cu.end(), n)) {
+                    met++;
+                    continue;
+                }
+                if (n >= bv.node_count)
+                    return (why = "TLAS entry cut out of range", PT_EDEVICE);
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t ch = nodes[n].child[k];
+                    if (ch == CHILD_EMPTY)
+                        continue;
+                    if (ch & CHILD_LEAF)
+                        return (why = "instance " + std::to_string(i) + ": triangles outside its TLAS entries",
+                                PT_EDEVICE);
+                    stack.push_back(ch);
+                }
+                if (stack.size() > 64)
+                    return (why = "TLAS entries of instance " + std::to_string(i) + " are not a cut of its BLAS",
+                            PT_EDEVICE);
+            }
+            if (met != cu.size())
+                return (why = "TLAS entries of instance " + std::to_string(i) + " are not a cut of its BLAS",
+                        PT_EDEVICE);
+        }
         // world bounds (union of the root's children) normalise the radiance cache's positions
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         const Node4& root = nodes[bv.tlas_root];
@@ -353,6 +408,9 @@ int build_scene_bvh(Ctx& c)
     bp.blas_leaf_size = c.leaf_size;
     bp.node_width = c.node_width;
     bp.builder = c.bvh_builder;
+    bp.tlas_open = c.tlas_open;
+    if (const char* env = getenv("PT_TLAS_OPEN_RATIO"))  // tuning of the opening criterion
+        bp.tlas_open_ratio = float(std::max(0.0, std::min(16.0, atof(env))));
     if (const char* env = getenv("PT_BVH_SAH_TRI_COST"))  // tuning of the SAH cost model
         bp.sah_tri_cost = float(std::max(0.01, std::min(100.0, atof(env))));
     HIP_TRY(&c,
@@ -370,7 +428,7 @@ int build_scene_bvh(Ctx& c)
     c.scene.nodes = c.bvh.nodes;
     c.scene.tris = c.bvh.tris;
     c.scene.inst_inv = c.bvh.inst_inv;
-    c.scene.inst_blas_root = c.bvh.inst_blas_root;
+    c.scene.tlas_entries = c.bvh.tlas_entries;
     c.scene.tlas_root = c.bvh.tlas_root;
     c.graphs.destroy();
     return PT_OK;
@@ -629,7 +687,7 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              s.nodes = c.bvh.nodes;
              s.tris = c.bvh.tris;
              s.inst_inv = c.bvh.inst_inv;
-             s.inst_blas_root = c.bvh.inst_blas_root;
+             s.tlas_entries = c.bvh.tlas_entries;
              s.inst_opacity = c.d_inst_opacity;
              s.tlas_root = c.bvh.tlas_root;
              s.envcol = make_float3(scene->envcol[0], scene->envcol[1], scene->envcol[2]);
@@ -968,6 +1026,7 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
         return PT_OK;
     case PT_OPT_LEAF_SIZE:
     case PT_OPT_NODE_WIDTH:
+    case PT_OPT_TLAS_OPEN:
     case PT_OPT_BVH_BUILDER: {
         if (option == PT_OPT_LEAF_SIZE && (value < 1 || value > 16))
             return fail(&c, PT_EINVAL, "leaf size must be 1..16");
@@ -975,9 +1034,13 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
             return fail(&c, PT_EINVAL, "node width must be 2..4");
         if (option == PT_OPT_BVH_BUILDER && value != BVH_BUILDER_PLOC && value != BVH_BUILDER_LBVH)
             return fail(&c, PT_EINVAL, "BVH builder must be 0 (PLOC) or 1 (radix tree)");
+        if (option == PT_OPT_TLAS_OPEN && (value < 0 || value > 2))
+            return fail(&c, PT_EINVAL, "TLAS opening depth must be 0..2");
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
-        uint32_t& field = option == PT_OPT_LEAF_SIZE ? c.leaf_size : option == PT_OPT_NODE_WIDTH ? c.node_width
-                                                                                                 : c.bvh_builder;
+        uint32_t& field = option == PT_OPT_LEAF_SIZE    ? c.leaf_size
+                          : option == PT_OPT_NODE_WIDTH ? c.node_width
+                          : option == PT_OPT_TLAS_OPEN  ? c.tlas_open
+                                                        : c.bvh_builder;
         field = uint32_t(value);
         if (int v = build_scene_bvh(c))
             return v;
@@ -1539,13 +1602,14 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         return fail(&c, PT_ESTATE, "instrumentation options are on");
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
     struct Cfg {
-        int64_t bpc, pool, lds, leaf, width;
+        int64_t bpc, pool, lds, leaf, width, open;
         bool operator==(const Cfg& o) const
         {
-            return bpc == o.bpc && pool == o.pool && lds == o.lds && leaf == o.leaf && width == o.width;
+            return bpc == o.bpc && pool == o.pool && lds == o.lds && leaf == o.leaf && width == o.width &&
+                   open == o.open;
         }
     };
-    const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size, c.node_width};
+    const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size, c.node_width, c.tlas_open};
     uint32_t trials = 0;
     // device nanoseconds per traced ray of spp samples of the current view (best of two runs
     // after a first one that warms the caches and captures the graphs)
@@ -1563,6 +1627,9 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
                 return v;
         if (g.width != int64_t(c.node_width))
             if (int v = pt_set_option(h, PT_OPT_NODE_WIDTH, g.width))
+                return v;
+        if (g.open != int64_t(c.tlas_open))
+            if (int v = pt_set_option(h, PT_OPT_TLAS_OPEN, g.open))
                 return v;
         ns = 1e300;
         for (int rep = 0; rep < 3; rep++) {
@@ -1597,13 +1664,24 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
             pools.push_back(p);
     const std::vector<int64_t> leaves = {2, 4, 8};
     const std::vector<int64_t> widths = {4, 3, 2};
-    // coordinate descent over (blocks per CU, LDS stack depth, pool slots, node width, leaf size)
-    for (int param = 0; param < 5; param++) {
-        const std::vector<int64_t>& vals =
-            param == 0 ? bpcs : param == 1 ? ldss : param == 2 ? pools : param == 3 ? widths : leaves;
+    const std::vector<int64_t> opens = {0, 1, 2};
+    // coordinate descent over (blocks per CU, LDS stack depth, pool slots, node width, leaf size,
+    // TLAS opening depth)
+    for (int param = 0; param < 6; param++) {
+        const std::vector<int64_t>& vals = param == 0   ? bpcs
+                                           : param == 1 ? ldss
+                                           : param == 2 ? pools
+                                           : param == 3 ? widths
+                                           : param == 4 ? leaves
+                                                        : opens;
         for (int64_t v : vals) {
             Cfg g = best;
-            (param == 0 ? g.bpc : param == 1 ? g.lds : param == 2 ? g.pool : param == 3 ? g.width : g.leaf) = v;
+            (param == 0   ? g.bpc
+             : param == 1 ? g.lds
+             : param == 2 ? g.pool
+             : param == 3 ? g.width
+             : param == 4 ? g.leaf
+                          : g.open) = v;
             if (g == best)
                 continue;
             double ns;
@@ -1630,6 +1708,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
         r.lds_stack = uint32_t(best.lds);
         r.leaf_size = uint32_t(best.leaf);
         r.node_width = uint32_t(best.width);
+        r.tlas_open = uint32_t(best.open);
         r.trials = trials;
         r.ns_per_ray_start = base_ns;
         r.ns_per_ray_best = best_ns;

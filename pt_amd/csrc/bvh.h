@@ -1,6 +1,7 @@
 // Device-built BVH: Morton codes -> radix sort -> binary tree (PLOC clustering, or Karras'
 // (2012) radix tree) -> SAH-optimal collapse into 4-wide 128-byte nodes. One BLAS per mesh
-// (all meshes in one batched build) and a TLAS over the instances.
+// (all meshes in one batched build) and a TLAS over the instances (or over subtrees of their
+// BLASes: tlas_open).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +19,12 @@ struct BvhBuildParams {
     uint32_t blas_leaf_size = 4;  // max triangles per BLAS leaf (1..16)
     uint32_t node_width = 4;      // children per node (2..4) the SAH collapse may use
     uint32_t builder = BVH_BUILDER_PLOC;
+    // Levels of each instance's BLAS the TLAS may open (0..2): an instance is then entered
+    // through the subtrees of its BLAS whose transformed boxes are tighter than its own (partial
+    // re-braiding, Benthin et al. 2017); a node opens when all its children are internal nodes
+    // and their world boxes' surface areas sum to less than tlas_open_ratio x its own.
+    uint32_t tlas_open = 0;
+    float tlas_open_ratio = 1.5f;
     // SAH cost of entering a node, testing a triangle, entering an instance
     float sah_node_cost = 1.0f;
     float sah_tri_cost = 0.5f;
@@ -28,7 +35,8 @@ struct BvhBuffers {
     Node4* nodes = nullptr;         // [blas nodes | tlas nodes]
     TriRec* tris = nullptr;
     InstXform* inst_inv = nullptr;
-    uint32_t* inst_blas_root = nullptr;
+    uint2* tlas_entries = nullptr;  // what a TLAS leaf enters: (instance, BLAS node or CHILD_EMPTY)
+    uint32_t entry_count = 0;
     uint32_t node_count = 0;
     uint32_t node_capacity = 0;
     uint32_t tri_count = 0;

@@ -669,37 +669,126 @@ __global__ void k_tri_records(uint32_t n, const uint32_t* perm, const uint32_t* 
     }
 }
 
-// World bounds of each instance from its BLAS root box (8 transformed corners).
-__global__ void k_instance_bounds(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
-                                  const float* mesh_box, const uint32_t* mesh_tris, float4* lo, float4* hi,
-                                  uint32_t* seg)
+constexpr uint32_t MAX_ENTRIES = 16;  // TLAS entries of one instance (two opened levels of 4)
+
+// World box of an object-space box under an instance transform: the 8 corners transformed,
+// padded for the rounding of the transform.
+__device__ void world_box(const InstanceData& d, float x0, float y0, float z0, float x1, float y1, float z1,
+                          float4& lo, float4& hi)
+{
+    float3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
+    for (int c = 0; c < 8; c++) {
+        const float px = (c & 1) ? x1 : x0, py = (c & 2) ? y1 : y0, pz = (c & 4) ? z1 : z0;
+        const float wx = d.m[0][0] * px + d.m[0][1] * py + d.m[0][2] * pz + d.m[0][3];
+        const float wy = d.m[1][0] * px + d.m[1][1] * py + d.m[1][2] * pz + d.m[1][3];
+        const float wz = d.m[2][0] * px + d.m[2][1] * py + d.m[2][2] * pz + d.m[2][3];
+        mn = f3(fminf(mn.x, wx), fminf(mn.y, wy), fminf(mn.z, wz));
+        mx = f3(fmaxf(mx.x, wx), fmaxf(mx.y, wy), fmaxf(mx.z, wz));
+    }
+    const float3 ext = f3(fabsf(mn.x) + fabsf(mx.x), fabsf(mn.y) + fabsf(mx.y), fabsf(mn.z) + fabsf(mx.z));
+    const float eps = 1e-6f;
+    lo = make_float4(mn.x - eps * ext.x, mn.y - eps * ext.y, mn.z - eps * ext.z, 0.0f);
+    hi = make_float4(mx.x + eps * ext.x, mx.y + eps * ext.y, mx.z + eps * ext.z, 0.0f);
+}
+
+__device__ __forceinline__ float half_area(float4 lo, float4 hi)
+{
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// The TLAS entries of each instance (at most MAX_ENTRIES, from slot i * MAX_ENTRIES): its BLAS
+// root, or - for up to `depth` levels - the children of an entry that opens: every child an
+// internal node, and the surface areas of their world boxes summing to less than ratio x that
+// of the entry's own world box. An instance of an empty mesh gets one entry that enters nothing.
+__global__ void k_open_instances(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
+                                 const uint32_t* mesh_tris, const float* mesh_box, const Node4* nodes, uint32_t depth,
+                                 float ratio, uint32_t* count, uint2* ent, float4* lo, float4* hi)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t m = inst_mesh[i];
-        const uint32_t nt = mesh_tris[m];
-        seg[i] = 0;
-        if (nt == 0) {  // empty mesh: a degenerate box far from everything is never entered
-            lo[i] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
-            hi[i] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        const uint32_t m = inst_mesh[i], base = i * MAX_ENTRIES;
+        if (mesh_tris[m] == 0) {
+            ent[base] = make_uint2(i, CHILD_EMPTY);
+            lo[base] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+            hi[base] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+            count[i] = 1;
             continue;
         }
-        const float* b = &mesh_box[size_t(m) * 6];
         const InstanceData& d = inst[i];
-        float3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
-        for (int c = 0; c < 8; c++) {
-            const float px = (c & 1) ? b[3] : b[0], py = (c & 2) ? b[4] : b[1], pz = (c & 4) ? b[5] : b[2];
-            const float wx = d.m[0][0] * px + d.m[0][1] * py + d.m[0][2] * pz + d.m[0][3];
-            const float wy = d.m[1][0] * px + d.m[1][1] * py + d.m[1][2] * pz + d.m[1][3];
-            const float wz = d.m[2][0] * px + d.m[2][1] * py + d.m[2][2] * pz + d.m[2][3];
-            mn = f3(fminf(mn.x, wx), fminf(mn.y, wy), fminf(mn.z, wz));
-            mx = f3(fmaxf(mx.x, wx), fmaxf(mx.y, wy), fmaxf(mx.z, wz));
+        uint32_t nodes_of[MAX_ENTRIES];
+        float4 blo[MAX_ENTRIES], bhi[MAX_ENTRIES];
+        uint32_t cnt = 1;
+        nodes_of[0] = m;  // the BLAS root of mesh m is node m
+        const float* b = &mesh_box[size_t(m) * 6];
+        world_box(d, b[0], b[1], b[2], b[3], b[4], b[5], blo[0], bhi[0]);
+        for (uint32_t level = 0; level < depth; level++) {
+            uint32_t next_nodes[MAX_ENTRIES];
+            float4 nlo[MAX_ENTRIES], nhi[MAX_ENTRIES];
+            uint32_t nc = 0;
+            for (uint32_t e = 0; e < cnt; e++) {
+                const Node4& nd = nodes[nodes_of[e]];
+                float4 clo[4], chi[4];
+                uint32_t cn[4], k_used = 0;
+                bool open = true;
+                float sum = 0.0f;
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t ch = nd.child[k];
+                    if (ch == CHILD_EMPTY)
+                        continue;
+                    if (ch & CHILD_LEAF) {
+                        open = false;  // a triangle leaf stays reachable only through this node
+                        break;
+                    }
+                    world_box(d, nd.lo_x[k], nd.lo_y[k], nd.lo_z[k], nd.hi_x[k], nd.hi_y[k], nd.hi_z[k], clo[k_used],
+                              chi[k_used]);
+                    sum += half_area(clo[k_used], chi[k_used]);
+                    cn[k_used++] = ch;
+                }
+                open = open && k_used > 0 && sum < ratio * half_area(blo[e], bhi[e]) &&
+                       nc + k_used + (cnt - e - 1) <= MAX_ENTRIES;
+                if (open) {
+                    for (uint32_t k = 0; k < k_used; k++) {
+                        next_nodes[nc] = cn[k];
+                        nlo[nc] = clo[k];
+                        nhi[nc] = chi[k];
+                        nc++;
+                    }
+                } else {
+                    next_nodes[nc] = nodes_of[e];
+                    nlo[nc] = blo[e];
+                    nhi[nc] = bhi[e];
+                    nc++;
+                }
+            }
+            for (uint32_t e = 0; e < nc; e++) {
+                nodes_of[e] = next_nodes[e];
+                blo[e] = nlo[e];
+                bhi[e] = nhi[e];
+            }
+            cnt = nc;
         }
-        // pad for the rounding of the transform
-        const float3 ext = f3(fabsf(mn.x) + fabsf(mx.x), fabsf(mn.y) + fabsf(mx.y), fabsf(mn.z) + fabsf(mx.z));
-        const float eps = 1e-6f;
-        lo[i] = make_float4(mn.x - eps * ext.x, mn.y - eps * ext.y, mn.z - eps * ext.z, 0.0f);
-        hi[i] = make_float4(mx.x + eps * ext.x, mx.y + eps * ext.y, mx.z + eps * ext.z, 0.0f);
+        for (uint32_t e = 0; e < cnt; e++) {
+            ent[base + e] = make_uint2(i, nodes_of[e]);
+            lo[base + e] = blo[e];
+            hi[base + e] = bhi[e];
+        }
+        count[i] = cnt;
     }
+}
+
+// Packs the entries of every instance at its offset (exclusive prefix sum of the counts).
+__global__ void k_pack_entries(uint32_t n, const uint32_t* count, const uint32_t* offset, const uint2* ent,
+                               const float4* lo, const float4* hi, uint2* out_ent, float4* out_lo, float4* out_hi,
+                               uint32_t* seg)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (uint32_t e = 0; e < count[i]; e++) {
+            const uint32_t src = i * MAX_ENTRIES + e, dst = offset[i] + e;
+            out_ent[dst] = ent[src];
+            out_lo[dst] = lo[src];
+            out_hi[dst] = hi[src];
+            seg[dst] = 0;
+        }
 }
 
 inline unsigned grid_for(uint32_t n, unsigned block = 256)
@@ -879,7 +968,7 @@ void free_bvh(BvhBuffers& b)
     (void)hipFree(b.nodes);
     (void)hipFree(b.tris);
     (void)hipFree(b.inst_inv);
-    (void)hipFree(b.inst_blas_root);
+    (void)hipFree(b.tlas_entries);
     b = BvhBuffers{};
 }
 
@@ -902,9 +991,12 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
     const Sah blas_sah{width, leaf, params.sah_node_cost, params.sah_tri_cost};
     const Sah tlas_sah{width, 1, params.sah_node_cost, params.sah_instance_cost};
 
-    // Node budget: BLAS roots + at most one node per binary internal node; TLAS likewise.
+    // Node budget: BLAS roots + at most one node per binary internal node; TLAS likewise, over
+    // at most MAX_ENTRIES entries per instance when instances may open.
     const uint32_t blas_cap = M + T + 1;
-    const uint32_t tlas_cap = instance_count + 2;
+    const uint32_t open_depth = std::min<uint32_t>(params.tlas_open, 2);
+    const uint32_t per_inst = open_depth == 0 ? 1u : (open_depth == 1 ? 4u : MAX_ENTRIES);
+    const uint32_t tlas_cap = instance_count * per_inst + 2;
     out.node_capacity = blas_cap + tlas_cap;
     CHECK(hipMalloc(&out.nodes, size_t(out.node_capacity) * sizeof(Node4)));
     CHECK(hipMalloc(&out.tris, std::max<size_t>(T, 1) * sizeof(TriRec)));
@@ -968,7 +1060,6 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         CHECK(hipStreamSynchronize(st));
     }
     std::vector<InstXform> xf(std::max<uint32_t>(instance_count, 1));
-    std::vector<uint32_t> roots(std::max<uint32_t>(instance_count, 1), CHILD_EMPTY);
     for (uint32_t i = 0; i < instance_count; i++) {
         const InstanceData& d = inst[i];
         double a[3][3], t[3];
@@ -996,35 +1087,57 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
             rows[r] = make_float4(float(iv[r][0]), float(iv[r][1]), float(iv[r][2]), float(tr));
         }
         xf[i] = InstXform{rows[0], rows[1], rows[2]};
-        const uint32_t m = inst_mesh[i];
-        if (m >= M)
+        if (inst_mesh[i] >= M)
             return hipErrorInvalidValue;
-        roots[i] = meshes[m].tri_count ? m : CHILD_EMPTY;
     }
     CHECK(hipMalloc(&out.inst_inv, xf.size() * sizeof(InstXform)));
-    CHECK(hipMalloc(&out.inst_blas_root, roots.size() * 4));
     CHECK(hipMemcpyAsync(out.inst_inv, xf.data(), xf.size() * sizeof(InstXform), hipMemcpyHostToDevice, st));
-    CHECK(hipMemcpyAsync(out.inst_blas_root, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, st));
 
-    // TLAS over instance world bounds.
+    // TLAS over the instances' entries (world bounds of their BLAS roots or opened subtrees).
     out.tlas_root = CHILD_EMPTY;
+    out.entry_count = 0;
     if (instance_count > 0) {
+        float* dummy_box = nullptr;
+        CHECK(sc.alloc(dummy_box, 6));
+        uint32_t *d_cnt, *d_off;
+        uint2* d_ent;
+        float4 *d_lo, *d_hi;
+        const size_t slots = size_t(instance_count) * MAX_ENTRIES;
+        CHECK(sc.alloc(d_cnt, instance_count));
+        CHECK(sc.alloc(d_off, instance_count));
+        CHECK(sc.alloc(d_ent, slots));
+        CHECK(sc.alloc(d_lo, slots));
+        CHECK(sc.alloc(d_hi, slots));
+        hipLaunchKernelGGL(k_open_instances, dim3(grid_for(instance_count, 64)), dim3(64), 0, st, instance_count,
+                           d_instances, d_instance_mesh, d_mesh_tris, mesh_box ? mesh_box : dummy_box, out.nodes,
+                           T > 0 ? open_depth : 0u, params.tlas_open_ratio, d_cnt, d_ent, d_lo, d_hi);
+        CHECK(hipGetLastError());
+        std::vector<uint32_t> cnt(instance_count), off(instance_count);
+        CHECK(hipMemcpyAsync(cnt.data(), d_cnt, size_t(instance_count) * 4, hipMemcpyDeviceToHost, st));
+        CHECK(hipStreamSynchronize(st));
+        uint32_t E = 0;
+        for (uint32_t i = 0; i < instance_count; i++) {
+            if (cnt[i] == 0 || cnt[i] > MAX_ENTRIES)
+                return hipErrorUnknown;
+            off[i] = E;
+            E += cnt[i];
+        }
+        CHECK(hipMemcpyAsync(d_off, off.data(), size_t(instance_count) * 4, hipMemcpyHostToDevice, st));
+        CHECK(hipMalloc(&out.tlas_entries, size_t(E) * sizeof(uint2)));
+        out.entry_count = E;
         BuildSet tl;
-        tl.n = instance_count;
+        tl.n = E;
         tl.segs = 1;
-        std::vector<uint32_t> sb = {0, instance_count};
+        std::vector<uint32_t> sb = {0, E};
         uint32_t* d_sb;
         CHECK(sc.alloc(d_sb, 2));
         CHECK(hipMemcpyAsync(d_sb, sb.data(), 8, hipMemcpyHostToDevice, st));
         tl.seg_begin = d_sb;
-        CHECK(sc.alloc(tl.lo, instance_count));
-        CHECK(sc.alloc(tl.hi, instance_count));
-        CHECK(sc.alloc(tl.seg, instance_count));
-        float* dummy_box = nullptr;
-        CHECK(sc.alloc(dummy_box, 6));
-        hipLaunchKernelGGL(k_instance_bounds, dim3(grid_for(instance_count)), dim3(256), 0, st, instance_count,
-                           d_instances, d_instance_mesh, mesh_box ? mesh_box : dummy_box, d_mesh_tris, tl.lo, tl.hi,
-                           tl.seg);
+        CHECK(sc.alloc(tl.lo, E));
+        CHECK(sc.alloc(tl.hi, E));
+        CHECK(sc.alloc(tl.seg, E));
+        hipLaunchKernelGGL(k_pack_entries, dim3(grid_for(instance_count)), dim3(256), 0, st, instance_count, d_cnt,
+                           d_off, d_ent, d_lo, d_hi, out.tlas_entries, tl.lo, tl.hi, tl.seg);
         const uint32_t tlas_slot = blas_nodes;
         counter = blas_nodes + 1;
         CHECK(hipMemcpyAsync(d_counter, &counter, 4, hipMemcpyHostToDevice, st));

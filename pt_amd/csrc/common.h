@@ -68,7 +68,7 @@ struct DeviceScene {
     const Node4* nodes;
     const TriRec* tris;
     const InstXform* inst_inv;
-    const uint32_t* inst_blas_root;   // root node of each instance's BLAS (CHILD_EMPTY: none)
+    const uint2* tlas_entries;        // TLAS leaf k enters instance .x at BLAS node .y (CHILD_EMPTY: none)
     const float* inst_opacity;        // base-colour alpha of transparent instances, 1 = opaque,
                                       // -1 = alpha from the base-colour map at the hit
     uint32_t tlas_root;               // CHILD_EMPTY when the scene has no instances

@@ -262,6 +262,7 @@ struct Ctx {
     uint32_t leaf_size = 4;
     uint32_t node_width = 4;
     uint32_t bvh_builder = 0;  // BVH_BUILDER_*
+    uint32_t tlas_open = 0;    // PT_OPT_TLAS_OPEN
     bool sort_hits = false;
     uint32_t job_paths = 0;  // capacity of the per-job radiance buffer
     std::vector<pt_mesh_desc> meshes;  // host copy (BVH rebuilds)

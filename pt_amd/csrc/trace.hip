@@ -355,8 +355,8 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 next = stk.pop();
             }
             if ((next & CHILD_LEAF) && inst == NONE) {  // TLAS leaf: enter the instance
-                const uint32_t id = next & ~CHILD_LEAF;
-                const uint32_t root = at32(s.inst_blas_root, id);
+                const uint2 entry = at32(s.tlas_entries, next & ~CHILD_LEAF);
+                const uint32_t id = entry.x, root = entry.y;
                 next = NONE;
                 if (root == CHILD_EMPTY)
                     continue;
@@ -374,7 +374,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                     fc->entries++;
                 if (ANY && ALPHA)
                     opacity = s.inst_opacity[id];
-                // Roots are single nodes; continue with the BLAS root as an internal node.
+                // continue with the entered BLAS node (its root, or a subtree of it) as an internal node
                 next = root;
             }
             break;

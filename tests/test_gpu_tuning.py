@@ -54,10 +54,20 @@ def test_launch_options_leave_the_image_unchanged(inst, opts):
     [(pt_amd.PT_OPT_NODE_WIDTH, 3), (pt_amd.PT_OPT_LEAF_SIZE, 8)],
     [(pt_amd.PT_OPT_BVH_BUILDER, 1)],
     [(pt_amd.PT_OPT_BVH_BUILDER, 1), (pt_amd.PT_OPT_NODE_WIDTH, 2), (pt_amd.PT_OPT_LEAF_SIZE, 1)],
-], ids=["leaf1", "leaf2", "leaf16", "width2", "width3-leaf8", "radix", "radix-width2-leaf1"])
-def test_hierarchy_rebuild(inst, opts):
+    [(pt_amd.PT_OPT_TLAS_OPEN, 1)],
+    [(pt_amd.PT_OPT_TLAS_OPEN, 2)],
+    [("PT_TLAS_OPEN_RATIO", "8"), (pt_amd.PT_OPT_TLAS_OPEN, 2), (pt_amd.PT_OPT_LEAF_SIZE, 1)],
+], ids=["leaf1", "leaf2", "leaf16", "width2", "width3-leaf8", "radix", "radix-width2-leaf1", "open1", "open2",
+        "open2-all-leaf1"])
+def test_hierarchy_rebuild(inst, opts, monkeypatch):
+    """Another hierarchy (leaf size, node width, builder, TLAS entries at opened BLAS subtrees,
+    where a ratio of 8 opens every instance whose top two levels are internal nodes) gives the
+    same image up to exact ties; pt_create validates each hierarchy on the host first."""
     scene, cam, ref = inst
-    img, st = _render(scene, cam, opts)
+    for k, v in opts:
+        if isinstance(k, str):
+            monkeypatch.setenv(k, v)
+    img, st = _render(scene, cam, [o for o in opts if not isinstance(o[0], str)])
     assert st["tri_record_count"] == scene.info()["total_triangles"]
     # another hierarchy may only resolve exact ties differently
     same = np.all(img == ref, axis=2)
@@ -69,7 +79,8 @@ def test_option_checks(inst):
     t = pt_amd.Tracer(scene, 16, 16)
     for k, v in [(pt_amd.PT_OPT_BLOCKS_PER_CU, 0), (pt_amd.PT_OPT_BLOCKS_PER_CU, 65),
                  (pt_amd.PT_OPT_POOL_SLOTS, 100), (pt_amd.PT_OPT_LDS_STACK, 12), (pt_amd.PT_OPT_LEAF_SIZE, 17),
-                 (pt_amd.PT_OPT_NODE_WIDTH, 1), (pt_amd.PT_OPT_NODE_WIDTH, 5), (pt_amd.PT_OPT_BVH_BUILDER, 2)]:
+                 (pt_amd.PT_OPT_NODE_WIDTH, 1), (pt_amd.PT_OPT_NODE_WIDTH, 5), (pt_amd.PT_OPT_BVH_BUILDER, 2),
+                 (pt_amd.PT_OPT_TLAS_OPEN, 3), (pt_amd.PT_OPT_SPILL_DEPTH, 113)]:
         with pytest.raises(pt_amd.PtError):
             t.set_option(k, v)
     with pytest.raises(pt_amd.PtError, match="no camera"):
