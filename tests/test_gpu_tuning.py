@@ -105,7 +105,7 @@ def test_autotune_picks_a_configuration(inst):
     assert t.stats()["sample_index"] == 0
     t.render(4)  # the accumulation restarted: the tuned renderer gives the same image
     same = np.all(t.read() == ref, axis=2)
-    assert same.mean() >= (1.0 if (r["leaf_size"], r["node_width"]) == (4, 4) else 0.999)
+    assert same.mean() >= (1.0 if (r["leaf_size"], r["node_width"], r["tlas_open"]) == (4, 4, 0) else 0.999)
     t.close()
 
 
