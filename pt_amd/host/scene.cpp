@@ -751,44 +751,56 @@ Scene load_gltf(const std::string& path)
 }
 
 // ---------------------------------------------------------------------------------------
-// .scn parsing (src/scene/scenebuild.cpp, src/io/tokenstream.cpp, src/io/parseutil.cpp)
+// .scn parsing. The grammar, defaults and messages are the reference's (SceneBuilder,
+// src/scene/scenebuild.cpp:55-365; TokenStream, src/io/tokenstream.cpp; parse_vector etc.,
+// src/io/parseutil.cpp); the reader is a line/word state machine: every non-empty line (carriage
+// returns and tabs deleted, words split at spaces) goes to the handler of the section it is in.
 
 namespace {
 
-struct Tokens {
-    std::vector<std::string> t;
-    size_t i = 0;
-    Tokens(const std::string& raw, char delim)
+// The words of one line and a cursor over them.
+struct Words {
+    std::vector<std::string> w;
+    size_t at = 0;
+    size_t left() const { return w.size() - at; }
+    const std::string& take()
     {
-        std::stringstream ss(raw);
-        std::string tok;
-        while (std::getline(ss, tok, delim))
-            if (!tok.empty())
-                t.push_back(tok);
-    }
-    bool done() const { return i >= t.size(); }
-    size_t remaining() const { return t.size() - i; }
-    const std::string& next()
-    {
-        if (done())
+        if (at >= w.size())
             throw std::runtime_error("Token stream has no next token");
-        return t[i++];
-    }
-    void rollback(size_t n)
-    {
-        if (i < n)
-            throw std::runtime_error("Cannot roll further back than the start");
-        i -= n;
+        return w[at++];
     }
 };
 
-float stof_checked(const std::string& s)
+std::vector<Words> scn_lines(std::string text)
 {
-    // std::stof: leading whitespace skipped, trailing garbage ignored, no digits -> throw.
+    text.erase(std::remove_if(text.begin(), text.end(), [](char c) { return c == '\r' || c == '\t'; }), text.end());
+    std::vector<Words> lines;
+    size_t pos = 0;
+    while (pos <= text.size()) {
+        const size_t nl = std::min(text.find('\n', pos), text.size());
+        Words line;
+        size_t p = pos;
+        while (p < nl) {
+            const size_t sp = std::min(text.find(' ', p), nl);
+            if (sp > p)
+                line.w.emplace_back(text, p, sp - p);
+            p = sp + 1;
+        }
+        if (!line.w.empty())
+            lines.push_back(std::move(line));
+        pos = nl + 1;
+    }
+    return lines;
+}
+
+// std::stof's contract: leading whitespace skipped, trailing characters ignored, no digits or
+// out of range -> exception.
+float number(const std::string& s)
+{
     const char* b = s.c_str();
     char* e = nullptr;
     errno = 0;
-    float v = std::strtof(b, &e);
+    const float v = std::strtof(b, &e);
     if (e == b)
         throw std::invalid_argument("stof");
     if (errno == ERANGE)
@@ -796,152 +808,165 @@ float stof_checked(const std::string& s)
     return v;
 }
 
-V3 parse_vector(Tokens& tk)
+// Three numbers; `exact` demands that nothing follows them on the line.
+V3 triple(Words& ws, bool exact, const char* err)
 {
-    if (tk.remaining() < 3)
-        throw std::runtime_error("Vectors need three components");
-    V3 v;
-    v.x = stof_checked(tk.next());
-    v.y = stof_checked(tk.next());
-    v.z = stof_checked(tk.next());
-    return v;
+    if (exact ? ws.left() != 3 : ws.left() < 3)
+        throw std::runtime_error(err);
+    const float x = number(ws.take()), y = number(ws.take()), z = number(ws.take());
+    return V3(x, y, z);
 }
 
-V3 parse_exact3(Tokens& tk, const char* what)
-{
-    if (tk.remaining() != 3)
-        throw std::runtime_error(what);
-    V3 v;
-    v.x = stof_checked(tk.next());
-    v.y = stof_checked(tk.next());
-    v.z = stof_checked(tk.next());
-    return v;
-}
+class ScnReader {
+public:
+    enum class Section { NONE, CAMERA, OBJECTS };
 
-struct SceneInstance {
-    M4 transform;
-};
-struct Subscene {
-    std::string identifier, path;
-    std::vector<SceneInstance> instances;
-};
-
-struct ParserState {
-    std::optional<std::string> current_object;
-    M4 current_transform = M4::identity();
-    std::vector<Subscene> subscenes;
-    std::unordered_map<std::string, uint32_t> subscene_indices;
-    std::vector<PointLight> point_lights;
-    std::vector<DirLight> dir_lights;
-    std::optional<CameraSection> camera;
-    V3 envcol{0, 0, 0};
-    std::string envmap;
-};
-
-bool is_section_label(const std::string& tok) { return tok[tok.size() - 1] == ':'; }
-
-void parse_camera(Tokens& lines, ParserState& st)
-{
-    if (st.camera)
-        throw std::runtime_error("Camera already set.");
-    CameraSection cam;
-    while (!lines.done()) {
-        Tokens tk(lines.next(), ' ');
-        if (tk.done())
-            continue;
-        const std::string tok = tk.next();
-        if (tok == "position") cam.position = parse_vector(tk);
-        else if (tok == "lookdir") cam.lookdir = parse_vector(tk);
-        else if (tok == "up") cam.up = parse_vector(tk);
-        else if (tok == "fov") cam.fov = stof_checked(tk.next());
-        else if (tok == "near") cam.near_clip = stof_checked(tk.next());
-        else if (tok == "far") cam.far_clip = stof_checked(tk.next());
-        else if (tok == "focusdist") cam.focus_dist = stof_checked(tk.next());
-        else if (tok == "lensradius") cam.lens_radius = stof_checked(tk.next());
-        else if (tok == "exposure") cam.exposure = stof_checked(tk.next());
-        else if (tok == "envcol") st.envcol = from_srgb(parse_vector(tk));
-        else if (tok == "envmap") st.envmap = tk.next();
-        else if (is_section_label(tok)) {
-            lines.rollback(1);
+    void line(Words& ws)
+    {
+        const std::string& head = ws.take();
+        if (head.back() == ':') {  // a section label ends the section that is open
+            close_section();
+            if (head == "camera:")
+                open_camera();
+            else if (head == "objects:")
+                section_ = Section::OBJECTS;
+            else
+                throw std::runtime_error("Unrecognised token " + head);
+            return;
+        }
+        switch (section_) {
+        case Section::NONE:
+            throw std::runtime_error("Unrecognised token " + head);
+        case Section::CAMERA:
+            camera_word(head, ws);
             break;
-        } else
-            throw std::runtime_error("Unrecognised token in camera section " + tok);
-    }
-    st.camera = cam;
-}
-
-void finish_instance(ParserState& st)
-{
-    if (st.current_object.has_value())
-        st.subscenes[st.subscene_indices[st.current_object.value()]].instances.push_back({st.current_transform});
-}
-
-void parse_light(Tokens& tk, ParserState& st, bool point)
-{
-    if (tk.remaining() != 7)
-        throw std::runtime_error("Invalid light syntax.");
-    V3 a = parse_vector(tk);
-    if (tk.next() != ",")
-        throw std::runtime_error("Invalid light syntax.");
-    V3 col = from_srgb(parse_vector(tk));
-    if (point)
-        st.point_lights.push_back({a, col});
-    else
-        st.dir_lights.push_back({a, col});
-}
-
-void parse_objects(Tokens& lines, ParserState& st)
-{
-    while (!lines.done()) {
-        Tokens tk(lines.next(), ' ');
-        if (tk.done())
-            continue;
-        const std::string tok = tk.next();
-        if (tok == "t") {
-            if (!st.current_object)
-                throw std::runtime_error("No instance to translate.");
-            st.current_transform = translation(parse_exact3(tk, "Translation did not have three coordinates")) *
-                                   st.current_transform;
-        } else if (tok == "r") {
-            if (!st.current_object)
-                throw std::runtime_error("No instance to rotate.");
-            if (tk.remaining() != 4)
-                throw std::runtime_error("Rotation did not have three coordinates and angle");
-            V3 u;
-            u.x = stof_checked(tk.next());
-            u.y = stof_checked(tk.next());
-            u.z = stof_checked(tk.next());
-            float angle = stof_checked(tk.next());
-            st.current_transform = rotation(u, angle) * st.current_transform;
-        } else if (tok == "s") {
-            if (!st.current_object)
-                throw std::runtime_error("No instance to scale.");
-            st.current_transform = scaling(parse_exact3(tk, "Scaling did not have three coordinates")) *
-                                   st.current_transform;
-        } else if (tok == "basecol" || tok == "roughness" || tok == "metalness" || tok == "specular") {
-            // Accepted and ignored, as in the reference (scenebuild.cpp:222-237).
-            if (!st.current_object)
-                throw std::runtime_error("No instance to apply " + tok + " parameter to.");
-        } else if (tok == "plight") {
-            parse_light(tk, st, true);
-        } else if (tok == "dlight") {
-            parse_light(tk, st, false);
-        } else if (is_section_label(tok)) {
-            lines.rollback(1);
+        case Section::OBJECTS:
+            object_word(head, ws);
             break;
-        } else if (tk.done()) {
-            if (!st.subscene_indices.count(tok))
-                throw std::runtime_error("Object " + tok + " not recognised.");
-            finish_instance(st);
-            st.current_object = tok;
-            st.current_transform = M4::identity();
-        } else {
-            st.subscene_indices[tok] = uint32_t(st.subscenes.size());
-            st.subscenes.push_back({tok, tk.next(), {}});
         }
     }
-    finish_instance(st);
-}
+
+    void close_section()
+    {
+        // an objects section adds the instance that is still being described (the reference adds
+        // each instance when the next one starts, and the last one when its section ends)
+        if (section_ == Section::OBJECTS && current_)
+            objects[index_.at(*current_)].instances.push_back(transform_);
+        section_ = Section::NONE;
+    }
+
+    struct Object {
+        std::string file;
+        std::vector<M4> instances;
+    };
+    std::optional<CameraSection> camera;
+    V3 envcol{0, 0, 0};
+    std::vector<PointLight> point_lights;
+    std::vector<DirLight> dir_lights;
+    std::vector<Object> objects;  // in declaration order; a name declared again names a new object
+
+private:
+    void open_camera()
+    {
+        if (camera)
+            throw std::runtime_error("Camera already set.");
+        camera = CameraSection{};
+        section_ = Section::CAMERA;
+    }
+
+    void camera_word(const std::string& key, Words& ws)
+    {
+        CameraSection& c = *camera;
+        static const std::unordered_map<std::string, V3 CameraSection::*> vectors = {
+            {"position", &CameraSection::position}, {"lookdir", &CameraSection::lookdir}, {"up", &CameraSection::up}};
+        static const std::unordered_map<std::string, float CameraSection::*> scalars = {
+            {"fov", &CameraSection::fov},
+            {"near", &CameraSection::near_clip},
+            {"far", &CameraSection::far_clip},
+            {"focusdist", &CameraSection::focus_dist},
+            {"lensradius", &CameraSection::lens_radius},
+            {"exposure", &CameraSection::exposure}};
+        if (auto v = vectors.find(key); v != vectors.end())
+            c.*(v->second) = triple(ws, false, "Vectors need three components");
+        else if (auto f = scalars.find(key); f != scalars.end())
+            c.*(f->second) = number(ws.take());
+        else if (key == "envcol")
+            envcol = from_srgb(triple(ws, false, "Vectors need three components"));
+        else if (key == "envmap")
+            (void)ws.take();  // the environment map path is read and not used
+        else
+            throw std::runtime_error("Unrecognised token in camera section " + key);
+    }
+
+    // Transform keywords: the matrix each makes, multiplied onto the left of the instance's.
+    void transform_word(char kind, Words& ws)
+    {
+        static const char* const none[3] = {"No instance to translate.", "No instance to rotate.",
+                                            "No instance to scale."};
+        const int k = kind == 't' ? 0 : kind == 'r' ? 1 : 2;
+        if (!current_)
+            throw std::runtime_error(none[k]);
+        M4 m;
+        if (kind == 't') {
+            m = translation(triple(ws, true, "Translation did not have three coordinates"));
+        } else if (kind == 's') {
+            m = scaling(triple(ws, true, "Scaling did not have three coordinates"));
+        } else {
+            if (ws.left() != 4)
+                throw std::runtime_error("Rotation did not have three coordinates and angle");
+            const V3 axis = triple(ws, false, "Rotation did not have three coordinates and angle");
+            m = rotation(axis, number(ws.take()));
+        }
+        transform_ = m * transform_;
+    }
+
+    void light_word(bool point, Words& ws)
+    {
+        if (ws.left() != 7)
+            throw std::runtime_error("Invalid light syntax.");
+        const V3 a = triple(ws, false, "Vectors need three components");
+        if (ws.take() != ",")  // the reference's messages differ in their final period here
+            throw std::runtime_error(point ? "Invalid light syntax." : "Invalid light syntax");
+        const V3 col = from_srgb(triple(ws, false, "Vectors need three components"));
+        if (point)
+            point_lights.push_back({a, col});
+        else
+            dir_lights.push_back({a, col});
+    }
+
+    void object_word(const std::string& word, Words& ws)
+    {
+        // material keywords are accepted and have no effect (scenebuild.cpp:222-237)
+        static const std::unordered_map<std::string, const char*> material = {
+            {"basecol", "base colour"}, {"roughness", "roughness"}, {"metalness", "metalness"},
+            {"specular", "specular"}};
+        if (word.size() == 1 && (word[0] == 't' || word[0] == 'r' || word[0] == 's')) {
+            transform_word(word[0], ws);
+        } else if (auto m = material.find(word); m != material.end()) {
+            if (!current_)
+                throw std::runtime_error(std::string("No instance to apply ") + m->second + " parameter to.");
+        } else if (word == "plight" || word == "dlight") {
+            light_word(word == "plight", ws);
+        } else if (ws.left() == 0) {
+            // a name alone starts an instance of that object; the one before it is complete
+            if (!index_.count(word))
+                throw std::runtime_error("Object " + word + " not recognised.");
+            if (current_)
+                objects[index_.at(*current_)].instances.push_back(transform_);
+            current_ = word;
+            transform_ = M4::identity();
+        } else {
+            // "name file": declares an object; instances made from here on use this declaration
+            index_[word] = objects.size();
+            objects.push_back(Object{ws.take(), {}});
+        }
+    }
+
+    Section section_ = Section::NONE;
+    std::unordered_map<std::string, size_t> index_;  // object name -> its latest declaration
+    std::optional<std::string> current_;
+    M4 transform_ = M4::identity();
+};
 
 }  // namespace
 
@@ -949,46 +974,33 @@ Scene load_scn(const std::string& path)
 {
     Scene scene;
     scene.resource_directory = directory_of(path);
-    std::string text = read_file(path, false);
-    text.erase(std::remove_if(text.begin(), text.end(), [](char c) { return c == '\r' || c == '\t'; }),
-               text.end());
-    Tokens lines(text, '\n');
-    ParserState st;
-    while (!lines.done()) {
-        Tokens tk(lines.next(), ' ');
-        if (tk.done())
-            continue;
-        const std::string tok = tk.next();
-        if (tok == "camera:")
-            parse_camera(lines, st);
-        else if (tok == "objects:")
-            parse_objects(lines, st);
-        else
-            throw std::runtime_error("Unrecognised token " + tok);
-    }
-    if (!st.camera)
+    ScnReader rd;
+    for (Words& ws : scn_lines(read_file(path, false)))
+        rd.line(ws);
+    rd.close_section();
+    if (!rd.camera)
         throw std::runtime_error("No camera section in scene description.");
-    scene.camera = *st.camera;
-    scene.envcol = st.envcol;
-    scene.point_lights = st.point_lights;
-    scene.dir_lights = st.dir_lights;
+    scene.camera = *rd.camera;
+    scene.envcol = rd.envcol;
+    scene.point_lights = rd.point_lights;
+    scene.dir_lights = rd.dir_lights;
 
-    for (const Subscene& sub : st.subscenes) {
-        if (sub.instances.empty())
+    for (const ScnReader::Object& obj : rd.objects) {
+        if (obj.instances.empty())
             continue;
         const uint32_t mesh_offset = uint32_t(scene.meshes.size());
         const uint32_t material_offset = uint32_t(scene.materials.size());
-        Scene loaded = load_gltf(scene.resource_directory + sub.path);
+        Scene loaded = load_gltf(scene.resource_directory + obj.file);
         for (Mesh& m : loaded.meshes)
             scene.meshes.push_back(std::move(m));
         // the object's lights, once per instance, in the instance's frame (scenebuild.cpp:323-335)
-        for (const SceneInstance& si : sub.instances) {
+        for (const M4& xf : obj.instances) {
             for (const PointLight& pl : loaded.point_lights) {
-                const V4 p = si.transform * V4(pl.pos.x, pl.pos.y, pl.pos.z, 1.0f);
+                const V4 p = xf * V4(pl.pos.x, pl.pos.y, pl.pos.z, 1.0f);
                 scene.point_lights.push_back({V3{p.x, p.y, p.z}, pl.col});
             }
             for (const DirLight& dl : loaded.dir_lights) {
-                const V4 d = si.transform * V4(dl.dir.x, dl.dir.y, dl.dir.z, 0.0f);
+                const V4 d = xf * V4(dl.dir.x, dl.dir.y, dl.dir.z, 0.0f);
                 scene.dir_lights.push_back({V3{d.x, d.y, d.z}, dl.col});
             }
         }
@@ -996,10 +1008,10 @@ Scene load_scn(const std::string& path)
             Variant v;
             v.mesh = lv.mesh + mesh_offset;
             v.material = lv.material + material_offset;
-            v.instances.reserve(lv.instances.size() * sub.instances.size());
-            for (const SceneInstance& si : sub.instances)
+            v.instances.reserve(lv.instances.size() * obj.instances.size());
+            for (const M4& xf : obj.instances)
                 for (const M4& oi : lv.instances)
-                    v.instances.push_back(si.transform * oi);
+                    v.instances.push_back(xf * oi);
             scene.variants.push_back(std::move(v));
         }
         // PbrMaterial::offset_maps_by (src/scene/material.cpp:16-42) for the maps this loader sets

@@ -71,10 +71,48 @@ def test_scn_rotation_below_threshold_is_identity(tmp_path):
     ("camera:\nobjects:\n plight 1 2 3 4 5 6\n", "Invalid light syntax"),
     ("camera:\n position 1 2\n", "Vectors need three components"),
     ("camera:\nobjects:\n tri missing.gltf\n tri\n", "Could not open file"),
+    ("camera:\nobjects:\n basecol 1 0 0\n", "No instance to apply base colour parameter to."),
+    ("camera:\nobjects:\n metalness 1\n", "No instance to apply metalness parameter to."),
+    ("camera:\nobjects:\n r 1 0 0 1\n", "No instance to rotate."),
+    ("camera:\nobjects:\n s 1 1 1\n", "No instance to scale."),
+    ("camera:\nobjects:\n dlight 1 2 3 x 4 5 6\n", "Invalid light syntax"),
+    ("camera:\nobjects:\n plight 1 2 3 x 4 5 6\n", "Invalid light syntax."),
+    ("camera:\n fov\n", "Token stream has no next token"),
+    ("camera:\n fov abc\n", "stof"),
+    ("camera:\nobjects:\n tri tri.gltf\n tri\n t 1 2 3 4\n", "Translation did not have three coordinates"),
+    ("camera: junk\nobjects:\n tri tri.gltf\n tri\n objects\n", "Object objects not recognised."),
+    ("camera:\nobjects:\n tri tri.gltf\n tri\nfoo:\n", "Unrecognised token foo:"),
 ])
 def test_scn_errors(tmp_path, text, msg):
     with pytest.raises(pt_amd.PtError, match=re.escape(msg)):
         pt_amd.Scene(_scn(tmp_path, text))
+
+
+def test_scn_light_comma_messages_differ_in_their_period(tmp_path):
+    # scenebuild.cpp:70 vs :87: the directional light's comma check has no final period
+    for kw, want in (("plight", "Invalid light syntax."), ("dlight", "Invalid light syntax")):
+        with pytest.raises(pt_amd.PtError) as e:
+            pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n %s 1 2 3 x 4 5 6\n" % kw))
+        assert str(e.value).endswith(want), str(e.value)
+
+
+def test_scn_declarations_and_instance_flushing(tmp_path):
+    """A name followed by any word declares an object (so 'position 3 0 0' in the objects section
+    declares 'position' with file '3', unused); a name declared again is a new object whose
+    instances use the new file; the instance being described when an objects section ends is
+    added then, and again at the end of a later objects section, as the reference adds it
+    (scenebuild.cpp:98-125, 245-257)."""
+    s = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n position 3 0 0\n tri\n  t 1 0 0\n"))
+    assert s.info()["instances"] == 1
+    # a second objects section adds the pending instance once more
+    s2 = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n tri\n  t 2 0 0\nobjects:\n"))
+    inst = s2.arrays()["instances"].view(np.float32).reshape(-1, 32)
+    assert len(inst) == 2 and inst[0, 7] == 2.0 and inst[1, 7] == 2.0
+    # re-declaring tri: instances before it keep the old file (a missing one here would fail the load)
+    import shutil
+    shutil.copy(str(tmp_path / "tri.gltf"), str(tmp_path / "tri2.gltf"))
+    s3 = pt_amd.Scene(_scn(tmp_path, "camera:\nobjects:\n tri tri.gltf\n tri\n tri tri2.gltf\n tri\n  t 0 1 0\n"))
+    assert s3.info()["instances"] == 2
 
 
 def test_point_and_directional_lights_in_light_table(tmp_path):
