@@ -243,6 +243,15 @@ int pt_trace_primary(pt_ctx* ctx, uint32_t sample_index, pt_hit* hits);
  * pathtrace.ch.hlsl:199 with ACCEPT_FIRST_HIT_AND_END_SEARCH). Synchronises. */
 int pt_trace_rays(pt_ctx* ctx, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit);
 
+/* pt_trace_rays without any device allocation: no counterpart in the reference. For callers
+ * that cannot allocate (tight serving loops, code that must not call hipMalloc): the rays and
+ * hits are staged through a fixed device array of the library (one per GPU, PT_NOALLOC_MAX_RAYS
+ * rays and hits, about 13 MiB, held until the process ends). count > PT_NOALLOC_MAX_RAYS ->
+ * PT_EINVAL. Results equal pt_trace_rays'. Calls that overlap on one GPU (from several contexts
+ * or threads) race for the array. Synchronises. */
+#define PT_NOALLOC_MAX_RAYS 262144u
+int pt_trace_rays_noalloc(pt_ctx* ctx, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit);
+
 /* Counters and timings (the reference prints its timing in ptdispatch.cpp:65-70). */
 int pt_get_stats(pt_ctx* ctx, pt_stats* out);
 

@@ -190,6 +190,7 @@ def hip_lib():
         lib.pt_read_tonemapped.argtypes = [P, C.c_void_p, C.c_size_t]
         lib.pt_trace_primary.argtypes = [P, C.c_uint32, C.c_void_p]
         lib.pt_trace_rays.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
+        lib.pt_trace_rays_noalloc.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
         lib.pt_get_stats.argtypes = [P, C.POINTER(Stats)]
         lib.pt_synchronize.argtypes = [P]
         lib.pt_set_option.argtypes = [P, C.c_uint32, C.c_int64]
@@ -427,6 +428,14 @@ class Tracer:
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         out = np.zeros(len(rays), HIT_DTYPE)
         self._check(self._lib.pt_trace_rays(self._h, rays.ctypes.data, len(rays), out.ctypes.data, int(any_hit)))
+        return out
+
+    def trace_rays_noalloc(self, rays, any_hit=False):
+        """trace_rays through the library's fixed staging array (no device allocation)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        out = np.zeros(len(rays), HIT_DTYPE)
+        self._check(self._lib.pt_trace_rays_noalloc(self._h, rays.ctypes.data, len(rays), out.ctypes.data,
+                                                    int(any_hit)))
         return out
 
     def stats(self):

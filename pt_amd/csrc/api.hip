@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -14,6 +15,41 @@
 #include "pthip.h"
 
 using namespace ptk;
+
+// Staging area of pt_trace_rays_noalloc: the rays of a call, then their hits, in device memory
+// that exists from load time on, so that call allocates nothing. One copy per GPU (each device
+// holds its own array until the process ends); calls that overlap on one GPU race for it.
+static __device__ __align__(256) unsigned char scratch[size_t(PT_NOALLOC_MAX_RAYS) * (sizeof(pt_ray) + sizeof(pt_hit))];
+
+namespace {
+
+// The array's device address on each device: hipGetSymbolAddress, called once per device with
+// that device current (by pt_create); the host-side name `scratch` is not a device address.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+hipError_t scratch_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device])
+        return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess)
+        g_scratch_ptr[device] = p;
+    return e;
+}
+
+void* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return (device >= 0 && device < MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
+}  // namespace
 
 struct pt_ctx {
     Ctx c;
@@ -635,6 +671,7 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
     int code;
     if ((code = [&]() -> int {
              HIP_TRY(&c, hipGetDevice(&c.device), "hipGetDevice");
+             HIP_TRY(&c, scratch_setup(c.device), "staging area of pt_trace_rays_noalloc");
              HIP_TRY(&c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
              HIP_TRY(&c, hipEventCreate(&c.ev_start), "hipEventCreate");
              HIP_TRY(&c, hipEventCreate(&c.ev_stop), "hipEventCreate");
@@ -904,6 +941,34 @@ int pt_trace_rays(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, i
         (void)hipFree(dh);
     if (e != hipSuccess)
         return hip_fail(&c, e, "trace rays");
+    return query_overflow(c);
+}
+
+int pt_trace_rays_noalloc(pt_ctx* h, const pt_ray* rays, uint32_t count, pt_hit* hits, int any_hit)
+{
+    if (!h)
+        return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
+    Ctx& c = h->c;
+    if (count == 0)
+        return PT_OK;
+    if (!rays || !hits)
+        return fail(&c, PT_EINVAL, "null ray or hit array");
+    if (count > PT_NOALLOC_MAX_RAYS)
+        return fail(&c, PT_EINVAL,
+                    "at most " + std::to_string(PT_NOALLOC_MAX_RAYS) + " rays per pt_trace_rays_noalloc call");
+    char* base = static_cast<char*>(scratch_ptr(c.device));
+    if (!base)
+        return fail(&c, PT_ESTATE, "staging area of pt_trace_rays_noalloc not set up on this device");
+    pt_ray* dr = reinterpret_cast<pt_ray*>(base);
+    pt_hit* dh = reinterpret_cast<pt_hit*>(base + size_t(PT_NOALLOC_MAX_RAYS) * sizeof(pt_ray));
+    HIP_TRY(&c, hipMemsetAsync(c.wf.counters + CTR(9), 0, 4, c.stream), "error flags");
+    HIP_TRY(&c, hipMemcpyAsync(dr, rays, size_t(count) * sizeof(pt_ray), hipMemcpyHostToDevice, c.stream), "upload");
+    HIP_TRY(&c, trace_rays(c, dr, count, dh, any_hit != 0), "trace rays");
+    HIP_TRY(&c, hipMemcpyAsync(hits, dh, size_t(count) * sizeof(pt_hit), hipMemcpyDeviceToHost, c.stream),
+            "readback");
+    HIP_TRY(&c, hipStreamSynchronize(c.stream), "trace rays");
     return query_overflow(c);
 }
 
