@@ -300,10 +300,11 @@ void pt_batch_destroy(pt_batch* batch);
 /* ---- Learned radiance cache ---------------------------------------------------------------
  * No counterpart in the reference: an optional estimator on top of its path tracer. A small
  * MLP (96 encoded inputs, five hidden layers of 64 ReLU units, RGB out; BF16 matrix cores,
- * FP32 accumulation and FP32 Adam) learns, online, the radiance leaving the second path
- * vertex from the tracer's own training paths. With query != 0 every other path ends at
- * that vertex with the cache's prediction (a biased, lower-variance estimate); with
- * query == 0 the renderer only trains, and its images equal the plain tracer's. */
+ * FP32 accumulation and FP32 Adam) learns, online, the radiance leaving the second (or
+ * query_vertex-th) path vertex from the tracer's own training paths. With query != 0 every
+ * other path ends at that vertex with the cache's prediction (a biased, lower-variance
+ * estimate); with query == 0 the renderer only trains, and its images equal the plain
+ * tracer's. */
 #define PT_CACHE_FEATURES 96 /* encoded input width of pt_cache_forward / pt_cache_train_batch */
 
 typedef struct pt_cache_config {
@@ -323,7 +324,14 @@ typedef struct pt_cache_config {
                             * it goes on with probability p and weight 1/p while the prediction
                             * is added with weight 1 - 1/p, a control variate (unbiased; per-sample
                             * sums may then be negative and are accumulated as they are) */
-    uint32_t reserved;
+    uint32_t query_vertex; /* path vertex whose outgoing radiance the cache learns and answers
+                            * (2..8; 0: the second vertex). Later vertices leave more of each
+                            * path unbiased and weigh the cache's error by the throughput to them */
+    float ema_decay;       /* 0: queries use the trained weights. d in (0, 1): queries use the
+                            * exponential moving average e <- d e + (1 - d) w of the weights after
+                            * each optimiser step, from e = 0, divided by 1 - d^steps (restarted
+                            * by pt_cache_configure and pt_cache_set_params) */
+    uint32_t reserved;     /* must be 0 */
 } pt_cache_config;
 
 typedef struct pt_cache_stats {
@@ -348,6 +356,9 @@ int pt_cache_get_stats(pt_ctx* ctx, pt_cache_stats* out);
 uint32_t pt_cache_param_count(void);
 int pt_cache_get_params(pt_ctx* ctx, float* params, size_t count);
 int pt_cache_set_params(pt_ctx* ctx, const float* params, size_t count);
+/* The fp32 weights the renderer's queries read: the trained ones, or with ema_decay > 0 the
+ * bias-corrected moving average (equal to the trained weights before the first step). */
+int pt_cache_get_query_params(pt_ctx* ctx, float* params, size_t count);
 
 /* Predictions of n explicit feature rows (n x PT_CACHE_FEATURES floats, host memory) into
  * out (n x 3). Inputs and weights are rounded to BF16 as in the renderer. */

@@ -123,7 +123,7 @@ class CacheConfig(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("query", C.c_uint32), ("train_stride", C.c_uint32),
                 ("train_steps", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
                 ("seed", C.c_uint32), ("frame_spp", C.c_uint32), ("continue_prob", C.c_float),
-                ("reserved", C.c_uint32)]
+                ("query_vertex", C.c_uint32), ("ema_decay", C.c_float), ("reserved", C.c_uint32)]
 
 
 class CacheStats(C.Structure):
@@ -217,6 +217,7 @@ def hip_lib():
         lib.pt_cache_param_count.argtypes = []
         lib.pt_cache_param_count.restype = C.c_uint32
         lib.pt_cache_get_params.argtypes = [P, C.c_void_p, C.c_size_t]
+        lib.pt_cache_get_query_params.argtypes = [P, C.c_void_p, C.c_size_t]
         lib.pt_cache_set_params.argtypes = [P, C.c_void_p, C.c_size_t]
         lib.pt_cache_forward.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
         lib.pt_cache_encode.argtypes = [P, C.c_void_p, C.c_uint32, C.c_void_p]
@@ -460,9 +461,9 @@ class Tracer:
     # ---- learned radiance cache (no counterpart in the reference)
 
     def cache_configure(self, enable=True, query=True, train_stride=32, train_steps=4, batch=16384,
-                        learning_rate=1e-3, seed=1, frame_spp=1, continue_prob=0.0):
+                        learning_rate=1e-3, seed=1, frame_spp=1, continue_prob=0.0, query_vertex=0, ema_decay=0.0):
         cfg = CacheConfig(int(enable), int(query), train_stride, train_steps, batch, learning_rate, seed, frame_spp,
-                          continue_prob, 0)
+                          continue_prob, query_vertex, ema_decay, 0)
         self._check(self._lib.pt_cache_configure(self._h, C.byref(cfg)))
 
     def cache_stats(self):
@@ -477,6 +478,12 @@ class Tracer:
         n = self._lib.pt_cache_param_count()
         out = np.zeros(n, np.float32)
         self._check(self._lib.pt_cache_get_params(self._h, out.ctypes.data, n))
+        return out
+
+    def cache_query_params(self):
+        n = self._lib.pt_cache_param_count()
+        out = np.zeros(n, np.float32)
+        self._check(self._lib.pt_cache_get_query_params(self._h, out.ctypes.data, n))
         return out
 
     def cache_set_params(self, params):

@@ -1134,7 +1134,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
         return PT_OK;
     }
     if (cfg->train_stride == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f) || cfg->batch > (1u << 24) ||
-        !(cfg->continue_prob >= 0.0f && cfg->continue_prob <= 1.0f))
+        !(cfg->continue_prob >= 0.0f && cfg->continue_prob <= 1.0f) || cfg->query_vertex == 1 ||
+        cfg->query_vertex > 8 || !(cfg->ema_decay >= 0.0f && cfg->ema_decay < 1.0f) || cfg->reserved != 0)
         return fail(&c, PT_EINVAL, "invalid cache configuration");
     CacheState& cs = c.cache;
     cs.query = cfg->query != 0;
@@ -1145,6 +1146,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
     cs.seed = cfg->seed;
     cs.frame_spp = cfg->frame_spp;
     cs.cont_prob = cfg->continue_prob;
+    cs.query_vertex = cfg->query_vertex ? cfg->query_vertex : 2u;
+    cs.ema_decay = cfg->ema_decay;
     cs.records_total = 0;
     cs.last_records = 0;
     // one training record per training path of a job
@@ -1207,6 +1210,21 @@ int pt_cache_get_params(pt_ctx* h, float* params, size_t count)
     if (!params || count != size_t(mlp_param_count()))
         return fail(&c, PT_EINVAL, "parameter buffer must hold pt_cache_param_count() floats");
     HIP_TRY(&c, cache_get_params(c.cache, params, c.stream), "cache parameters");
+    return PT_OK;
+}
+
+int pt_cache_get_query_params(pt_ctx* h, float* params, size_t count)
+{
+    if (!h)
+        return PT_EINVAL;
+    if (int v = settle(h->c))
+        return v;
+    Ctx& c = h->c;
+    if (int v = need_cache(c))
+        return v;
+    if (!params || count != size_t(mlp_param_count()))
+        return fail(&c, PT_EINVAL, "parameter buffer must hold pt_cache_param_count() floats");
+    HIP_TRY(&c, cache_get_query_params(c.cache, params, c.stream), "cache parameters");
     return PT_OK;
 }
 

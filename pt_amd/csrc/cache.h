@@ -3,7 +3,9 @@
 // Network: 96 inputs (frequency-encoded position + direction, normal, roughness, albedo),
 // five hidden layers of 64 ReLU units, 3 linear outputs (RGB). Weights are kept as fp32
 // master copies (Adam state alongside) and as bf16 copies in two layouts for the MFMA
-// kernels: row-major W[out][in] (forward) and W^T[in][out] (input gradients).
+// kernels: row-major W[out][in] (forward) and W^T[in][out] (input gradients). Optionally the
+// renderer's queries read an exponential moving average of the weights (fp32 and bf16
+// W[out][in]), which damps the step-to-step jitter of training while the paths use it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,6 +62,9 @@ struct CacheState {
     uint32_t seed = 1;
     uint32_t frame_spp = 1;      // samples per render job while training
     float cont_prob = 0.0f;      // control-variate continuation probability (0: terminate)
+    uint32_t query_vertex = 2;   // path vertex where records are taken and queries answered
+    float ema_decay = 0.0f;      // > 0: queries read the moving average of the weights
+    uint32_t ema_steps = 0;      // steps averaged since the average restarted
     uint32_t steps_done = 0;
     float last_loss = 0.0f;
     uint64_t records_total = 0;
@@ -80,6 +85,9 @@ struct CacheState {
     uint32_t query_capacity = 0;
     uint32_t* counters = nullptr;  // [0] training records, [1] queries
     float* loss_acc = nullptr;
+    float* ema = nullptr;        // moving average of the weights (from zero)
+    float* ema_hat = nullptr;    // its bias-corrected value, ema / (1 - decay^ema_steps)
+    uint16_t* ema_wb = nullptr;  // bf16 W[out][in] of ema_hat
 };
 
 }  // namespace ptk

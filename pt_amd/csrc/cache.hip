@@ -354,7 +354,10 @@ __global__ void k_encode(CacheArgs a, const float* rec13, uint32_t n, float* out
     }
 }
 
-__global__ void k_adam(float* params, float* grads, float* m, float* v, int count, float lr, float b1t, float b2t)
+// Adam step; with ema != nullptr also the moving average of the weights, e <- d e + (1 - d) w
+// from e = 0, and its bias-corrected value e / (1 - d^t) into ema_hat (dt = d^t).
+__global__ void k_adam(float* params, float* grads, float* m, float* v, int count, float lr, float b1t, float b2t,
+                       float* ema, float* ema_hat, float d, float dt)
 {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -364,12 +367,18 @@ __global__ void k_adam(float* params, float* grads, float* m, float* v, int coun
         m[i] = mi;
         v[i] = vi;
         const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
-        params[i] -= lr * mh / (sqrtf(vh) + eps);
+        const float w = params[i] - lr * mh / (sqrtf(vh) + eps);
+        params[i] = w;
         grads[i] = 0.0f;
+        if (ema) {
+            const float e = d * ema[i] + (1.0f - d) * w;
+            ema[i] = e;
+            ema_hat[i] = e / (1.0f - dt);
+        }
     }
 }
 
-// bf16 copies of the weights in both layouts (padding rows/columns are zero).
+// bf16 copies of the weights in both layouts (padding rows/columns are zero); wt may be null.
 __global__ void k_sync_bf16(const float* params, uint16_t* wb, uint16_t* wt)
 {
     for (int l = 0; l < MLP_LAYERS; l++) {
@@ -379,7 +388,7 @@ __global__ void k_sync_bf16(const float* params, uint16_t* wb, uint16_t* wt)
             const int o = e / K, k = e % K;
             wb[LAYOUT.wb[l] + e] = o < NV ? f2bf(W[o * K + k]) : uint16_t(0);
         }
-        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * NT; e += gridDim.x * blockDim.x) {
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; wt && e < K * NT; e += gridDim.x * blockDim.x) {
             const int k = e / NT, o = e % NT;
             wt[LAYOUT.wt[l] + e] = o < NV ? f2bf(W[o * K + k]) : uint16_t(0);
         }
@@ -402,6 +411,18 @@ CacheArgs args_of(const CacheState& c)
     return a;
 }
 
+// The weights the renderer's queries read: the trained ones, or their moving average.
+CacheArgs query_args_of(const CacheState& c)
+{
+    CacheArgs a = args_of(c);
+    if (c.ema_decay > 0.0f) {
+        a.params = c.ema_hat;
+        a.wb = c.ema_wb;
+        a.wt = nullptr;
+    }
+    return a;
+}
+
 }  // namespace
 
 int mlp_param_count() { return LAYOUT.count; }
@@ -409,7 +430,7 @@ int mlp_param_count() { return LAYOUT.count; }
 void cache_free(CacheState& c)
 {
     void* ptrs[] = {c.params, c.grads, c.adam_m, c.adam_v, c.wb, c.wt, c.records, c.queries,
-                    c.query_slot, c.counters, c.loss_acc};
+                    c.query_slot, c.counters, c.loss_acc, c.ema, c.ema_hat, c.ema_wb};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -423,7 +444,19 @@ void cache_free(CacheState& c)
 hipError_t cache_sync_weights(CacheState& c, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.params, c.wb, c.wt);
+    if (c.ema_decay > 0.0f)
+        hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.ema_hat, c.ema_wb, nullptr);
     return hipGetLastError();
+}
+
+// Restarts the moving average at the current weights (no step averaged yet).
+static hipError_t cache_reset_ema(CacheState& c, hipStream_t st)
+{
+    c.ema_steps = 0;
+    hipError_t e = hipMemsetAsync(c.ema, 0, size_t(LAYOUT.count) * 4, st);
+    if (e != hipSuccess)
+        return e;
+    return hipMemcpyAsync(c.ema_hat, c.params, size_t(LAYOUT.count) * 4, hipMemcpyDeviceToDevice, st);
 }
 
 // He-uniform initialisation from a Splitmix32 stream; biases zero.
@@ -443,6 +476,9 @@ hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capaci
         A(c.wt, size_t(LAYOUT.wt_count) * 2);
         A(c.counters, 8 * 4);
         A(c.loss_acc, 4);
+        A(c.ema, size_t(LAYOUT.count) * 4);
+        A(c.ema_hat, size_t(LAYOUT.count) * 4);
+        A(c.ema_wb, size_t(LAYOUT.wb_count) * 2);
     }
     if (e == hipSuccess && record_capacity > c.record_capacity) {
         if (c.records)
@@ -487,6 +523,8 @@ hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capaci
         return e;
     if ((e = hipMemsetAsync(c.counters, 0, 8 * 4, st)) != hipSuccess)
         return e;
+    if ((e = cache_reset_ema(c, st)) != hipSuccess)
+        return e;
     c.steps_done = 0;
     c.configured = true;
     return cache_sync_weights(c, st);
@@ -497,7 +535,18 @@ hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st)
     hipError_t e = hipMemcpyAsync(c.params, host, size_t(LAYOUT.count) * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess)
         return e;
+    if ((e = cache_reset_ema(c, st)) != hipSuccess)
+        return e;
     return cache_sync_weights(c, st);
+}
+
+hipError_t cache_get_query_params(CacheState& c, float* host, hipStream_t st)
+{
+    const float* src = c.ema_decay > 0.0f ? c.ema_hat : c.params;
+    hipError_t e = hipMemcpyAsync(host, src, size_t(LAYOUT.count) * 4, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess)
+        return e;
+    return hipStreamSynchronize(st);
 }
 
 hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st)
@@ -526,7 +575,7 @@ hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* 
 hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad, float4* rad_out, unsigned blocks,
                        hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, args_of(c), c.queries, c.query_slot, d_count,
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, query_args_of(c), c.queries, c.query_slot, d_count,
                        next_rad, rad_out, nullptr, 0u, nullptr);
     return hipGetLastError();
 }
@@ -548,8 +597,12 @@ hipError_t cache_adam(CacheState& c, hipStream_t st)
 {
     c.steps_done++;
     const float b1t = std::pow(0.9f, float(c.steps_done)), b2t = std::pow(0.999f, float(c.steps_done));
+    const bool ema = c.ema_decay > 0.0f;
+    if (ema)
+        c.ema_steps++;
+    const float dt = ema ? float(std::pow(double(c.ema_decay), double(c.ema_steps))) : 0.0f;
     hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, st, c.params, c.grads, c.adam_m, c.adam_v, LAYOUT.count, c.lr,
-                       b1t, b2t);
+                       b1t, b2t, ema ? c.ema : nullptr, c.ema_hat, c.ema_decay, dt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

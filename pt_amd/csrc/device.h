@@ -77,7 +77,8 @@ struct CacheDev {
     float* queries;
     uint32_t* query_slot;     // radiance target of each query (TARGET_ENDED | path, or next entry)
     uint32_t train_stride;
-    uint32_t collect;         // record training paths at the 2nd bounce
+    uint32_t collect;         // record training paths at the cache's vertex
+    uint32_t segment;         // segment arriving at that vertex (query_vertex - 1)
     uint32_t query;           // end other paths there with the cache's prediction
     float cont_prob;          // > 0: queried paths go on with this probability (control variate)
 };
@@ -334,6 +335,7 @@ void cache_free(CacheState& c);
 hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capacity, hipStream_t st);
 hipError_t cache_set_params(CacheState& c, const float* host, hipStream_t st);
 hipError_t cache_get_params(CacheState& c, float* host, hipStream_t st);
+hipError_t cache_get_query_params(CacheState& c, float* host, hipStream_t st);
 hipError_t cache_forward(CacheState& c, const float* d_features, uint32_t n, float* d_out, hipStream_t st);
 hipError_t cache_encode(CacheState& c, const float* d_rec13, uint32_t n, float* d_out, hipStream_t st);
 // ACES tone mapping + 8-bit conversion of the PNG output path (post.hip): width x height

@@ -737,8 +737,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     const float3 nrd = -rd;
                     const float3 o = normalize3(f3(dot3(wx, nrd), dot3(wy, nrd), dot3(wn, nrd)));
                     bool cached = false;
-                    if (o.z > 0.0f && cd.collect && segment == 1) {
-                        // Second-bounce vertex: a training path records the cache input here
+                    if (o.z > 0.0f && cd.collect && segment == cd.segment) {
+                        // The cache's vertex (the second by default): a training path records the cache input here
                         // (its target is completed when the path ends); any other path ends
                         // here with the cache's prediction of the radiance leaving it.
                         const uint32_t pix = py * vw_width + px;
@@ -1405,6 +1405,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     memcpy(&cp, &cd.cont_prob, 4);
     mix(cp);
     mix(cd.train_stride);
+    mix(cd.segment);
     mix(reinterpret_cast<uintptr_t>(cd.records));
     mix(reinterpret_cast<uintptr_t>(cd.queries));
     return h | 1ull;
@@ -1542,6 +1543,7 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
         cd.query_slot = cs->query_slot;
         cd.train_stride = std::max<uint32_t>(1, cs->train_stride);
         cd.collect = 1;
+        cd.segment = cs->query_vertex - 1;  // the path segment that arrives at the cache's vertex
         cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
         cd.cont_prob = cs->cont_prob;
     }

@@ -104,6 +104,34 @@ def test_adam_step_matches_numpy(box):
     assert tr.cache_stats()["steps"] == 3
 
 
+def test_query_weights_are_the_moving_average(box):
+    """ema_decay d: after t Adam steps the queries read sum_k (1 - d) d^(t-k) w_k / (1 - d^t), the
+    normalised weighted mean of the trained weights w_1..w_t (closed form, float64), restarted by
+    pt_cache_set_params; with d = 0 they read the trained weights."""
+    _, _, tr = box
+    d = 0.75
+    tr.cache_configure(query=False, seed=3, ema_decay=d)
+    np.testing.assert_array_equal(tr.cache_query_params(), tr.cache_params())
+    X = _features(256, 6)
+    T = np.random.default_rng(6).uniform(0, 1, (256, 3)).astype(np.float32)
+    for restart in range(2):
+        ws = []
+        for t in range(1, 6):
+            tr.cache_train_batch(X, T, apply=True)
+            ws.append(tr.cache_params().astype(np.float64))
+            expect = sum((1 - d) * d ** (t - k) * ws[k - 1] for k in range(1, t + 1)) / (1 - d ** t)
+            np.testing.assert_allclose(tr.cache_query_params(), expect, rtol=1e-5, atol=1e-6)
+        assert np.abs(tr.cache_query_params() - ws[-1]).max() > 1e-4  # the mean lags the last step
+        tr.cache_set_params(_params(14))
+        np.testing.assert_array_equal(tr.cache_query_params(), _params(14))
+    tr.cache_configure(query=False, seed=3)
+    tr.cache_train_batch(X, T, apply=True)
+    np.testing.assert_array_equal(tr.cache_query_params(), tr.cache_params())
+    for bad in (1.0, -0.5):
+        with pytest.raises(pt_amd.PtError, match="cache configuration"):
+            tr.cache_configure(ema_decay=bad)
+
+
 def test_encoding_matches_numpy(box):
     _, _, tr = box
     r = np.random.default_rng(8)
@@ -166,20 +194,45 @@ def test_inflight_training_leaves_the_image_unchanged(box):
     plain.close()
 
 
+@pytest.mark.parametrize("vertex", [3, 8])
+def test_training_at_a_later_vertex_leaves_the_image_unchanged(box, vertex):
+    """Records taken at a later path vertex: the image still equals the plain tracer's, and
+    fewer paths reach vertex 8 than vertex 3 (and some reach neither)."""
+    scene, cam, tr = box
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    plain.render(4)
+    counts = {}
+    for v in (2, vertex):
+        tr.cache_configure(query=False, seed=1, train_stride=4, train_steps=1, batch=4096, query_vertex=v)
+        tr.set_camera(cam)
+        for _ in range(4):
+            tr.render(1)
+        np.testing.assert_array_equal(tr.read(), plain.read())
+        counts[v] = tr.cache_stats()["records_total"]
+    assert 0 < counts[vertex] < counts[2], counts
+    for bad in (1, 9):
+        with pytest.raises(pt_amd.PtError, match="cache configuration"):
+            tr.cache_configure(query_vertex=bad)
+    plain.close()
+
+
 def test_control_variate_with_certain_continuation_is_the_plain_estimate(box):
     """continue_prob = 1: every queried path goes on with weight 1 and the prediction enters
     with weight 0, so the image is the plain one."""
     scene, cam, tr = box
     plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
     plain.set_camera(cam)
-    tr.cache_configure(query=True, seed=6, train_stride=8, train_steps=2, batch=4096, continue_prob=1.0)
-    tr.set_camera(cam)
-    tr.render(2)  # trains; the first frame is not queried
-    tr.set_camera(cam)
-    for _ in range(8):
-        tr.render(1)
     plain.render(8)
-    np.testing.assert_array_equal(tr.read(), plain.read())
+    for vertex in (2, 4):
+        tr.cache_configure(query=True, seed=6, train_stride=8, train_steps=2, batch=4096, continue_prob=1.0,
+                           query_vertex=vertex)
+        tr.set_camera(cam)
+        tr.render(2)  # trains; the first frame is not queried
+        tr.set_camera(cam)
+        for _ in range(8):
+            tr.render(1)
+        np.testing.assert_array_equal(tr.read(), plain.read(), err_msg="vertex %d" % vertex)
     plain.close()
 
 
