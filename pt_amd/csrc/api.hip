@@ -447,6 +447,8 @@ int build_scene_bvh(Ctx& c)
     bp.tlas_open = c.tlas_open;
     if (const char* env = getenv("PT_TLAS_OPEN_RATIO"))  // tuning of the opening criterion
         bp.tlas_open_ratio = float(std::max(0.0, std::min(16.0, atof(env))));
+    if (const char* env = getenv("PT_TLAS_BOX_LEVELS"))
+        bp.tlas_box_levels = uint32_t(std::max(0, std::min(2, atoi(env))));
     if (const char* env = getenv("PT_BVH_SAH_TRI_COST"))  // tuning of the SAH cost model
         bp.sah_tri_cost = float(std::max(0.01, std::min(100.0, atof(env))));
     HIP_TRY(&c,

@@ -25,6 +25,10 @@ struct BvhBuildParams {
     // and their world boxes' surface areas sum to less than tlas_open_ratio x its own.
     uint32_t tlas_open = 0;
     float tlas_open_ratio = 1.5f;
+    // TLAS entry boxes: 0 = the 8 transformed corners of the entry node's box; 1 or 2 = the
+    // union of the transformed boxes of its descendants that many levels down (tighter under
+    // rotation)
+    uint32_t tlas_box_levels = 0;
     // SAH cost of entering a node, testing a triangle, entering an instance
     float sah_node_cost = 1.0f;
     float sah_tri_cost = 0.5f;

@@ -691,6 +691,37 @@ __device__ void world_box(const InstanceData& d, float x0, float y0, float z0, f
     hi = make_float4(mx.x + eps * ext.x, mx.y + eps * ext.y, mx.z + eps * ext.z, 0.0f);
 }
 
+// World box of BLAS node `node` under an instance transform, tightened: the union of the world
+// boxes of its descendants `levels` (1 or 2) levels down, a triangle leaf ending its branch with
+// its own box. Each descendant box lies inside the node's box and together they hold all of its
+// triangles, so the union contains the node's geometry and is no larger than the node box's own.
+__device__ void tight_world_box(const InstanceData& d, const Node4* nodes, uint32_t node, uint32_t levels, float4& lo,
+                                float4& hi)
+{
+    lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    auto add = [&](const Node4& nd, int k) {
+        float4 bl, bh;
+        world_box(d, nd.lo_x[k], nd.lo_y[k], nd.lo_z[k], nd.hi_x[k], nd.hi_y[k], nd.hi_z[k], bl, bh);
+        lo = make_float4(fminf(lo.x, bl.x), fminf(lo.y, bl.y), fminf(lo.z, bl.z), 0.0f);
+        hi = make_float4(fmaxf(hi.x, bh.x), fmaxf(hi.y, bh.y), fmaxf(hi.z, bh.z), 0.0f);
+    };
+    const Node4& nd = nodes[node];
+    for (int k = 0; k < 4; k++) {
+        const uint32_t ch = nd.child[k];
+        if (ch == CHILD_EMPTY)
+            continue;
+        if (levels < 2 || (ch & CHILD_LEAF)) {
+            add(nd, k);
+            continue;
+        }
+        const Node4& g = nodes[ch];
+        for (int j = 0; j < 4; j++)
+            if (g.child[j] != CHILD_EMPTY)
+                add(g, j);
+    }
+}
+
 __device__ __forceinline__ float half_area(float4 lo, float4 hi)
 {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -701,9 +732,10 @@ __device__ __forceinline__ float half_area(float4 lo, float4 hi)
 // root, or - for up to `depth` levels - the children of an entry that opens: every child an
 // internal node, and the surface areas of their world boxes summing to less than ratio x that
 // of the entry's own world box. An instance of an empty mesh gets one entry that enters nothing.
+// With box_levels > 0 each entry's box is then tightened (tight_world_box).
 __global__ void k_open_instances(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
                                  const uint32_t* mesh_tris, const float* mesh_box, const Node4* nodes, uint32_t depth,
-                                 float ratio, uint32_t* count, uint2* ent, float4* lo, float4* hi)
+                                 float ratio, uint32_t box_levels, uint32_t* count, uint2* ent, float4* lo, float4* hi)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t m = inst_mesh[i], base = i * MAX_ENTRIES;
@@ -768,6 +800,8 @@ __global__ void k_open_instances(uint32_t n, const InstanceData* inst, const uin
             cnt = nc;
         }
         for (uint32_t e = 0; e < cnt; e++) {
+            if (box_levels > 0)
+                tight_world_box(d, nodes, nodes_of[e], box_levels, blo[e], bhi[e]);
             ent[base + e] = make_uint2(i, nodes_of[e]);
             lo[base + e] = blo[e];
             hi[base + e] = bhi[e];
@@ -1110,7 +1144,8 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         CHECK(sc.alloc(d_hi, slots));
         hipLaunchKernelGGL(k_open_instances, dim3(grid_for(instance_count, 64)), dim3(64), 0, st, instance_count,
                            d_instances, d_instance_mesh, d_mesh_tris, mesh_box ? mesh_box : dummy_box, out.nodes,
-                           T > 0 ? open_depth : 0u, params.tlas_open_ratio, d_cnt, d_ent, d_lo, d_hi);
+                           T > 0 ? open_depth : 0u, params.tlas_open_ratio, T > 0 ? params.tlas_box_levels : 0u,
+                           d_cnt, d_ent, d_lo, d_hi);
         CHECK(hipGetLastError());
         std::vector<uint32_t> cnt(instance_count), off(instance_count);
         CHECK(hipMemcpyAsync(cnt.data(), d_cnt, size_t(instance_count) * 4, hipMemcpyDeviceToHost, st));

@@ -163,8 +163,14 @@ __device__ __forceinline__ bool intersect_tri(const RayPre& r, float3 v0, float3
     return true;
 }
 
+// Rows of a lane's LDS column after its stack ring: state of the ray that traversal reads
+// rarely, kept out of registers (the ray in world space, read when an instance is entered or
+// left; the closest hit's barycentrics, instance and primitive, written when a triangle is hit).
+constexpr uint32_t COLD_ROWS = 10;
+enum ColdRow : uint32_t { COLD_WO = 0, COLD_WD = 3, COLD_U = 6, COLD_V = 7, COLD_INST = 8, COLD_PRIM = 9 };
+
 // Per-lane traversal stack: the newest D entries in an LDS ring (this lane's column of the
-// block's array), older ones spilled to global memory.
+// block's array of D + COLD_ROWS rows), older ones spilled to global memory.
 template <uint32_t D>
 struct StackT {
     static_assert((D & (D - 1)) == 0, "LDS stack depth is a power of two");
@@ -197,6 +203,18 @@ struct StackT {
             lds[slot] = spill[size_t(sp - D) * spill_stride];
         return v;
     }
+    __device__ __forceinline__ void put(uint32_t row, uint32_t v) { lds[(D + row) * TRACE_BLOCK] = v; }
+    __device__ __forceinline__ uint32_t get(uint32_t row) const { return lds[(D + row) * TRACE_BLOCK]; }
+    __device__ __forceinline__ void put3(uint32_t row, float3 v)
+    {
+        put(row, float_as_u32(v.x));
+        put(row + 1, float_as_u32(v.y));
+        put(row + 2, float_as_u32(v.z));
+    }
+    __device__ __forceinline__ float3 get3(uint32_t row) const
+    {
+        return f3(u32_as_float(get(row)), u32_as_float(get(row + 1)), u32_as_float(get(row + 2)));
+    }
 };
 using Stack = StackT<LDS_STACK>;
 
@@ -211,30 +229,60 @@ struct FetchCount {
     uint32_t nodes = 0, tris = 0, tlas_nodes = 0, entries = 0;
 };
 
-// Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For any-hit rays the
-// transmittance through transparent instances is returned in trans (shadowray.ah.hlsl);
-// ALPHA = false compiles that out for scenes without transparent instances (every hit occludes).
-template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true>
-__device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
-                         float& trans, FetchCount* fc = nullptr)
-{
-    hit.inst = NONE;
-    trans = 1.0f;
-    if (s.tlas_root == CHILD_EMPTY)
-        return false;
-    float best = tmax;
+// One ray's traversal: closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]. For
+// any-hit rays the transmittance through transparent instances is kept in trans
+// (shadowray.ah.hlsl); ALPHA = false compiles that out for scenes without transparent instances
+// (every hit occludes). start() sets a ray up; each step() visits one internal node (testing
+// the triangle leaves among its children at once) and picks the next one, entering and leaving
+// instances, until it returns true: the traversal is over, found() tells whether it hit and
+// result() gives the hit. The world-space ray and the closest hit's attributes live in the
+// lane's LDS rows (ColdRow); the any-hit record is only made when the ray ends.
+template <bool ANY, bool COUNT, typename Stk, bool ALPHA>
+struct Walker {
     RayPre r;
-    precompute(r, wo, wd);
-    uint32_t inst = NONE;
-    uint32_t blas_sp = NONE;
-    float opacity = 1.0f;
-    uint32_t node = s.tlas_root;
-    stk.sp = 0;
-    // Conservative slab far distance: each slab distance fl(fl(p - o) * inv), inv within 1 ulp,
-    // is within 4u of exact (u = 2^-24), so a near/far pair within 8u; 1 + 2^-20 = 1 + 16u.
-    const float cons = 1.00000095367431640625f;
+    float tmin, best, opacity, trans;
+    uint32_t inst, blas_sp, node;
+    HitOut hit;  // any hit
 
-    while (true) {
+    // false: nothing to traverse (an empty scene); the ray is then finished without a hit
+    __device__ __forceinline__ bool start(const DeviceScene& s, float3 o, float3 d, float t0, float t1, Stk& stk)
+    {
+        hit.inst = NONE;
+        trans = 1.0f;
+        stk.put3(COLD_WO, o);
+        stk.put3(COLD_WD, d);
+        if (!ANY)
+            stk.put(COLD_INST, NONE);
+        tmin = t0;
+        best = t1;
+        inst = NONE;
+        blas_sp = NONE;
+        opacity = 1.0f;
+        node = s.tlas_root;
+        stk.sp = 0;
+        if (node == CHILD_EMPTY)
+            return false;
+        precompute(r, o, d);
+        return true;
+    }
+
+    __device__ __forceinline__ bool found(const Stk& stk) const
+    {
+        return (ANY ? hit.inst : stk.get(COLD_INST)) != NONE;
+    }
+    __device__ __forceinline__ HitOut result(const Stk& stk) const
+    {
+        if (ANY)
+            return hit;
+        return HitOut{best, u32_as_float(stk.get(COLD_U)), u32_as_float(stk.get(COLD_V)), stk.get(COLD_INST),
+                      stk.get(COLD_PRIM)};
+    }
+
+    __device__ __forceinline__ bool step(const DeviceScene& s, Stk& stk, FetchCount* fc)
+    {
+        // Conservative slab far distance: each slab distance fl(fl(p - o) * inv), inv within 1 ulp,
+        // is within 4u of exact (u = 2^-24), so a near/far pair within 8u; 1 + 2^-20 = 1 + 16u.
+        const float cons = 1.00000095367431640625f;
         // ---- visit an internal node
         if (COUNT) {
             fc->nodes++;
@@ -322,10 +370,13 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                             continue;
                         }
                         hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
-                        return true;
+                        return true;  // occluded
                     }
                     best = t;
-                    hit = HitOut{t, b1, b2, inst, float_as_u32(tr.v0.w)};
+                    stk.put(COLD_U, float_as_u32(b1));
+                    stk.put(COLD_V, float_as_u32(b2));
+                    stk.put(COLD_INST, inst);
+                    stk.put(COLD_PRIM, float_as_u32(tr.v0.w));
                 }
             }
         }
@@ -348,10 +399,10 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 if (stk.sp == blas_sp) {  // the current BLAS is finished: back to world space
                     inst = NONE;
                     blas_sp = NONE;
-                    precompute(r, wo, wd);
+                    precompute(r, stk.get3(COLD_WO), stk.get3(COLD_WD));
                 }
                 if (stk.sp == 0)
-                    return hit.inst != NONE;
+                    return true;  // nothing left to visit
                 next = stk.pop();
             }
             if ((next & CHILD_LEAF) && inst == NONE) {  // TLAS leaf: enter the instance
@@ -361,6 +412,7 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
                 if (root == CHILD_EMPTY)
                     continue;
                 const InstXform x = at32(s.inst_inv, id);
+                const float3 wo = stk.get3(COLD_WO), wd = stk.get3(COLD_WD);
                 const float3 oo = f3(x.r0.x * wo.x + x.r0.y * wo.y + x.r0.z * wo.z + x.r0.w,
                                      x.r1.x * wo.x + x.r1.y * wo.y + x.r1.z * wo.z + x.r1.w,
                                      x.r2.x * wo.x + x.r2.y * wo.y + x.r2.z * wo.z + x.r2.w);
@@ -380,7 +432,21 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
             break;
         }
         node = next;
+        return false;
     }
+};
+
+template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true>
+__device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
+                         float& trans, FetchCount* fc = nullptr)
+{
+    Walker<ANY, COUNT, Stk, ALPHA> w;
+    if (w.start(s, wo, wd, tmin, tmax, stk))
+        while (!w.step(s, stk, fc)) {
+        }
+    hit = w.result(stk);
+    trans = w.trans;
+    return w.found(stk);
 }
 
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid)
@@ -484,7 +550,7 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
 template <bool COUNT, bool MULTI, uint32_t D>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
-    __shared__ uint32_t lds[D * TRACE_BLOCK];
+    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(q)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
@@ -498,9 +564,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
-        if (stk.overflow)
-            atomicOr(&wf.counters[CTR(3)], 1u);
     }
+    if (stk.overflow)
+        atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
         atomicAdd(&wf.totals[2], (unsigned long long)fc.nodes);
         atomicAdd(&wf.totals[3], (unsigned long long)fc.tris);
@@ -515,12 +581,23 @@ template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
-    __shared__ uint32_t lds[D * TRACE_BLOCK];
+    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(2)];
     const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
+    // an unoccluded ray adds its contribution, times the transmittance, where its path keeps it
+    auto add = [&](uint32_t i, float trans) {
+        const uint32_t tgt = wf.s_target[i];
+        float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
+        const float4 c = wf.s_c[i];
+        float4 r = *dst;
+        r.x += c.x * trans;
+        r.y += c.y * trans;
+        r.z += c.z * trans;
+        *dst = r;
+    };
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
         const DeviceScene& s = MULTI ? scenes[wf.s_scene[i]] : s0;
@@ -528,19 +605,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         float trans;
         const bool occluded =
             traverse<true, COUNT, StackT<D>, ALPHA>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &fc);
-        if (!occluded && trans >= 1e-5f) {
-            const uint32_t tgt = wf.s_target[i];
-            float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
-            const float4 c = wf.s_c[i];
-            float4 r = *dst;
-            r.x += c.x * trans;
-            r.y += c.y * trans;
-            r.z += c.z * trans;
-            *dst = r;
-        }
-        if (stk.overflow)
-            atomicOr(&wf.counters[CTR(3)], 1u);
+        if (!occluded && trans >= 1e-5f)
+            add(i, trans);
     }
+    if (stk.overflow)
+        atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
         atomicAdd(&wf.totals[4], (unsigned long long)fc.nodes);
         atomicAdd(&wf.totals[5], (unsigned long long)fc.tris);
@@ -1116,7 +1185,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, Cam
                                                               uint32_t height, uint32_t sample_index, pt_hit* hits,
                                                               Wavefront wf)
 {
-    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    __shared__ uint32_t lds[(LDS_STACK + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t n = width * height;
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
@@ -1140,7 +1209,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_hits(DeviceScene s, Cam
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene s, const pt_ray* rays, uint32_t n, pt_hit* hits,
                                                             int any_hit, Wavefront wf)
 {
-    __shared__ uint32_t lds[LDS_STACK * TRACE_BLOCK];
+    __shared__ uint32_t lds[(LDS_STACK + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     for (uint32_t p = gtid; p < n; p += gridDim.x * TRACE_BLOCK) {
