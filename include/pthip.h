@@ -190,6 +190,13 @@ typedef struct pt_stats {
                                    * an instance whose BLAS children have much tighter world boxes than
                                    * its own is entered through those subtrees (partial re-braiding);
                                    * rebuilds the BVH */
+#define PT_OPT_TLAS_BOXES 13u     /* boxes of the TLAS leaves (0..3, default 3): 0 the transformed corners
+                                   * of the BLAS node's box; 1, 2 the union of its transformed descendant
+                                   * boxes that many levels down; 3 an instance's transformed vertices
+                                   * (level 2 for entries at opened subtrees): rebuilds the BVH */
+#define PT_OPT_TLAS_BUILDER 14u   /* TLAS builder: 0 the BLAS builder on the device, 1 (default) a full
+                                   * SAH sweep with an SAH-optimal collapse on the host, for up to 16384
+                                   * TLAS leaves (more use 0): rebuilds the BVH */
 #define PT_OPT_SPILL_DEPTH 11u    /* traversal stack entries per lane spilled to global memory beyond the
                                    * LDS ones (0..112, default 112); a ray whose stack would grow deeper
                                    * loses nodes, which the calls below report as PT_EDEVICE */

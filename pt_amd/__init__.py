@@ -104,6 +104,8 @@ PT_OPT_NODE_WIDTH = 9
 PT_OPT_BVH_BUILDER = 10
 PT_OPT_SPILL_DEPTH = 11
 PT_OPT_TLAS_OPEN = 12
+PT_OPT_TLAS_BOXES = 13
+PT_OPT_TLAS_BUILDER = 14
 
 
 class TuneResult(C.Structure):

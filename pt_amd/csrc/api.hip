@@ -447,8 +447,8 @@ int build_scene_bvh(Ctx& c)
     bp.tlas_open = c.tlas_open;
     if (const char* env = getenv("PT_TLAS_OPEN_RATIO"))  // tuning of the opening criterion
         bp.tlas_open_ratio = float(std::max(0.0, std::min(16.0, atof(env))));
-    if (const char* env = getenv("PT_TLAS_BOX_LEVELS"))
-        bp.tlas_box_levels = uint32_t(std::max(0, std::min(2, atoi(env))));
+    bp.tlas_box_levels = c.tlas_boxes;
+    bp.tlas_builder = c.tlas_builder;
     if (const char* env = getenv("PT_BVH_SAH_TRI_COST"))  // tuning of the SAH cost model
         bp.sah_tri_cost = float(std::max(0.01, std::min(100.0, atof(env))));
     HIP_TRY(&c,
@@ -1094,6 +1094,8 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
     case PT_OPT_LEAF_SIZE:
     case PT_OPT_NODE_WIDTH:
     case PT_OPT_TLAS_OPEN:
+    case PT_OPT_TLAS_BOXES:
+    case PT_OPT_TLAS_BUILDER:
     case PT_OPT_BVH_BUILDER: {
         if (option == PT_OPT_LEAF_SIZE && (value < 1 || value > 16))
             return fail(&c, PT_EINVAL, "leaf size must be 1..16");
@@ -1103,10 +1105,16 @@ int pt_set_option(pt_ctx* h, uint32_t option, int64_t value)
             return fail(&c, PT_EINVAL, "BVH builder must be 0 (PLOC) or 1 (radix tree)");
         if (option == PT_OPT_TLAS_OPEN && (value < 0 || value > 2))
             return fail(&c, PT_EINVAL, "TLAS opening depth must be 0..2");
+        if (option == PT_OPT_TLAS_BOXES && (value < 0 || value > 3))
+            return fail(&c, PT_EINVAL, "TLAS entry boxes must be 0..3");
+        if (option == PT_OPT_TLAS_BUILDER && value != BVH_TLAS_DEVICE && value != BVH_TLAS_HOST)
+            return fail(&c, PT_EINVAL, "TLAS builder must be 0 (device) or 1 (host SAH sweep)");
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
         uint32_t& field = option == PT_OPT_LEAF_SIZE    ? c.leaf_size
                           : option == PT_OPT_NODE_WIDTH ? c.node_width
                           : option == PT_OPT_TLAS_OPEN  ? c.tlas_open
+                          : option == PT_OPT_TLAS_BOXES ? c.tlas_boxes
+                          : option == PT_OPT_TLAS_BUILDER ? c.tlas_builder
                                                         : c.bvh_builder;
         field = uint32_t(value);
         if (int v = build_scene_bvh(c))

@@ -14,6 +14,9 @@ namespace ptk {
 
 constexpr uint32_t BVH_BUILDER_PLOC = 0;  // parallel locally-ordered clustering (Meister & Bittner 2018)
 constexpr uint32_t BVH_BUILDER_LBVH = 1;  // Karras (2012) binary radix tree over the Morton codes
+constexpr uint32_t BVH_TLAS_DEVICE = 0;
+constexpr uint32_t BVH_TLAS_HOST = 1;
+constexpr uint32_t HOST_TLAS_MAX = 16384;
 
 struct BvhBuildParams {
     uint32_t blas_leaf_size = 4;  // max triangles per BLAS leaf (1..16)
@@ -27,8 +30,13 @@ struct BvhBuildParams {
     float tlas_open_ratio = 1.5f;
     // TLAS entry boxes: 0 = the 8 transformed corners of the entry node's box; 1 or 2 = the
     // union of the transformed boxes of its descendants that many levels down (tighter under
-    // rotation)
+    // rotation); 3 = an instance entered at its BLAS root gets the box of its transformed
+    // vertices, opened entries that of level 2
     uint32_t tlas_box_levels = 0;
+    // TLAS builder: BVH_TLAS_DEVICE (the BLAS builder's) or BVH_TLAS_HOST (full-sweep SAH and
+    // SAH-optimal collapse on the host, for at most HOST_TLAS_MAX entries; more fall back to
+    // the device builder)
+    uint32_t tlas_builder = BVH_TLAS_DEVICE;
     // SAH cost of entering a node, testing a triangle, entering an instance
     float sah_node_cost = 1.0f;
     float sah_tri_cost = 0.5f;

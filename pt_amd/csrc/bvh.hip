@@ -722,6 +722,50 @@ __device__ void tight_world_box(const InstanceData& d, const Node4* nodes, uint3
     }
 }
 
+// World box of every instance's transformed vertices (one block per instance), padded for the
+// rounding of the transform as world_box pads: it holds all of the instance's triangles and,
+// for a rotated instance, is far tighter than the transformed corners of the mesh box.
+__global__ __launch_bounds__(256) void k_vertex_boxes(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
+                                                      const pt_mesh_desc* meshes, const float4* vertices, float4* lo,
+                                                      float4* hi)
+{
+    __shared__ float red[6][256];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const InstanceData& d = inst[i];
+        const pt_mesh_desc md = meshes[inst_mesh[i]];
+        float3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t v = t; v < md.vertex_count; v += 256) {
+            const float4 p = vertices[size_t(md.vertex_start + v) * 2];
+            const float wx = d.m[0][0] * p.x + d.m[0][1] * p.y + d.m[0][2] * p.z + d.m[0][3];
+            const float wy = d.m[1][0] * p.x + d.m[1][1] * p.y + d.m[1][2] * p.z + d.m[1][3];
+            const float wz = d.m[2][0] * p.x + d.m[2][1] * p.y + d.m[2][2] * p.z + d.m[2][3];
+            mn = f3(fminf(mn.x, wx), fminf(mn.y, wy), fminf(mn.z, wz));
+            mx = f3(fmaxf(mx.x, wx), fmaxf(mx.y, wy), fmaxf(mx.z, wz));
+        }
+        red[0][t] = mn.x, red[1][t] = mn.y, red[2][t] = mn.z;
+        red[3][t] = mx.x, red[4][t] = mx.y, red[5][t] = mx.z;
+        __syncthreads();
+        for (uint32_t h = 128; h > 0; h >>= 1) {
+            if (t < h) {
+                for (int k = 0; k < 3; k++) {
+                    red[k][t] = fminf(red[k][t], red[k][t + h]);
+                    red[3 + k][t] = fmaxf(red[3 + k][t], red[3 + k][t + h]);
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            const float3 a = f3(red[0][0], red[1][0], red[2][0]), b = f3(red[3][0], red[4][0], red[5][0]);
+            const float3 ext = f3(fabsf(a.x) + fabsf(b.x), fabsf(a.y) + fabsf(b.y), fabsf(a.z) + fabsf(b.z));
+            const float eps = 1e-6f;
+            lo[i] = make_float4(a.x - eps * ext.x, a.y - eps * ext.y, a.z - eps * ext.z, 0.0f);
+            hi[i] = make_float4(b.x + eps * ext.x, b.y + eps * ext.y, b.z + eps * ext.z, 0.0f);
+        }
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ float half_area(float4 lo, float4 hi)
 {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -732,10 +776,12 @@ __device__ __forceinline__ float half_area(float4 lo, float4 hi)
 // root, or - for up to `depth` levels - the children of an entry that opens: every child an
 // internal node, and the surface areas of their world boxes summing to less than ratio x that
 // of the entry's own world box. An instance of an empty mesh gets one entry that enters nothing.
-// With box_levels > 0 each entry's box is then tightened (tight_world_box).
+// With box_levels > 0 each entry's box is then tightened (tight_world_box, at most two levels);
+// with vertex boxes (vlo, vhi: k_vertex_boxes) an entry at the BLAS root takes its instance's.
 __global__ void k_open_instances(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
                                  const uint32_t* mesh_tris, const float* mesh_box, const Node4* nodes, uint32_t depth,
-                                 float ratio, uint32_t box_levels, uint32_t* count, uint2* ent, float4* lo, float4* hi)
+                                 float ratio, uint32_t box_levels, const float4* vlo, const float4* vhi,
+                                 uint32_t* count, uint2* ent, float4* lo, float4* hi)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t m = inst_mesh[i], base = i * MAX_ENTRIES;
@@ -800,8 +846,12 @@ __global__ void k_open_instances(uint32_t n, const InstanceData* inst, const uin
             cnt = nc;
         }
         for (uint32_t e = 0; e < cnt; e++) {
-            if (box_levels > 0)
-                tight_world_box(d, nodes, nodes_of[e], box_levels, blo[e], bhi[e]);
+            if (vlo && nodes_of[e] == m) {
+                blo[e] = vlo[i];
+                bhi[e] = vhi[i];
+            } else if (box_levels > 0) {
+                tight_world_box(d, nodes, nodes_of[e], std::min(box_levels, 2u), blo[e], bhi[e]);
+            }
             ent[base + e] = make_uint2(i, nodes_of[e]);
             lo[base + e] = blo[e];
             hi[base + e] = bhi[e];
@@ -1006,6 +1056,190 @@ void free_bvh(BvhBuffers& b)
     b = BvhBuffers{};
 }
 
+// The TLAS built on the host, for a modest number of entries: a binary tree split by the full
+// SAH sweep (every position of the centroid order along every axis; a split costs
+// area(left) x count(left) + area(right) x count(right)), then collapsed SAH-optimally into nodes
+// of at most sah.width children. The collapse is a dynamic program over the binary tree: f(b, k)
+// is the cheapest cover of b's subtree by k child roots, where a root costs c_leaf x area for an
+// entry and c_node x area plus its own best cover of 2..width roots for an internal node. Nodes
+// are numbered from slot0 (the root) in the order they are made.
+static void host_tlas(const std::vector<float4>& lo, const std::vector<float4>& hi, const Sah& sah, uint32_t slot0,
+                      std::vector<Node4>& out)
+{
+    const uint32_t E = uint32_t(lo.size());
+    const uint32_t W = std::max(2u, std::min(sah.width, 4u));
+    struct B {
+        float lo[3], hi[3];
+        int32_t l, r;  // children (binary node ids), or -1 with entry id in l for a leaf
+    };
+    std::vector<B> bn;
+    bn.reserve(2 * E);
+    auto area = [](const B& b) {
+        const double dx = std::max(0.0, double(b.hi[0] - b.lo[0])), dy = std::max(0.0, double(b.hi[1] - b.lo[1])),
+                     dz = std::max(0.0, double(b.hi[2] - b.lo[2]));
+        return dx * dy + dy * dz + dz * dx;
+    };
+    auto box_of = [&](const uint32_t* ids, size_t n, B& b) {
+        for (int k = 0; k < 3; k++) {
+            b.lo[k] = INFINITY;
+            b.hi[k] = -INFINITY;
+        }
+        for (size_t i = 0; i < n; i++) {
+            const float4 a = lo[ids[i]], c = hi[ids[i]];
+            b.lo[0] = std::min(b.lo[0], a.x), b.lo[1] = std::min(b.lo[1], a.y), b.lo[2] = std::min(b.lo[2], a.z);
+            b.hi[0] = std::max(b.hi[0], c.x), b.hi[1] = std::max(b.hi[1], c.y), b.hi[2] = std::max(b.hi[2], c.z);
+        }
+    };
+    std::vector<uint32_t> ids(E), tmp(E), best_order;
+    for (uint32_t i = 0; i < E; i++)
+        ids[i] = i;
+    std::vector<double> suffix(E + 1);
+    // iterative top-down build over [begin, end) ranges of ids
+    struct Task {
+        uint32_t begin, end;
+        int32_t node;
+    };
+    bn.push_back(B{});
+    std::vector<Task> work{{0, E, 0}};
+    while (!work.empty()) {
+        const Task tk = work.back();
+        work.pop_back();
+        B& nb = bn[tk.node];
+        box_of(&ids[tk.begin], tk.end - tk.begin, nb);
+        const uint32_t n = tk.end - tk.begin;
+        if (n == 1) {
+            nb.l = int32_t(ids[tk.begin]);
+            nb.r = -1;
+            continue;
+        }
+        double best = INFINITY;
+        uint32_t best_split = n / 2;
+        int best_axis = -1;
+        for (int ax = 0; ax < 3; ax++) {
+            std::copy(ids.begin() + tk.begin, ids.begin() + tk.end, tmp.begin());
+            auto cen = [&](uint32_t e) {
+                const float c = ax == 0 ? lo[e].x + hi[e].x : (ax == 1 ? lo[e].y + hi[e].y : lo[e].z + hi[e].z);
+                return c == c ? c : 0.0f;  // an empty box (an instance of an empty mesh) sorts at 0
+            };
+            std::stable_sort(tmp.begin(), tmp.begin() + n, [&](uint32_t a, uint32_t b) { return cen(a) < cen(b); });
+            B acc;
+            box_of(&tmp[n - 1], 1, acc);
+            suffix[n - 1] = area(acc);
+            for (uint32_t i = n - 1; i-- > 0;) {
+                B one;
+                box_of(&tmp[i], 1, one);
+                for (int k = 0; k < 3; k++)
+                    acc.lo[k] = std::min(acc.lo[k], one.lo[k]), acc.hi[k] = std::max(acc.hi[k], one.hi[k]);
+                suffix[i] = area(acc);
+            }
+            box_of(&tmp[0], 1, acc);
+            for (uint32_t i = 1; i < n; i++) {  // left = [0, i), right = [i, n)
+                const double c = area(acc) * double(i) + suffix[i] * double(n - i);
+                if (c < best) {
+                    best = c;
+                    best_split = i;
+                    best_axis = ax;
+                }
+                B one;
+                box_of(&tmp[i], 1, one);
+                for (int k = 0; k < 3; k++)
+                    acc.lo[k] = std::min(acc.lo[k], one.lo[k]), acc.hi[k] = std::max(acc.hi[k], one.hi[k]);
+            }
+            if (best_axis == ax)
+                best_order.assign(tmp.begin(), tmp.begin() + n);
+        }
+        if (best_axis >= 0)
+            std::copy(best_order.begin(), best_order.end(), ids.begin() + tk.begin);
+        const int32_t l = int32_t(bn.size()), r = l + 1;
+        bn.push_back(B{});
+        bn.push_back(B{});
+        bn[tk.node].l = l;
+        bn[tk.node].r = r;
+        work.push_back({tk.begin, tk.begin + best_split, l});
+        work.push_back({tk.begin + best_split, tk.end, r});
+    }
+    // collapse: children are made after parents, so a reverse sweep sees children first
+    const size_t NB = bn.size();
+    std::vector<double> f(NB * 5, INFINITY);
+    std::vector<uint8_t> split(NB * 5, 0), own(NB, 0);  // k1 of f(b, k); best root count of b as a node
+    for (size_t b = NB; b-- > 0;) {
+        const B& x = bn[b];
+        if (x.r < 0) {
+            f[b * 5 + 1] = double(sah.c_leaf) * area(x);
+            continue;
+        }
+        for (uint32_t k = 2; k <= W; k++)
+            for (uint32_t k1 = 1; k1 < k; k1++) {
+                const double c = f[size_t(x.l) * 5 + k1] + f[size_t(x.r) * 5 + (k - k1)];
+                if (c < f[b * 5 + k]) {
+                    f[b * 5 + k] = c;
+                    split[b * 5 + k] = uint8_t(k1);
+                }
+            }
+        double g = INFINITY;
+        for (uint32_t k = 2; k <= W; k++)
+            if (f[b * 5 + k] < g) {
+                g = f[b * 5 + k];
+                own[b] = uint8_t(k);
+            }
+        f[b * 5 + 1] = double(sah.c_node) * area(x) + g;
+    }
+    // emit
+    out.clear();
+    auto empty_node = []() {
+        Node4 nd;
+        for (int k = 0; k < 4; k++) {
+            nd.lo_x[k] = nd.lo_y[k] = nd.lo_z[k] = INFINITY;
+            nd.hi_x[k] = nd.hi_y[k] = nd.hi_z[k] = -INFINITY;
+            nd.child[k] = CHILD_EMPTY;
+            nd.pad[k] = 0;
+        }
+        return nd;
+    };
+    if (bn[0].r < 0) {  // one entry: a root with one leaf child
+        Node4 nd = empty_node();
+        const B& x = bn[0];
+        nd.lo_x[0] = x.lo[0], nd.lo_y[0] = x.lo[1], nd.lo_z[0] = x.lo[2];
+        nd.hi_x[0] = x.hi[0], nd.hi_y[0] = x.hi[1], nd.hi_z[0] = x.hi[2];
+        nd.child[0] = CHILD_LEAF | uint32_t(x.l);
+        out.push_back(nd);
+        return;
+    }
+    std::vector<std::pair<int32_t, uint32_t>> q{{0, 0}};  // (binary node, output node)
+    out.push_back(empty_node());
+    for (size_t qi = 0; qi < q.size(); qi++) {
+        const int32_t b = q[qi].first;
+        const uint32_t o = q[qi].second;
+        std::vector<int32_t> roots;
+        std::vector<std::pair<int32_t, uint32_t>> cover{{b, own[b]}};
+        while (!cover.empty()) {
+            const auto c = cover.back();
+            cover.pop_back();
+            if (c.second == 1) {
+                roots.push_back(c.first);
+                continue;
+            }
+            const uint32_t k1 = split[size_t(c.first) * 5 + c.second];
+            cover.push_back({bn[c.first].r, c.second - k1});
+            cover.push_back({bn[c.first].l, k1});
+        }
+        for (size_t k = 0; k < roots.size(); k++) {
+            const B& x = bn[roots[k]];
+            Node4& nd = out[o];
+            nd.lo_x[k] = x.lo[0], nd.lo_y[k] = x.lo[1], nd.lo_z[k] = x.lo[2];
+            nd.hi_x[k] = x.hi[0], nd.hi_y[k] = x.hi[1], nd.hi_z[k] = x.hi[2];
+            if (x.r < 0) {
+                nd.child[k] = CHILD_LEAF | uint32_t(x.l);
+            } else {
+                const uint32_t id = uint32_t(out.size());
+                out[o].child[k] = slot0 + id;
+                out.push_back(empty_node());
+                q.push_back({roots[k], id});
+            }
+        }
+    }
+}
+
 hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d_indices, const pt_mesh_desc* d_meshes,
                      const std::vector<pt_mesh_desc>& meshes, const InstanceData* d_instances,
                      const uint32_t* d_instance_mesh, uint32_t instance_count, const MaterialRec* d_materials,
@@ -1142,10 +1376,17 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         CHECK(sc.alloc(d_ent, slots));
         CHECK(sc.alloc(d_lo, slots));
         CHECK(sc.alloc(d_hi, slots));
+        float4 *d_vlo = nullptr, *d_vhi = nullptr;
+        if (T > 0 && params.tlas_box_levels >= 3) {
+            CHECK(sc.alloc(d_vlo, instance_count));
+            CHECK(sc.alloc(d_vhi, instance_count));
+            hipLaunchKernelGGL(k_vertex_boxes, dim3(std::min<uint32_t>(instance_count, 4096)), dim3(256), 0, st,
+                               instance_count, d_instances, d_instance_mesh, d_meshes, d_vertices, d_vlo, d_vhi);
+        }
         hipLaunchKernelGGL(k_open_instances, dim3(grid_for(instance_count, 64)), dim3(64), 0, st, instance_count,
                            d_instances, d_instance_mesh, d_mesh_tris, mesh_box ? mesh_box : dummy_box, out.nodes,
                            T > 0 ? open_depth : 0u, params.tlas_open_ratio, T > 0 ? params.tlas_box_levels : 0u,
-                           d_cnt, d_ent, d_lo, d_hi);
+                           d_vlo, d_vhi, d_cnt, d_ent, d_lo, d_hi);
         CHECK(hipGetLastError());
         std::vector<uint32_t> cnt(instance_count), off(instance_count);
         CHECK(hipMemcpyAsync(cnt.data(), d_cnt, size_t(instance_count) * 4, hipMemcpyDeviceToHost, st));
@@ -1174,14 +1415,28 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         hipLaunchKernelGGL(k_pack_entries, dim3(grid_for(instance_count)), dim3(256), 0, st, instance_count, d_cnt,
                            d_off, d_ent, d_lo, d_hi, out.tlas_entries, tl.lo, tl.hi, tl.seg);
         const uint32_t tlas_slot = blas_nodes;
-        counter = blas_nodes + 1;
-        CHECK(hipMemcpyAsync(d_counter, &counter, 4, hipMemcpyHostToDevice, st));
-        CHECK(build_set(st, sc, tl, sb, tlas_sah, params.builder, 1, out.nodes, tlas_slot, d_counter,
-                        out.node_capacity));
-        CHECK(hipMemcpyAsync(&counter, d_counter, 4, hipMemcpyDeviceToHost, st));
-        CHECK(hipStreamSynchronize(st));
-        if (counter > out.node_capacity)
-            return hipErrorOutOfMemory;
+        if (params.tlas_builder == BVH_TLAS_HOST && E <= HOST_TLAS_MAX) {
+            std::vector<float4> hlo(E), hhi(E);
+            CHECK(hipMemcpyAsync(hlo.data(), tl.lo, size_t(E) * sizeof(float4), hipMemcpyDeviceToHost, st));
+            CHECK(hipMemcpyAsync(hhi.data(), tl.hi, size_t(E) * sizeof(float4), hipMemcpyDeviceToHost, st));
+            CHECK(hipStreamSynchronize(st));
+            std::vector<Node4> tn;
+            host_tlas(hlo, hhi, tlas_sah, tlas_slot, tn);
+            counter = tlas_slot + uint32_t(tn.size());
+            if (counter > out.node_capacity)
+                return hipErrorOutOfMemory;
+            CHECK(hipMemcpyAsync(out.nodes + tlas_slot, tn.data(), tn.size() * sizeof(Node4), hipMemcpyHostToDevice,
+                                 st));
+        } else {
+            counter = blas_nodes + 1;
+            CHECK(hipMemcpyAsync(d_counter, &counter, 4, hipMemcpyHostToDevice, st));
+            CHECK(build_set(st, sc, tl, sb, tlas_sah, params.builder, 1, out.nodes, tlas_slot, d_counter,
+                            out.node_capacity));
+            CHECK(hipMemcpyAsync(&counter, d_counter, 4, hipMemcpyDeviceToHost, st));
+            CHECK(hipStreamSynchronize(st));
+            if (counter > out.node_capacity)
+                return hipErrorOutOfMemory;
+        }
         out.tlas_root = tlas_slot;
     }
     out.node_count = counter;

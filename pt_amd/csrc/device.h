@@ -264,6 +264,8 @@ struct Ctx {
     uint32_t node_width = 4;
     uint32_t bvh_builder = 0;  // BVH_BUILDER_*
     uint32_t tlas_open = 0;    // PT_OPT_TLAS_OPEN
+    uint32_t tlas_boxes = 3;   // PT_OPT_TLAS_BOXES
+    uint32_t tlas_builder = BVH_TLAS_HOST;  // PT_OPT_TLAS_BUILDER
     bool sort_hits = false;
     uint32_t job_paths = 0;  // capacity of the per-job radiance buffer
     std::vector<pt_mesh_desc> meshes;  // host copy (BVH rebuilds)

@@ -57,11 +57,17 @@ def test_launch_options_leave_the_image_unchanged(inst, opts):
     [(pt_amd.PT_OPT_TLAS_OPEN, 1)],
     [(pt_amd.PT_OPT_TLAS_OPEN, 2)],
     [("PT_TLAS_OPEN_RATIO", "8"), (pt_amd.PT_OPT_TLAS_OPEN, 2), (pt_amd.PT_OPT_LEAF_SIZE, 1)],
+    [(pt_amd.PT_OPT_TLAS_BUILDER, 0)],
+    [(pt_amd.PT_OPT_TLAS_BUILDER, 0), (pt_amd.PT_OPT_TLAS_BOXES, 0)],
+    [(pt_amd.PT_OPT_TLAS_BOXES, 1)],
+    [(pt_amd.PT_OPT_TLAS_BOXES, 2), (pt_amd.PT_OPT_NODE_WIDTH, 2)],
+    [(pt_amd.PT_OPT_TLAS_BOXES, 3), (pt_amd.PT_OPT_TLAS_OPEN, 2), (pt_amd.PT_OPT_NODE_WIDTH, 3)],
 ], ids=["leaf1", "leaf2", "leaf16", "width2", "width3-leaf8", "radix", "radix-width2-leaf1", "open1", "open2",
-        "open2-all-leaf1"])
+        "open2-all-leaf1", "tlas-device", "tlas-device-corners", "boxes1", "boxes2-width2", "boxes3-open2-width3"])
 def test_hierarchy_rebuild(inst, opts, monkeypatch):
     """Another hierarchy (leaf size, node width, builder, TLAS entries at opened BLAS subtrees,
-    where a ratio of 8 opens every instance whose top two levels are internal nodes) gives the
+    where a ratio of 8 opens every instance whose top two levels are internal nodes, the TLAS
+    from the device builder instead of the host's SAH sweep, looser TLAS leaf boxes) gives the
     same image up to exact ties; pt_create validates each hierarchy on the host first."""
     scene, cam, ref = inst
     for k, v in opts:
@@ -80,7 +86,8 @@ def test_option_checks(inst):
     for k, v in [(pt_amd.PT_OPT_BLOCKS_PER_CU, 0), (pt_amd.PT_OPT_BLOCKS_PER_CU, 65),
                  (pt_amd.PT_OPT_POOL_SLOTS, 100), (pt_amd.PT_OPT_LDS_STACK, 12), (pt_amd.PT_OPT_LEAF_SIZE, 17),
                  (pt_amd.PT_OPT_NODE_WIDTH, 1), (pt_amd.PT_OPT_NODE_WIDTH, 5), (pt_amd.PT_OPT_BVH_BUILDER, 2),
-                 (pt_amd.PT_OPT_TLAS_OPEN, 3), (pt_amd.PT_OPT_SPILL_DEPTH, 113)]:
+                 (pt_amd.PT_OPT_TLAS_OPEN, 3), (pt_amd.PT_OPT_SPILL_DEPTH, 113), (pt_amd.PT_OPT_TLAS_BOXES, 4),
+                 (pt_amd.PT_OPT_TLAS_BUILDER, 2)]:
         with pytest.raises(pt_amd.PtError):
             t.set_option(k, v)
     with pytest.raises(pt_amd.PtError, match="no camera"):
