@@ -1755,7 +1755,7 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
     for (int64_t p : {1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20})
         if (uint64_t(p) <= std::max<uint64_t>(npix * spp, 1u << 20))
             pools.push_back(p);
-    const std::vector<int64_t> leaves = {2, 4, 8};
+    const std::vector<int64_t> leaves = {1, 2, 4, 8};
     const std::vector<int64_t> widths = {4, 3, 2};
     const std::vector<int64_t> opens = {0, 1, 2};
     // coordinate descent over (blocks per CU, LDS stack depth, pool slots, node width, leaf size,
