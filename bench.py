@@ -241,6 +241,8 @@ def run_instanced(a):
         tr.set_option(pt_amd.PT_OPT_POOL_SLOTS, a.pool_slots)
     if a.tlas_open >= 0:
         tr.set_option(pt_amd.PT_OPT_TLAS_OPEN, a.tlas_open)
+    if a.node_width:
+        tr.set_option(pt_amd.PT_OPT_NODE_WIDTH, a.node_width)
     st0 = tr.stats()
     tune = None
     if not a.no_autotune:
@@ -571,6 +573,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="traversal blocks per CU to start from")
     ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")
     ap.add_argument("--tlas-open", type=int, default=-1, help="TLAS opening depth to start from (-1: the library's)")
+    ap.add_argument("--node-width", type=int, default=0, help="BVH node width to start from (0: the library's)")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
