@@ -174,12 +174,12 @@ def convergence(tr, cam, threshold=1e-3, ref_spp=4096):
 
 def profile_figure(name, kernel):
     """A figure of a kernel from the committed rocprofv3 summaries (machine-readable lines
-    `name kernel value` in profiles/*.txt), or None."""
+    `name kernel value` in profiles/*.txt and *.md), or None."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
     for fn in sorted(os.listdir(d)):
-        if not fn.endswith(".txt"):
+        if not fn.endswith((".txt", ".md")):
             continue
         for line in open(os.path.join(d, fn)):
             f = line.split()
