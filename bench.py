@@ -462,10 +462,13 @@ def run_scenes(a):
     rays = st["extension_rays"] + st["shadow_rays"]
     batch.close()
     # each scene's views on their own
+    seq_s = 0.0
     for k in range(16):
         vk = [(0,) + v[1:] for v in views if v[0] == k]
         b = pt_amd.Batch([tracers[k]], vk)
         ek, sk = _timed_batch(b, 1, 1)
+        seq_s += ek
+        scenes_info[k]["alone_s"] = round(ek, 4)
         scenes_info[k]["mrays_s"] = round((sk["extension_rays"] + sk["shadow_rays"]) / ek / 1e6, 2)
         scenes_info[k]["l2_hit_pct"] = profile_figure("l2_hit_pct", "k_extend/scene%d" % k)
         b.close()
@@ -476,6 +479,9 @@ def run_scenes(a):
         "config": {"workload": "16 instanced scenes (10k-20M triangles) x 4 views in one batch, depth 8",
                    "pixel_samples_per_step": paths},
         "pixel_samples_per_s": round(paths * a.steps / el, 1),
+        # one step of the batch against the same 16 scenes' views rendered scene by scene
+        "batched_s_per_step": round(el / a.steps, 4), "sequential_s_per_step": round(seq_s, 4),
+        "batched_speedup": round(seq_s / (el / a.steps), 3) if el > 0 else None,
         "scenes": scenes_info,
     }
 

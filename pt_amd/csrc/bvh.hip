@@ -722,20 +722,27 @@ __device__ void tight_world_box(const InstanceData& d, const Node4* nodes, uint3
     }
 }
 
-// World box of every instance's transformed vertices (one block per instance), padded for the
-// rounding of the transform as world_box pads: it holds all of the instance's triangles and,
-// for a rotated instance, is far tighter than the transformed corners of the mesh box.
-__global__ __launch_bounds__(256) void k_vertex_boxes(uint32_t n, const InstanceData* inst, const uint32_t* inst_mesh,
-                                                      const pt_mesh_desc* meshes, const float4* vertices, float4* lo,
-                                                      float4* hi)
+// World box of every instance's transformed vertices, padded for the rounding of the transform
+// as world_box pads: it holds all of the instance's triangles and, for a rotated instance, is
+// far tighter than the transformed corners of the mesh box. Each instance's vertex range is
+// split into VBOX_PARTS slices; k_vertex_box_parts reduces each slice in one block, and
+// k_vertex_boxes combines the slices of each instance.
+constexpr uint32_t VBOX_PARTS = 16;
+
+__global__ __launch_bounds__(256) void k_vertex_box_parts(uint32_t n, const InstanceData* inst,
+                                                          const uint32_t* inst_mesh, const pt_mesh_desc* meshes,
+                                                          const float4* vertices, float4* part_lo, float4* part_hi)
 {
     __shared__ float red[6][256];
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (uint32_t b = blockIdx.x; b < n * VBOX_PARTS; b += gridDim.x) {
+        const uint32_t i = b / VBOX_PARTS, part = b % VBOX_PARTS;
         const InstanceData& d = inst[i];
         const pt_mesh_desc md = meshes[inst_mesh[i]];
+        const uint32_t per = (md.vertex_count + VBOX_PARTS - 1) / VBOX_PARTS;
+        const uint32_t v0 = min(md.vertex_count, part * per), v1 = min(md.vertex_count, v0 + per);
         float3 mn = f3(INFINITY, INFINITY, INFINITY), mx = f3(-INFINITY, -INFINITY, -INFINITY);
-        for (uint32_t v = t; v < md.vertex_count; v += 256) {
+        for (uint32_t v = v0 + t; v < v1; v += 256) {
             const float4 p = vertices[size_t(md.vertex_start + v) * 2];
             const float wx = d.m[0][0] * p.x + d.m[0][1] * p.y + d.m[0][2] * p.z + d.m[0][3];
             const float wy = d.m[1][0] * p.x + d.m[1][1] * p.y + d.m[1][2] * p.z + d.m[1][3];
@@ -756,13 +763,26 @@ __global__ __launch_bounds__(256) void k_vertex_boxes(uint32_t n, const Instance
             __syncthreads();
         }
         if (t == 0) {
-            const float3 a = f3(red[0][0], red[1][0], red[2][0]), b = f3(red[3][0], red[4][0], red[5][0]);
-            const float3 ext = f3(fabsf(a.x) + fabsf(b.x), fabsf(a.y) + fabsf(b.y), fabsf(a.z) + fabsf(b.z));
-            const float eps = 1e-6f;
-            lo[i] = make_float4(a.x - eps * ext.x, a.y - eps * ext.y, a.z - eps * ext.z, 0.0f);
-            hi[i] = make_float4(b.x + eps * ext.x, b.y + eps * ext.y, b.z + eps * ext.z, 0.0f);
+            part_lo[b] = make_float4(red[0][0], red[1][0], red[2][0], 0.0f);
+            part_hi[b] = make_float4(red[3][0], red[4][0], red[5][0], 0.0f);
         }
         __syncthreads();
+    }
+}
+
+__global__ void k_vertex_boxes(uint32_t n, const float4* part_lo, const float4* part_hi, float4* lo, float4* hi)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float3 a = f3(INFINITY, INFINITY, INFINITY), b = f3(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t k = 0; k < VBOX_PARTS; k++) {
+            const float4 l = part_lo[i * VBOX_PARTS + k], h = part_hi[i * VBOX_PARTS + k];
+            a = f3(fminf(a.x, l.x), fminf(a.y, l.y), fminf(a.z, l.z));
+            b = f3(fmaxf(b.x, h.x), fmaxf(b.y, h.y), fmaxf(b.z, h.z));
+        }
+        const float3 ext = f3(fabsf(a.x) + fabsf(b.x), fabsf(a.y) + fabsf(b.y), fabsf(a.z) + fabsf(b.z));
+        const float eps = 1e-6f;
+        lo[i] = make_float4(a.x - eps * ext.x, a.y - eps * ext.y, a.z - eps * ext.z, 0.0f);
+        hi[i] = make_float4(b.x + eps * ext.x, b.y + eps * ext.y, b.z + eps * ext.z, 0.0f);
     }
 }
 
@@ -1378,10 +1398,16 @@ hipError_t build_bvh(hipStream_t st, const float4* d_vertices, const uint32_t* d
         CHECK(sc.alloc(d_hi, slots));
         float4 *d_vlo = nullptr, *d_vhi = nullptr;
         if (T > 0 && params.tlas_box_levels >= 3) {
+            float4 *d_plo, *d_phi;
             CHECK(sc.alloc(d_vlo, instance_count));
             CHECK(sc.alloc(d_vhi, instance_count));
-            hipLaunchKernelGGL(k_vertex_boxes, dim3(std::min<uint32_t>(instance_count, 4096)), dim3(256), 0, st,
-                               instance_count, d_instances, d_instance_mesh, d_meshes, d_vertices, d_vlo, d_vhi);
+            CHECK(sc.alloc(d_plo, size_t(instance_count) * VBOX_PARTS));
+            CHECK(sc.alloc(d_phi, size_t(instance_count) * VBOX_PARTS));
+            hipLaunchKernelGGL(k_vertex_box_parts, dim3(std::min<uint32_t>(instance_count * VBOX_PARTS, 65536)),
+                               dim3(256), 0, st, instance_count, d_instances, d_instance_mesh, d_meshes, d_vertices,
+                               d_plo, d_phi);
+            hipLaunchKernelGGL(k_vertex_boxes, dim3(grid_for(instance_count)), dim3(256), 0, st, instance_count,
+                               d_plo, d_phi, d_vlo, d_vhi);
         }
         hipLaunchKernelGGL(k_open_instances, dim3(grid_for(instance_count, 64)), dim3(64), 0, st, instance_count,
                            d_instances, d_instance_mesh, d_mesh_tris, mesh_box ? mesh_box : dummy_box, out.nodes,
