@@ -461,6 +461,16 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// A batch's rays carry their scene; queue entries of one view, and so of one scene, stay
+// mostly together. True when every active lane of the wave has the scene of the first active
+// lane: the callers then index the scene table by that wave-uniform (scalar) value, so the
+// scene record is read with scalar loads and held in SGPRs for the whole traversal, as for a
+// single scene.
+__device__ __forceinline__ bool same_scene(uint32_t lane_sid, uint32_t first)
+{
+    return __ballot(lane_sid != first) == 0;
+}
+
 // Positions in NC queues for the threads of a block that want one, with one atomic per
 // queue and block: device-scope atomics on one address serialise across the whole chip, so
 // the shade and refill stages aggregate over the block instead of the wave. Every thread of
@@ -558,10 +568,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     FetchCount fc;
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
-        const DeviceScene& s = MULTI ? scenes[wf.q_scene[q][i]] : s0;
         HitOut h;
         float trans;
-        traverse<false, COUNT>(s, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
+        const uint32_t sid = MULTI ? wf.q_scene[q][i] : 0u, first = __builtin_amdgcn_readfirstlane(sid);
+        if (MULTI && !same_scene(sid, first))
+            traverse<false, COUNT>(scenes[sid], xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
+        else
+            traverse<false, COUNT>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
     }
@@ -600,11 +613,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     };
     for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
-        const DeviceScene& s = MULTI ? scenes[wf.s_scene[i]] : s0;
         HitOut h;
         float trans;
-        const bool occluded =
-            traverse<true, COUNT, StackT<D>, ALPHA>(s, xyz(o), xyz(d), 0.0f, o.w, stk, h, trans, &fc);
+        const uint32_t sid = MULTI ? wf.s_scene[i] : 0u, first = __builtin_amdgcn_readfirstlane(sid);
+        bool occluded;
+        if (MULTI && !same_scene(sid, first))
+            occluded = traverse<true, COUNT, StackT<D>, ALPHA>(scenes[sid], xyz(o), xyz(d), 0.0f, o.w, stk, h, trans,
+                                                               &fc);
+        else
+            occluded = traverse<true, COUNT, StackT<D>, ALPHA>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), 0.0f, o.w,
+                                                               stk, h, trans, &fc);
         if (!occluded && trans >= 1e-5f)
             add(i, trans);
     }
