@@ -94,7 +94,8 @@ struct ViewDev {
 };
 
 // A job is a run of the path pool over consecutive segments; a segment is a range of
-// samples of one view, its paths numbered sample-major from path_offset.
+// samples of one view, its paths numbered sample-major from path_offset (within a sample in
+// the 8x8-pixel tile order of trace.hip's tile_pixel).
 constexpr uint32_t MAX_SEGS = 1024;
 struct SegDev {
     uint32_t view;

@@ -532,14 +532,30 @@ __device__ __forceinline__ uint32_t find_segment(const JobDev* job, uint32_t p)
 
 // Starts path p of the job in slot `slot` as entry pos of queue par: camera ray of its view
 // (pathtrace.rg.hlsl:18-33) and a fresh payload.
+// Pixel of the u-th path of a sample: paths run through 8x8-pixel tiles (64 consecutive
+// paths, one wave, share a tile), tiles left to right within bands of 8 rows, bands top to
+// bottom; a band's last tile is narrower and the last band shorter when the image sides are
+// not multiples of 8. A bijection of [0, width x height) onto the pixels.
+__device__ __forceinline__ uint2 tile_pixel(uint32_t u, uint32_t width, uint32_t height)
+{
+    const uint32_t band = u / (8u * width);
+    const uint32_t rows = min(8u, height - 8u * band);
+    const uint32_t rem = u - band * 8u * width;
+    const uint32_t col = rem / (rows * 8u);
+    const uint32_t cw = min(8u, width - 8u * col);
+    const uint32_t within = rem - col * rows * 8u;
+    return make_uint2(8u * col + within % cw, 8u * band + within / cw);
+}
+
 __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, uint32_t slot, const Wavefront& wf,
                                            int par, uint32_t pos, uint32_t p)
 {
     const SegDev sg = job->seg[find_segment(job, p)];
     const ViewDev& vw = views[sg.view];
     const uint32_t local = p - sg.path_offset;
-    const uint32_t pix = local % vw.npix, sample = sg.first_sample + local / vw.npix;
-    const uint32_t px = pix % vw.width, py = pix / vw.width;
+    const uint32_t sample = sg.first_sample + local / vw.npix;
+    const uint2 pxy = tile_pixel(local % vw.npix, vw.width, vw.height);
+    const uint32_t px = pxy.x, py = pxy.y;
     Sampler smp;
     smp.start(px, py, sample, 0);
     const float2 up = smp.s2();
@@ -1171,11 +1187,13 @@ __global__ __launch_bounds__(256) void k_accumulate(const ViewDev* views, const 
         }
         const SegDev sg = job->seg[lo];
         const ViewDev& vw = views[sg.view];
-        const uint32_t pix = t - sg.pix_offset;
+        const uint32_t u = t - sg.pix_offset;  // the pixel's place in the paths' tile order
+        const uint2 pxy = tile_pixel(u, vw.width, vw.height);
+        const uint32_t pix = pxy.y * vw.width + pxy.x;
         const float exposure = vw.cam.exposure;
         float4 acc = vw.accum[pix];
         for (uint32_t sl = 0; sl < sg.samples; sl++) {
-            const float4 r4 = rad[sg.path_offset + size_t(sl) * vw.npix + pix];
+            const float4 r4 = rad[sg.path_offset + size_t(sl) * vw.npix + u];
             float rx, ry, rz;
             if (signed_radiance) {
                 rx = r4.x * exposure, ry = r4.y * exposure, rz = r4.z * exposure;
