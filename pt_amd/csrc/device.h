@@ -94,14 +94,14 @@ struct ViewDev {
 };
 
 // A job is a run of the path pool over consecutive segments; a segment is a range of
-// samples of one view, its paths numbered sample-major from path_offset (within a sample in
-// the 8x8-pixel tile order of trace.hip's tile_pixel).
+// samples of one view, its paths numbered pixel-major from path_offset (a pixel's samples
+// consecutive, pixels in the 8x8-pixel tile order of trace.hip's tile_pixel).
 constexpr uint32_t MAX_SEGS = 1024;
 struct SegDev {
     uint32_t view;
     uint32_t first_sample;  // sample index of the segment's first sample
     uint32_t samples;
-    uint32_t path_offset;   // job path index of (first_sample, pixel 0)
+    uint32_t path_offset;   // job path index of (pixel 0 of the tile order, first_sample)
     uint32_t pix_offset;    // sum of the pixel counts of the earlier segments
     uint32_t pad[3];
 };

@@ -553,8 +553,11 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     const SegDev sg = job->seg[find_segment(job, p)];
     const ViewDev& vw = views[sg.view];
     const uint32_t local = p - sg.path_offset;
-    const uint32_t sample = sg.first_sample + local / vw.npix;
-    const uint2 pxy = tile_pixel(local % vw.npix, vw.width, vw.height);
+    // pixel-major: the segment's samples of one pixel are consecutive paths, pixels in the
+    // tile order, so a wave traces nearly the same ray 64 times (or a few pixels of one tile
+    // when the segment has fewer samples)
+    const uint32_t sample = sg.first_sample + local % sg.samples;
+    const uint2 pxy = tile_pixel(local / sg.samples, vw.width, vw.height);
     const uint32_t px = pxy.x, py = pxy.y;
     Sampler smp;
     smp.start(px, py, sample, 0);
@@ -1171,49 +1174,67 @@ __global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
 }
 
 // Running mean per pixel in sample order (pathtrace.rg.hlsl:61-67, one sample per dispatch),
-// for every segment of the job.
-__global__ __launch_bounds__(256) void k_accumulate(const ViewDev* views, const JobDev* job, const float4* rad)
+// for every segment of the job. A block (one wave) takes 64 consecutive pixels of the tile
+// order, whose paths are one contiguous run (each pixel's samples in a row): chunks of ACC_C
+// samples of the 64 pixels are read with the lanes along the run (whole 128-byte lines) into
+// LDS, and each lane then folds its pixel's samples in order.
+constexpr uint32_t ACC_C = 8;
+
+__global__ __launch_bounds__(64) void k_accumulate(const ViewDev* views, const JobDev* job, const float4* rad)
 {
-    const uint32_t total = job->pix_total, nseg = job->nseg;
+    __shared__ float4 buf[64 * (ACC_C + 1)];  // a pixel's chunk per row, padded against bank conflicts
+    const uint32_t lane = threadIdx.x;
     const bool signed_radiance = job->signed_radiance != 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = nseg;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (job->seg[mid].pix_offset <= t)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const SegDev sg = job->seg[lo];
+    for (uint32_t s = 0; s < job->nseg; s++) {
+        const SegDev sg = job->seg[s];
         const ViewDev& vw = views[sg.view];
-        const uint32_t u = t - sg.pix_offset;  // the pixel's place in the paths' tile order
-        const uint2 pxy = tile_pixel(u, vw.width, vw.height);
-        const uint32_t pix = pxy.y * vw.width + pxy.x;
+        const uint32_t groups = (vw.npix + 63u) / 64u;
         const float exposure = vw.cam.exposure;
-        float4 acc = vw.accum[pix];
-        for (uint32_t sl = 0; sl < sg.samples; sl++) {
-            const float4 r4 = rad[sg.path_offset + size_t(sl) * vw.npix + u];
-            float rx, ry, rz;
-            if (signed_radiance) {
-                rx = r4.x * exposure, ry = r4.y * exposure, rz = r4.z * exposure;
-                rx = isfinite(rx) ? rx : 0.0f, ry = isfinite(ry) ? ry : 0.0f, rz = isfinite(rz) ? rz : 0.0f;
-            } else {
-                rx = correct_component(r4.x * exposure);
-                ry = correct_component(r4.y * exposure);
-                rz = correct_component(r4.z * exposure);
+        for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+            const uint32_t npx = min(64u, vw.npix - g * 64u);  // pixels of this group
+            uint32_t pix = 0;
+            float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (lane < npx) {
+                const uint2 pxy = tile_pixel(g * 64u + lane, vw.width, vw.height);
+                pix = pxy.y * vw.width + pxy.x;
+                acc = vw.accum[pix];
             }
-            const uint32_t si = sg.first_sample + sl;
-            if (si == 0) {
-                acc = make_float4(rx, ry, rz, 1.0f);
-            } else {
-                const float n = float(si + 1u);
-                acc.x = acc.x + (rx - 1.0f * acc.x) / n;
-                acc.y = acc.y + (ry - 1.0f * acc.y) / n;
-                acc.z = acc.z + (rz - 1.0f * acc.z) / n;
+            const float4* src = rad + sg.path_offset + size_t(g) * 64u * sg.samples;  // the group's first path
+            for (uint32_t c0 = 0; c0 < sg.samples; c0 += ACC_C) {
+                const uint32_t cn = min(ACC_C, sg.samples - c0);
+                for (uint32_t e = lane; e < npx * cn; e += 64u) {
+                    const uint32_t p = e / cn, k = e - p * cn;
+                    buf[p * (ACC_C + 1) + k] = src[size_t(p) * sg.samples + c0 + k];
+                }
+                __syncthreads();
+                if (lane < npx) {
+                    for (uint32_t k = 0; k < cn; k++) {
+                        const float4 r4 = buf[lane * (ACC_C + 1) + k];
+                        float rx, ry, rz;
+                        if (signed_radiance) {
+                            rx = r4.x * exposure, ry = r4.y * exposure, rz = r4.z * exposure;
+                            rx = isfinite(rx) ? rx : 0.0f, ry = isfinite(ry) ? ry : 0.0f, rz = isfinite(rz) ? rz : 0.0f;
+                        } else {
+                            rx = correct_component(r4.x * exposure);
+                            ry = correct_component(r4.y * exposure);
+                            rz = correct_component(r4.z * exposure);
+                        }
+                        const uint32_t si = sg.first_sample + c0 + k;
+                        if (si == 0) {
+                            acc = make_float4(rx, ry, rz, 1.0f);
+                        } else {
+                            const float n = float(si + 1u);
+                            acc.x = acc.x + (rx - 1.0f * acc.x) / n;
+                            acc.y = acc.y + (ry - 1.0f * acc.y) / n;
+                            acc.z = acc.z + (rz - 1.0f * acc.z) / n;
+                        }
+                    }
+                }
+                __syncthreads();
             }
+            if (lane < npx)
+                vw.accum[pix] = acc;
         }
-        vw.accum[pix] = acc;
     }
 }
 
@@ -1682,7 +1703,7 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
     // later job's paths can be written there: stream order), train between frames.
     auto finish = [&](int s) -> hipError_t {
         const uint32_t pix_total = io.h_job[s].pix_total;
-        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(pix_total, 256, 65535u * 4)), dim3(256), 0, st, io.d_views,
+        hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(pix_total, 64, 65535u * 4)), dim3(64), 0, st, io.d_views,
                            io.d_job + s, wf.rad_out + size_t(s) * wf.rad_out_capacity);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess)
