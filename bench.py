@@ -513,7 +513,7 @@ def run_cache(a):
     def run(cached):
         frames, errs, losses, records = [], {}, [], []
         if cached:
-            tr.cache_configure(**CACHE_PATH)
+            tr.cache_configure(**dict(CACHE_PATH, ema_decay=a.cache_ema))
         rays = 0
         for f, (spp, cam) in enumerate(path):
             tr.set_camera(cam)
@@ -547,7 +547,7 @@ def run_cache(a):
         "vs_baseline": None, "dtype": "f32 (cache network bf16 weights, fp32 accumulation)",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": "instanced scene, 64 frames of 1920x1080 at 1-4 spp on a keyframed camera path, "
-                               "depth 8, learned radiance cache trained online", "cache": CACHE_PATH},
+                               "depth 8, learned radiance cache trained online", "cache": dict(CACHE_PATH, ema_decay=a.cache_ema)},
         "frame_ms": {"plain_mean": round(1e3 * np.mean(pf), 3), "cached_mean": round(1e3 * np.mean(cf), 3)},
         "frame_spp": [spp for spp, _ in path],
         "relmse_vs_4096spp": {"frames": check, "plain": [perr[f] for f in check], "cached": [cerr[f] for f in check]},
@@ -580,6 +580,8 @@ def main():
     ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")
     ap.add_argument("--tlas-open", type=int, default=-1, help="TLAS opening depth to start from (-1: the library's)")
     ap.add_argument("--node-width", type=int, default=0, help="BVH node width to start from (0: the library's)")
+    ap.add_argument("--cache-ema", type=float, default=0.0,
+                    help="camera-path workload: decay of the weights' moving average the queries read (0: none)")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
     ap.add_argument("--sort-hits", type=int, default=0, help="1: shade hits in material order")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
