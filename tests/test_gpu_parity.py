@@ -69,6 +69,24 @@ def test_render_matches_oracle_per_pixel(box):
     assert np.all(g[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("w,h,spp", [(37, 23, 3), (1, 333, 10), (251, 1, 1), (9, 97, 17)])
+def test_image_sides_off_the_tile_grid_match_oracle(box, w, h, spp):
+    """Paths run pixel-major through 8x8 tiles (narrower and shorter at the right and bottom
+    edges) and are accumulated 8 samples at a time: every pixel of images whose sides are not
+    multiples of 8, with sample counts around the chunk size, gets its own samples."""
+    scene, _, _, orc = box
+    cam = scene.camera(w, h)
+    tr = pt_amd.Tracer(scene, w, h, max_bounces=3)
+    tr.set_camera(cam)
+    tr.render(spp)
+    g = tr.read()
+    tr.close()
+    o, _ = orc.render(cam, w, h, spp, 3)
+    assert np.all(g[..., 3] == 1.0)  # every pixel's first sample was accumulated
+    close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * (1.0 + np.abs(o[..., :3]))
+    assert close.all(axis=-1).mean() >= 0.97, close.all(axis=-1).mean()
+
+
 def test_render_converges_to_oracle(box):
     scene, cam, tracer, orc = box
     tracer.set_camera(cam)
