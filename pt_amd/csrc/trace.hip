@@ -660,7 +660,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 // Miss / closest-hit / Russian roulette of one segment for every entry of queue q. Continuing
 // paths move, with their state, to the front of queue 1-q; an ended path's radiance goes to
 // rad_out; next-event rays go to the shadow queue with the place their contribution is added.
-// WPE: waves per SIMD the register allocation must allow (3 = no limit beyond the block).
+// WPE: waves per SIMD the register allocation must allow (2: up to 256 registers per lane).
 template <bool MULTI, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_shade(
     DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
@@ -1393,13 +1393,14 @@ ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha)
 using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
                         const uint32_t*);
 
-// Register budget of the shade stage: PT_SHADE_WAVES (2..6 waves per SIMD; tuning) or 3,
-// the fastest measured (168 registers, hardly any spilling).
+// Register budget of the shade stage: PT_SHADE_WAVES (2..6 waves per SIMD; tuning) or 2,
+// the fastest measured with paths in pixel order (256 registers, no spilling; 3 waves spill
+// 28 registers and ran 4.5 % slower, 4 and 5 waves slower still).
 int shade_waves()
 {
     static const int w = [] {
         const char* env = getenv("PT_SHADE_WAVES");
-        return env ? std::max(2, std::min(6, atoi(env))) : 3;
+        return env ? std::max(2, std::min(6, atoi(env))) : 2;
     }();
     return w;
 }
