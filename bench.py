@@ -130,13 +130,15 @@ def time_to_threshold(tr, cam, ref, chunk, limit_spp, limit_s, threshold, before
 
 # The learned-cache estimator of the convergence runs: in-flight training (three Adam steps of
 # 8192 records after every wavefront iteration), one path in 16 trains; the others end at their
-# second vertex with the prediction of the weights' moving average (decay 0.9). Learning rate
-# 3e-3: the best of a sweep over 1e-3..1e-2, 2..4 steps, batches of 8K..16K records, strides
-# 4..16, continuation probabilities 0..0.75 and decays 0..0.999 on the instanced scene. The
-# same estimator queried at the fourth vertex is timed beside it: a later vertex leaves less of
-# each path to the cache's error, which pays where the plain tracer converges fast (the box).
+# second vertex with the prediction of the weights' moving average (decay 0.9), unless they
+# arrived there by a direction sampled with a pdf above 2 (a sharp glossy lobe, which goes on).
+# Learning rate 3e-3: the best of a sweep over 1e-3..1e-2, 2..4 steps, batches of 8K..16K
+# records, strides 4..16, continuation probabilities 0..0.75, decays 0..0.999 and pdf limits
+# 0..5 on the instanced scene. The same estimator queried at the fourth vertex is timed beside
+# it: a later vertex leaves less of each path to the cache's error, which pays where the plain
+# tracer converges fast (the box).
 CACHE_CONVERGENCE = dict(query=True, train_stride=16, train_steps=3, batch=8192, learning_rate=3e-3, frame_spp=0,
-                         continue_prob=0.0, seed=1, query_vertex=2, ema_decay=0.9)
+                         continue_prob=0.0, seed=1, query_vertex=2, ema_decay=0.9, query_max_pdf=2.0)
 CACHE_LATER_VERTEX = 4
 
 

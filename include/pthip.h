@@ -338,7 +338,10 @@ typedef struct pt_cache_config {
                             * exponential moving average e <- d e + (1 - d) w of the weights after
                             * each optimiser step, from e = 0, divided by 1 - d^steps (restarted
                             * by pt_cache_configure and pt_cache_set_params) */
-    uint32_t reserved;     /* must be 0 */
+    float query_max_pdf;   /* 0: every non-training path at the vertex ends with the prediction.
+                            * q > 0: only those that arrived by a direction sampled with a pdf
+                            * (solid angle) below q; paths from sharp glossy or specular lobes,
+                            * whose radiance a blurred prediction would miss, go on unbiased */
 } pt_cache_config;
 
 typedef struct pt_cache_stats {

@@ -125,7 +125,7 @@ class CacheConfig(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("query", C.c_uint32), ("train_stride", C.c_uint32),
                 ("train_steps", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
                 ("seed", C.c_uint32), ("frame_spp", C.c_uint32), ("continue_prob", C.c_float),
-                ("query_vertex", C.c_uint32), ("ema_decay", C.c_float), ("reserved", C.c_uint32)]
+                ("query_vertex", C.c_uint32), ("ema_decay", C.c_float), ("query_max_pdf", C.c_float)]
 
 
 class CacheStats(C.Structure):
@@ -463,9 +463,10 @@ class Tracer:
     # ---- learned radiance cache (no counterpart in the reference)
 
     def cache_configure(self, enable=True, query=True, train_stride=32, train_steps=4, batch=16384,
-                        learning_rate=1e-3, seed=1, frame_spp=1, continue_prob=0.0, query_vertex=0, ema_decay=0.0):
+                        learning_rate=1e-3, seed=1, frame_spp=1, continue_prob=0.0, query_vertex=0, ema_decay=0.0,
+                        query_max_pdf=0.0):
         cfg = CacheConfig(int(enable), int(query), train_stride, train_steps, batch, learning_rate, seed, frame_spp,
-                          continue_prob, query_vertex, ema_decay, 0)
+                          continue_prob, query_vertex, ema_decay, query_max_pdf)
         self._check(self._lib.pt_cache_configure(self._h, C.byref(cfg)))
 
     def cache_stats(self):

@@ -1145,7 +1145,8 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
     }
     if (cfg->train_stride == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f) || cfg->batch > (1u << 24) ||
         !(cfg->continue_prob >= 0.0f && cfg->continue_prob <= 1.0f) || cfg->query_vertex == 1 ||
-        cfg->query_vertex > 8 || !(cfg->ema_decay >= 0.0f && cfg->ema_decay < 1.0f) || cfg->reserved != 0)
+        cfg->query_vertex > 8 || !(cfg->ema_decay >= 0.0f && cfg->ema_decay < 1.0f) ||
+        !(cfg->query_max_pdf >= 0.0f))
         return fail(&c, PT_EINVAL, "invalid cache configuration");
     CacheState& cs = c.cache;
     cs.query = cfg->query != 0;
@@ -1158,6 +1159,7 @@ int pt_cache_configure(pt_ctx* h, const pt_cache_config* cfg)
     cs.cont_prob = cfg->continue_prob;
     cs.query_vertex = cfg->query_vertex ? cfg->query_vertex : 2u;
     cs.ema_decay = cfg->ema_decay;
+    cs.query_max_pdf = cfg->query_max_pdf;
     cs.records_total = 0;
     cs.last_records = 0;
     // one training record per training path of a job

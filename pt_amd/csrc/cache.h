@@ -64,6 +64,7 @@ struct CacheState {
     float cont_prob = 0.0f;      // control-variate continuation probability (0: terminate)
     uint32_t query_vertex = 2;   // path vertex where records are taken and queries answered
     float ema_decay = 0.0f;      // > 0: queries read the moving average of the weights
+    float query_max_pdf = 0.0f;  // > 0: query only paths whose last direction had a pdf below it
     uint32_t ema_steps = 0;      // steps averaged since the average restarted
     uint32_t steps_done = 0;
     float last_loss = 0.0f;

@@ -81,6 +81,7 @@ struct CacheDev {
     uint32_t segment;         // segment arriving at that vertex (query_vertex - 1)
     uint32_t query;           // end other paths there with the cache's prediction
     float cont_prob;          // > 0: queried paths go on with this probability (control variate)
+    float max_pdf;            // > 0: only paths that arrived by a direction of pdf below it are queried
 };
 
 // One view of a render: its camera, image and accumulation buffer (device table).

@@ -858,7 +858,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                         cL = L;
                         if (hash_u32(pix ^ (sample * 0x9E3779B9u)) % cd.train_stride == 0) {
                             want_train = true;
-                        } else if (cd.query) {
+                        } else if (cd.query && (cd.max_pdf <= 0.0f || brdf_pdf < cd.max_pdf)) {
                             want_query = true;
                             const float uc = float(hash_u32(hash_u32(pix ^ 0xA511E9B3u) + sample) >> 8) * 0x1p-24f;
                             if (uc < cd.cont_prob) {
@@ -1533,6 +1533,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
     mix(cp);
     mix(cd.train_stride);
     mix(cd.segment);
+    mix(float_as_u32(cd.max_pdf));
     mix(reinterpret_cast<uintptr_t>(cd.records));
     mix(reinterpret_cast<uintptr_t>(cd.queries));
     return h | 1ull;
@@ -1673,6 +1674,7 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
         cd.segment = cs->query_vertex - 1;  // the path segment that arrives at the cache's vertex
         cd.query = (cs->query && cs->steps_done > 0) ? 1u : 0u;  // an untrained cache is not queried
         cd.cont_prob = cs->cont_prob;
+        cd.max_pdf = cs->query_max_pdf;
     }
     // (the library radix sort is launched directly: it may record memsets)
     const bool graphs = io.graphs && !io.timer->enabled && !train_inflight && !io.sort_hits;

@@ -236,6 +236,27 @@ def test_control_variate_with_certain_continuation_is_the_plain_estimate(box):
     plain.close()
 
 
+def test_query_limit_below_every_pdf_queries_nothing(box):
+    """query_max_pdf: only paths that arrived by a direction of pdf below the limit are ended
+    with the prediction; with a limit below every sampled pdf no path is, and the image is the
+    plain tracer's even though queries are on and the network is trained."""
+    scene, cam, tr = box
+    plain = pt_amd.Tracer(scene, W, H, max_bounces=8)
+    plain.set_camera(cam)
+    plain.render(6)
+    tr.cache_configure(query=True, seed=6, train_stride=8, train_steps=2, batch=4096, query_max_pdf=1e-30)
+    tr.set_camera(cam)
+    tr.render(2)  # trains
+    tr.set_camera(cam)
+    for _ in range(6):
+        tr.render(1)
+    assert tr.cache_stats()["steps"] > 0
+    np.testing.assert_array_equal(tr.read(), plain.read())
+    with pytest.raises(pt_amd.PtError, match="cache configuration"):
+        tr.cache_configure(query_max_pdf=-1.0)
+    plain.close()
+
+
 def test_cached_estimate_gains_at_equal_samples(box):
     """The cached estimator (paths end at their 2nd vertex with the prediction) has a lower
     error against the converged plain image than the plain estimator at the same sample
