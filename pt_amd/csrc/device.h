@@ -35,8 +35,8 @@ struct Wavefront {
     float4* q_d[2] = {nullptr, nullptr};       // direction, tmax
     // job path index, py << 16 | px of its pixel, sample index, view << 16 | segment
     uint4* q_ps[2] = {nullptr, nullptr};
-    float4* q_thr[2] = {nullptr, nullptr};     // throughput rgb, brdf pdf
-    float4* q_rad[2] = {nullptr, nullptr};     // radiance rgb, bits of its training record (~0: none)
+    float4* q_thr[2] = {nullptr, nullptr};     // throughput rgb, brdf pdf (segments >= 1; implied for camera rays)
+    float4* q_rad[2] = {nullptr, nullptr};     // radiance rgb, training record bits (~0: none; segments >= 1)
     uint32_t* q_scene[2] = {nullptr, nullptr}; // scene of each queued ray (scene batches)
     float4* hit = nullptr;         // t, u, v, instance bits (by queue position)
     uint32_t* hit_prim = nullptr;

@@ -567,9 +567,9 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     camera_ray(vw.cam, px, py, vw.width, vw.height, up, ul, o, d);
     wf.q_o[par][pos] = make_float4(o.x, o.y, o.z, vw.cam.near_clip);
     wf.q_d[par][pos] = make_float4(d.x, d.y, d.z, vw.cam.far_clip);
-    wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, (sg.view << 16) | (slot << 15));  // segment 0
-    wf.q_thr[par][pos] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    wf.q_rad[par][pos] = make_float4(0.0f, 0.0f, 0.0f, u32_as_float(NONE));  // no training record
+    // segment 0; its throughput (1, pdf -1) and radiance (0, no training record) are implied,
+    // not written: the shade stage reads them only for later segments
+    wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, (sg.view << 16) | (slot << 15));
     wf.q_scene[par][pos] = vw.scene;
 }
 
@@ -693,8 +693,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             // then, for next-event estimation, (5-7) an emitting object's triangle table,
             // emitter instance, vertices and emission.
             ps = wf.q_ps[q][qi];  // job path, py << 16 | px, sample, view << 16 | segment
-            const float4 tp = wf.q_thr[q][qi];
-            const float4 rr = wf.q_rad[q][qi];
+            const bool camera_ray = (ps.w & 0x7FFFu) == 0;  // a new path: throughput 1, no radiance yet
+            const float4 tp = camera_ray ? make_float4(1.0f, 1.0f, 1.0f, -1.0f) : wf.q_thr[q][qi];
+            const float4 rr = camera_ray ? make_float4(0.0f, 0.0f, 0.0f, u32_as_float(NONE)) : wf.q_rad[q][qi];
             const float4 hh = wf.hit[qi];
             const uint32_t prim = wf.hit_prim[qi];
             const float4 ro4 = wf.q_o[q][qi], rd4 = wf.q_d[q][qi];
