@@ -245,6 +245,8 @@ def run_instanced(a):
         tr.set_option(pt_amd.PT_OPT_TLAS_OPEN, a.tlas_open)
     if a.node_width:
         tr.set_option(pt_amd.PT_OPT_NODE_WIDTH, a.node_width)
+    if a.lds_stack:
+        tr.set_option(pt_amd.PT_OPT_LDS_STACK, a.lds_stack)
     st0 = tr.stats()
     tune = None
     if not a.no_autotune:
@@ -582,6 +584,7 @@ def main():
     ap.add_argument("--pool-slots", type=int, default=0, help="path pool slots to start from")
     ap.add_argument("--tlas-open", type=int, default=-1, help="TLAS opening depth to start from (-1: the library's)")
     ap.add_argument("--node-width", type=int, default=0, help="BVH node width to start from (0: the library's)")
+    ap.add_argument("--lds-stack", type=int, default=0, help="LDS stack depth to start from (0: the library's)")
     ap.add_argument("--cache-ema", type=float, default=0.0,
                     help="camera-path workload: decay of the weights' moving average the queries read (0: none)")
     ap.add_argument("--only-scene", type=int, default=-1, help="scenes workload: render scene k alone")
