@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "lib")
 BIN = os.path.join(PKG, "bin")
 INC = os.path.join(ROOT, "include")
 
-HOST_SRCS = ["mathx.cpp", "json.cpp", "scene.cpp", "hdrio.cpp", "png.cpp", "jpeg.cpp", "api.cpp"]
+HOST_SRCS = ["mathx.cpp", "json.cpp", "import_steps.cpp", "scene.cpp", "hdrio.cpp", "png.cpp", "jpeg.cpp", "api.cpp"]
 HIP_SRCS = ["bvh.hip", "trace.hip", "cache.hip", "lights.hip", "post.hip", "api.hip"]
 HIP_HDRS = ["common.h", "device.h", "bvh.h", "shading.h", "cache.h"]
 
@@ -59,7 +59,7 @@ def build_host(force=True):
     out = os.path.join(LIB, "libpthost.so")
     deps = srcs + [os.path.join(PKG, "host", h) for h in os.listdir(os.path.join(PKG, "host")) if h.endswith(".h")]
     if force or _newer(out, deps):
-        run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + INC, "-o", out + ".tmp"] + srcs)
+        run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I" + INC, "-o", out + ".tmp"] + srcs)
         os.replace(out + ".tmp", out)
     return out
 

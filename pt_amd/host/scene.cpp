@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <unordered_map>
 
+#include "import_steps.h"
 #include "json.h"
 #include "png.h"
 
@@ -318,64 +319,392 @@ Gltf open_gltf(const std::string& path)
     return g;
 }
 
-// aiProcess_GenNormals for a primitive without normals (Assimp's GenFaceNormals, which the
-// reference's import requests, src/scene/scene.cpp:537-550; glTF 2.0 also prescribes flat
-// normals then): every face gets its own three vertices carrying the face normal
-// (v1 - v0) x (v2 - v0), divided by its length when that is not zero. Faces left as zero
-// triangles keep indices 0, 0, 0.
-void flat_normals(Mesh& mesh)
-{
-    std::vector<Vertex> out;
-    out.reserve(mesh.tris.size() * 3);
-    for (Tri& t : mesh.tris) {
-        if (t.i[0] == 0 && t.i[1] == 0 && t.i[2] == 0)
-            continue;
-        const Vertex& a = mesh.vertices[t.i[0]];
-        const Vertex& b = mesh.vertices[t.i[1]];
-        const Vertex& c = mesh.vertices[t.i[2]];
-        const float e1[3] = {b.pos[0] - a.pos[0], b.pos[1] - a.pos[1], b.pos[2] - a.pos[2]};
-        const float e2[3] = {c.pos[0] - a.pos[0], c.pos[1] - a.pos[1], c.pos[2] - a.pos[2]};
-        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-        const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        if (len > 0.0f)
-            n[0] /= len, n[1] /= len, n[2] /= len;
-        const uint32_t base = uint32_t(out.size());
-        for (const Vertex* v : {&a, &b, &c}) {
-            Vertex w = *v;
-            memcpy(w.normal, n, sizeof(n));
-            out.push_back(w);
-        }
-        t = Tri{{base, base + 1, base + 2}};
-    }
-    if (out.empty())
-        out.push_back(mesh.vertices.empty() ? Vertex{} : mesh.vertices[0]);
-    mesh.vertices = std::move(out);
-}
-
-M4 node_matrix(const Json& node)
+// GetNodeTransform of Assimp's glTF importer: the node's `matrix` (column-major), or the
+// identity multiplied on the right by the translation, rotation and scale that are present,
+// in Assimp's matrix arithmetic.
+steps::AiMat4 node_transform(const Json& node)
 {
     if (node.has("matrix")) {
-        M4 m;
+        steps::AiMat4 m;
         for (int i = 0; i < 16; i++)
-            m.c[i / 4][i % 4] = float(node["matrix"][size_t(i)].number(0));
+            m.m[i % 4][i / 4] = float(node["matrix"][size_t(i)].number(0));
         return m;
     }
-    M4 t = M4::identity(), r = M4::identity(), s = M4::identity();
-    if (node.has("translation"))
-        t = translation(V3(float(node["translation"][0].number(0)), float(node["translation"][1].number(0)),
-                           float(node["translation"][2].number(0))));
-    if (node.has("rotation")) {
-        Quat q;
-        q.x = float(node["rotation"][0].number(0));
-        q.y = float(node["rotation"][1].number(0));
-        q.z = float(node["rotation"][2].number(0));
-        q.w = float(node["rotation"][3].number(1));
-        r = mat4_from_quat(q);
+    steps::AiMat4 m = steps::ai_identity();
+    if (node.has("translation")) {
+        const Json& t = node["translation"];
+        m = steps::ai_mul(m, steps::ai_translation(float(t[size_t(0)].number(0)), float(t[size_t(1)].number(0)),
+                                                   float(t[size_t(2)].number(0))));
     }
-    if (node.has("scale"))
-        s = scaling(V3(float(node["scale"][0].number(1)), float(node["scale"][1].number(1)),
-                       float(node["scale"][2].number(1))));
-    return t * r * s;
+    if (node.has("rotation")) {
+        const Json& r = node["rotation"];
+        m = steps::ai_mul(m, steps::ai_rotation(float(r[size_t(0)].number(0)), float(r[size_t(1)].number(0)),
+                                                float(r[size_t(2)].number(0)), float(r[size_t(3)].number(1))));
+    }
+    if (node.has("scale")) {
+        const Json& s = node["scale"];
+        m = steps::ai_mul(m, steps::ai_scaling(float(s[size_t(0)].number(1)), float(s[size_t(1)].number(1)),
+                                               float(s[size_t(2)].number(1))));
+    }
+    return m;
+}
+
+// The texture path Assimp gives a glTF texture reference: the image URI, or "*n" for an image
+// held in the file (buffer view or data URI).
+std::string image_key(const Gltf& g, const Json& ref)
+{
+    const Json& t = g.doc["textures"][json_index(ref["index"])];
+    if (t.kind != Json::OBJ || !t.has("source"))
+        throw std::runtime_error("glTF: texture without an image source");
+    const size_t ii = json_index(t["source"]);
+    const Json& img = g.doc["images"][ii];
+    if (img.kind != Json::OBJ)
+        throw std::runtime_error("glTF: bad image index");
+    const bool external = img.has("uri") && img["uri"].str.compare(0, 5, "data:") != 0;
+    return external ? img["uri"].str : "*" + std::to_string(ii);
+}
+
+std::string float_text(double v)
+{
+    char b[32];
+    snprintf(b, sizeof(b), "%.9g", double(float(v)));
+    return b;
+}
+
+// A material's properties as Assimp's glTF importer stores them, as text: equal texts for
+// materials RemoveRedundantMaterials would find equal. Core values with the glTF defaults filled
+// in (the importer writes them whether the file gives them or not); a texture as its image path,
+// sampler, coordinate set, scale/strength and extensions; extension objects as given (their
+// presence is a property); the name and extras left out.
+std::string material_key(const Gltf& g, const Json& m)
+{
+    std::function<std::string(const Json&)> text;
+    auto tex = [&](const Json& ref) -> std::string {
+        if (ref.kind != Json::OBJ || !ref.has("index"))
+            return "-";
+        const Json& t = g.doc["textures"][json_index(ref["index"])];
+        return "tex(" + image_key(g, ref) + "," + (t.has("sampler") ? std::to_string(json_index(t["sampler"])) : "-") +
+               ",uv" + float_text(ref["texCoord"].number(0)) + ",s" + float_text(ref["scale"].number(1)) + ",k" +
+               float_text(ref["strength"].number(1)) + ",x" + text(ref["extensions"]) + ")";
+    };
+    text = [&](const Json& j) -> std::string {
+        switch (j.kind) {
+        case Json::NUL:
+            return "null";
+        case Json::BOOL:
+            return j.b ? "true" : "false";
+        case Json::NUM:
+            return float_text(j.num);
+        case Json::STR:
+            return "\"" + j.str + "\"";
+        case Json::ARR: {
+            std::string s = "[";
+            for (const Json& e : j.arr)
+                s += text(e) + ",";
+            return s + "]";
+        }
+        case Json::OBJ: {
+            std::string s = "{";
+            for (const auto& kv : j.obj) {
+                const bool texture_ref = kv.first.size() >= 7 &&
+                                         kv.first.compare(kv.first.size() - 7, 7, "Texture") == 0 &&
+                                         kv.second.kind == Json::OBJ && kv.second.has("index");
+                s += kv.first + "=" + (texture_ref ? tex(kv.second) : text(kv.second)) + ";";
+            }
+            return s + "}";
+        }
+        }
+        return "";
+    };
+    const Json& pbr = m["pbrMetallicRoughness"];
+    std::string k = "bc";
+    for (int i = 0; i < 4; i++)
+        k += float_text(pbr["baseColorFactor"][size_t(i)].number(1)) + ",";
+    k += "|bct" + tex(pbr["baseColorTexture"]) + "|mf" + float_text(pbr["metallicFactor"].number(1)) + "|rf" +
+         float_text(pbr["roughnessFactor"].number(1)) + "|mrt" + tex(pbr["metallicRoughnessTexture"]) + "|nt" +
+         tex(m["normalTexture"]) + "|ot" + tex(m["occlusionTexture"]) + "|et" + tex(m["emissiveTexture"]) + "|ef";
+    for (int i = 0; i < 3; i++)
+        k += float_text(m["emissiveFactor"][size_t(i)].number(0)) + ",";
+    k += "|am" + (m["alphaMode"].kind == Json::STR ? m["alphaMode"].str : std::string("OPAQUE")) + "|ac" +
+         float_text(m["alphaCutoff"].number(0.5)) + "|ds" +
+         (m["doubleSided"].kind == Json::BOOL && m["doubleSided"].b ? "1" : "0") + "|x" + text(m["extensions"]);
+    return k;
+}
+
+// Faces of a primitive as Assimp's glTF importer builds them (indices into the primitive's
+// vertex list): triangle lists as given; strips (i, i+1, i+2) for even i and (i+1, i, i+2) for odd
+// i; fans (0, i+1, i+2); points, lines, line strips and line loops as faces of one or two
+// corners. Returns the aiPrimitiveType of the mode.
+uint32_t primitive_faces(size_t mode, const std::vector<uint32_t>& idx, std::vector<steps::Face>& faces)
+{
+    auto add = [&](std::initializer_list<uint32_t> corners) {
+        steps::Face f;
+        for (uint32_t c : corners)
+            f.i[f.n++] = c;
+        faces.push_back(f);
+    };
+    const size_t n = idx.size();
+    switch (mode) {
+    case 0:
+        for (size_t i = 0; i < n; i++)
+            add({idx[i]});
+        return steps::PRIM_POINT;
+    case 1:
+        for (size_t i = 0; i + 1 < n; i += 2)
+            add({idx[i], idx[i + 1]});
+        return steps::PRIM_LINE;
+    case 2:
+    case 3:
+        if (n >= 2) {
+            add({idx[0], idx[1]});
+            for (size_t i = 2; i < n; i++)
+                add({idx[i - 1], idx[i]});
+            if (mode == 2)
+                add({idx[n - 1], idx[0]});
+        }
+        return steps::PRIM_LINE;
+    case 4:
+        if (n % 3 != 0)
+            throw std::runtime_error("glTF: index count is not a multiple of 3");
+        for (size_t i = 0; i < n; i += 3)
+            add({idx[i], idx[i + 1], idx[i + 2]});
+        return steps::PRIM_TRIANGLE;
+    case 5:
+        for (size_t i = 0; i + 2 < n; i++) {
+            if ((i + 1) % 2 == 0)
+                add({idx[i + 1], idx[i], idx[i + 2]});
+            else
+                add({idx[i], idx[i + 1], idx[i + 2]});
+        }
+        return steps::PRIM_TRIANGLE;
+    default:
+        for (size_t i = 0; i + 2 < n; i++)
+            add({idx[0], idx[i + 1], idx[i + 2]});
+        return steps::PRIM_TRIANGLE;
+    }
+}
+
+// What Assimp's glTF importer hands the post-processing steps for one file: meshes (one per
+// primitive, every face with its own vertices, texture v flipped), the materials (the file's,
+// then the importer's default one for primitives without a material) and the node graph (one
+// node per visit of the scene's hierarchy; a single scene root is the graph's root, several get
+// a "ROOT" parent).
+steps::Model import_model(const Gltf& g)
+{
+    steps::Model model;
+    const Json& doc = g.doc;
+    const Json& mats = doc["materials"];
+    for (size_t i = 0; i < mats.size(); i++) {
+        model.material_keys.push_back(material_key(g, mats[i]));
+        model.material_source.push_back(int(i));
+    }
+    Json no_material;
+    no_material.kind = Json::OBJ;
+    model.material_keys.push_back(material_key(g, no_material));
+    model.material_source.push_back(-1);
+    const uint32_t default_material = uint32_t(mats.size());
+
+    const Json& meshes = doc["meshes"];
+    std::vector<uint32_t> first_of_mesh(meshes.size() + 1, 0);
+    for (size_t mi = 0; mi < meshes.size(); mi++) {
+        first_of_mesh[mi] = uint32_t(model.meshes.size());
+        const Json& prims = meshes[mi]["primitives"];
+        const std::string mesh_name = meshes[mi]["name"].kind == Json::STR && !meshes[mi]["name"].str.empty()
+                                          ? meshes[mi]["name"].str
+                                          : "meshes_" + std::to_string(mi);
+        for (size_t pi = 0; pi < prims.size(); pi++) {
+            const Json& prim = prims[pi];
+            const size_t mode = json_index(prim["mode"], 4);
+            if (mode > 6)
+                throw std::runtime_error("glTF: unknown primitive mode");
+            const Json& attr = prim["attributes"];
+            if (!attr.has("POSITION"))
+                throw std::runtime_error("glTF: primitives need POSITION");
+            const std::vector<float> pos = g.floats(attr["POSITION"], 3);
+            const size_t nv = pos.size() / 3;
+            auto read = [&](const char* name, int comps) {
+                std::vector<float> v = g.floats(attr[name], comps);
+                if (v.size() / size_t(comps) != nv)
+                    throw std::runtime_error("glTF: attribute counts differ");
+                return v;
+            };
+            std::vector<float> nrm, tan;
+            if (attr.has("NORMAL")) {
+                nrm = read("NORMAL", 3);
+                if (attr.has("TANGENT"))  // the importer reads tangents only beside normals
+                    tan = read("TANGENT", 4);
+            }
+            std::vector<std::vector<float>> uvs, cols;
+            std::vector<int> col_comps;
+            for (int c = 0; c < 8 && attr.has("TEXCOORD_" + std::to_string(c)); c++)
+                uvs.push_back(read(("TEXCOORD_" + std::to_string(c)).c_str(), 2));
+            for (int c = 0; c < 8 && attr.has("COLOR_" + std::to_string(c)); c++) {
+                const std::string name = "COLOR_" + std::to_string(c);
+                const Json& acc = doc["accessors"][json_index(attr[name])];
+                const int comps = acc["type"].kind == Json::STR && acc["type"].str == "VEC3" ? 3 : 4;
+                cols.push_back(read(name.c_str(), comps));
+                col_comps.push_back(comps);
+            }
+            std::vector<uint32_t> idx;
+            if (prim.has("indices")) {
+                idx = g.uints(prim["indices"]);
+            } else {
+                idx.resize(nv);
+                std::iota(idx.begin(), idx.end(), 0u);
+            }
+            for (uint32_t v : idx)
+                if (v >= nv)
+                    throw std::runtime_error("glTF: index out of range");
+            std::vector<steps::Face> faces;
+            steps::Mesh mesh;
+            mesh.ptypes = primitive_faces(mode, idx, faces);
+            mesh.name = prims.size() > 1 ? mesh_name + "-" + std::to_string(pi) : mesh_name;
+            if (prim.has("material")) {
+                const size_t mat = json_index(prim["material"]);
+                if (mat >= mats.size())
+                    throw std::runtime_error("glTF: bad material index");
+                mesh.material = uint32_t(mat);
+            } else {
+                mesh.material = default_material;
+            }
+            mesh.bones = attr.has("JOINTS_0");
+            mesh.uv.resize(uvs.size());
+            mesh.uv_comps.assign(uvs.size(), 2);
+            mesh.col.resize(cols.size());
+            // one vertex per face corner (the importer's MakeVerboseFormat)
+            for (steps::Face& f : faces) {
+                for (uint32_t k = 0; k < f.n; k++) {
+                    const size_t v = f.i[k];
+                    f.i[k] = uint32_t(mesh.pos.size());
+                    mesh.pos.push_back({pos[v * 3], pos[v * 3 + 1], pos[v * 3 + 2]});
+                    if (!nrm.empty())
+                        mesh.nrm.push_back({nrm[v * 3], nrm[v * 3 + 1], nrm[v * 3 + 2]});
+                    if (!tan.empty()) {
+                        const steps::AiVec n = mesh.nrm.back();
+                        const steps::AiVec t{tan[v * 4], tan[v * 4 + 1], tan[v * 4 + 2]};
+                        const float w = tan[v * 4 + 3];
+                        mesh.tan.push_back(t);
+                        mesh.bitan.push_back({(n.y * t.z - n.z * t.y) * w, (n.z * t.x - n.x * t.z) * w,
+                                              (n.x * t.y - n.y * t.x) * w});
+                    }
+                    for (size_t c = 0; c < uvs.size(); c++) {
+                        // the importer stores 1 - v and FlipUVs takes it back: 1 - (1 - v)
+                        const float flipped = 1.0f - uvs[c][v * 2 + 1];
+                        mesh.uv[c].push_back({uvs[c][v * 2], 1.0f - flipped, 0.0f});
+                    }
+                    for (size_t c = 0; c < cols.size(); c++) {
+                        const int n = col_comps[c];
+                        mesh.col[c].push_back({cols[c][v * n], cols[c][v * n + 1], cols[c][v * n + 2],
+                                               n == 4 ? cols[c][v * n + 3] : 1.0f});
+                    }
+                }
+            }
+            mesh.faces = std::move(faces);
+            model.meshes.push_back(std::move(mesh));
+        }
+    }
+    first_of_mesh[meshes.size()] = uint32_t(model.meshes.size());
+
+    // Node names as the importer gives them (the name, else "nodes_<i>"); cameras and lights
+    // take the name of a node that references them, and those names, the nodes animations
+    // drive and the joints of skins are kept by OptimizeGraph.
+    const Json& nodes = doc["nodes"];
+    auto node_name = [&](size_t i) {
+        const Json& nm = nodes[i]["name"];
+        return nm.kind == Json::STR && !nm.str.empty() ? nm.str : "nodes_" + std::to_string(i);
+    };
+    std::vector<std::string> camera_names(doc["cameras"].size()), light_names;
+    for (size_t c = 0; c < camera_names.size(); c++)
+        camera_names[c] = doc["cameras"][c]["name"].str;
+    light_names.resize(doc["extensions"]["KHR_lights_punctual"]["lights"].size());
+    for (size_t l = 0; l < light_names.size(); l++)
+        light_names[l] = doc["extensions"]["KHR_lights_punctual"]["lights"][l]["name"].str;
+
+    std::function<uint32_t(size_t, int)> visit = [&](size_t ni, int depth) -> uint32_t {
+        if (depth > 256)
+            throw std::runtime_error("glTF node hierarchy too deep (cycle?)");
+        const Json& node = nodes[ni];
+        if (node.kind != Json::OBJ)
+            throw std::runtime_error("glTF: bad node index");
+        steps::Node n;
+        n.name = node_name(ni);
+        n.transform = node_transform(node);
+        if (node.has("mesh")) {
+            const size_t mi = json_index(node["mesh"]);
+            if (mi >= meshes.size())
+                throw std::runtime_error("glTF: bad mesh index");
+            for (uint32_t k = first_of_mesh[mi]; k < first_of_mesh[mi + 1]; k++)
+                n.meshes.push_back(k);
+        }
+        if (node.has("camera")) {
+            const size_t c = json_index(node["camera"]);
+            if (c < camera_names.size())
+                camera_names[c] = n.name;
+        }
+        const Json& light = node["extensions"]["KHR_lights_punctual"]["light"];
+        if (light.kind == Json::NUM) {
+            const size_t l = json_index(light);
+            if (l < light_names.size())
+                light_names[l] = n.name;
+        }
+        const uint32_t self = uint32_t(model.nodes.size());
+        model.nodes.push_back(n);
+        const Json& ch = node["children"];
+        for (size_t i = 0; i < ch.size(); i++) {
+            const uint32_t c = visit(json_index(ch[i]), depth + 1);
+            model.nodes[self].children.push_back(c);
+        }
+        return self;
+    };
+    std::vector<size_t> roots;
+    if (doc.has("scenes") && doc["scenes"].size() > 0) {
+        const Json& sc = doc["scenes"][json_index(doc["scene"], 0)];
+        for (size_t i = 0; i < sc["nodes"].size(); i++)
+            roots.push_back(json_index(sc["nodes"][i]));
+    } else {
+        std::vector<bool> is_child(nodes.size(), false);
+        for (size_t n = 0; n < nodes.size(); n++)
+            for (size_t c = 0; c < nodes[n]["children"].size(); c++) {
+                const size_t ci = json_index(nodes[n]["children"][c]);
+                if (ci < is_child.size())
+                    is_child[ci] = true;
+            }
+        for (size_t n = 0; n < is_child.size(); n++)
+            if (!is_child[n])
+                roots.push_back(n);
+    }
+    if (roots.size() == 1) {
+        model.root = visit(roots[0], 0);
+    } else {
+        steps::Node top;
+        top.name = "ROOT";
+        top.transform = steps::ai_identity();
+        model.nodes.push_back(top);
+        model.root = uint32_t(model.nodes.size() - 1);
+        for (size_t r : roots) {
+            const uint32_t c = visit(r, 1);
+            model.nodes[model.root].children.push_back(c);
+        }
+    }
+    for (const std::string& s : camera_names)
+        model.locked.insert(s);
+    for (const std::string& s : light_names)
+        model.locked.insert(s);
+    const Json& anims = doc["animations"];
+    for (size_t a = 0; a < anims.size(); a++)
+        for (size_t c = 0; c < anims[a]["channels"].size(); c++) {
+            const Json& target = anims[a]["channels"][c]["target"]["node"];
+            if (target.kind == Json::NUM && json_index(target) < nodes.size())
+                model.locked.insert(node_name(json_index(target)));
+        }
+    const Json& skins = doc["skins"];
+    for (size_t s = 0; s < skins.size(); s++)
+        for (size_t j = 0; j < skins[s]["joints"].size(); j++) {
+            const size_t ji = json_index(skins[s]["joints"][j]);
+            if (ji < nodes.size())
+                model.locked.insert(node_name(ji));
+        }
+    return model;
 }
 
 bool has_texture(const Json& j) { return j.kind == Json::OBJ && j.has("index"); }
@@ -522,27 +851,6 @@ void mark_transparency(Material& m, const std::vector<Texture>& textures)
         m.set_map_bits(bits | MATERIAL_IS_TRANSPARENT_BIT);
 }
 
-void walk_nodes(const Json& doc, size_t node_index, const M4& parent, std::vector<std::vector<M4>>& mesh_instances,
-                const std::vector<std::vector<uint32_t>>& mesh_prims, int depth)
-{
-    if (depth > 256)
-        throw std::runtime_error("glTF node hierarchy too deep (cycle?)");
-    const Json& node = doc["nodes"][node_index];
-    if (node.kind != Json::OBJ)
-        throw std::runtime_error("glTF: bad node index");
-    const M4 m = parent * node_matrix(node);
-    if (node.has("mesh")) {
-        size_t mi = json_index(node["mesh"]);
-        if (mi >= mesh_prims.size())
-            throw std::runtime_error("glTF: bad mesh index");
-        for (uint32_t prim_mesh : mesh_prims[mi])
-            mesh_instances[prim_mesh].push_back(m);
-    }
-    const Json& ch = node["children"];
-    for (size_t i = 0; i < ch.size(); i++)
-        walk_nodes(doc, json_index(ch[i]), m, mesh_instances, mesh_prims, depth + 1);
-}
-
 }  // namespace
 
 Scene load_gltf(const std::string& path)
@@ -555,15 +863,10 @@ Scene load_gltf(const std::string& path)
     // embedded image): one entry per distinct image, in order of first use.
     std::map<std::string, uint32_t> tex_index;
     auto tex = [&](const Json& ref, bool srgb) -> uint32_t {
-        const Json& t = g.doc["textures"][json_index(ref["index"])];
-        if (t.kind != Json::OBJ || !t.has("source"))
-            throw std::runtime_error("glTF: texture without an image source");
-        const size_t ii = json_index(t["source"]);
+        const std::string key = image_key(g, ref);
+        const size_t ii = json_index(g.doc["textures"][json_index(ref["index"])]["source"]);
         const Json& img = g.doc["images"][ii];
-        if (img.kind != Json::OBJ)
-            throw std::runtime_error("glTF: bad image index");
-        const bool external = img.has("uri") && img["uri"].str.compare(0, 5, "data:") != 0;
-        const std::string key = external ? img["uri"].str : "*" + std::to_string(ii);
+        const bool external = key[0] != '*';
         auto it = tex_index.find(key);
         if (it != tex_index.end())
             return it->second;
@@ -599,132 +902,69 @@ Scene load_gltf(const std::string& path)
         tex_index[key] = index;
         return index;
     };
+    // Assimp's import and the post-processing the reference requests (scene.cpp:537-550),
+    // then what Scene::from_file makes of the result.
+    steps::Model model = import_model(g);
+    try {
+        steps::run_steps(model);
+    } catch (const std::runtime_error& e) {
+        throw std::runtime_error("Could not load scene at " + path + ": " + e.what());
+    }
+
+    // process_materials (scene.cpp:356-396): the materials left, in order; textures are numbered
+    // by first use as they are converted
     const Json& mats = g.doc["materials"];
-    for (size_t i = 0; i < mats.size(); i++)
-        scene.materials.push_back(convert_material(mats[i], tex));
+    Json no_material;
+    no_material.kind = Json::OBJ;
+    for (int src : model.material_source)
+        scene.materials.push_back(convert_material(src >= 0 ? mats[size_t(src)] : no_material, tex));
     for (Material& m : scene.materials)
         mark_transparency(m, scene.textures);
 
-    const Json& meshes = g.doc["meshes"];
-    std::vector<std::vector<uint32_t>> mesh_prims(meshes.size());
-    for (size_t mi = 0; mi < meshes.size(); mi++) {
-        const Json& prims = meshes[mi]["primitives"];
-        for (size_t pi = 0; pi < prims.size(); pi++) {
-            const Json& prim = prims[pi];
-            const size_t mode = json_index(prim["mode"], 4);
-            if (mode > 6)
-                throw std::runtime_error("glTF: unknown primitive mode");
-            const Json& attr = prim["attributes"];
-            if (!attr.has("POSITION"))
-                throw std::runtime_error("glTF: primitives need POSITION");
-            const bool has_normals = attr.has("NORMAL");
-            if (!prim.has("material"))
-                throw std::runtime_error("glTF: primitives need a material");
-            std::vector<float> pos = g.floats(attr["POSITION"], 3);
-            std::vector<float> nrm;
-            if (has_normals)
-                nrm = g.floats(attr["NORMAL"], 3);
-            std::vector<float> uv;
-            if (attr.has("TEXCOORD_0"))
-                uv = g.floats(attr["TEXCOORD_0"], 2);
-            else
-                std::cerr << "Warning: mesh \"" << meshes[mi]["name"].str << "\" has no UVs." << std::endl;
-            const size_t nv = pos.size() / 3;
-            if ((has_normals && nrm.size() / 3 != nv) || (!uv.empty() && uv.size() / 2 != nv))
-                throw std::runtime_error("glTF: attribute counts differ");
-            Mesh mesh;
-            mesh.vertices.resize(nv);
-            for (size_t v = 0; v < nv; v++) {
-                Vertex& vx = mesh.vertices[v];
-                for (int k = 0; k < 3; k++) {
-                    vx.pos[k] = pos[v * 3 + k];
-                    vx.normal[k] = has_normals ? nrm[v * 3 + k] : 0.0f;
-                }
-                // Assimp's glTF importer flips v and aiProcess_FlipUVs flips it back.
-                vx.uv[0] = uv.empty() ? 0.0f : uv[v * 2 + 0];
-                vx.uv[1] = uv.empty() ? 0.0f : uv[v * 2 + 1];
-            }
-            std::vector<uint32_t> idx;
-            if (prim.has("indices")) {
-                idx = g.uints(prim["indices"]);
-            } else {
-                idx.resize(nv);
-                std::iota(idx.begin(), idx.end(), 0u);
-            }
-            // Triangle lists as they are; strips and fans as glTF 2.0 (3.7.2.1) unrolls them;
-            // points and lines are faces of fewer than three corners, which process_meshes
-            // leaves as all-zero triangles (scene.cpp:433-437).
-            size_t zero_faces = 0;
-            if (mode == 5 || mode == 6) {
-                std::vector<uint32_t> list;
-                for (size_t i = 0; i + 2 < idx.size(); i++) {
-                    const uint32_t t[3] = {mode == 5 ? idx[i] : idx[i + 1],
-                                           mode == 5 ? idx[i + 1 + i % 2] : idx[i + 2],
-                                           mode == 5 ? idx[i + 2 - i % 2] : idx[0]};
-                    list.insert(list.end(), t, t + 3);
-                }
-                idx.swap(list);
-            } else if (mode != 4) {
-                const size_t n = idx.size();
-                zero_faces = mode == 0 ? n : mode == 1 ? n / 2 : mode == 2 ? (n >= 2 ? n : 0) : (n >= 2 ? n - 1 : 0);
-                idx.clear();
-            }
-            if (idx.size() % 3 != 0)
-                throw std::runtime_error("glTF: index count is not a multiple of 3");
-            mesh.tris.resize(idx.size() / 3);
-            for (size_t t = 0; t < mesh.tris.size(); t++) {
-                uint32_t a = idx[t * 3], b = idx[t * 3 + 1], c = idx[t * 3 + 2];
-                if (a >= nv || b >= nv || c >= nv)
-                    throw std::runtime_error("glTF: index out of range");
-                // FindDegenerates without removal turns a face with coincident corners into a
-                // line/point, which process_meshes leaves as an all-zero triangle
-                // (scene.cpp:433-437).
-                auto same = [&](uint32_t p, uint32_t q) {
-                    return memcmp(mesh.vertices[p].pos, mesh.vertices[q].pos, 12) == 0;
-                };
-                if (same(a, b) || same(b, c) || same(a, c))
-                    mesh.tris[t] = Tri{{0, 0, 0}};
-                else
-                    mesh.tris[t] = Tri{{a, b, c}};
-            }
-            mesh.tris.resize(mesh.tris.size() + zero_faces, Tri{{0, 0, 0}});
-            if (!has_normals)
-                flat_normals(mesh);
-            const size_t mat = json_index(prim["material"]);
-            if (mat >= scene.materials.size())
-                throw std::runtime_error("glTF: bad material index");
-            mesh_prims[mi].push_back(uint32_t(scene.meshes.size()));
-            Variant var;
-            var.mesh = uint32_t(scene.meshes.size());
-            var.material = uint32_t(mat);
-            scene.meshes.push_back(std::move(mesh));
-            scene.variants.push_back(var);
+    // process_meshes (scene.cpp:398-447): one variant per mesh; texture set 0 or zero; a face of
+    // fewer than three corners leaves its triangle all zero
+    for (const steps::Mesh& am : model.meshes) {
+        Mesh mesh;
+        mesh.vertices.resize(am.pos.size());
+        for (size_t v = 0; v < am.pos.size(); v++) {
+            Vertex& vx = mesh.vertices[v];
+            const steps::AiVec n = am.nrm.empty() ? steps::AiVec{} : am.nrm[v];
+            const float p[3] = {am.pos[v].x, am.pos[v].y, am.pos[v].z}, q[3] = {n.x, n.y, n.z};
+            memcpy(vx.pos, p, sizeof(p));
+            memcpy(vx.normal, q, sizeof(q));
+            vx.uv[0] = am.uv.empty() ? 0.0f : am.uv[0][v].x;
+            vx.uv[1] = am.uv.empty() ? 0.0f : am.uv[0][v].y;
         }
+        if (am.uv.empty())
+            std::cerr << "Warning: mesh \"" << am.name << "\" has no UVs." << std::endl;
+        mesh.tris.resize(am.faces.size(), Tri{{0, 0, 0}});
+        for (size_t t = 0; t < am.faces.size(); t++)
+            if (am.faces[t].n == 3)
+                mesh.tris[t] = Tri{{am.faces[t].i[0], am.faces[t].i[1], am.faces[t].i[2]}};
+        if (mesh.vertices.empty() && !mesh.tris.empty())
+            mesh.vertices.push_back(Vertex{});  // keeps the all-zero triangles' index in range
+        Variant var;
+        var.mesh = uint32_t(scene.meshes.size());
+        var.material = am.material;
+        scene.meshes.push_back(std::move(mesh));
+        scene.variants.push_back(var);
     }
 
-    std::vector<std::vector<M4>> mesh_instances(scene.meshes.size());
-    std::vector<size_t> roots;
-    if (g.doc.has("scenes") && g.doc["scenes"].size() > 0) {
-        const size_t si = json_index(g.doc["scene"], 0);
-        const Json& sc = g.doc["scenes"][si];
-        for (size_t i = 0; i < sc["nodes"].size(); i++)
-            roots.push_back(json_index(sc["nodes"][i]));
-    } else {
-        std::vector<bool> is_child(g.doc["nodes"].size(), false);
-        for (size_t n = 0; n < g.doc["nodes"].size(); n++)
-            for (size_t c = 0; c < g.doc["nodes"][n]["children"].size(); c++) {
-                size_t ci = json_index(g.doc["nodes"][n]["children"][c]);
-                if (ci < is_child.size())
-                    is_child[ci] = true;
-            }
-        for (size_t n = 0; n < is_child.size(); n++)
-            if (!is_child[n])
-                roots.push_back(n);
-    }
-    for (size_t r : roots)
-        walk_nodes(g.doc, r, M4::identity(), mesh_instances, mesh_prims, 0);
-    for (size_t i = 0; i < scene.variants.size(); i++)
-        scene.variants[i].instances = std::move(mesh_instances[i]);
+    // process_scene_node (scene.cpp:476-488): an instance of every mesh a node references, with
+    // the product of the transforms from the root down (parent on the left)
+    std::function<void(uint32_t, const M4&)> place = [&](uint32_t n, const M4& parent) {
+        const steps::Node& node = model.nodes[n];
+        M4 local;
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++)
+                local.at(c, r) = node.transform.m[r][c];
+        const M4 t = parent * local;
+        for (uint32_t mi : node.meshes)
+            scene.variants[mi].instances.push_back(t);
+        for (uint32_t c : node.children)
+            place(c, t);
+    };
+    place(model.root, M4::identity());
 
     // Punctual lights (process_lights, src/scene/scene.cpp:449-474, reading Assimp's import of
     // KHR_lights_punctual): point and directional lights in the extension's order, colour =

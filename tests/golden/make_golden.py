@@ -1,4 +1,5 @@
-"""Regenerate tests/golden/ref_host.json from the reference's own host code.
+"""Regenerate tests/golden/ref_host.json from the reference's own host code, and
+tests/golden/assimp_math.json from the Assimp inline math the reference vendors.
 
 Builds ``oracle/_ref`` (``make -C oracle ref``: the reference's rng.cpp, aliastable.h,
 colour.cpp, lalgebra.cpp, scene/camera.cpp, io/imagewrite.cpp and util/meansq/meansq.cpp,
@@ -17,6 +18,7 @@ import tempfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 OUT = os.path.join(ROOT, "tests", "golden", "ref_host.json")
+OUT_AI = os.path.join(ROOT, "tests", "golden", "assimp_math.json")
 
 
 def main():
@@ -43,6 +45,13 @@ def main():
     with open(OUT, "w") as f:
         json.dump(data, f, indent=0, sort_keys=True)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    ai = json.loads(subprocess.check_output([os.path.join(ref, "aiharness")], text=True))
+    ai["_source"] = ("oracle/assimp_harness.cpp compiled by oracle/Makefile `ref` against the reference's "
+                     "include/assimp/{vector3,matrix3x3,matrix4x4,quaternion}.inl (Assimp 5.4.3); "
+                     "floats are bit patterns")
+    with open(OUT_AI, "w") as f:
+        json.dump(ai, f, indent=0, sort_keys=True)
+    print("wrote", OUT_AI, os.path.getsize(OUT_AI), "bytes")
 
 
 if __name__ == "__main__":

@@ -284,8 +284,9 @@ def test_pt_create_argument_checks():
 def test_gltf_without_normals_gets_flat_normals(tmp_path):
     """A primitive without NORMAL loads with Assimp's generated face normals (the reference's
     import requests aiProcess_GenNormals, src/scene/scene.cpp:537-550; glTF 2.0 prescribes flat
-    normals): each face gets its own vertices with normal (v1 - v0) x (v2 - v0) / |...|, and the
-    triangles' corners stay where they were."""
+    normals): every corner of a face carries (v1 - v0) x (v2 - v0) / |...|, the triangles'
+    corners stay where they were, and JoinIdenticalVertices keeps one vertex per distinct
+    (position, normal, uv): a corner is shared only by faces of one plane."""
     import json
     from pt_amd.scenes import generate
     generate.box_scene(str(tmp_path))
@@ -299,7 +300,9 @@ def test_gltf_without_normals_gets_flat_normals(tmp_path):
     got = pt_amd.Scene(str(tmp_path / "flat.scn")).arrays()
     assert len(ref["meshes"]) == len(got["meshes"])
     for (rv0, ri0, rn, _), (gv0, gi0, gn, gvn) in zip(ref["meshes"], got["meshes"]):
-        assert rn == gn and gvn == 3 * gn
+        assert rn == gn
+        corners = {tuple(r) for r in got["vertices"][gv0 + got["indices"][gi0:gi0 + 3 * gn]].tolist()}
+        assert gvn == len(corners)
         rv, gv = ref["vertices"], got["vertices"]
         rt = ref["indices"][ri0:ri0 + 3 * rn].reshape(-1, 3)
         gt = got["indices"][gi0:gi0 + 3 * gn].reshape(-1, 3)
@@ -327,9 +330,13 @@ def _gltf_doc(buffers_bytes, accessors, views, prims):
 
 
 def test_gltf_modes_quantised_and_sparse_accessors(tmp_path):
-    """glTF primitive modes (glTF 2.0 3.7.2.1: strips and fans unrolled, points and lines left
-    as all-zero triangles as process_meshes leaves faces of fewer than three corners,
-    scene.cpp:433-437), normalised integer texture coordinates and a sparse accessor."""
+    """glTF primitive modes as Assimp's glTF importer builds faces (strips (i, i+1, i+2) for even i
+    and (i+1, i, i+2) for odd i, fans (0, i+1, i+2); points and lines are faces of fewer than three
+    corners, which process_meshes leaves as all-zero triangles, scene.cpp:433-437), normalised
+    integer texture coordinates and a sparse accessor. The four primitives share one material and
+    vertex layout in one node, so OptimizeMeshes merges them: the later ones in its scan order
+    (the fan, then the last primitive moved into the scanned slot, then the lines), the scanning
+    one (the strip) last; JoinIdenticalVertices then numbers the distinct vertices by first use."""
     import json
     pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
     nrm = np.tile(np.array([0, 0, 1], np.float32), (4, 1))
@@ -360,15 +367,20 @@ def test_gltf_modes_quantised_and_sparse_accessors(tmp_path):
     json.dump(_gltf_doc(b"".join(parts), acc, views, prims), open(tmp_path / "m.gltf", "w"))
     (tmp_path / "m.scn").write_text("camera:\nobjects:\n m m.gltf\n\n m\n")
     a = pt_amd.Scene(str(tmp_path / "m.scn")).arrays()
-    tris = [a["indices"][i0:i0 + 3 * n].reshape(-1, 3).tolist() for _, i0, n, _ in a["meshes"]]
-    assert tris[0] == [[0, 1, 2], [1, 3, 2]]  # strip: v_i, v_(i+1+i%2), v_(i+2-i%2)
-    assert tris[1] == [[1, 3, 0], [3, 2, 0]]  # fan: v_(i+1), v_(i+2), v_0
-    assert tris[2] == [[0, 0, 0], [0, 0, 0]]  # two lines
-    v = a["vertices"]
-    v0 = a["meshes"][0][0]
-    np.testing.assert_array_equal(v[v0:v0 + 4, 6:8], uv8 / np.float32(255))
-    v3 = a["meshes"][3][0]
-    np.testing.assert_array_equal(v[v3:v3 + 4, :3], np.vstack([pos[:3], sp_val]))
+    assert len(a["meshes"]) == 1
+    v0, i0, n, nv = a["meshes"][0]
+    tris = a["indices"][i0:i0 + 3 * n].reshape(-1, 3).tolist()
+    assert tris == [[0, 1, 2], [0, 2, 3],    # fan over vertices 0 1 3 2: (0, 1, 3), (0, 3, 2)
+                    [0, 1, 3], [3, 1, 4],    # sparse strip: (0, 1, 2), (2, 1, 3'), 3' moved to (2, 2, 0)
+                    [0, 0, 0], [0, 0, 0],    # two lines
+                    [0, 1, 3], [3, 1, 2]]    # strip: (0, 1, 2), (2, 1, 3)
+    v = a["vertices"][v0:v0 + nv]
+    order = [0, 1, 3, 2, 3]
+    np.testing.assert_array_equal(v[:, :3], np.vstack([pos[[0, 1, 3, 2]], sp_val]))
+    one = np.float32(1)
+    uv = uv8[order].astype(np.float32) / np.float32(255)
+    uv[:, 1] = one - (one - uv[:, 1])  # the importer's v flip and FlipUVs' flip back
+    np.testing.assert_array_equal(v[:, 6:8], uv)
 
 
 @pytest.mark.parametrize("edit, msg", [
