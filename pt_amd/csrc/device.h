@@ -50,7 +50,8 @@ struct Wavefront {
     uint32_t rad_out_capacity = 0; // paths
     // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags of the path
     // stages, [4] ended recorded paths, [5],[6] next path to start of job slot 0, 1, [7],[8]
-    // ended paths of job slot 0, 1, [9] error flags of the ray queries (pt_trace_*)
+    // ended paths of job slot 0, 1, [9] error flags of the ray queries (pt_trace_*), [10], [11],
+    // [12] chunk counters of the extend, shadow and shade launches (one per XCD, 16 words apart)
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays, [6]/[7] TLAS node fetches / instances entered by extension

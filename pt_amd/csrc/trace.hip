@@ -449,16 +449,6 @@ __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin,
     return w.found(stk);
 }
 
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid)
-{
-    // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous share of the
-    // logical blocks so neighbouring rays share an L2.
-    const uint32_t per = grid / 8u;
-    if (per == 0 || grid % 8u != 0)
-        return b;
-    return (b % 8u) * per + b / 8u;
-}
-
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // A batch's rays carry their scene; queue entries of one view, and so of one scene, stay
@@ -573,6 +563,41 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     wf.q_scene[par][pos] = vw.scene;
 }
 
+// Queue entries of a traversal launch in chunks of 64 consecutive entries, which the waves of
+// each XCD take in order from their XCD's counter (zeroed before the launch; blocks are dealt
+// to the 8 XCDs round-robin). Chunk k of an XCD is its k-th chunk of the rounds of the
+// launch: every round of gridDim.x x TRACE_BLOCK entries gives each XCD one contiguous share,
+// so neighbouring rays share an XCD's L2 and all XCDs trace one window of the queue at a time
+// (shared BVH nodes in the Infinity Cache), while a wave that drew slow rays (grazing ones
+// visit hundreds of nodes) takes fewer chunks instead of holding up the end of the launch.
+// Each XCD's chunks cover its shares exactly once, and every wave leaves once its XCD's next
+// chunk starts past the count.
+template <typename F>
+__device__ __forceinline__ void for_each_chunk(uint32_t* ctr, uint32_t count, F&& body)
+{
+    const uint32_t grid = gridDim.x;
+    const bool split = grid % 8u == 0u;
+    const uint32_t xcd = split ? blockIdx.x % 8u : 0u;
+    const uint32_t share = (split ? grid / 8u : grid) * (TRACE_BLOCK / 64u);  // chunks of an XCD per round
+    const uint32_t round = grid * TRACE_BLOCK;
+    uint32_t* const c = ctr + xcd * 16u;  // the XCDs' counters 64 bytes apart
+    auto first = [&](uint32_t k) { return (k / share) * round + (xcd * share + k % share) * 64u; };
+    const uint32_t lane = lane_id();
+    uint32_t k = 0;
+    if (lane == 0)
+        k = atomicAdd(c, 1u);
+    uint32_t base = first(__builtin_amdgcn_readfirstlane(k));
+    while (base < count) {
+        uint32_t next = 0;
+        if (lane == 0)
+            next = atomicAdd(c, 1u);  // the next chunk, fetched while this one is traced
+        const uint32_t i = base + lane;
+        if (i < count)
+            body(i);
+        base = first(__builtin_amdgcn_readfirstlane(next));
+    }
+}
+
 // Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
 // waves in flight hide more of the dependent node fetches (measured: +3% over 5 waves; 8 waves
 // spill too much).
@@ -581,11 +606,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(q)];
-    const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
-    for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
+    for_each_chunk(&wf.counters[CTR(10)], count, [&](uint32_t i) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
         HitOut h;
         float trans;
@@ -596,7 +620,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
             traverse<false, COUNT>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
-    }
+    });
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
@@ -615,7 +639,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(2)];
-    const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
@@ -630,7 +653,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         r.z += c.z * trans;
         *dst = r;
     };
-    for (uint32_t i = gb * TRACE_BLOCK + threadIdx.x; i < count; i += gridDim.x * TRACE_BLOCK) {
+    for_each_chunk(&wf.counters[CTR(11)], count, [&](uint32_t i) {
         const float4 o = wf.s_o[i], d = wf.s_d[i];
         HitOut h;
         float trans;
@@ -644,7 +667,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
                                                                stk, h, trans, &fc);
         if (!occluded && trans >= 1e-5f)
             add(i, trans);
-    }
+    });
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
@@ -668,8 +691,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                                                const uint32_t* order)
 {
     const uint32_t count = wf.counters[CTR(q)];
-    const uint32_t rounded = (count + 255u) & ~255u;  // whole blocks take part in the compaction
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rounded; i += gridDim.x * blockDim.x) {
+    // Whole blocks take part in the compaction. A block takes chunks of 256 entries from its
+    // XCD's counter, the traversal launches' scheme by block (chunk k of an XCD is its share of
+    // round k / share), so a block that drew hits (shading, next-event estimation) takes fewer
+    // chunks than one that drew misses. Every block leaves once its XCD's next chunk starts
+    // past the count.
+    __shared__ uint32_t chunk_next;
+    const uint32_t grid = gridDim.x;
+    const bool split = grid % 8u == 0u;
+    const uint32_t xcd = split ? blockIdx.x % 8u : 0u;
+    const uint32_t share = split ? grid / 8u : grid;
+    uint32_t* const chunk_ctr = &wf.counters[CTR(12) + xcd * 16u];
+    auto first = [&](uint32_t k) { return ((k / share) * grid + xcd * share + k % share) * 256u; };
+    if (threadIdx.x == 0)
+        chunk_next = atomicAdd(chunk_ctr, 1u);
+    __syncthreads();
+    for (uint32_t k = chunk_next; first(k) < count; k = chunk_next) {
+        __syncthreads();  // every thread has read chunk_next
+        uint32_t k_after = 0;
+        if (threadIdx.x == 0)
+            k_after = atomicAdd(chunk_ctr, 1u);  // the next chunk, fetched while this one is shaded
+        const uint32_t i = first(k) + threadIdx.x;
         const bool valid = i < count;
         bool cont = false, shadow = false;
         bool want_train = false, want_query = false;
@@ -1036,6 +1078,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 cd.query_slot[qpos] = target;
             }
         }
+        if (threadIdx.x == 0)
+            chunk_next = k_after;
+        __syncthreads();
     }
 }
 
@@ -1165,6 +1210,11 @@ __global__ void k_slot_reset(Wavefront wf, uint32_t s)
 // Clears the queue counters an iteration is about to fill.
 __global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
 {
+    if (threadIdx.x < 8u) {  // the XCDs' chunk counters of the extend, shadow and shade launches
+        wf.counters[CTR(10) + threadIdx.x * 16u] = 0;
+        wf.counters[CTR(11) + threadIdx.x * 16u] = 0;
+        wf.counters[CTR(12) + threadIdx.x * 16u] = 0;
+    }
     if (threadIdx.x == 0) {
         wf.counters[CTR(1 - q)] = 0;
         wf.counters[CTR(2)] = 0;
