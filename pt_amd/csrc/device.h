@@ -127,6 +127,10 @@ struct StreamState {
     uint64_t launched = 0;   // iterations queued so far
     uint32_t seq = 0;
     bool overflow = false;   // a traversal stack overflowed (seen in a snapshot)
+    // the shadow queue holds rays of the last iteration queued, to be traced by the next one
+    // (a pipelined iteration's; a job's paths end with their last shade stage, and its
+    // accumulation is queued only after the iteration that traces those rays)
+    bool shadow_pending = false;
     uint64_t flags_from = 0; // the first iteration queued after the error flags were last cleared
     struct Slot {
         bool busy = false;
@@ -136,6 +140,13 @@ struct StreamState {
         uint32_t ended = 0;   // paths ended, as last read back
     } slot[2];
     bool idle() const { return !slot[0].busy && !slot[1].busy; }
+};
+
+// How an iteration traces its shadow rays (launch_iteration).
+struct IterMode {
+    bool pipelined = false;  // with the next iteration's extension rays
+    bool consume = false;    // this iteration traces the previous one's shadow rays with its own
+    bool flush = false;      // a non-pipelined iteration after a pipelined one: those rays first
 };
 
 // Executable graphs of one wavefront iteration (queue parity 0 and 1), captured on first use

@@ -598,17 +598,12 @@ __device__ __forceinline__ void for_each_chunk(uint32_t* ctr, uint32_t count, F&
     }
 }
 
-// Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
-// waves in flight hide more of the dependent node fetches (measured: +3% over 5 waves; 8 waves
-// spill too much).
+// Closest hits of the extension rays of queue q (count at CTR(q); chunk counters at CTR(10)).
 template <bool COUNT, bool MULTI, uint32_t D>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
+__device__ __forceinline__ void extend_queue(const DeviceScene& s0, const DeviceScene* scenes, const Wavefront& wf,
+                                             int q, StackT<D>& stk, FetchCount& fc)
 {
-    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(q)];
-    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
-    FetchCount fc;
     for_each_chunk(&wf.counters[CTR(10)], count, [&](uint32_t i) {
         const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
         HitOut h;
@@ -621,31 +616,19 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
     });
-    if (stk.overflow)
-        atomicOr(&wf.counters[CTR(3)], 1u);
-    if (COUNT && (fc.nodes | fc.tris)) {
-        atomicAdd(&wf.totals[2], (unsigned long long)fc.nodes);
-        atomicAdd(&wf.totals[3], (unsigned long long)fc.tris);
-        atomicAdd(&wf.totals[6], (unsigned long long)fc.tlas_nodes);
-        atomicAdd(&wf.totals[7], (unsigned long long)fc.entries);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(&wf.totals[0], (unsigned long long)count);
 }
 
+// Any hits of the shadow rays the shade stage of parity qs queued (count at CTR(2); chunk
+// counters at CTR(11)): an unoccluded ray adds its contribution, times the transmittance, where
+// its path keeps its radiance (the next queue's entry, or the ended path's slot).
 template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
-    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
+__device__ __forceinline__ void shadow_queue(const DeviceScene& s0, const DeviceScene* scenes, const Wavefront& wf,
+                                             int qs, StackT<D>& stk, FetchCount& fc)
 {
-    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t count = wf.counters[CTR(2)];
-    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
-    FetchCount fc;
-    // an unoccluded ray adds its contribution, times the transmittance, where its path keeps it
     auto add = [&](uint32_t i, float trans) {
         const uint32_t tgt = wf.s_target[i];
-        float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - q][tgt];
+        float4* dst = (tgt & TARGET_ENDED) ? &wf.rad_out[tgt & ~TARGET_ENDED] : &wf.q_rad[1 - qs][tgt];
         const float4 c = wf.s_c[i];
         float4 r = *dst;
         r.x += c.x * trans;
@@ -668,6 +651,40 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (!occluded && trans >= 1e-5f)
             add(i, trans);
     });
+}
+
+// Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
+// waves in flight hide more of the dependent node fetches (measured: +3% over 5 waves; 8 waves
+// spill too much).
+template <bool COUNT, bool MULTI, uint32_t D>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
+{
+    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
+    FetchCount fc;
+    extend_queue<COUNT, MULTI, D>(s0, scenes, wf, q, stk, fc);
+    if (stk.overflow)
+        atomicOr(&wf.counters[CTR(3)], 1u);
+    if (COUNT && (fc.nodes | fc.tris)) {
+        atomicAdd(&wf.totals[2], (unsigned long long)fc.nodes);
+        atomicAdd(&wf.totals[3], (unsigned long long)fc.tris);
+        atomicAdd(&wf.totals[6], (unsigned long long)fc.tlas_nodes);
+        atomicAdd(&wf.totals[7], (unsigned long long)fc.entries);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&wf.totals[0], (unsigned long long)wf.counters[CTR(q)]);
+}
+
+template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
+    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
+{
+    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
+    FetchCount fc;
+    shadow_queue<COUNT, MULTI, D, ALPHA>(s0, scenes, wf, q, stk, fc);
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
@@ -677,7 +694,40 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         atomicAdd(&wf.totals[9], (unsigned long long)fc.entries);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(&wf.totals[1], (unsigned long long)count);
+        atomicAdd(&wf.totals[1], (unsigned long long)wf.counters[CTR(2)]);
+}
+
+// The previous iteration's shadow rays (when `consume`), then this iteration's extension rays,
+// in one launch: waves that run out of shadow chunks go straight on to extension chunks, so the
+// end of the shadow work leaves no part of the chip idle. The shadow rays add to the radiance
+// of queue q's entries, which the extension part does not touch.
+template <bool MULTI, uint32_t D, bool ALPHA>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_both(
+    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q, int consume)
+{
+    __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
+    FetchCount fc;
+    if (consume)
+        shadow_queue<false, MULTI, D, ALPHA>(s0, scenes, wf, 1 - q, stk, fc);
+    extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc);
+    if (stk.overflow)
+        atomicOr(&wf.counters[CTR(3)], 1u);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&wf.totals[0], (unsigned long long)wf.counters[CTR(q)]);
+        if (consume)
+            atomicAdd(&wf.totals[1], (unsigned long long)wf.counters[CTR(2)]);
+    }
+}
+
+// Empties the shadow queue once its rays are traced (count and the XCDs' chunk counters).
+__global__ void k_clear_shadow(Wavefront wf)
+{
+    if (threadIdx.x < 8u)
+        wf.counters[CTR(11) + threadIdx.x * 16u] = 0;
+    if (threadIdx.x == 0)
+        wf.counters[CTR(2)] = 0;
 }
 
 // Miss / closest-hit / Russian roulette of one segment for every entry of queue q. Continuing
@@ -1208,7 +1258,9 @@ __global__ void k_slot_reset(Wavefront wf, uint32_t s)
 }
 
 // Clears the queue counters an iteration is about to fill.
-__global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
+// Counters an iteration starts from. The shadow queue is kept (clear_shadow = 0) when it still
+// holds the previous iteration's rays, which this iteration traces first.
+__global__ void k_prepare(Wavefront wf, int q, CacheDev cd, int clear_shadow)
 {
     if (threadIdx.x < 8u) {  // the XCDs' chunk counters of the extend, shadow and shade launches
         wf.counters[CTR(10) + threadIdx.x * 16u] = 0;
@@ -1217,7 +1269,8 @@ __global__ void k_prepare(Wavefront wf, int q, CacheDev cd)
     }
     if (threadIdx.x == 0) {
         wf.counters[CTR(1 - q)] = 0;
-        wf.counters[CTR(2)] = 0;
+        if (clear_shadow)
+            wf.counters[CTR(2)] = 0;
         wf.counters[CTR(4)] = 0;
         if (cd.collect)
             cd.counters[1] = 0;
@@ -1440,6 +1493,18 @@ ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha)
     return depth == 8 ? pick_shadow_d<8>(count, multi, alpha)
                       : (depth == 32 ? pick_shadow_d<32>(count, multi, alpha) : pick_shadow_d<16>(count, multi, alpha));
 }
+typedef void (*BothFn)(DeviceScene, const DeviceScene*, Wavefront, int, int);
+template <uint32_t D>
+BothFn pick_both_d(bool multi, bool alpha)
+{
+    return multi ? (alpha ? k_trace_both<true, D, true> : k_trace_both<true, D, false>)
+                 : (alpha ? k_trace_both<false, D, true> : k_trace_both<false, D, false>);
+}
+BothFn pick_both(bool multi, uint32_t depth, bool alpha)
+{
+    return depth == 8 ? pick_both_d<8>(multi, alpha)
+                      : (depth == 32 ? pick_both_d<32>(multi, alpha) : pick_both_d<16>(multi, alpha));
+}
 
 using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
                         const uint32_t*);
@@ -1497,7 +1562,11 @@ unsigned shade_grid(bool multi)
 
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
 // cache queries, refill, and copy the counts the host steers by to pinned memory.
-hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
+// Pipelined (mode.pipelined): the shade stage's shadow rays are traced in the next iteration,
+// in one launch with its extension rays (k_trace_both; mode.consume when the queue holds the
+// previous iteration's rays). mode.flush: a non-pipelined iteration first traces the rays a
+// pipelined one left.
+hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks, IterMode mode)
 {
     Wavefront& wf = *io.wf;
     hipStream_t st = io.st;
@@ -1505,13 +1574,25 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     const bool multi = io.d_scenes != nullptr;
     const DeviceScene* tab = io.d_scenes;
     hipError_t e;
-    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd);
-    if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
-        return e;
     const dim3 tg(io.grid_blocks), tb(TRACE_BLOCK);
-    hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
-    if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
-        return e;
+    const bool keep_shadow = mode.pipelined || mode.flush;
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd, keep_shadow ? 0 : 1);
+    if (mode.flush) {
+        hipLaunchKernelGGL(pick_shadow(false, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf,
+                           1 - q);
+        hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
+    }
+    if (mode.pipelined) {
+        hipLaunchKernelGGL(pick_both(multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf, q,
+                           mode.consume ? 1 : 0);
+        hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
+    } else {
+        if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
+        if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
+            return e;
+    }
     const uint32_t* order = nullptr;
     if (io.sort_hits) {
         // shade in (scene, material) order: radix sort of the hit keys over the whole pool
@@ -1534,12 +1615,14 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
                        wf, q, cd, order);
     if ((e = timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
         return e;
-    if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
-        return e;
-    hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene,
-                       tab, wf, q);
-    if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
-        return e;
+    if (!mode.pipelined) {
+        if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st,
+                           io.scene, tab, wf, q);
+        if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
+            return e;
+    }
     if (cd.query) {
         if ((e = timer.mark(st, KERNEL_CACHE, true)) != hipSuccess)
             return e;
@@ -1561,10 +1644,11 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
 }
 
 // Launch configuration an iteration graph depends on.
-uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
+uint64_t graph_signature(const RenderIO& io, const CacheDev& cd, IterMode mode)
 {
     uint64_t h = 1469598103934665603ull;
     auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    mix((mode.pipelined ? 1u : 0u) | (mode.consume ? 2u : 0u) | (mode.flush ? 4u : 0u));
     mix(io.count_fetches);
     mix(io.lds_stack);
     mix(io.alpha_shadows);
@@ -1591,10 +1675,10 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd)
 }
 
 // Runs one iteration through its captured graph (captured on first use).
-hipError_t graph_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks)
+hipError_t graph_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks, IterMode mode)
 {
     IterGraphs& g = *io.graphs;
-    const uint64_t sig = graph_signature(io, cd);
+    const uint64_t sig = graph_signature(io, cd, mode);
     hipError_t e;
     if (!g.exec[q] || g.sig[q] != sig) {
         if (g.exec[q]) {
@@ -1603,7 +1687,7 @@ hipError_t graph_iteration(const RenderIO& io, const CacheDev& cd, int q, unsign
         }
         if ((e = hipStreamBeginCapture(io.st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return e;
-        hipError_t le = launch_iteration(io, cd, q, shade_blocks);
+        hipError_t le = launch_iteration(io, cd, q, shade_blocks, mode);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(io.st, &graph);
         if (le != hipSuccess || e != hipSuccess) {
@@ -1729,6 +1813,10 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
     }
     // (the library radix sort is launched directly: it may record memsets)
     const bool graphs = io.graphs && !io.timer->enabled && !train_inflight && !io.sort_hits;
+    // Shadow rays traced with the next iteration's extension rays: in the production path (the
+    // profiled pass times every stage on its own; fetch counting and the cache, whose training
+    // targets need every path's radiance complete when it ends, trace them in their iteration)
+    const bool pipelined = graphs && !io.count_fetches && !cs;
     size_t next_job = 0;
     auto load = [&](int s) -> hipError_t {
         const JobPlan& plan = jobs[next_job++];
@@ -1798,9 +1886,15 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
                 break;
         }
         // One iteration. With an empty pool it only refills, so a stream starts by itself.
-        e = graphs ? graph_iteration(io, cd, S.q, shade_blocks) : launch_iteration(io, cd, S.q, shade_blocks);
+        IterMode mode;
+        mode.pipelined = pipelined;
+        mode.consume = pipelined && S.shadow_pending;
+        mode.flush = !pipelined && S.shadow_pending;
+        e = graphs ? graph_iteration(io, cd, S.q, shade_blocks, mode)
+                   : launch_iteration(io, cd, S.q, shade_blocks, mode);
         if (e != hipSuccess)
             return e;
+        S.shadow_pending = pipelined;
         if (train_inflight) {
             if ((e = io.timer->mark(st, KERNEL_CACHE, true)) != hipSuccess)
                 return e;

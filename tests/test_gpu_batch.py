@@ -113,3 +113,34 @@ def test_graph_replay_equals_direct_launches(scenes, kind):
         imgs.append(t.read())
         t.close()
     np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
+@pytest.mark.parametrize("kind", ["box", "inst"])
+def test_pipelined_shadow_rays_across_mode_changes(scenes, kind):
+    """With graphs, an iteration's shadow rays are traced in the next iteration's launch, with
+    its extension rays; iterations that time their stages or count fetches trace them in their
+    own iteration and first trace any the previous iteration left. Renders that switch between
+    the two (a small pool, so paths stream across calls) equal renders in either mode alone."""
+    scene, _ = scenes[kind]
+    w, h = 96, 64
+    cam = scene.camera(w, h)
+
+    def run(plan):
+        t = pt_amd.Tracer(scene, w, h, max_bounces=8)
+        t.set_option(pt_amd.PT_OPT_POOL_SLOTS, 4096)
+        t.set_camera(cam)
+        for option, value, spp in plan:
+            if option is not None:
+                t.set_option(option, value)
+            t.render(spp)
+        img = t.read()
+        t.close()
+        return img
+
+    pipelined = run([(None, 0, 3), (None, 0, 5), (None, 0, 4)])
+    per_stage = run([(pt_amd.PT_OPT_PROFILE_KERNELS, 1, 3), (None, 0, 5), (None, 0, 4)])
+    mixed = run([(None, 0, 3), (pt_amd.PT_OPT_PROFILE_KERNELS, 1, 5), (pt_amd.PT_OPT_PROFILE_KERNELS, 0, 1),
+                 (pt_amd.PT_OPT_COUNT_FETCHES, 1, 2), (pt_amd.PT_OPT_COUNT_FETCHES, 0, 1)])
+    assert np.isfinite(pipelined).all() and pipelined[..., :3].mean() > 0
+    np.testing.assert_array_equal(pipelined, per_stage)
+    np.testing.assert_array_equal(pipelined, mixed)

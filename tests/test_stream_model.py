@@ -13,11 +13,16 @@ random schedules that:
     exactly once;
   * a job is accumulated only after all its paths have ended, none of them still in the pool;
   * jobs are accumulated in the order they were loaded;
-  * the stream drains (every job finishes).
+  * the stream drains (every job finishes);
+  * every shadow ray is traced exactly once, and a job is accumulated only after the shadow
+    rays of all its paths are traced: pipelined iterations (the production path with graphs)
+    trace an iteration's shadow rays in the next iteration, before its shade stage; other
+    iterations trace them in their own, after tracing any a pipelined one left (flush).
 
 A variant with the defect found when the two-slot design first ran on the GPU (a freed slot
 kept its start counter and the refill's unsigned `paths - started` wrapped) must violate the
-first property, which shows the checks see that class of error.
+first property, and variants that skip the flush or trace the queue a non-pipelined iteration
+already traced must violate the last, which shows the checks see those classes of error.
 """
 import random
 
@@ -46,6 +51,16 @@ class Device:
         self.q = 0
         self.issued = {}      # job id -> set of issued paths
         self.done = {}        # job id -> set of ended paths
+        self.shadow = []      # the shadow queue: (job id, path, ray number), until it is cleared
+        self.queued = {}      # job id -> shadow rays ever queued for its paths
+        self.traced = set()   # shadow rays traced
+        self.rays = 0
+
+    def trace_shadow(self):
+        for ray in self.shadow:
+            if ray in self.traced:
+                raise Violation("shadow ray %r traced twice" % (ray,))
+            self.traced.add(ray)
 
     # k_slot_reset: a job was loaded into slot s
     def slot_reset(self, s, paths, seq, job_id):
@@ -60,10 +75,20 @@ class Device:
             self.next[s] = 0
             self.ended[s] = 0
 
-    def iteration(self):
-        # shade: every queued path continues or ends (ended paths counted per slot)
+    def iteration(self, pipelined=False, consume=False, flush=False):
+        # the previous iteration's shadow rays: with this iteration's extension rays (consume)
+        # or before it (flush); a queue not handed over is cleared by k_prepare
+        if consume or flush:
+            self.trace_shadow()
+        self.shadow = []  # k_clear_shadow after tracing them, else k_prepare
+        # shade: every queued path continues or ends (ended paths counted per slot), most
+        # with a shadow ray towards a light
         cont = []
         for slot, jid, p in self.pool:
+            if self.rng.random() < 0.7:
+                self.shadow.append((jid, p, self.rays))
+                self.queued.setdefault(jid, set()).add((jid, p, self.rays))
+                self.rays += 1
             if self.rng.random() < 0.45:
                 cont.append((slot, jid, p))
             else:
@@ -94,6 +119,8 @@ class Device:
                     raise Violation("path %d of job %s issued twice" % (p, jid))
                 iss.add(p)
                 cont.append((s, jid, p))
+        if not pipelined:
+            self.trace_shadow()
         # k_refill_commit
         for s in (0, 1):
             self.next[s] += n[s]
@@ -102,12 +129,14 @@ class Device:
         self.q = 1 - self.q
 
 
-def run_stream(rng, calls, capacity, clear_on_free=True, clamp=True, max_iters=100000):
-    """Host side of render_views over a list of render calls (each a list of job sizes);
-    returns the job ids in the order they were accumulated."""
+def run_stream(rng, calls, capacity, clear_on_free=True, clamp=True, max_iters=100000, modes=None, flush=True,
+               consume_always=False):
+    """Host side of render_views over a list of render calls (each a list of job sizes, traced
+    pipelined or not as `modes` says, default not); returns the job ids in load order and in
+    the order they were accumulated."""
     dev = Device(capacity, rng, clear_on_free, clamp)
     slots = [dict(busy=False, paths=0, seq=0, first=0, issued=0, ended=0, id=None) for _ in range(2)]
-    st = {"launched": 0, "seq": 0, "q": 0}
+    st = {"launched": 0, "seq": 0, "q": 0, "pending": False}
     finished, loaded = [], []
     next_id = [0]
 
@@ -129,11 +158,13 @@ def run_stream(rng, calls, capacity, clear_on_free=True, clamp=True, max_iters=1
             raise Violation("job %s accumulated with %d of %d paths ended" % (jid, len(dev.done.get(jid, ())), paths))
         if any(j == jid for _, j, _ in dev.pool):
             raise Violation("job %s accumulated while paths of it are in the pool" % jid)
+        if not dev.queued.get(jid, set()) <= dev.traced:
+            raise Violation("job %s accumulated before its shadow rays were traced" % jid)
         finished.append(jid)
         dev.slot_free(s)
         slots[s]["busy"] = False
 
-    def render(jobs, drain):
+    def render(jobs, drain, pipelined=False):
         jobs = list(jobs)
         iters = 0
         while True:
@@ -145,7 +176,9 @@ def run_stream(rng, calls, capacity, clear_on_free=True, clamp=True, max_iters=1
             if not drain and not jobs:
                 if all(not sl["busy"] or sl["issued"] == sl["paths"] for sl in slots):
                     return
-            dev.iteration()
+            consume = pipelined and (st["pending"] or consume_always)
+            dev.iteration(pipelined, consume, flush and not pipelined and st["pending"])
+            st["pending"] = pipelined
             st["q"] = 1 - st["q"]
             st["launched"] += 1
             iters += 1
@@ -169,9 +202,12 @@ def run_stream(rng, calls, capacity, clear_on_free=True, clamp=True, max_iters=1
                     break
                 finish(oldest)
 
-    for jobs in calls:
-        render(jobs, drain=False)
-    render([], drain=True)  # pt_synchronize / readback
+    modes = modes or [False] * len(calls)
+    for jobs, pipelined in zip(calls, modes):
+        render(jobs, drain=False, pipelined=pipelined)
+        if rng.random() < 0.3:  # an option change settles the stream first
+            render([], drain=True, pipelined=pipelined)
+    render([], drain=True, pipelined=modes[-1] if modes else False)  # pt_synchronize / readback
     for jid in loaded:
         if len(dev.issued.get(jid, ())) != len(dev.done.get(jid, ())):
             raise Violation("job %s: issued and ended paths differ" % jid)
@@ -215,3 +251,29 @@ def test_either_fix_alone_keeps_the_protocol(clear, clamp):
         calls, cap = _schedule(rng)
         loaded, finished = run_stream(rng, calls, cap, clear_on_free=clear, clamp=clamp)
         assert finished == loaded
+
+
+def test_pipelined_shadow_rays_over_random_mode_changes():
+    rng = random.Random(2024)
+    for _ in range(600):
+        calls, cap = _schedule(rng)
+        modes = [rng.random() < 0.6 for _ in calls]
+        loaded, finished = run_stream(rng, calls, cap, modes=modes)
+        assert finished == loaded
+
+
+@pytest.mark.parametrize("defect", ["no flush", "consume a traced queue"])
+def test_model_detects_mishandled_shadow_queues(defect):
+    rng = random.Random(5)
+    seen = False
+    for _ in range(300):
+        calls, cap = _schedule(rng)
+        modes = [rng.random() < 0.5 for _ in calls]
+        try:
+            run_stream(rng, calls, cap, modes=modes, flush=defect != "no flush",
+                       consume_always=defect == "consume a traced queue")
+        except Violation as e:
+            assert "shadow" in str(e)
+            seen = True
+            break
+    assert seen
