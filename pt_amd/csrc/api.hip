@@ -1706,8 +1706,11 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
     };
     const Cfg start{c.blocks_per_cu, c.pool_slots, c.lds_stack, c.leaf_size, c.node_width, c.tlas_open};
     uint32_t trials = 0;
-    // device nanoseconds per traced ray of spp samples of the current view (best of two runs
-    // after a first one that warms the caches and captures the graphs)
+    // device nanoseconds per traced ray of spp samples of the current view in the streaming
+    // steady state the renders run in: calls that return once their paths have all started, the
+    // next call's paths entering while the previous ones drain (best of two timed calls after
+    // one that fills the pool, warms the caches and captures the graphs; the stream is drained
+    // only when the configuration changes)
     auto measure = [&](const Cfg& g, double& ns) -> int {
         if (g.bpc != int64_t(c.blocks_per_cu))
             if (int v = pt_set_option(h, PT_OPT_BLOCKS_PER_CU, g.bpc))
@@ -1727,21 +1730,26 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
             if (int v = pt_set_option(h, PT_OPT_TLAS_OPEN, g.open))
                 return v;
         ns = 1e300;
-        for (int rep = 0; rep < 3; rep++) {
+        c.sample_index = 0;
+        HIP_TRY(&c, render_samples(c, spp, false), "render");
+        c.sample_index += spp;
+        for (int rep = 0; rep < 2; rep++) {
             unsigned long long t0[2], t1[2];
+            HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
             HIP_TRY(&c, hipMemcpy(t0, c.wf.totals, sizeof(t0), hipMemcpyDeviceToHost), "stats");
-            c.sample_index = 0;
             HIP_TRY(&c, hipEventRecord(c.ev_start, c.stream), "event");
-            HIP_TRY(&c, render_samples(c, spp, true), "render");
+            HIP_TRY(&c, render_samples(c, spp, false), "render");
+            c.sample_index += spp;
             HIP_TRY(&c, hipEventRecord(c.ev_stop, c.stream), "event");
             HIP_TRY(&c, hipEventSynchronize(c.ev_stop), "event");
             float ms = 0.0f;
             HIP_TRY(&c, hipEventElapsedTime(&ms, c.ev_start, c.ev_stop), "event");
             HIP_TRY(&c, hipMemcpy(t1, c.wf.totals, sizeof(t1), hipMemcpyDeviceToHost), "stats");
             const double rays = double(t1[0] - t0[0]) + double(t1[1] - t0[1]);
-            if (rep > 0 && rays > 0)
+            if (rays > 0)
                 ns = std::min(ns, ms * 1e6 / rays);
         }
+        HIP_TRY(&c, drain_stream(c), "render");
         trials++;
         return PT_OK;
     };
