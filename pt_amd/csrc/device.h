@@ -51,7 +51,9 @@ struct Wavefront {
     // at CTR(k): [0],[1] extension queues, [2] shadow queue, [3] error flags of the path
     // stages, [4] ended recorded paths, [5],[6] next path to start of job slot 0, 1, [7],[8]
     // ended paths of job slot 0, 1, [9] error flags of the ray queries (pt_trace_*), [10], [11],
-    // [12] chunk counters of the extend, shadow and shade launches (one per XCD, 16 words apart)
+    // [12] chunk counters of the extend, shadow and shade launches (one per XCD, 16 words apart),
+    // [13] the last refill plan (6 words: continuing entries, older slot, first new path and new
+    // paths per slot)
     uint32_t* counters = nullptr;
     // [0] extension rays, [1] shadow rays, [2]/[3] node/triangle fetches of extension
     // rays, [4]/[5] of shadow rays, [6]/[7] TLAS node fetches / instances entered by extension
@@ -146,7 +148,8 @@ struct StreamState {
 struct IterMode {
     bool pipelined = false;  // with the next iteration's extension rays
     bool consume = false;    // this iteration traces the previous one's shadow rays with its own
-    bool flush = false;      // a non-pipelined iteration after a pipelined one: those rays first
+                             // and starts the new paths it planned
+    bool flush = false;      // a non-pipelined iteration after a pipelined one: those first
 };
 
 // Executable graphs of one wavefront iteration (queue parity 0 and 1), captured on first use

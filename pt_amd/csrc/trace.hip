@@ -538,7 +538,8 @@ __device__ __forceinline__ uint2 tile_pixel(uint32_t u, uint32_t width, uint32_t
 }
 
 __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* job, uint32_t slot, const Wavefront& wf,
-                                           int par, uint32_t pos, uint32_t p)
+                                           int par, uint32_t pos, uint32_t p, float4* ro = nullptr,
+                                           float4* rd = nullptr)
 {
     const SegDev sg = job->seg[find_segment(job, p)];
     const ViewDev& vw = views[sg.view];
@@ -555,12 +556,50 @@ __device__ __forceinline__ void start_path(const ViewDev* views, const JobDev* j
     const float2 ul = smp.s2();
     float3 o, d;
     camera_ray(vw.cam, px, py, vw.width, vw.height, up, ul, o, d);
-    wf.q_o[par][pos] = make_float4(o.x, o.y, o.z, vw.cam.near_clip);
-    wf.q_d[par][pos] = make_float4(d.x, d.y, d.z, vw.cam.far_clip);
+    const float4 o4 = make_float4(o.x, o.y, o.z, vw.cam.near_clip), d4 = make_float4(d.x, d.y, d.z, vw.cam.far_clip);
+    wf.q_o[par][pos] = o4;
+    wf.q_d[par][pos] = d4;
+    if (ro) {
+        *ro = o4;
+        *rd = d4;
+    }
     // segment 0; its throughput (1, pdf -1) and radiance (0, no training record) are implied,
     // not written: the shade stage reads them only for later segments
     wf.q_ps[par][pos] = make_uint4(p, (py << 16) | px, sample, (sg.view << 16) | (slot << 15));
     wf.q_scene[par][pos] = vw.scene;
+}
+
+// Which new paths an iteration's refill appends to the next queue (behind its n_cont
+// continuing entries): the older job slot's first, then the other's. k_refill_commit stores the
+// plan at CTR(13) when it advances the counters; the entries are then written by k_refill, or,
+// in a pipelined iteration, by the next iteration's traversal launch, which traces them at once.
+struct RefillPlan {
+    uint32_t n_cont, first;   // continuing entries, the older slot
+    uint32_t next[2], n[2];   // per slot: first new path id, new paths
+};
+
+__device__ __forceinline__ RefillPlan stored_plan(const Wavefront& wf)
+{
+    const uint32_t* w = wf.counters + CTR(13);
+    RefillPlan r;
+    r.n_cont = w[0];
+    r.first = w[1];
+    r.next[0] = w[2];
+    r.next[1] = w[3];
+    r.n[0] = w[4];
+    r.n[1] = w[5];
+    return r;
+}
+
+// Starts the new path planned for entry pos (>= n_cont) of queue par.
+__device__ __forceinline__ void issue_planned(const ViewDev* views, const JobDev* j0, const JobDev* j1,
+                                              const RefillPlan& r, const Wavefront& wf, int par, uint32_t pos,
+                                              float4* ro = nullptr, float4* rd = nullptr)
+{
+    const uint32_t k = pos - r.n_cont;
+    const uint32_t s = k < r.n[r.first] ? r.first : 1u - r.first;
+    const uint32_t p = r.next[s] + (s == r.first ? k : k - r.n[r.first]);
+    start_path(views, s ? j1 : j0, s, wf, par, pos, p, ro, rd);
 }
 
 // Queue entries of a traversal launch in chunks of 64 consecutive entries, which the waves of
@@ -599,13 +638,21 @@ __device__ __forceinline__ void for_each_chunk(uint32_t* ctr, uint32_t count, F&
 }
 
 // Closest hits of the extension rays of queue q (count at CTR(q); chunk counters at CTR(10)).
+// With `issue` (the views and job slots given), the entries the stored refill plan appended are
+// started here: their camera rays are written for the shade stage and traced without being read.
 template <bool COUNT, bool MULTI, uint32_t D>
 __device__ __forceinline__ void extend_queue(const DeviceScene& s0, const DeviceScene* scenes, const Wavefront& wf,
-                                             int q, StackT<D>& stk, FetchCount& fc)
+                                             int q, StackT<D>& stk, FetchCount& fc, const ViewDev* views = nullptr,
+                                             const JobDev* j0 = nullptr, const JobDev* j1 = nullptr)
 {
     const uint32_t count = wf.counters[CTR(q)];
+    const RefillPlan plan = views ? stored_plan(wf) : RefillPlan{count, 0, {0, 0}, {0, 0}};
     for_each_chunk(&wf.counters[CTR(10)], count, [&](uint32_t i) {
-        const float4 o = wf.q_o[q][i], d = wf.q_d[q][i];
+        float4 o, d;
+        if (i >= plan.n_cont)
+            issue_planned(views, j0, j1, plan, wf, q, i, &o, &d);
+        else
+            o = wf.q_o[q][i], d = wf.q_d[q][i];
         HitOut h;
         float trans;
         const uint32_t sid = MULTI ? wf.q_scene[q][i] : 0u, first = __builtin_amdgcn_readfirstlane(sid);
@@ -703,20 +750,24 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 // of queue q's entries, which the extension part does not touch.
 template <bool MULTI, uint32_t D, bool ALPHA>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_both(
-    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q, int consume)
+    DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q, const ViewDev* views, const JobDev* j0,
+    const JobDev* j1, int carry)
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
-    if (consume)
+    if (carry) {  // the previous iteration was pipelined: its shadow rays and its new paths
         shadow_queue<false, MULTI, D, ALPHA>(s0, scenes, wf, 1 - q, stk, fc);
-    extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc);
+        extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc, views, j0, j1);
+    } else {
+        extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc);
+    }
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&wf.totals[0], (unsigned long long)wf.counters[CTR(q)]);
-        if (consume)
+        if (carry)
             atomicAdd(&wf.totals[1], (unsigned long long)wf.counters[CTR(2)]);
     }
 }
@@ -1162,14 +1213,9 @@ __global__ __launch_bounds__(256) void k_hit_keys(DeviceScene s0, const DeviceSc
 
 // Ends an iteration: completes the training records of recorded paths that ended in it
 // (their radiance is final once the shadow rays and cache queries have added theirs), then
-// appends new paths behind the continuing ones in queue 1-q: the remaining paths of the older
-// job slot first, then those of the newer one, consecutive path ids into consecutive entries,
-// so no atomics. k_refill_commit then advances the counters.
-struct RefillPlan {
-    uint32_t n_cont, first;   // continuing entries, the older slot
-    uint32_t next[2], n[2];   // per slot: first new path id, new paths
-};
-
+// writes the new paths k_refill_commit planned behind the continuing ones in queue 1-q: the
+// remaining paths of the older job slot first, then those of the newer one, consecutive path ids
+// into consecutive entries, so no atomics.
 __device__ __forceinline__ RefillPlan refill_plan(const JobDev* j0, const JobDev* j1, const Wavefront& wf, int q)
 {
     RefillPlan r;
@@ -1209,20 +1255,28 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev* views, const JobD
             rec[19] = 1.0f;  // complete
         }
     }
-    const RefillPlan r = refill_plan(j0, j1, wf, q);
-    const uint32_t s0 = r.first, s1 = 1u - r.first;
-    for (uint32_t k = tid; k < r.n[s0]; k += stride)
-        start_path(views, s0 ? j1 : j0, s0, wf, 1 - q, r.n_cont + k, r.next[s0] + k);
-    for (uint32_t k = tid; k < r.n[s1]; k += stride)
-        start_path(views, s1 ? j1 : j0, s1, wf, 1 - q, r.n_cont + r.n[s0] + k, r.next[s1] + k);
+    // the new paths k_refill_commit planned
+    const RefillPlan r = stored_plan(wf);
+    const uint32_t n_new = r.n[0] + r.n[1];
+    for (uint32_t k = tid; k < n_new; k += stride)
+        issue_planned(views, j0, j1, r, wf, 1 - q, r.n_cont + k);
 }
 
-// Advances the counters by what k_refill appended and leaves the counts the host steers by
-// (next queue, paths ended and paths started per slot) in wf.snapshot[q].
+// Plans the refill (the job's next paths behind the continuing ones, stored at CTR(13)),
+// advances the counters by it and leaves the counts the host steers by (next queue, paths ended
+// and paths started per slot) in wf.snapshot[q]. The planned entries are written by k_refill or
+// by the next pipelined traversal launch.
 __global__ void k_refill_commit(const JobDev* j0, const JobDev* j1, Wavefront wf, int q)
 {
     if (threadIdx.x == 0) {
         const RefillPlan r = refill_plan(j0, j1, wf, q);
+        uint32_t* w = wf.counters + CTR(13);
+        w[0] = r.n_cont;
+        w[1] = r.first;
+        w[2] = r.next[0];
+        w[3] = r.next[1];
+        w[4] = r.n[0];
+        w[5] = r.n[1];
         const uint32_t count = r.n_cont + r.n[0] + r.n[1];
         wf.counters[CTR(1 - q)] = count;
         wf.counters[CTR(5)] = r.next[0] + r.n[0];
@@ -1493,7 +1547,8 @@ ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha)
     return depth == 8 ? pick_shadow_d<8>(count, multi, alpha)
                       : (depth == 32 ? pick_shadow_d<32>(count, multi, alpha) : pick_shadow_d<16>(count, multi, alpha));
 }
-typedef void (*BothFn)(DeviceScene, const DeviceScene*, Wavefront, int, int);
+typedef void (*BothFn)(DeviceScene, const DeviceScene*, Wavefront, int, const ViewDev*, const JobDev*, const JobDev*,
+                       int);
 template <uint32_t D>
 BothFn pick_both_d(bool multi, bool alpha)
 {
@@ -1563,9 +1618,10 @@ unsigned shade_grid(bool multi)
 // One wavefront iteration with queue parity q: clear the counters, extend, shade, shadow,
 // cache queries, refill, and copy the counts the host steers by to pinned memory.
 // Pipelined (mode.pipelined): the shade stage's shadow rays are traced in the next iteration,
-// in one launch with its extension rays (k_trace_both; mode.consume when the queue holds the
-// previous iteration's rays). mode.flush: a non-pipelined iteration first traces the rays a
-// pipelined one left.
+// in one launch with its extension rays, and the new paths the refill plans are started by that
+// launch too (k_trace_both; mode.consume when the previous iteration left both). mode.flush: a
+// non-pipelined iteration first traces the shadow rays and writes the new paths a pipelined one
+// left.
 hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsigned shade_blocks, IterMode mode)
 {
     Wavefront& wf = *io.wf;
@@ -1577,14 +1633,18 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     const dim3 tg(io.grid_blocks), tb(TRACE_BLOCK);
     const bool keep_shadow = mode.pipelined || mode.flush;
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd, keep_shadow ? 0 : 1);
-    if (mode.flush) {
+    if (mode.flush) {  // what the pipelined iteration before left: its shadow rays, its new paths
         hipLaunchKernelGGL(pick_shadow(false, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf,
                            1 - q);
         hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
+        CacheDev none = cd;
+        none.collect = 0;
+        hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, io.d_job + 1, wf,
+                           1 - q, none);
     }
     if (mode.pipelined) {
         hipLaunchKernelGGL(pick_both(multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf, q,
-                           mode.consume ? 1 : 0);
+                           io.d_views, io.d_job, io.d_job + 1, mode.consume ? 1 : 0);
         hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
     } else {
         if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
@@ -1634,8 +1694,10 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     }
     if ((e = timer.mark(st, KERNEL_REFILL, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, io.d_job + 1, wf, q, cd);
     hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, st, io.d_job, io.d_job + 1, wf, q);
+    if (!mode.pipelined)
+        hipLaunchKernelGGL(k_refill, dim3(shade_blocks), dim3(256), 0, st, io.d_views, io.d_job, io.d_job + 1, wf, q,
+                           cd);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if ((e = timer.mark(st, KERNEL_REFILL, false)) != hipSuccess)
