@@ -785,12 +785,14 @@ __global__ void k_clear_shadow(Wavefront wf)
 // paths move, with their state, to the front of queue 1-q; an ended path's radiance goes to
 // rad_out; next-event rays go to the shadow queue with the place their contribution is added.
 // WPE: waves per SIMD the register allocation must allow (2: up to 256 registers per lane).
-template <bool MULTI, int WPE>
+template <bool MULTI, int WPE, bool CACHE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_shade(
     DeviceScene s0, const DeviceScene* scenes, const ViewDev* views,
                                                const JobDev* job, Wavefront wf, int q, CacheDev cd,
                                                const uint32_t* order)
 {
+    // CACHE = false: the plain tracer's variant, with the radiance cache's code compiled out
+    const bool collect = CACHE && cd.collect;
     const uint32_t count = wf.counters[CTR(q)];
     // Whole blocks take part in the compaction. A block takes chunks of 256 entries from its
     // XCD's counter, the traversal launches' scheme by block (chunk k of an XCD is its share of
@@ -987,7 +989,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     const float3 nrd = -rd;
                     const float3 o = normalize3(f3(dot3(wx, nrd), dot3(wy, nrd), dot3(wn, nrd)));
                     bool cached = false;
-                    if (o.z > 0.0f && cd.collect && segment == cd.segment) {
+                    if (o.z > 0.0f && collect && segment == cd.segment) {
                         // The cache's vertex (the second by default): a training path records the cache input here
                         // (its target is completed when the path ends); any other path ends
                         // here with the cache's prediction of the radiance leaving it.
@@ -1120,7 +1122,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const uint32_t slot = (ps.w >> 15) & 1u;  // job slot of the path
         // ended paths are counted per job slot: a job is complete when all its paths ended
         uint32_t apos[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (cd.collect) {  // uniform: training records, cache queries and ended records are appended too
+        if (collect) {  // uniform: training records, cache queries and ended records are appended too
             block_append<7>({cont, shadow, want_train, want_query, ended_rec, ended && slot == 0u, ended && slot == 1u},
                             {&wf.counters[CTR(1 - q)], &wf.counters[CTR(2)], &cd.counters[0], &cd.counters[1],
                              &wf.counters[CTR(4)], &wf.counters[CTR(7)], &wf.counters[CTR(8)]},
@@ -1160,7 +1162,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             if (MULTI)
                 wf.s_scene[spos] = sid;
         }
-        if (cd.collect) {
+        if (collect) {
             // training record: inputs, throughput and radiance so far (target filled at the end)
             if (want_train && tpos < cd.record_capacity) {
                 float* r = cd.records + size_t(tpos) * MLP_REC_FLOATS;
@@ -1576,37 +1578,41 @@ int shade_waves()
     return w;
 }
 
-template <bool MULTI>
+template <bool MULTI, bool CACHE>
 ShadeFn pick_shade_m()
 {
     switch (shade_waves()) {
     case 2:
-        return k_shade<MULTI, 2>;
+        return k_shade<MULTI, 2, CACHE>;
     case 3:
-        return k_shade<MULTI, 3>;
+        return k_shade<MULTI, 3, CACHE>;
     case 5:
-        return k_shade<MULTI, 5>;
+        return k_shade<MULTI, 5, CACHE>;
     case 6:
-        return k_shade<MULTI, 6>;
+        return k_shade<MULTI, 6, CACHE>;
     default:
-        return k_shade<MULTI, 4>;
+        return k_shade<MULTI, 4, CACHE>;
     }
 }
 
-ShadeFn pick_shade(bool multi) { return multi ? pick_shade_m<true>() : pick_shade_m<false>(); }
+ShadeFn pick_shade(bool multi, bool cache)
+{
+    return multi ? (cache ? pick_shade_m<true, true>() : pick_shade_m<true, false>())
+                 : (cache ? pick_shade_m<false, true>() : pick_shade_m<false, false>());
+}
 
 // Blocks of 256 threads of the shade stage resident on the whole chip at once: a
 // grid-stride launch of exactly that many finishes in one round, with no partial round.
-unsigned shade_grid(bool multi)
+unsigned shade_grid(bool multi, bool cache)
 {
-    static unsigned g[2] = {0, 0};
-    unsigned& v = g[multi ? 1 : 0];
+    static unsigned g[4] = {0, 0, 0, 0};
+    unsigned& v = g[(multi ? 1 : 0) + (cache ? 2 : 0)];
     if (v == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
             cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pick_shade(multi)), 256,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pick_shade(multi, cache)), 256,
                                                          0) != hipSuccess ||
             per_cu < 1)
             per_cu = 2;
@@ -1671,7 +1677,8 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     }
     if ((e = timer.mark(st, KERNEL_SHADE, true)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(pick_shade(multi), dim3(shade_blocks), dim3(256), 0, st, io.scene, tab, io.d_views, io.d_job,
+    hipLaunchKernelGGL(pick_shade(multi, cd.collect != 0), dim3(shade_blocks), dim3(256), 0, st, io.scene, tab,
+                       io.d_views, io.d_job,
                        wf, q, cd, order);
     if ((e = timer.mark(st, KERNEL_SHADE, false)) != hipSuccess)
         return e;
@@ -1858,7 +1865,7 @@ hipError_t render_views(const RenderIO& io, const std::vector<ViewWork>& work, b
     if (cs)
         drain = true;
     // grid-stride stages: the blocks that fill the chip in one round, not one per pool slot
-    const unsigned shade_blocks = blocks_for(wf.capacity, 256, shade_grid(io.d_scenes != nullptr));
+    const unsigned shade_blocks = blocks_for(wf.capacity, 256, shade_grid(io.d_scenes != nullptr, cs != nullptr));
     CacheDev cd{};
     if (cs) {
         cd.records = cs->records;
