@@ -1566,9 +1566,10 @@ BothFn pick_both(bool multi, uint32_t depth, bool alpha)
 using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
                         const uint32_t*);
 
-// Register budget of the shade stage: PT_SHADE_WAVES (2..6 waves per SIMD; tuning) or 2,
-// the fastest measured with paths in pixel order (256 registers, no spilling; 3 waves spill
-// 28 registers and ran 4.5 % slower, 4 and 5 waves slower still).
+// Register budget of the shade stage: PT_SHADE_WAVES (2..6 waves per SIMD; tuning) or 2, which
+// lets the compiler use what the code needs without spilling: the plain tracer's variant takes
+// 162 registers (3 waves per SIMD), the radiance cache's 185 (2 waves). Forcing 4 waves spills
+// the plain variant and ran its stage 8 % slower.
 int shade_waves()
 {
     static const int w = [] {
