@@ -81,6 +81,8 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 struct RayPre {
     float3 o, d, inv;
+    float3 oi;    // o x inv (rounded): slab distances as one fused multiply-add, fma(p, inv, -oi)
+    float slack;  // absolute error bound of a near/far pair of those distances
     int kx, ky, kz;
     float sx, sy, sz;
 };
@@ -101,6 +103,10 @@ __device__ __forceinline__ void precompute(RayPre& r, float3 o, float3 d)
     const float ey = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
     const float ez = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
     r.inv = f3(rcp1(ex), rcp1(ey), rcp1(ez));
+    r.oi = f3(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+    // fma(p, inv, -oi) = (p - o) inv within 3u relative (the factor below covers it) plus u x |oi|
+    // absolute (oi's rounding, u = 2^-24); a pair within 2u max|oi|, the slack is twice that
+    r.slack = fmaxf(fmaxf(fabsf(r.oi.x), fabsf(r.oi.y)), fabsf(r.oi.z)) * 0x1p-22f;
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = 0;
     if (ay > ax)
@@ -280,8 +286,9 @@ struct Walker {
 
     __device__ __forceinline__ bool step(const DeviceScene& s, Stk& stk, FetchCount* fc)
     {
-        // Conservative slab far distance: each slab distance fl(fl(p - o) * inv), inv within 1 ulp,
-        // is within 4u of exact (u = 2^-24), so a near/far pair within 8u; 1 + 2^-20 = 1 + 16u.
+        // Conservative slab far distance: each slab distance fma(p, inv, -oi), inv within 1 ulp, is
+        // within 3u of exact (u = 2^-24) plus u x |oi| (RayPre::slack), so a near/far pair within
+        // 6u relative plus the slack; 1 + 2^-20 = 1 + 16u.
         const float cons = 1.00000095367431640625f;
         // ---- visit an internal node
         if (COUNT) {
@@ -305,10 +312,11 @@ struct Walker {
         const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const float t0 = fmaxf(fmaxf((nxa[k] - r.o.x) * r.inv.x, (nya[k] - r.o.y) * r.inv.y),
-                                   fmaxf((nza[k] - r.o.z) * r.inv.z, tmin));
-            const float t1 = fminf(fminf((fxa[k] - r.o.x) * r.inv.x, (fya[k] - r.o.y) * r.inv.y),
-                                   fminf((fza[k] - r.o.z) * r.inv.z, best)) * cons;
+            const float t0 = fmaxf(fmaxf(fmaf(nxa[k], r.inv.x, -r.oi.x), fmaf(nya[k], r.inv.y, -r.oi.y)),
+                                   fmaxf(fmaf(nza[k], r.inv.z, -r.oi.z), tmin));
+            const float t1 = fmaf(fminf(fminf(fmaf(fxa[k], r.inv.x, -r.oi.x), fmaf(fya[k], r.inv.y, -r.oi.y)),
+                                        fminf(fmaf(fza[k], r.inv.z, -r.oi.z), best)),
+                                  cons, r.slack);
             const bool h = cc[k] != CHILD_EMPTY && t0 <= t1;
             tn[k] = h ? t0 : INFINITY;
             if (!h)
