@@ -604,9 +604,12 @@ __device__ __forceinline__ void issue_planned(const ViewDev* views, const JobDev
                                               const RefillPlan& r, const Wavefront& wf, int par, uint32_t pos,
                                               float4* ro = nullptr, float4* rd = nullptr)
 {
+    // (selects, not indexing by the slot: the plan stays in registers)
     const uint32_t k = pos - r.n_cont;
-    const uint32_t s = k < r.n[r.first] ? r.first : 1u - r.first;
-    const uint32_t p = r.next[s] + (s == r.first ? k : k - r.n[r.first]);
+    const uint32_t n_first = r.first ? r.n[1] : r.n[0];
+    const bool older = k < n_first;
+    const uint32_t s = older ? r.first : 1u - r.first;
+    const uint32_t p = (s ? r.next[1] : r.next[0]) + (older ? k : k - n_first);
     start_path(views, s ? j1 : j0, s, wf, par, pos, p, ro, rd);
 }
 
