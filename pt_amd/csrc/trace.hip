@@ -324,7 +324,15 @@ struct Walker {
         }
         // sort the (up to) four hits near to far (optimal 5-comparator network)
         auto cmpx = [&](int a, int b) {
-            if (tn[b] < tn[a]) {
+            if (ANY) {  // selects (measured: any-hit rays -2.5 %, closest-hit rays +1.3 %)
+                const bool sw = tn[b] < tn[a];
+                const float lo = fminf(tn[a], tn[b]), hi = fmaxf(tn[a], tn[b]);
+                const uint32_t ca = cc[a], cb = cc[b];
+                tn[a] = lo;
+                tn[b] = hi;
+                cc[a] = sw ? cb : ca;
+                cc[b] = sw ? ca : cb;
+            } else if (tn[b] < tn[a]) {
                 const float tt = tn[a];
                 tn[a] = tn[b];
                 tn[b] = tt;
