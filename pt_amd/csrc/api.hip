@@ -222,6 +222,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
         return (why = "BVH readback failed", PT_EDEVICE);
     std::vector<uint8_t> tri_seen(bv.tri_count, 0), prim_seen(bv.tri_count, 0);
     std::vector<uint32_t> stack;
+    uint32_t max_leaf = 0;
     uint32_t tri_base = 0;
     for (uint32_t m = 0; m < meshes.size(); m++) {
         const uint32_t nt = meshes[m].tri_count;
@@ -238,6 +239,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
                         continue;
                     if (ch & CHILD_LEAF) {
                         const uint32_t f = ch & LEAF_FIRST_MASK, cnt = ((ch >> 27) & 0xFu) + 1u;
+                        max_leaf = std::max(max_leaf, cnt);
                         if (f < tri_base || f + cnt > tri_base + nt)
                             return (why = "BLAS leaf outside its mesh", PT_EDEVICE);
                         for (uint32_t q = f; q < f + cnt; q++) {
@@ -264,6 +266,7 @@ int validate_bvh(Ctx& c, const std::vector<pt_mesh_desc>& meshes, std::string& w
     for (uint32_t q = 0; q < bv.tri_count; q++)
         if (!tri_seen[q])
             return (why = "triangle record not reachable", PT_EDEVICE);
+    c.single_tri_leaves = max_leaf <= 1;
     if (c.instance_count) {
         if (bv.tlas_root >= bv.node_count)
             return (why = "TLAS root out of range", PT_EDEVICE);
@@ -1352,6 +1355,7 @@ RenderIO batch_io(pt_batch& b)
                 b.d_job,    b.h_job, &b.stream_state,
                 b.use_graphs ? &b.graphs : nullptr, nullptr, b.lds_stack, b.sort_hits};
     io.alpha_shadows = std::any_of(b.scenes.begin(), b.scenes.end(), [](const pt_ctx* c) { return c->c.transparent; });
+    io.leaf1 = std::all_of(b.scenes.begin(), b.scenes.end(), [](const pt_ctx* c) { return c->c.single_tri_leaves; });
     return io;
 }
 

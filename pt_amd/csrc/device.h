@@ -277,6 +277,7 @@ struct Ctx {
     uint32_t lds_stack = LDS_STACK;
     uint32_t spill_depth = SPILL_STACK;  // PT_OPT_SPILL_DEPTH
     bool transparent = false;  // some instance's material is transparent
+    bool single_tri_leaves = false;  // every BLAS leaf of the built hierarchy holds one triangle
     uint32_t leaf_size = 4;
     uint32_t node_width = 4;
     uint32_t bvh_builder = 0;  // BVH_BUILDER_*
@@ -317,6 +318,7 @@ struct RenderIO {
     uint32_t lds_stack;             // LDS stack entries per lane of the traversal kernels
     bool sort_hits;                 // shade the extension hits in (scene, material) order
     bool alpha_shadows = true;      // some instance is transparent: shadow rays weigh alpha
+    bool leaf1 = false;             // every BLAS leaf holds one triangle (no leaf loop)
 };
 
 // Samples [sample_index, sample_index + spp) of one view.

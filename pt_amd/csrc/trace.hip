@@ -242,8 +242,10 @@ struct FetchCount {
 // the triangle leaves among its children at once) and picks the next one, entering and leaving
 // instances, until it returns true: the traversal is over, found() tells whether it hit and
 // result() gives the hit. The world-space ray and the closest hit's attributes live in the
-// lane's LDS rows (ColdRow); the any-hit record is only made when the ray ends.
-template <bool ANY, bool COUNT, typename Stk, bool ALPHA>
+// lane's LDS rows (ColdRow); the any-hit record is only made when the ray ends. LEAF1 compiles
+// the triangle loop of a leaf out for hierarchies whose BLAS leaves each hold one triangle (the
+// host validation finds the largest leaf).
+template <bool ANY, bool COUNT, typename Stk, bool ALPHA, bool LEAF1 = false>
 struct Walker {
     RayPre r;
     float tmin, best, opacity, trans;
@@ -355,7 +357,7 @@ struct Walker {
             if ((c & CHILD_LEAF) && inst != NONE) {
                 // triangle leaf of the current BLAS
                 const uint32_t first = c & LEAF_FIRST_MASK;
-                const uint32_t count = ((c >> 27) & 0xFu) + 1u;
+                const uint32_t count = LEAF1 ? 1u : ((c >> 27) & 0xFu) + 1u;
                 for (uint32_t q = 0; q < count; q++) {
                     if (COUNT)
                         fc->tris++;
@@ -452,11 +454,11 @@ struct Walker {
     }
 };
 
-template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true>
+template <bool ANY, bool COUNT = false, typename Stk, bool ALPHA = true, bool LEAF1 = false>
 __device__ bool traverse(const DeviceScene& s, float3 wo, float3 wd, float tmin, float tmax, Stk& stk, HitOut& hit,
                          float& trans, FetchCount* fc = nullptr)
 {
-    Walker<ANY, COUNT, Stk, ALPHA> w;
+    Walker<ANY, COUNT, Stk, ALPHA, LEAF1> w;
     if (w.start(s, wo, wd, tmin, tmax, stk))
         while (!w.step(s, stk, fc)) {
         }
@@ -659,7 +661,7 @@ __device__ __forceinline__ void for_each_chunk(uint32_t* ctr, uint32_t count, F&
 // Closest hits of the extension rays of queue q (count at CTR(q); chunk counters at CTR(10)).
 // With `issue` (the views and job slots given), the entries the stored refill plan appended are
 // started here: their camera rays are written for the shade stage and traced without being read.
-template <bool COUNT, bool MULTI, uint32_t D>
+template <bool COUNT, bool MULTI, uint32_t D, bool LEAF1>
 __device__ __forceinline__ void extend_queue(const DeviceScene& s0, const DeviceScene* scenes, const Wavefront& wf,
                                              int q, StackT<D>& stk, FetchCount& fc, const ViewDev* views = nullptr,
                                              const JobDev* j0 = nullptr, const JobDev* j1 = nullptr)
@@ -676,9 +678,10 @@ __device__ __forceinline__ void extend_queue(const DeviceScene& s0, const Device
         float trans;
         const uint32_t sid = MULTI ? wf.q_scene[q][i] : 0u, first = __builtin_amdgcn_readfirstlane(sid);
         if (MULTI && !same_scene(sid, first))
-            traverse<false, COUNT>(scenes[sid], xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
+            traverse<false, COUNT, StackT<D>, true, LEAF1>(scenes[sid], xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
         else
-            traverse<false, COUNT>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), o.w, d.w, stk, h, trans, &fc);
+            traverse<false, COUNT, StackT<D>, true, LEAF1>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), o.w, d.w, stk, h,
+                                                           trans, &fc);
         wf.hit[i] = make_float4(h.t, h.u, h.v, u32_as_float(h.inst));
         wf.hit_prim[i] = h.prim;
     });
@@ -687,7 +690,7 @@ __device__ __forceinline__ void extend_queue(const DeviceScene& s0, const Device
 // Any hits of the shadow rays the shade stage of parity qs queued (count at CTR(2); chunk
 // counters at CTR(11)): an unoccluded ray adds its contribution, times the transmittance, where
 // its path keeps its radiance (the next queue's entry, or the ended path's slot).
-template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
+template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA, bool LEAF1>
 __device__ __forceinline__ void shadow_queue(const DeviceScene& s0, const DeviceScene* scenes, const Wavefront& wf,
                                              int qs, StackT<D>& stk, FetchCount& fc)
 {
@@ -709,11 +712,11 @@ __device__ __forceinline__ void shadow_queue(const DeviceScene& s0, const Device
         const uint32_t sid = MULTI ? wf.s_scene[i] : 0u, first = __builtin_amdgcn_readfirstlane(sid);
         bool occluded;
         if (MULTI && !same_scene(sid, first))
-            occluded = traverse<true, COUNT, StackT<D>, ALPHA>(scenes[sid], xyz(o), xyz(d), 0.0f, o.w, stk, h, trans,
-                                                               &fc);
+            occluded = traverse<true, COUNT, StackT<D>, ALPHA, LEAF1>(scenes[sid], xyz(o), xyz(d), 0.0f, o.w, stk, h,
+                                                                      trans, &fc);
         else
-            occluded = traverse<true, COUNT, StackT<D>, ALPHA>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), 0.0f, o.w,
-                                                               stk, h, trans, &fc);
+            occluded = traverse<true, COUNT, StackT<D>, ALPHA, LEAF1>(MULTI ? scenes[first] : s0, xyz(o), xyz(d), 0.0f,
+                                                                      o.w, stk, h, trans, &fc);
         if (!occluded && trans >= 1e-5f)
             add(i, trans);
     });
@@ -722,14 +725,14 @@ __device__ __forceinline__ void shadow_queue(const DeviceScene& s0, const Device
 // Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
 // waves in flight hide more of the dependent node fetches (measured: +3% over 5 waves; 8 waves
 // spill too much).
-template <bool COUNT, bool MULTI, uint32_t D>
+template <bool COUNT, bool MULTI, uint32_t D, bool LEAF1>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
-    extend_queue<COUNT, MULTI, D>(s0, scenes, wf, q, stk, fc);
+    extend_queue<COUNT, MULTI, D, LEAF1>(s0, scenes, wf, q, stk, fc);
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         atomicAdd(&wf.totals[0], (unsigned long long)wf.counters[CTR(q)]);
 }
 
-template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA>
+template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA, bool LEAF1>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
@@ -750,7 +753,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
-    shadow_queue<COUNT, MULTI, D, ALPHA>(s0, scenes, wf, q, stk, fc);
+    shadow_queue<COUNT, MULTI, D, ALPHA, LEAF1>(s0, scenes, wf, q, stk, fc);
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
     if (COUNT && (fc.nodes | fc.tris)) {
@@ -767,7 +770,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 // in one launch: waves that run out of shadow chunks go straight on to extension chunks, so the
 // end of the shadow work leaves no part of the chip idle. The shadow rays add to the radiance
 // of queue q's entries, which the extension part does not touch.
-template <bool MULTI, uint32_t D, bool ALPHA>
+template <bool MULTI, uint32_t D, bool ALPHA, bool LEAF1>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_both(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q, const ViewDev* views, const JobDev* j0,
     const JobDev* j1, int carry)
@@ -777,10 +780,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     StackT<D> stk{&lds[threadIdx.x], wf.spill + gtid, wf.spill_threads, wf.spill_depth, 0, false};
     FetchCount fc;
     if (carry) {  // the previous iteration was pipelined: its shadow rays and its new paths
-        shadow_queue<false, MULTI, D, ALPHA>(s0, scenes, wf, 1 - q, stk, fc);
-        extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc, views, j0, j1);
+        shadow_queue<false, MULTI, D, ALPHA, LEAF1>(s0, scenes, wf, 1 - q, stk, fc);
+        extend_queue<false, MULTI, D, LEAF1>(s0, scenes, wf, q, stk, fc, views, j0, j1);
     } else {
-        extend_queue<false, MULTI, D>(s0, scenes, wf, q, stk, fc);
+        extend_queue<false, MULTI, D, LEAF1>(s0, scenes, wf, q, stk, fc);
     }
     if (stk.overflow)
         atomicOr(&wf.counters[CTR(3)], 1u);
@@ -1540,46 +1543,60 @@ namespace {
 typedef void (*ExtendFn)(DeviceScene, const DeviceScene*, Wavefront, int);
 typedef void (*ShadowFn)(DeviceScene, const DeviceScene*, Wavefront, int);
 
+// Traversal kernel variants by fetch counting, scene table, LDS stack depth (8, 16 or 32
+// entries per lane), alpha-weighted shadow rays and single-triangle leaves (the counting
+// kernels keep the general leaf loop).
 template <uint32_t D>
-ExtendFn pick_extend_d(bool count, bool multi)
+ExtendFn pick_extend_d(bool count, bool multi, bool leaf1)
 {
-    return count ? (multi ? k_extend<true, true, D> : k_extend<true, false, D>)
-                 : (multi ? k_extend<false, true, D> : k_extend<false, false, D>);
+    if (count)
+        return multi ? k_extend<true, true, D, false> : k_extend<true, false, D, false>;
+    return multi ? (leaf1 ? k_extend<false, true, D, true> : k_extend<false, true, D, false>)
+                 : (leaf1 ? k_extend<false, false, D, true> : k_extend<false, false, D, false>);
 }
 template <uint32_t D, bool ALPHA>
-ShadowFn pick_shadow_da(bool count, bool multi)
+ShadowFn pick_shadow_da(bool count, bool multi, bool leaf1)
 {
-    return count ? (multi ? k_shadow<true, true, D, ALPHA> : k_shadow<true, false, D, ALPHA>)
-                 : (multi ? k_shadow<false, true, D, ALPHA> : k_shadow<false, false, D, ALPHA>);
+    if (count)
+        return multi ? k_shadow<true, true, D, ALPHA, false> : k_shadow<true, false, D, ALPHA, false>;
+    return multi ? (leaf1 ? k_shadow<false, true, D, ALPHA, true> : k_shadow<false, true, D, ALPHA, false>)
+                 : (leaf1 ? k_shadow<false, false, D, ALPHA, true> : k_shadow<false, false, D, ALPHA, false>);
 }
 template <uint32_t D>
-ShadowFn pick_shadow_d(bool count, bool multi, bool alpha)
+ShadowFn pick_shadow_d(bool count, bool multi, bool alpha, bool leaf1)
 {
-    return alpha ? pick_shadow_da<D, true>(count, multi) : pick_shadow_da<D, false>(count, multi);
+    return alpha ? pick_shadow_da<D, true>(count, multi, leaf1) : pick_shadow_da<D, false>(count, multi, leaf1);
 }
-// Traversal kernel variants by LDS stack depth (8, 16 or 32 entries per lane).
-ExtendFn pick_extend(bool count, bool multi, uint32_t depth)
+ExtendFn pick_extend(bool count, bool multi, uint32_t depth, bool leaf1)
 {
-    return depth == 8 ? pick_extend_d<8>(count, multi)
-                      : (depth == 32 ? pick_extend_d<32>(count, multi) : pick_extend_d<16>(count, multi));
+    return depth == 8    ? pick_extend_d<8>(count, multi, leaf1)
+           : depth == 32 ? pick_extend_d<32>(count, multi, leaf1)
+                         : pick_extend_d<16>(count, multi, leaf1);
 }
-ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha)
+ShadowFn pick_shadow(bool count, bool multi, uint32_t depth, bool alpha, bool leaf1)
 {
-    return depth == 8 ? pick_shadow_d<8>(count, multi, alpha)
-                      : (depth == 32 ? pick_shadow_d<32>(count, multi, alpha) : pick_shadow_d<16>(count, multi, alpha));
+    return depth == 8    ? pick_shadow_d<8>(count, multi, alpha, leaf1)
+           : depth == 32 ? pick_shadow_d<32>(count, multi, alpha, leaf1)
+                         : pick_shadow_d<16>(count, multi, alpha, leaf1);
 }
 typedef void (*BothFn)(DeviceScene, const DeviceScene*, Wavefront, int, const ViewDev*, const JobDev*, const JobDev*,
                        int);
-template <uint32_t D>
-BothFn pick_both_d(bool multi, bool alpha)
+template <uint32_t D, bool ALPHA>
+BothFn pick_both_da(bool multi, bool leaf1)
 {
-    return multi ? (alpha ? k_trace_both<true, D, true> : k_trace_both<true, D, false>)
-                 : (alpha ? k_trace_both<false, D, true> : k_trace_both<false, D, false>);
+    return multi ? (leaf1 ? k_trace_both<true, D, ALPHA, true> : k_trace_both<true, D, ALPHA, false>)
+                 : (leaf1 ? k_trace_both<false, D, ALPHA, true> : k_trace_both<false, D, ALPHA, false>);
 }
-BothFn pick_both(bool multi, uint32_t depth, bool alpha)
+template <uint32_t D>
+BothFn pick_both_d(bool multi, bool alpha, bool leaf1)
 {
-    return depth == 8 ? pick_both_d<8>(multi, alpha)
-                      : (depth == 32 ? pick_both_d<32>(multi, alpha) : pick_both_d<16>(multi, alpha));
+    return alpha ? pick_both_da<D, true>(multi, leaf1) : pick_both_da<D, false>(multi, leaf1);
+}
+BothFn pick_both(bool multi, uint32_t depth, bool alpha, bool leaf1)
+{
+    return depth == 8    ? pick_both_d<8>(multi, alpha, leaf1)
+           : depth == 32 ? pick_both_d<32>(multi, alpha, leaf1)
+                         : pick_both_d<16>(multi, alpha, leaf1);
 }
 
 using ShadeFn = void (*)(DeviceScene, const DeviceScene*, const ViewDev*, const JobDev*, Wavefront, int, CacheDev,
@@ -1660,7 +1677,7 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     const bool keep_shadow = mode.pipelined || mode.flush;
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, st, wf, q, cd, keep_shadow ? 0 : 1);
     if (mode.flush) {  // what the pipelined iteration before left: its shadow rays, its new paths
-        hipLaunchKernelGGL(pick_shadow(false, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf,
+        hipLaunchKernelGGL(pick_shadow(false, multi, io.lds_stack, io.alpha_shadows, io.leaf1), tg, tb, 0, st, io.scene, tab, wf,
                            1 - q);
         hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
         CacheDev none = cd;
@@ -1669,13 +1686,13 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
                            1 - q, none);
     }
     if (mode.pipelined) {
-        hipLaunchKernelGGL(pick_both(multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st, io.scene, tab, wf, q,
+        hipLaunchKernelGGL(pick_both(multi, io.lds_stack, io.alpha_shadows, io.leaf1), tg, tb, 0, st, io.scene, tab, wf, q,
                            io.d_views, io.d_job, io.d_job + 1, mode.consume ? 1 : 0);
         hipLaunchKernelGGL(k_clear_shadow, dim3(1), dim3(64), 0, st, wf);
     } else {
         if ((e = timer.mark(st, KERNEL_EXTEND, true)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack), tg, tb, 0, st, io.scene, tab, wf, q);
+        hipLaunchKernelGGL(pick_extend(io.count_fetches, multi, io.lds_stack, io.leaf1), tg, tb, 0, st, io.scene, tab, wf, q);
         if ((e = timer.mark(st, KERNEL_EXTEND, false)) != hipSuccess)
             return e;
     }
@@ -1705,7 +1722,8 @@ hipError_t launch_iteration(const RenderIO& io, const CacheDev& cd, int q, unsig
     if (!mode.pipelined) {
         if ((e = timer.mark(st, KERNEL_SHADOW, true)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack, io.alpha_shadows), tg, tb, 0, st,
+        hipLaunchKernelGGL(pick_shadow(io.count_fetches, multi, io.lds_stack, io.alpha_shadows,
+                                       io.leaf1), tg, tb, 0, st,
                            io.scene, tab, wf, q);
         if ((e = timer.mark(st, KERNEL_SHADOW, false)) != hipSuccess)
             return e;
@@ -1741,6 +1759,7 @@ uint64_t graph_signature(const RenderIO& io, const CacheDev& cd, IterMode mode)
     mix(io.count_fetches);
     mix(io.lds_stack);
     mix(io.alpha_shadows);
+    mix(io.leaf1);
     mix(io.sort_hits);
     mix(reinterpret_cast<uintptr_t>(io.scene.nodes));
     mix(reinterpret_cast<uintptr_t>(io.d_scenes));
@@ -2044,6 +2063,7 @@ hipError_t render_samples(Ctx& ctx, uint32_t spp, bool drain)
                 ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
                 &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
     io.alpha_shadows = ctx.transparent;
+    io.leaf1 = ctx.single_tri_leaves;
     return render_views(io, spp ? std::vector<ViewWork>{ViewWork{0, v.npix, ctx.sample_index, spp}}
                                 : std::vector<ViewWork>{},
                         drain);
@@ -2058,6 +2078,7 @@ hipError_t drain_stream(Ctx& ctx)
                 ctx.d_job,        ctx.h_job,   &ctx.stream_state, ctx.use_graphs ? &ctx.graphs : nullptr,
                 &ctx.cache,       ctx.lds_stack,   ctx.sort_hits};
     io.alpha_shadows = ctx.transparent;
+    io.leaf1 = ctx.single_tri_leaves;
     return render_views(io, {}, true);
 }
 
