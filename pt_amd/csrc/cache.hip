@@ -97,43 +97,142 @@ __device__ __forceinline__ uint16_t* act_buf(uint16_t* base, int l)
     return base + (PING ? (l & 1) : l) * WS * ST;
 }
 
+// One layer of the forward pass over a 32-sample tile: Y = relu(X W^T + b) into LDS, or, for
+// the last layer (LAST), X W^T + b into y (lane holds rows 4g..4g+3 of tile mt for output r).
+// The wave's weight fragments (global, L2-resident) and activation fragments (LDS) are all
+// loaded before the first product, so the layer waits once for their latency.
+template <int K, int NO, bool LAST>
+__device__ __forceinline__ void forward_layer(const uint16_t* X, uint16_t* Y, const uint16_t* W, const float* bias,
+                                              int nv, f32x4_t y[2])
+{
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    constexpr int KC = K / 32, NT = NO / 16;
+    bf16x8_t bw[NT][KC], ax[2][KC];
+    float bo[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+            bw[nt][kc] = load8(W + (nt * 16 + r) * K + kc * 32 + 8 * g);
+        const int o = nt * 16 + r;
+        bo[nt] = o < nv ? bias[o] : 0.0f;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+            ax[mt][kc] = load8(X + (mt * 16 + r) * ST + kc * 32 + 8 * g);
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++)
+                acc = mfma(ax[mt][kc], bw[nt][kc], acc);
+            if (LAST) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    acc[i] += bo[nt];
+                y[mt] = acc;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    Y[(mt * 16 + 4 * g + i) * ST + nt * 16 + r] = f2bf(fmaxf(acc[i] + bo[nt], 0.0f));
+            }
+        }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Forward of one 32-sample tile whose encoded inputs are in act_buf(base, 0); the output of
 // the last layer is returned in y (lane holds rows 4g..4g+3 of tile mt for output r < 3).
 template <bool PING>
 __device__ __forceinline__ void forward_tile(uint16_t* base, const uint16_t* wb, const float* params, f32x4_t y[2])
 {
-    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    forward_layer<MLP_IN, MLP_H, false>(act_buf<PING>(base, 0), act_buf<PING>(base, 1), wb + LAYOUT.wb[0],
+                                        params + LAYOUT.b[0], MLP_H, y);
 #pragma unroll 1
-    for (int l = 0; l < MLP_LAYERS; l++) {
-        const int K = mlp_in(l), NO = mlp_outp(l);
-        const uint16_t* W = wb + LAYOUT.wb[l];
-        const uint16_t* X = act_buf<PING>(base, l);
-        uint16_t* Y = act_buf<PING>(base, l + 1);
-        for (int nt = 0; nt < NO / 16; nt++) {
-            const int o = nt * 16 + r;
-            const float bias = o < mlp_out(l) ? params[LAYOUT.b[l] + o] : 0.0f;
-            for (int mt = 0; mt < 2; mt++) {
-                f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int kc = 0; kc < K / 32; kc++) {
-                    const bf16x8_t a = load8(X + (mt * 16 + r) * ST + kc * 32 + 8 * g);
-                    const bf16x8_t b = load8(W + (nt * 16 + r) * K + kc * 32 + 8 * g);
-                    acc = mfma(a, b, acc);
-                }
-                if (l < MLP_LAYERS - 1) {
+    for (int l = 1; l < MLP_LAYERS - 1; l++)
+        forward_layer<MLP_H, MLP_H, false>(act_buf<PING>(base, l), act_buf<PING>(base, l + 1), wb + LAYOUT.wb[l],
+                                           params + LAYOUT.b[l], MLP_H, y);
+    constexpr int L = MLP_LAYERS - 1;
+    forward_layer<MLP_H, mlp_outp(L), true>(act_buf<PING>(base, L), nullptr, wb + LAYOUT.wb[L], params + LAYOUT.b[L],
+                                            MLP_OUT, y);
+}
+
+// One layer of the backward pass over a 32-sample tile, given G (the loss gradient at the
+// layer's output, bf16 rows in LDS) and X (its input): the weight and bias gradients are added
+// into gW / gb, and, with INPUT_GRAD, the gradient at the input, masked by the ReLU that made X,
+// is written to Gp. Fragments are loaded once per layer: the weight-gradient operands (columns
+// of G and X, gathered) once per row and column block, the W^T fragments before any product.
+template <int K, int NO, int NTT, bool INPUT_GRAD>
+__device__ __forceinline__ void backward_layer(const uint16_t* G, const uint16_t* X, uint16_t* Gp, const uint16_t* WT,
+                                               float* gW, float* gb, int nv)
+{
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    constexpr int OT = NO / 16, KT = K / 16;
+    {
+        // weight gradient tiles dW[o][k] = sum_s G[s][o] X[s][k]: rows o, columns k
+        bf16x8_t ga[OT], xb[KT];
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        Y[(mt * 16 + 4 * g + i) * ST + o] = f2bf(fmaxf(acc[i] + bias, 0.0f));
-                } else {
+        for (int ot = 0; ot < OT; ot++)
+            ga[ot] = gather8(G + (8 * g) * ST + ot * 16 + r, ST);
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        acc[i] += bias;
-                    y[mt] = acc;
+        for (int kt = 0; kt < KT; kt++)
+            xb[kt] = gather8(X + (8 * g) * ST + kt * 16 + r, ST);
+#pragma unroll
+        for (int ot = 0; ot < OT; ot++)
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++) {
+                const f32x4_t d = mfma(ga[ot], xb[kt], f32x4_t{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int o = ot * 16 + 4 * g + i;
+                    if (o < nv)
+                        atomicAdd(&gW[o * K + kt * 16 + r], d[i]);
                 }
             }
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
     }
+    // bias gradient: one output per lane
+    if (lane < nv) {
+        float sb = 0.0f;
+        for (int sr = 0; sr < WS; sr++)
+            sb += bf2f(G[sr * ST + lane]);
+        atomicAdd(&gb[lane], sb);
+    }
+    if (!INPUT_GRAD)
+        return;
+    // input gradient G'[s][k] = sum_o G[s][o] W[o][k] through W^T, masked by the ReLU of X
+    constexpr int OC = NTT / 32;
+    bf16x8_t bw[KT][OC], ag[2][OC];
+#pragma unroll
+    for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+        for (int oc = 0; oc < OC; oc++)
+            bw[kt][oc] = load8(WT + (kt * 16 + r) * NTT + oc * 32 + 8 * g);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int oc = 0; oc < OC; oc++)
+            ag[mt][oc] = load8(G + (mt * 16 + r) * ST + oc * 32 + 8 * g);
+#pragma unroll
+    for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int oc = 0; oc < OC; oc++)
+                acc = mfma(ag[mt][oc], bw[kt][oc], acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int srow = mt * 16 + 4 * g + i, k = kt * 16 + r;
+                const bool on = X[srow * ST + k] != 0;  // post-ReLU activation > 0
+                Gp[srow * ST + k] = on ? f2bf(acc[i]) : uint16_t(0);
+            }
+        }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
 }
 
 struct CacheArgs {
@@ -232,55 +331,19 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uin
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
 
-#pragma unroll 1
-        for (int l = MLP_LAYERS - 1; l >= 0; l--) {
-            const int K = mlp_in(l), NO = mlp_outp(l), NT = mlp_outt(l), NV = mlp_out(l);
-            uint16_t* Gl = gbase + (l & 1) * WS * ST;
-            const uint16_t* Xl = act_buf<false>(lds, l);
-            float* gW = a.grads + LAYOUT.w[l];
-            // weight gradient tiles: rows o (outputs), columns k (inputs), reduction over samples
-            for (int ot = 0; ot < NO / 16; ot++)
-                for (int kt = 0; kt < K / 16; kt++) {
-                    const bf16x8_t A = gather8(Gl + (8 * g) * ST + ot * 16 + r, ST);
-                    const bf16x8_t B = gather8(Xl + (8 * g) * ST + kt * 16 + r, ST);
-                    const f32x4_t d = mfma(A, B, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f});
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int o = ot * 16 + 4 * g + i;
-                        if (o < NV)
-                            atomicAdd(&gW[o * K + kt * 16 + r], d[i]);
-                    }
-                }
-            // bias gradient: one output per lane
-            if (lane < NV) {
-                float sb = 0.0f;
-                for (int sr = 0; sr < WS; sr++)
-                    sb += bf2f(Gl[sr * ST + lane]);
-                atomicAdd(&a.grads[LAYOUT.b[l] + lane], sb);
-            }
-            if (l == 0)
-                break;
-            // input gradient through W, masked by the ReLU of layer l-1
-            uint16_t* Gp = gbase + ((l - 1) & 1) * WS * ST;
-            const uint16_t* WT = a.wt + LAYOUT.wt[l];
-            for (int kt = 0; kt < K / 16; kt++)
-                for (int mt = 0; mt < 2; mt++) {
-                    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    for (int oc = 0; oc < NT / 32; oc++) {
-                        const bf16x8_t A = load8(Gl + (mt * 16 + r) * ST + oc * 32 + 8 * g);
-                        const bf16x8_t B = load8(WT + (kt * 16 + r) * NT + oc * 32 + 8 * g);
-                        acc = mfma(A, B, acc);
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int srow = mt * 16 + 4 * g + i, k = kt * 16 + r;
-                        const bool on = Xl[srow * ST + k] != 0;  // post-ReLU activation > 0
-                        Gp[srow * ST + k] = on ? f2bf(acc[i]) : uint16_t(0);
-                    }
-                }
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
+        {
+            constexpr int L = MLP_LAYERS - 1;
+            backward_layer<MLP_H, mlp_outp(L), mlp_outt(L), true>(
+                gbase + (L & 1) * WS * ST, act_buf<false>(lds, L), gbase + ((L - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[L],
+                a.grads + LAYOUT.w[L], a.grads + LAYOUT.b[L], MLP_OUT);
         }
+#pragma unroll 1
+        for (int l = MLP_LAYERS - 2; l >= 1; l--)
+            backward_layer<MLP_H, MLP_H, MLP_H, true>(gbase + (l & 1) * WS * ST, act_buf<false>(lds, l),
+                                                      gbase + ((l - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[l],
+                                                      a.grads + LAYOUT.w[l], a.grads + LAYOUT.b[l], MLP_H);
+        backward_layer<MLP_IN, MLP_H, MLP_H, false>(gbase, act_buf<false>(lds, 0), nullptr, nullptr,
+                                                    a.grads + LAYOUT.w[0], a.grads + LAYOUT.b[0], MLP_H);
     }
     // wave-reduce the loss
     for (int off = 32; off >= 1; off >>= 1)
@@ -354,10 +417,30 @@ __global__ void k_encode(CacheArgs a, const float* rec13, uint32_t n, float* out
     }
 }
 
+// Position of fp32 parameter i in the layout: a weight W[o][k] of layer l (true), or a bias.
+__device__ __forceinline__ bool weight_of(int i, int& l, int& o, int& k)
+{
+#pragma unroll
+    for (int j = 0; j < MLP_LAYERS; j++) {
+        if (i < LAYOUT.b[j]) {
+            const int e = i - LAYOUT.w[j];
+            l = j;
+            o = e / mlp_in(j);
+            k = e - o * mlp_in(j);
+            return true;
+        }
+        if (i < LAYOUT.b[j] + mlp_out(j))
+            return false;
+    }
+    return false;
+}
+
 // Adam step; with ema != nullptr also the moving average of the weights, e <- d e + (1 - d) w
-// from e = 0, and its bias-corrected value e / (1 - d^t) into ema_hat (dt = d^t).
+// from e = 0, and its bias-corrected value e / (1 - d^t) into ema_hat (dt = d^t). The bf16
+// copies of each updated weight (wb, wt; ema_wb of the average) are written in the same pass;
+// their zero padding never changes.
 __global__ void k_adam(float* params, float* grads, float* m, float* v, int count, float lr, float b1t, float b2t,
-                       float* ema, float* ema_hat, float d, float dt)
+                       float* ema, float* ema_hat, float d, float dt, uint16_t* wb, uint16_t* wt, uint16_t* ema_wb)
 {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -370,10 +453,19 @@ __global__ void k_adam(float* params, float* grads, float* m, float* v, int coun
         const float w = params[i] - lr * mh / (sqrtf(vh) + eps);
         params[i] = w;
         grads[i] = 0.0f;
+        float eh = 0.0f;
         if (ema) {
             const float e = d * ema[i] + (1.0f - d) * w;
             ema[i] = e;
-            ema_hat[i] = e / (1.0f - dt);
+            eh = e / (1.0f - dt);
+            ema_hat[i] = eh;
+        }
+        int l, o, k;
+        if (weight_of(i, l, o, k)) {
+            wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(w);
+            wt[LAYOUT.wt[l] + k * mlp_outt(l) + o] = f2bf(w);
+            if (ema)
+                ema_wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(eh);
         }
     }
 }
@@ -443,9 +535,10 @@ void cache_free(CacheState& c)
 
 hipError_t cache_sync_weights(CacheState& c, hipStream_t st)
 {
+    // (the average's copy too, whether or not queries read it: Adam steps then keep its weights
+    // current and its padding stays zero)
     hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.params, c.wb, c.wt);
-    if (c.ema_decay > 0.0f)
-        hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.ema_hat, c.ema_wb, nullptr);
+    hipLaunchKernelGGL(k_sync_bf16, dim3(64), dim3(256), 0, st, c.ema_hat, c.ema_wb, nullptr);
     return hipGetLastError();
 }
 
@@ -602,11 +695,8 @@ hipError_t cache_adam(CacheState& c, hipStream_t st)
         c.ema_steps++;
     const float dt = ema ? float(std::pow(double(c.ema_decay), double(c.ema_steps))) : 0.0f;
     hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, st, c.params, c.grads, c.adam_m, c.adam_v, LAYOUT.count, c.lr,
-                       b1t, b2t, ema ? c.ema : nullptr, c.ema_hat, c.ema_decay, dt);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return e;
-    return cache_sync_weights(c, st);
+                       b1t, b2t, ema ? c.ema : nullptr, c.ema_hat, c.ema_decay, dt, c.wb, c.wt, c.ema_wb);
+    return hipGetLastError();
 }
 
 // Steps on the records written so far in the current job (count read on the device; only
