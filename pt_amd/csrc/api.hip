@@ -1195,7 +1195,7 @@ int pt_cache_get_stats(pt_ctx* h, pt_cache_stats* out)
     s.param_count = uint32_t(mlp_param_count());
     if (cs.configured) {
         HIP_TRY(&c, hipStreamSynchronize(c.stream), "sync");
-        HIP_TRY(&c, hipMemcpy(&s.last_loss, cs.loss_acc, 4, hipMemcpyDeviceToHost), "cache loss");
+        HIP_TRY(&c, hipMemcpy(&s.last_loss, cs.loss_acc + 1, 4, hipMemcpyDeviceToHost), "cache loss");
         if (cs.frame_spp == 0) {  // in-flight training: records of the current job
             uint32_t n = 0;
             HIP_TRY(&c, hipMemcpy(&n, cs.counters, 4, hipMemcpyDeviceToHost), "cache records");

@@ -85,7 +85,8 @@ struct CacheState {
     uint32_t* query_slot = nullptr;  // radiance target of each query
     uint32_t query_capacity = 0;
     uint32_t* counters = nullptr;  // [0] training records, [1] queries
-    float* loss_acc = nullptr;
+    float* loss_acc = nullptr;   // [0] loss of the batch being trained, [1] of the last Adam step
+    float* partials = nullptr;   // per-block partial gradients of a training launch
     float* ema = nullptr;        // moving average of the weights (from zero)
     float* ema_hat = nullptr;    // its bias-corrected value, ema / (1 - decay^ema_steps)
     uint16_t* ema_wb = nullptr;  // bf16 W[out][in] of ema_hat

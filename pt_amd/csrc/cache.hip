@@ -26,8 +26,13 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int WS = 32;       // samples per wave
+// Rows of per-block partial gradients: a training launch has at most this many blocks, each
+// sums its tiles' weight and bias gradients into its own row (no atomics), and the Adam step
+// adds the rows in a fixed order (the gradient of a batch is the same on every run).
+constexpr int PARTIAL_ROWS = 512;
 constexpr int ST = 104;      // LDS row stride (bf16 elements) of every activation buffer
 constexpr MlpLayout LAYOUT{};
+constexpr int PSTRIDE = (LAYOUT.count + 63) / 64 * 64;  // floats per partial row
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -163,13 +168,21 @@ __device__ __forceinline__ void forward_tile(uint16_t* base, const uint16_t* wb,
 
 // One layer of the backward pass over a 32-sample tile, given G (the loss gradient at the
 // layer's output, bf16 rows in LDS) and X (its input): the weight and bias gradients are added
-// into gW / gb, and, with INPUT_GRAD, the gradient at the input, masked by the ReLU that made X,
+// into gW / gb (this block's partial row: stored by its first tile, first = true, added by
+// later ones; each element has one writer lane), and, with INPUT_GRAD, the gradient at the
+// input, masked by the ReLU that made X,
 // is written to Gp. Fragments are loaded once per layer: the weight-gradient operands (columns
 // of G and X, gathered) once per row and column block, the W^T fragments before any product.
 template <int K, int NO, int NTT, bool INPUT_GRAD>
 __device__ __forceinline__ void backward_layer(const uint16_t* G, const uint16_t* X, uint16_t* Gp, const uint16_t* WT,
-                                               float* gW, float* gb, int nv)
+                                               float* gW, float* gb, int nv, bool first)
 {
+    auto put = [first](float* p, float v) {
+        if (first)
+            *p = v;
+        else
+            *p += v;
+    };
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
     constexpr int OT = NO / 16, KT = K / 16;
     {
@@ -190,7 +203,7 @@ __device__ __forceinline__ void backward_layer(const uint16_t* G, const uint16_t
                 for (int i = 0; i < 4; i++) {
                     const int o = ot * 16 + 4 * g + i;
                     if (o < nv)
-                        atomicAdd(&gW[o * K + kt * 16 + r], d[i]);
+                        put(&gW[o * K + kt * 16 + r], d[i]);
                 }
             }
     }
@@ -199,7 +212,7 @@ __device__ __forceinline__ void backward_layer(const uint16_t* G, const uint16_t
         float sb = 0.0f;
         for (int sr = 0; sr < WS; sr++)
             sb += bf2f(G[sr * ST + lane]);
-        atomicAdd(&gb[lane], sb);
+        put(&gb[lane], sb);
     }
     if (!INPUT_GRAD)
         return;
@@ -239,7 +252,7 @@ struct CacheArgs {
     const float* params;
     const uint16_t* wb;
     const uint16_t* wt;
-    float* grads;
+    float* partials;  // PARTIAL_ROWS rows of PSTRIDE floats
     float* loss_acc;
     float lo[3], inv_ext[3];
 };
@@ -258,20 +271,26 @@ struct TrainSrc {
     const float* targets;
 };
 
-// Training: forward + backward over n samples; gradients (summed over the batch, already
-// scaled by the loss normalisation) are added into grads.
+// Training: forward + backward over n samples; the gradients (summed over the batch, already
+// scaled by the loss normalisation) of the tiles block b takes (b, b + grid, ...) go to row b of
+// the partial gradients, which k_apply_gradients sums. At most PARTIAL_ROWS blocks.
 __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uint32_t n)
 {
+    float* const prow = a.partials + size_t(blockIdx.x) * PSTRIDE;
     __shared__ __attribute__((aligned(16))) uint16_t lds[(MLP_LAYERS + 2) * WS * ST];
     uint16_t* const gbase = lds + MLP_LAYERS * WS * ST;  // two gradient buffers
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
     const float inv_n3 = 1.0f / (3.0f * float(n));
     float loss = 0.0f;
     const uint32_t n_rec = src.n_rec_dev ? min(*src.n_rec_dev, src.cap) : src.n_rec;
-    if (!src.features && n_rec == 0)
+    if (!src.features && n_rec == 0) {  // nothing recorded yet: a zero gradient
+        for (int i = lane; i < PSTRIDE; i += 64)
+            prow[i] = 0.0f;
         return;
+    }
 
     for (uint32_t base = blockIdx.x * WS; base < n; base += gridDim.x * WS) {
+        const bool first = base == blockIdx.x * WS;
         float tgt[3] = {0, 0, 0};
         float use = 0.0f;  // 1 for a sample that counts
         const uint32_t s = base + uint32_t(lane);
@@ -335,15 +354,15 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uin
             constexpr int L = MLP_LAYERS - 1;
             backward_layer<MLP_H, mlp_outp(L), mlp_outt(L), true>(
                 gbase + (L & 1) * WS * ST, act_buf<false>(lds, L), gbase + ((L - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[L],
-                a.grads + LAYOUT.w[L], a.grads + LAYOUT.b[L], MLP_OUT);
+                prow + LAYOUT.w[L], prow + LAYOUT.b[L], MLP_OUT, first);
         }
 #pragma unroll 1
         for (int l = MLP_LAYERS - 2; l >= 1; l--)
             backward_layer<MLP_H, MLP_H, MLP_H, true>(gbase + (l & 1) * WS * ST, act_buf<false>(lds, l),
                                                       gbase + ((l - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[l],
-                                                      a.grads + LAYOUT.w[l], a.grads + LAYOUT.b[l], MLP_H);
+                                                      prow + LAYOUT.w[l], prow + LAYOUT.b[l], MLP_H, first);
         backward_layer<MLP_IN, MLP_H, MLP_H, false>(gbase, act_buf<false>(lds, 0), nullptr, nullptr,
-                                                    a.grads + LAYOUT.w[0], a.grads + LAYOUT.b[0], MLP_H);
+                                                    prow + LAYOUT.w[0], prow + LAYOUT.b[0], MLP_H, first);
     }
     // wave-reduce the loss
     for (int off = 32; off >= 1; off >>= 1)
@@ -435,38 +454,81 @@ __device__ __forceinline__ bool weight_of(int i, int& l, int& o, int& k)
     return false;
 }
 
-// Adam step; with ema != nullptr also the moving average of the weights, e <- d e + (1 - d) w
-// from e = 0, and its bias-corrected value e / (1 - d^t) into ema_hat (dt = d^t). The bf16
-// copies of each updated weight (wb, wt; ema_wb of the average) are written in the same pass;
-// their zero padding never changes.
-__global__ void k_adam(float* params, float* grads, float* m, float* v, int count, float lr, float b1t, float b2t,
-                       float* ema, float* ema_hat, float d, float dt, uint16_t* wb, uint16_t* wt, uint16_t* ema_wb)
+// Sum of the partial-gradient rows [0, rows) of parameter i, added to grads[i]: a block takes 64
+// parameters, its four waves every fourth row each (16 loads in flight per lane), and the
+// waves' sums are added in wave order (the same result on every run).
+__device__ __forceinline__ float gradient_sum(const float* grads, const float* partials, uint32_t rows, int i,
+                                              int count)
 {
+    __shared__ float part[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float sum = 0.0f;
+    if (i < count) {
+        uint32_t b = uint32_t(w);
+        for (; b + 60 < rows; b += 64) {
+            float v[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                v[j] = partials[size_t(b + 4 * j) * PSTRIDE + i];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                sum += v[j];
+        }
+        for (; b < rows; b += 4)
+            sum += partials[size_t(b) * PSTRIDE + i];
+    }
+    part[w][lane] = sum;
+    __syncthreads();
+    return i < count ? grads[i] + part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane] : 0.0f;
+}
+
+// One Adam step on the gradient grads + the sum of the partial rows [0, rows) (blocks of 256
+// threads, 64 parameters each; wave 0 updates). With ema != nullptr also the moving average of
+// the weights, e <- d e + (1 - d) w from e = 0, and its bias-corrected value e / (1 - d^t)
+// into ema_hat (dt = d^t). The bf16 copies of each updated weight (wb, wt; ema_wb of the
+// average) are written in the same pass; their zero padding never changes. grads is cleared;
+// the batch loss moves from loss_acc[0] to loss_acc[1] and the accumulator is cleared for the
+// next batch. With adam = false the sum only goes to grads (the gradient of a batch).
+__global__ __launch_bounds__(256) void k_apply_gradients(float* params, float* grads, const float* partials,
+                                                         uint32_t rows, float* m, float* v, int count, int adam,
+                                                         float lr, float b1t, float b2t, float* ema, float* ema_hat,
+                                                         float d, float dt, uint16_t* wb, uint16_t* wt,
+                                                         uint16_t* ema_wb, float* loss_acc)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const float gr = gradient_sum(grads, partials, rows, i, count);
+    if (threadIdx.x >= 64 || i >= count)
+        return;
+    if (!adam) {
+        grads[i] = gr;
+        return;
+    }
+    if (i == 0) {
+        loss_acc[1] = loss_acc[0];
+        loss_acc[0] = 0.0f;
+    }
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const float gr = grads[i];
-        const float mi = b1 * m[i] + (1.0f - b1) * gr;
-        const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
-        m[i] = mi;
-        v[i] = vi;
-        const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
-        const float w = params[i] - lr * mh / (sqrtf(vh) + eps);
-        params[i] = w;
-        grads[i] = 0.0f;
-        float eh = 0.0f;
-        if (ema) {
-            const float e = d * ema[i] + (1.0f - d) * w;
-            ema[i] = e;
-            eh = e / (1.0f - dt);
-            ema_hat[i] = eh;
-        }
-        int l, o, k;
-        if (weight_of(i, l, o, k)) {
-            wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(w);
-            wt[LAYOUT.wt[l] + k * mlp_outt(l) + o] = f2bf(w);
-            if (ema)
-                ema_wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(eh);
-        }
+    const float mi = b1 * m[i] + (1.0f - b1) * gr;
+    const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
+    const float w = params[i] - lr * mh / (sqrtf(vh) + eps);
+    params[i] = w;
+    grads[i] = 0.0f;
+    float eh = 0.0f;
+    if (ema) {
+        const float e = d * ema[i] + (1.0f - d) * w;
+        ema[i] = e;
+        eh = e / (1.0f - dt);
+        ema_hat[i] = eh;
+    }
+    int l, o, k;
+    if (weight_of(i, l, o, k)) {
+        wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(w);
+        wt[LAYOUT.wt[l] + k * mlp_outt(l) + o] = f2bf(w);
+        if (ema)
+            ema_wb[LAYOUT.wb[l] + o * mlp_in(l) + k] = f2bf(eh);
     }
 }
 
@@ -493,7 +555,7 @@ CacheArgs args_of(const CacheState& c)
     a.params = c.params;
     a.wb = c.wb;
     a.wt = c.wt;
-    a.grads = c.grads;
+    a.partials = c.partials;
     a.loss_acc = c.loss_acc;
     for (int k = 0; k < 3; k++) {
         a.lo[k] = c.bounds_lo[k];
@@ -522,7 +584,7 @@ int mlp_param_count() { return LAYOUT.count; }
 void cache_free(CacheState& c)
 {
     void* ptrs[] = {c.params, c.grads, c.adam_m, c.adam_v, c.wb, c.wt, c.records, c.queries,
-                    c.query_slot, c.counters, c.loss_acc, c.ema, c.ema_hat, c.ema_wb};
+                    c.query_slot, c.counters, c.loss_acc, c.ema, c.ema_hat, c.ema_wb, c.partials};
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -568,7 +630,8 @@ hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capaci
         A(c.wb, size_t(LAYOUT.wb_count) * 2);
         A(c.wt, size_t(LAYOUT.wt_count) * 2);
         A(c.counters, 8 * 4);
-        A(c.loss_acc, 4);
+        A(c.loss_acc, 2 * 4);  // [0] the batch being trained, [1] the last one stepped
+        A(c.partials, size_t(PARTIAL_ROWS) * PSTRIDE * 4);
         A(c.ema, size_t(LAYOUT.count) * 4);
         A(c.ema_hat, size_t(LAYOUT.count) * 4);
         A(c.ema_wb, size_t(LAYOUT.wb_count) * 2);
@@ -615,6 +678,8 @@ hipError_t cache_init(CacheState& c, uint32_t pool_slots, uint32_t record_capaci
     if ((e = hipMemsetAsync(c.adam_v, 0, p.size() * 4, st)) != hipSuccess)
         return e;
     if ((e = hipMemsetAsync(c.counters, 0, 8 * 4, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemsetAsync(c.loss_acc, 0, 2 * 4, st)) != hipSuccess)
         return e;
     if ((e = cache_reset_ema(c, st)) != hipSuccess)
         return e;
@@ -673,47 +738,63 @@ hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad,
     return hipGetLastError();
 }
 
-// Gradients of the batch into c.grads (zeroed first) and the loss into *loss (device).
+// One training launch over n samples into the partial-gradient rows; returns the rows used.
+static uint32_t launch_train(CacheState& c, const TrainSrc& src, uint32_t n, hipStream_t st)
+{
+    const uint32_t rows = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((uint64_t(n) + WS - 1) / WS, PARTIAL_ROWS)));
+    hipLaunchKernelGGL(k_mlp_train, dim3(rows), dim3(64), 0, st, args_of(c), src, n);
+    return rows;
+}
+
+// Sums the partial rows into c.grads (adam = false), or takes the Adam step on that sum.
+static hipError_t apply_gradients(CacheState& c, uint32_t rows, bool adam, hipStream_t st)
+{
+    float b1t = 0.0f, b2t = 0.0f, dt = 0.0f;
+    const bool ema = adam && c.ema_decay > 0.0f;
+    if (adam) {
+        c.steps_done++;
+        b1t = std::pow(0.9f, float(c.steps_done));
+        b2t = std::pow(0.999f, float(c.steps_done));
+        if (ema)
+            c.ema_steps++;
+        dt = ema ? float(std::pow(double(c.ema_decay), double(c.ema_steps))) : 0.0f;
+    }
+    const unsigned blocks = unsigned((LAYOUT.count + 63) / 64);
+    hipLaunchKernelGGL(k_apply_gradients, dim3(blocks), dim3(256), 0, st, c.params, c.grads, c.partials, rows,
+                       c.adam_m, c.adam_v, LAYOUT.count, adam ? 1 : 0, c.lr, b1t, b2t, ema ? c.ema : nullptr,
+                       c.ema_hat, c.ema_decay, dt, c.wb, c.wt, c.ema_wb, c.loss_acc);
+    return hipGetLastError();
+}
+
+// Gradients of the batch added into c.grads and the loss into loss_acc[0] (device).
 hipError_t cache_backward(CacheState& c, const float* d_features, const float* d_targets, uint32_t n,
                           uint32_t n_rec, uint32_t start, uint32_t stride, hipStream_t st)
 {
     hipError_t e = hipMemsetAsync(c.loss_acc, 0, 4, st);
     if (e != hipSuccess)
         return e;
-    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(n) + WS - 1) / WS, 2048));
     TrainSrc src{c.records, n_rec, nullptr, c.record_capacity, start, stride, d_features, d_targets};
-    hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), src, n);
-    return hipGetLastError();
+    const uint32_t rows = launch_train(c, src, n, st);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    return apply_gradients(c, rows, false, st);
 }
 
-hipError_t cache_adam(CacheState& c, hipStream_t st)
-{
-    c.steps_done++;
-    const float b1t = std::pow(0.9f, float(c.steps_done)), b2t = std::pow(0.999f, float(c.steps_done));
-    const bool ema = c.ema_decay > 0.0f;
-    if (ema)
-        c.ema_steps++;
-    const float dt = ema ? float(std::pow(double(c.ema_decay), double(c.ema_steps))) : 0.0f;
-    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, st, c.params, c.grads, c.adam_m, c.adam_v, LAYOUT.count, c.lr,
-                       b1t, b2t, ema ? c.ema : nullptr, c.ema_hat, c.ema_decay, dt, c.wb, c.wt, c.ema_wb);
-    return hipGetLastError();
-}
+// Adam step on the gradient in c.grads.
+hipError_t cache_adam(CacheState& c, hipStream_t st) { return apply_gradients(c, 0, true, st); }
 
 // Steps on the records written so far in the current job (count read on the device; only
 // completed records count): lets the network learn while a job's paths are still in flight.
 hipError_t cache_train_inflight(CacheState& c, hipStream_t st)
 {
     hipError_t e;
-    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(c.batch) + WS - 1) / WS, 2048));
     for (uint32_t k = 0; k < c.train_steps; k++) {
-        if ((e = hipMemsetAsync(c.loss_acc, 0, 4, st)) != hipSuccess)
-            return e;
         TrainSrc src{c.records, 0, c.counters, c.record_capacity, (c.steps_done + 1) * 0x6C8E9CF5u, 0, nullptr,
                      nullptr};
-        hipLaunchKernelGGL(k_mlp_train, dim3(std::max(1u, blocks)), dim3(64), 0, st, args_of(c), src, c.batch);
+        const uint32_t rows = launch_train(c, src, c.batch, st);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        if ((e = cache_adam(c, st)) != hipSuccess)
+        if ((e = apply_gradients(c, rows, true, st)) != hipSuccess)
             return e;
     }
     return hipSuccess;
@@ -742,9 +823,11 @@ hipError_t cache_train_records(CacheState& c, uint32_t n_records, hipStream_t st
     hipError_t e;
     for (uint32_t k = 0; k < c.train_steps; k++) {
         const uint32_t start = uint32_t((uint64_t(c.steps_done + k) * 2654435761u) % n_records);
-        if ((e = cache_backward(c, nullptr, nullptr, batch, n_records, start, stride, st)) != hipSuccess)
+        TrainSrc src{c.records, n_records, nullptr, c.record_capacity, start, stride, nullptr, nullptr};
+        const uint32_t rows = launch_train(c, src, batch, st);
+        if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        if ((e = cache_adam(c, st)) != hipSuccess)
+        if ((e = apply_gradients(c, rows, true, st)) != hipSuccess)
             return e;
     }
     return hipSuccess;
