@@ -94,6 +94,34 @@ __device__ __forceinline__ void encode_row(const float* rec, const float* lo, co
         row[f++] = 0;
 }
 
+// Part w (0..3) of the encoded features of one record, as encode_row computes them: the bands
+// 3w..3w+2 (features 18w..18w+17), and for w = 0 also features 72..95.
+__device__ __forceinline__ void encode_part(const float* rec, const float* lo, const float* inv_ext, uint16_t* row,
+                                            int w)
+{
+    float p[3];
+    for (int c = 0; c < 3; c++)
+        p[c] = fminf(fmaxf((rec[c] - lo[c]) * inv_ext[c], 0.0f), 1.0f);
+    for (int b = 3 * w; b < 3 * w + 3; b++) {
+        const float s = float(1 << b) * 3.14159265f;
+        for (int c = 0; c < 3; c++) {
+            float sn, cs;
+            sincosf(s * p[c], &sn, &cs);
+            row[6 * b + 2 * c] = f2bf(sn);
+            row[6 * b + 2 * c + 1] = f2bf(cs);
+        }
+    }
+    if (w == 0) {
+        int f = 72;
+        for (int c = 0; c < 3; c++)
+            row[f++] = f2bf(p[c]);
+        for (int c = 3; c < 13; c++)
+            row[f++] = f2bf(rec[c]);
+        while (f < MLP_IN)
+            row[f++] = 0;
+    }
+}
+
 // LDS buffer holding the input of layer l: one buffer per layer (training keeps them all for
 // the backward pass) or two alternating buffers (inference).
 template <bool PING>
@@ -166,88 +194,6 @@ __device__ __forceinline__ void forward_tile(uint16_t* base, const uint16_t* wb,
                                             MLP_OUT, y);
 }
 
-// One layer of the backward pass over a 32-sample tile, given G (the loss gradient at the
-// layer's output, bf16 rows in LDS) and X (its input): the weight and bias gradients are added
-// into gW / gb (this block's partial row: stored by its first tile, first = true, added by
-// later ones; each element has one writer lane), and, with INPUT_GRAD, the gradient at the
-// input, masked by the ReLU that made X,
-// is written to Gp. Fragments are loaded once per layer: the weight-gradient operands (columns
-// of G and X, gathered) once per row and column block, the W^T fragments before any product.
-template <int K, int NO, int NTT, bool INPUT_GRAD>
-__device__ __forceinline__ void backward_layer(const uint16_t* G, const uint16_t* X, uint16_t* Gp, const uint16_t* WT,
-                                               float* gW, float* gb, int nv, bool first)
-{
-    auto put = [first](float* p, float v) {
-        if (first)
-            *p = v;
-        else
-            *p += v;
-    };
-    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
-    constexpr int OT = NO / 16, KT = K / 16;
-    {
-        // weight gradient tiles dW[o][k] = sum_s G[s][o] X[s][k]: rows o, columns k
-        bf16x8_t ga[OT], xb[KT];
-#pragma unroll
-        for (int ot = 0; ot < OT; ot++)
-            ga[ot] = gather8(G + (8 * g) * ST + ot * 16 + r, ST);
-#pragma unroll
-        for (int kt = 0; kt < KT; kt++)
-            xb[kt] = gather8(X + (8 * g) * ST + kt * 16 + r, ST);
-#pragma unroll
-        for (int ot = 0; ot < OT; ot++)
-#pragma unroll
-            for (int kt = 0; kt < KT; kt++) {
-                const f32x4_t d = mfma(ga[ot], xb[kt], f32x4_t{0.0f, 0.0f, 0.0f, 0.0f});
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int o = ot * 16 + 4 * g + i;
-                    if (o < nv)
-                        put(&gW[o * K + kt * 16 + r], d[i]);
-                }
-            }
-    }
-    // bias gradient: one output per lane
-    if (lane < nv) {
-        float sb = 0.0f;
-        for (int sr = 0; sr < WS; sr++)
-            sb += bf2f(G[sr * ST + lane]);
-        put(&gb[lane], sb);
-    }
-    if (!INPUT_GRAD)
-        return;
-    // input gradient G'[s][k] = sum_o G[s][o] W[o][k] through W^T, masked by the ReLU of X
-    constexpr int OC = NTT / 32;
-    bf16x8_t bw[KT][OC], ag[2][OC];
-#pragma unroll
-    for (int kt = 0; kt < KT; kt++)
-#pragma unroll
-        for (int oc = 0; oc < OC; oc++)
-            bw[kt][oc] = load8(WT + (kt * 16 + r) * NTT + oc * 32 + 8 * g);
-#pragma unroll
-    for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-        for (int oc = 0; oc < OC; oc++)
-            ag[mt][oc] = load8(G + (mt * 16 + r) * ST + oc * 32 + 8 * g);
-#pragma unroll
-    for (int kt = 0; kt < KT; kt++)
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int oc = 0; oc < OC; oc++)
-                acc = mfma(ag[mt][oc], bw[kt][oc], acc);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int srow = mt * 16 + 4 * g + i, k = kt * 16 + r;
-                const bool on = X[srow * ST + k] != 0;  // post-ReLU activation > 0
-                Gp[srow * ST + k] = on ? f2bf(acc[i]) : uint16_t(0);
-            }
-        }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-}
-
 struct CacheArgs {
     const float* params;
     const uint16_t* wb;
@@ -271,38 +217,154 @@ struct TrainSrc {
     const float* targets;
 };
 
-// Training: forward + backward over n samples; the gradients (summed over the batch, already
-// scaled by the loss normalisation) of the tiles block b takes (b, b + grid, ...) go to row b of
-// the partial gradients, which k_apply_gradients sums. At most PARTIAL_ROWS blocks.
-__global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uint32_t n)
+// The training kernel's layers, split over the four waves of a block that share one 32-sample
+// tile in LDS (the caller synchronises the block between layers).
+//
+// Forward: a 64-wide layer gives wave w output tile nt = w for both sample tiles; the last
+// layer (LAST, 16 padded outputs) gives waves 0 and 1 sample tile mt = w, into y.
+template <int K, int NO, bool LAST>
+__device__ __forceinline__ void forward_layer4(const uint16_t* X, uint16_t* Y, const uint16_t* W, const float* bias,
+                                               int nv, int w, f32x4_t& y)
+{
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    constexpr int KC = K / 32;
+    static_assert(LAST ? NO == 16 : NO == 64, "layer widths");
+    if (LAST && w >= 2)
+        return;
+    const int nt = LAST ? 0 : w;
+    bf16x8_t bw[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++)
+        bw[kc] = load8(W + (nt * 16 + r) * K + kc * 32 + 8 * g);
+    const int o = nt * 16 + r;
+    const float bo = o < nv ? bias[o] : 0.0f;
+    constexpr int MT = LAST ? 1 : 2;
+    bf16x8_t ax[MT][KC];
+#pragma unroll
+    for (int j = 0; j < MT; j++)
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+            ax[j][kc] = load8(X + ((LAST ? w : j) * 16 + r) * ST + kc * 32 + 8 * g);
+#pragma unroll
+    for (int j = 0; j < MT; j++) {
+        const int mt = LAST ? w : j;
+        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+            acc = mfma(ax[j][kc], bw[kc], acc);
+        if (LAST) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                acc[i] += bo;
+            y = acc;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                Y[(mt * 16 + 4 * g + i) * ST + o] = f2bf(fmaxf(acc[i] + bo, 0.0f));
+        }
+    }
+}
+
+// Backward: the weight-gradient tiles of column blocks kt = w, w + 4, ... (every row block) go
+// to the block's partial row (stored by its first tile, first = true, added by later ones; each
+// element has one writer lane), wave 0 adds the bias gradient, and with INPUT_GRAD wave w
+// writes input-gradient column block kt = w (masked by the ReLU that made X) to Gp.
+template <int K, int NO, int NTT, bool INPUT_GRAD>
+__device__ __forceinline__ void backward_layer4(const uint16_t* G, const uint16_t* X, uint16_t* Gp,
+                                                const uint16_t* WT, float* gW, float* gb, int nv, bool first, int w)
+{
+    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    constexpr int OT = NO / 16, KT = K / 16;
+    auto put = [first](float* p, float v) {
+        if (first)
+            *p = v;
+        else
+            *p += v;
+    };
+    bf16x8_t ga[OT];
+#pragma unroll
+    for (int ot = 0; ot < OT; ot++)
+        ga[ot] = gather8(G + (8 * g) * ST + ot * 16 + r, ST);
+    for (int kt = w; kt < KT; kt += 4) {
+        const bf16x8_t xb = gather8(X + (8 * g) * ST + kt * 16 + r, ST);
+#pragma unroll
+        for (int ot = 0; ot < OT; ot++) {
+            const f32x4_t d = mfma(ga[ot], xb, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int o = ot * 16 + 4 * g + i;
+                if (o < nv)
+                    put(&gW[o * K + kt * 16 + r], d[i]);
+            }
+        }
+    }
+    if (w == 0 && lane < nv) {
+        float sb = 0.0f;
+        for (int sr = 0; sr < WS; sr++)
+            sb += bf2f(G[sr * ST + lane]);
+        put(&gb[lane], sb);
+    }
+    if (!INPUT_GRAD)
+        return;
+    static_assert(!INPUT_GRAD || KT == 4, "one input-gradient column block per wave");
+    constexpr int OC = NTT / 32;
+    const int kt = w;
+    bf16x8_t bw[OC], ag[2][OC];
+#pragma unroll
+    for (int oc = 0; oc < OC; oc++)
+        bw[oc] = load8(WT + (kt * 16 + r) * NTT + oc * 32 + 8 * g);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int oc = 0; oc < OC; oc++)
+            ag[mt][oc] = load8(G + (mt * 16 + r) * ST + oc * 32 + 8 * g);
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int oc = 0; oc < OC; oc++)
+            acc = mfma(ag[mt][oc], bw[oc], acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int srow = mt * 16 + 4 * g + i, k = kt * 16 + r;
+            const bool on = X[srow * ST + k] != 0;  // post-ReLU activation > 0
+            Gp[srow * ST + k] = on ? f2bf(acc[i]) : uint16_t(0);
+        }
+    }
+}
+
+// Training: forward + backward over n samples, one 32-sample tile at a time per block of four
+// waves; the gradients (summed over the batch, already scaled by the loss normalisation) of the
+// tiles block b takes (b, b + grid, ...) go to row b of the partial gradients, which
+// k_apply_gradients sums. At most PARTIAL_ROWS blocks.
+__global__ __launch_bounds__(256) void k_mlp_train(CacheArgs a, TrainSrc src, uint32_t n)
 {
     float* const prow = a.partials + size_t(blockIdx.x) * PSTRIDE;
     __shared__ __attribute__((aligned(16))) uint16_t lds[(MLP_LAYERS + 2) * WS * ST];
+    __shared__ float4 tl[WS];  // targets and use (1: the sample counts) of the tile's samples
     uint16_t* const gbase = lds + MLP_LAYERS * WS * ST;  // two gradient buffers
-    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    const int w = threadIdx.x >> 6, lane = __lane_id(), r = lane & 15, g = lane >> 4;
     const float inv_n3 = 1.0f / (3.0f * float(n));
     float loss = 0.0f;
     const uint32_t n_rec = src.n_rec_dev ? min(*src.n_rec_dev, src.cap) : src.n_rec;
     if (!src.features && n_rec == 0) {  // nothing recorded yet: a zero gradient
-        for (int i = lane; i < PSTRIDE; i += 64)
+        for (int i = threadIdx.x; i < PSTRIDE; i += 256)
             prow[i] = 0.0f;
         return;
     }
-
+    constexpr int L = MLP_LAYERS - 1;
     for (uint32_t base = blockIdx.x * WS; base < n; base += gridDim.x * WS) {
         const bool first = base == blockIdx.x * WS;
-        float tgt[3] = {0, 0, 0};
-        float use = 0.0f;  // 1 for a sample that counts
         const uint32_t s = base + uint32_t(lane);
-        if (lane < WS) {
+        if (lane < WS) {  // the tile's inputs, a quarter of the features per wave
             uint16_t* row = lds + lane * ST;
+            float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (s < n) {
                 if (src.features) {
-                    for (int f = 0; f < MLP_IN; f++)
+                    for (int f = 24 * w; f < 24 * w + 24; f++)
                         row[f] = f2bf(src.features[size_t(s) * MLP_IN + f]);
-                    for (int c = 0; c < 3; c++)
-                        tgt[c] = src.targets[size_t(s) * 3 + c];
-                    use = 1.0f;
+                    t = make_float4(src.targets[size_t(s) * 3], src.targets[size_t(s) * 3 + 1],
+                                    src.targets[size_t(s) * 3 + 2], 1.0f);
                 } else {
                     uint32_t j;
                     if (src.n_rec_dev)
@@ -310,35 +372,42 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uin
                     else
                         j = uint32_t((uint64_t(src.start) + uint64_t(s) * src.stride) % n_rec);
                     const float* rec = src.records + size_t(j) * MLP_REC_FLOATS;
-                    encode_row(rec, a.lo, a.inv_ext, row);
-                    for (int c = 0; c < 3; c++)
-                        tgt[c] = rec[13 + c];
-                    use = rec[19] != 0.0f ? 1.0f : 0.0f;  // the target is complete
+                    encode_part(rec, a.lo, a.inv_ext, row, w);
+                    // the target is complete: the sample counts
+                    t = make_float4(rec[13], rec[14], rec[15], rec[19] != 0.0f ? 1.0f : 0.0f);
                 }
             } else {
-                for (int f = 0; f < MLP_IN; f++)
+                for (int f = 24 * w; f < 24 * w + 24; f++)
                     row[f] = 0;
             }
+            if (w == 0)
+                tl[lane] = t;
         }
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
-        f32x4_t y[2];
-        forward_tile<false>(lds, a.wb, a.params, y);
-
-        // dL/dy of the relative L2 loss mean((y - t)^2 / (sg(y)^2 + 0.01)), into G5 (32 wide).
-        // Targets live with lane s; the D layout holds sample 4g+i of tile mt in lane (r, g).
-        uint16_t* G = gbase + ((MLP_LAYERS - 1) & 1) * WS * ST;
-        for (int mt = 0; mt < 2; mt++)
+        __syncthreads();
+        f32x4_t y = {0.0f, 0.0f, 0.0f, 0.0f};
+        forward_layer4<MLP_IN, MLP_H, false>(act_buf<false>(lds, 0), act_buf<false>(lds, 1), a.wb + LAYOUT.wb[0],
+                                             a.params + LAYOUT.b[0], MLP_H, w, y);
+        __syncthreads();
+#pragma unroll 1
+        for (int l = 1; l < L; l++) {
+            forward_layer4<MLP_H, MLP_H, false>(act_buf<false>(lds, l), act_buf<false>(lds, l + 1),
+                                                a.wb + LAYOUT.wb[l], a.params + LAYOUT.b[l], MLP_H, w, y);
+            __syncthreads();
+        }
+        forward_layer4<MLP_H, mlp_outp(L), true>(act_buf<false>(lds, L), nullptr, a.wb + LAYOUT.wb[L],
+                                                 a.params + LAYOUT.b[L], MLP_OUT, w, y);
+        // dL/dy of the relative L2 loss mean((y - t)^2 / (sg(y)^2 + 0.01)) into G5 (32 wide): waves
+        // 0 and 1, sample tile mt = w, whose rows 4g..4g+3 the lane holds for output r
+        uint16_t* G = gbase + (L & 1) * WS * ST;
+        if (w < 2) {
+#pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int srow = mt * 16 + 4 * g + i;
-                float t_c = 0.0f;
-                // fetch target component r of sample srow from the lane that loaded it
-                const float t0 = __shfl(tgt[0], srow), t1 = __shfl(tgt[1], srow), t2 = __shfl(tgt[2], srow);
-                const float u = __shfl(use, srow);
-                t_c = r == 0 ? t0 : (r == 1 ? t1 : t2);
+                const int srow = w * 16 + 4 * g + i;
+                const float4 t = tl[srow];
+                const float t_c = r == 0 ? t.x : (r == 1 ? t.y : t.z);
                 float gr = 0.0f;
-                if (r < 3 && u != 0.0f) {
-                    const float yv = y[mt][i];
+                if (r < 3 && t.w != 0.0f) {
+                    const float yv = y[i];
                     const float den = yv * yv + 0.01f;
                     const float d = yv - t_c;
                     loss += d * d / den * inv_n3;
@@ -347,27 +416,27 @@ __global__ __launch_bounds__(64) void k_mlp_train(CacheArgs a, TrainSrc src, uin
                 G[srow * ST + r] = f2bf(gr);
                 G[srow * ST + 16 + r] = 0;
             }
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
-
-        {
-            constexpr int L = MLP_LAYERS - 1;
-            backward_layer<MLP_H, mlp_outp(L), mlp_outt(L), true>(
-                gbase + (L & 1) * WS * ST, act_buf<false>(lds, L), gbase + ((L - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[L],
-                prow + LAYOUT.w[L], prow + LAYOUT.b[L], MLP_OUT, first);
         }
+        __syncthreads();
+        backward_layer4<MLP_H, mlp_outp(L), mlp_outt(L), true>(G, act_buf<false>(lds, L),
+                                                               gbase + ((L - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[L],
+                                                               prow + LAYOUT.w[L], prow + LAYOUT.b[L], MLP_OUT, first, w);
+        __syncthreads();
 #pragma unroll 1
-        for (int l = MLP_LAYERS - 2; l >= 1; l--)
-            backward_layer<MLP_H, MLP_H, MLP_H, true>(gbase + (l & 1) * WS * ST, act_buf<false>(lds, l),
-                                                      gbase + ((l - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[l],
-                                                      prow + LAYOUT.w[l], prow + LAYOUT.b[l], MLP_H, first);
-        backward_layer<MLP_IN, MLP_H, MLP_H, false>(gbase, act_buf<false>(lds, 0), nullptr, nullptr,
-                                                    prow + LAYOUT.w[0], prow + LAYOUT.b[0], MLP_H, first);
+        for (int l = L - 1; l >= 1; l--) {
+            backward_layer4<MLP_H, MLP_H, MLP_H, true>(gbase + (l & 1) * WS * ST, act_buf<false>(lds, l),
+                                                       gbase + ((l - 1) & 1) * WS * ST, a.wt + LAYOUT.wt[l],
+                                                       prow + LAYOUT.w[l], prow + LAYOUT.b[l], MLP_H, first, w);
+            __syncthreads();
+        }
+        backward_layer4<MLP_IN, MLP_H, MLP_H, false>(gbase, act_buf<false>(lds, 0), nullptr, nullptr,
+                                                     prow + LAYOUT.w[0], prow + LAYOUT.b[0], MLP_H, first, w);
+        __syncthreads();  // the next tile's inputs overwrite what this one read
     }
     // wave-reduce the loss
     for (int off = 32; off >= 1; off >>= 1)
         loss += __shfl_xor(loss, off);
-    if (lane == 0)
+    if (lane == 0 && w < 2)
         atomicAdd(a.loss_acc, loss);
 }
 
@@ -742,7 +811,7 @@ hipError_t cache_query(CacheState& c, const uint32_t* d_count, float4* next_rad,
 static uint32_t launch_train(CacheState& c, const TrainSrc& src, uint32_t n, hipStream_t st)
 {
     const uint32_t rows = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((uint64_t(n) + WS - 1) / WS, PARTIAL_ROWS)));
-    hipLaunchKernelGGL(k_mlp_train, dim3(rows), dim3(64), 0, st, args_of(c), src, n);
+    hipLaunchKernelGGL(k_mlp_train, dim3(rows), dim3(256), 0, st, args_of(c), src, n);
     return rows;
 }
 
