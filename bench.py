@@ -354,6 +354,8 @@ def run_instanced(a):
                                  "shadow_nodes": round(n_node_sh, 3), "shadow_tris": round(n_tri_sh, 3)}, **topo),
         "stages_ms": dict({k: round(v["ms"], 3) for k, v in stages.items()}, refill=round(sp["refill_ms"], 3)),
         "stage_launches": dict({k: v["launches"] for k, v in stages.items()}, refill=sp["refill_launches"]),
+        # rays of the profiled pass (shade and refill work per extension ray)
+        "stage_rays": {"extension": sp["extension_rays"], "shadow": sp["shadow_rays"]},
         "bvh_build_ms": round(st0["bvh_build_ms"], 2),
         "autotune": tune,
         "sort_hits": bool(a.sort_hits),

@@ -175,10 +175,10 @@ typedef struct pt_stats {
  * size, node width, builder) can resolve exact ties between equally distant triangles
  * differently. */
 #define PT_OPT_BLOCKS_PER_CU 4u   /* traversal grid: blocks of 256 threads per CU (1..64, default 6) */
-#define PT_OPT_POOL_SLOTS 5u      /* path pool slots (1024..2^26; default min(8M, max(4 px, 64K)); a batch's
+#define PT_OPT_POOL_SLOTS 5u      /* path pool slots (1024..2^26; default min(16M, max(8 px, 64K)); a batch's
                                    * min(16M, max(4 x its pixels, 64K)) */
 #define PT_OPT_LDS_STACK 6u       /* traversal stack entries per lane kept in LDS (8, 16, 32) */
-#define PT_OPT_LEAF_SIZE 7u       /* max triangles per BLAS leaf (1..16, default 4): rebuilds the BVH */
+#define PT_OPT_LEAF_SIZE 7u       /* max triangles per BLAS leaf (1..16, default 1): rebuilds the BVH */
 #define PT_OPT_SORT_HITS 8u       /* value != 0: shade extension hits in (scene, material) order
                                    * after a radix sort (default 0; iterations then launch
                                    * without graphs) */

@@ -748,9 +748,9 @@ int pt_create(const pt_scene_blob* scene, uint32_t width, uint32_t height, pt_ct
              if (const char* env = getenv("PT_BLOCKS_PER_CU"))
                  c.blocks_per_cu = uint32_t(std::max(1, std::min(64, atoi(env))));
              const uint32_t npix = width * height;
-             // Path pool: 8M slots (fewer for small images); per-job radiance buffer for up to
-             // 2^27 paths (64 spp at 1920x1080).
-             c.pool_slots = std::min<uint32_t>(8u << 20, std::max<uint32_t>(npix * 4u, 65536u));
+             // Path pool: 16M slots (fewer for small images; the autotuner's usual pick at 1080p);
+             // per-job radiance buffer for up to 2^27 paths (64 spp at 1920x1080).
+             c.pool_slots = std::min<uint32_t>(16u << 20, std::max<uint32_t>(npix * 8u, 65536u));
              if (const char* env = getenv("PT_POOL_SLOTS"))
                  c.pool_slots = uint32_t(std::max(1024, atoi(env)));
              const uint32_t job_spp = std::max<uint32_t>(1, std::min<uint32_t>(1024, (1u << 27) / npix));
@@ -1773,32 +1773,39 @@ int pt_autotune(pt_ctx* h, uint32_t spp, pt_tune_result* out)
     const std::vector<int64_t> widths = {4, 3, 2};
     const std::vector<int64_t> opens = {0, 1, 2};
     // coordinate descent over (blocks per CU, LDS stack depth, pool slots, node width, leaf size,
-    // TLAS opening depth)
-    for (int param = 0; param < 6; param++) {
-        const std::vector<int64_t>& vals = param == 0   ? bpcs
-                                           : param == 1 ? ldss
-                                           : param == 2 ? pools
-                                           : param == 3 ? widths
-                                           : param == 4 ? leaves
-                                                        : opens;
-        for (int64_t v : vals) {
-            Cfg g = best;
-            (param == 0   ? g.bpc
-             : param == 1 ? g.lds
-             : param == 2 ? g.pool
-             : param == 3 ? g.width
-             : param == 4 ? g.leaf
-                          : g.open) = v;
-            if (g == best)
-                continue;
-            double ns;
-            if (int e = measure(g, ns))
-                return e;
-            if (ns < best_ns) {
-                best_ns = ns;
-                best = g;
+    // TLAS opening depth), repeated while a round changes the configuration (at most three
+    // rounds: a parameter tried early is tried again once the others have moved); a trial
+    // replaces the best only when it is 0.5 % faster, so timing noise does not move it
+    for (int round = 0; round < 3; round++) {
+        const Cfg round_start = best;
+        for (int param = 0; param < 6; param++) {
+            const std::vector<int64_t>& vals = param == 0   ? bpcs
+                                               : param == 1 ? ldss
+                                               : param == 2 ? pools
+                                               : param == 3 ? widths
+                                               : param == 4 ? leaves
+                                                            : opens;
+            for (int64_t v : vals) {
+                Cfg g = best;
+                (param == 0   ? g.bpc
+                 : param == 1 ? g.lds
+                 : param == 2 ? g.pool
+                 : param == 3 ? g.width
+                 : param == 4 ? g.leaf
+                              : g.open) = v;
+                if (g == best)
+                    continue;
+                double ns;
+                if (int e = measure(g, ns))
+                    return e;
+                if (ns < best_ns * 0.995) {
+                    best_ns = ns;
+                    best = g;
+                }
             }
         }
+        if (best == round_start)
+            break;
     }
     double final_ns;
     if (int v = measure(best, final_ns))

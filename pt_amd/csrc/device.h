@@ -278,7 +278,7 @@ struct Ctx {
     uint32_t spill_depth = SPILL_STACK;  // PT_OPT_SPILL_DEPTH
     bool transparent = false;  // some instance's material is transparent
     bool single_tri_leaves = false;  // every BLAS leaf of the built hierarchy holds one triangle
-    uint32_t leaf_size = 4;
+    uint32_t leaf_size = 1;  // single-triangle leaves: the traversal kernels' leaf-loop-free variants
     uint32_t node_width = 4;
     uint32_t bvh_builder = 0;  // BVH_BUILDER_*
     uint32_t tlas_open = 0;    // PT_OPT_TLAS_OPEN

@@ -112,7 +112,8 @@ def test_autotune_picks_a_configuration(inst):
     assert t.stats()["sample_index"] == 0
     t.render(4)  # the accumulation restarted: the tuned renderer gives the same image
     same = np.all(t.read() == ref, axis=2)
-    assert same.mean() >= (1.0 if (r["leaf_size"], r["node_width"], r["tlas_open"]) == (4, 4, 0) else 0.999)
+    # the default hierarchy (leaves of 1, width 4, no TLAS opening) gives the reference image bit for bit
+    assert same.mean() >= (1.0 if (r["leaf_size"], r["node_width"], r["tlas_open"]) == (1, 4, 0) else 0.999)
     t.close()
 
 
