@@ -1336,10 +1336,12 @@ int pt_cache_train_batch(pt_ctx* h, const float* features, const float* targets,
                 "readback");
     if (loss)
         HIP_TRY(&c, hipMemcpyAsync(loss, c.cache.loss_acc, 4, hipMemcpyDeviceToHost, c.stream), "readback");
-    if (apply)
+    if (apply) {
         HIP_TRY(&c, cache_adam(c.cache, c.stream), "cache update");
-    else
+    } else {  // gradient and loss discarded: training steps start from zero
         HIP_TRY(&c, hipMemsetAsync(c.cache.grads, 0, size_t(mlp_param_count()) * 4, c.stream), "cache gradients");
+        HIP_TRY(&c, hipMemsetAsync(c.cache.loss_acc, 0, 4, c.stream), "cache loss");
+    }
     HIP_TRY(&c, hipStreamSynchronize(c.stream), "cache train");
     return PT_OK;
 }
