@@ -5,8 +5,9 @@ Bars (from the task's correctness statement):
     mismatch is confirmed in fp64 to be a tie (tests/tie_check.py): the ray passes within 64
     fp32 ulps of the contested triangle's edge, or both triangles are hit at the same depth;
   * with the same seeds the GPU and oracle images agree per pixel (fp32 rounding aside; a path
-    whose decision flips on an ulp is a rare outlier), and the converged images agree within
-    relative MSE 1e-4.
+    whose decision flips on an ulp is a rare outlier: the pixels it moves are bounded in number
+    and, summed over the image, in size), and the converged images agree within relative MSE
+    1e-4.
 """
 import numpy as np
 import pytest
@@ -59,9 +60,12 @@ def test_render_matches_oracle_per_pixel(box):
     tracer.render(4)
     g = tracer.read()
     o, counts = orc.render(cam, W, H, 4, 4)
-    close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * (1.0 + np.abs(o[..., :3]))
+    d = np.abs(g[..., :3] - o[..., :3])
+    close = d <= 1e-3 * (1.0 + np.abs(o[..., :3]))
     frac = close.all(axis=-1).mean()
-    assert frac >= 0.99, frac
+    # measured: every pixel agrees, mean difference 1.4e-7 of the mean radiance
+    assert frac >= 0.999, frac
+    assert d.mean() <= 1e-5 * np.abs(o[..., :3]).mean(), d.mean()
     st = tracer.stats()
     # identical path decisions -> ray counts within a fraction of a percent
     assert abs(st["extension_rays"] - counts["extension_rays"]) <= 0.005 * counts["extension_rays"]
@@ -83,8 +87,10 @@ def test_image_sides_off_the_tile_grid_match_oracle(box, w, h, spp):
     tr.close()
     o, _ = orc.render(cam, w, h, spp, 3)
     assert np.all(g[..., 3] == 1.0)  # every pixel's first sample was accumulated
-    close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * (1.0 + np.abs(o[..., :3]))
-    assert close.all(axis=-1).mean() >= 0.97, close.all(axis=-1).mean()
+    d = np.abs(g[..., :3] - o[..., :3])
+    close = d <= 1e-3 * (1.0 + np.abs(o[..., :3]))
+    assert close.all(axis=-1).mean() >= 0.99, close.all(axis=-1).mean()
+    assert d.mean() <= 1e-3 * np.abs(o[..., :3]).mean(), d.mean()
 
 
 def test_render_converges_to_oracle(box):
@@ -120,6 +126,8 @@ def test_point_and_directional_lights_match_oracle(box_scn):
     t.render(4)
     g = t.read()
     o, _ = ob.Oracle(scene).render(cam, w, h, 4, 4)
-    close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * (1.0 + np.abs(o[..., :3]))
+    d = np.abs(g[..., :3] - o[..., :3])
+    close = d <= 1e-3 * (1.0 + np.abs(o[..., :3]))
     assert close.all(axis=-1).mean() >= 0.99
+    assert d.mean() <= 1e-3 * np.abs(o[..., :3]).mean(), d.mean()
     t.close()

@@ -86,8 +86,12 @@ def test_render_instanced_matches_oracle(inst):
     tr.render(4)
     g = tr.read()
     o, counts = orc.render(cam, W, H, 4, 8)
-    close = np.abs(g[..., :3] - o[..., :3]) <= 1e-3 * (1.0 + np.abs(o[..., :3]))
-    assert close.all(axis=-1).mean() >= 0.99
+    d = np.abs(g[..., :3] - o[..., :3])
+    close = d <= 1e-3 * (1.0 + np.abs(o[..., :3]))
+    # measured: 99.89 % of pixels agree; the paths that flipped a decision move the image's mean
+    # absolute difference to 9.7e-5 of its mean radiance
+    assert close.all(axis=-1).mean() >= 0.995
+    assert d.mean() <= 1e-3 * np.abs(o[..., :3]).mean(), d.mean()
     st = tr.stats()
     assert abs(st["extension_rays"] - counts["extension_rays"]) <= 0.005 * counts["extension_rays"]
 

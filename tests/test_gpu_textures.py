@@ -27,9 +27,12 @@ def test_textured_render_matches_oracle_per_pixel(tex):
     t.render(4)
     g = t.read()
     o, counts = orc.render(cam, W, H, 4, 4)
-    close = np.abs(g[..., :3] - o[..., :3]) <= 2e-3 * (1.0 + np.abs(o[..., :3]))
+    d = np.abs(g[..., :3] - o[..., :3])
+    close = d <= 2e-3 * (1.0 + np.abs(o[..., :3]))
     frac = close.all(axis=-1).mean()
-    assert frac >= 0.98, frac
+    # measured: 99.996 % of pixels agree, mean difference 2e-6 of the mean radiance
+    assert frac >= 0.999, frac
+    assert d.mean() <= 1e-4 * np.abs(o[..., :3]).mean(), d.mean()
     st = t.stats()
     assert abs(st["extension_rays"] - counts["extension_rays"]) <= 0.01 * counts["extension_rays"]
     assert abs(st["shadow_rays"] - counts["shadow_rays"]) <= 0.01 * counts["shadow_rays"]
