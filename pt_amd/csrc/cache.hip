@@ -1,8 +1,9 @@
 // Learned radiance cache: fused MLP inference and training on BF16 MFMA (gfx950
 // v_mfma_f32_16x16x32_bf16, FP32 accumulation) and an FP32 Adam update.
 //
-// One wave evaluates 32 samples through all six layers; activations stay in LDS as bf16
-// (sample-major rows). Every product is a 16x16x32 MFMA tile:
+// A 32-sample tile goes through all six layers with its activations in LDS as bf16 (sample-major
+// rows): one wave per tile for queries, the four waves of a block per tile for training. Every
+// product is a 16x16x32 MFMA tile:
 //   forward       Y[s][o]  = sum_k X[s][k] W[o][k]      A = X rows, B = W rows
 //   input grad    G'[s][k] = sum_o G[s][o] W[o][k]      A = G rows, B = W^T rows
 //   weight grad   dW[o][k] = sum_s G[s][o] X[s][k]      A = G columns, B = X columns
