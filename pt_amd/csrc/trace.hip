@@ -724,9 +724,11 @@ __device__ __forceinline__ void shadow_queue(const DeviceScene& s0, const Device
 
 // Traversal kernels keep at most 80 registers (6 waves per SIMD, a few bytes spilled): more
 // waves in flight hide more of the dependent node fetches (measured: +3% over 5 waves; 8 waves
-// spill too much).
+// spill too much). With a 32-entry LDS stack a block's LDS (42 KiB) lets only 3 waves per SIMD
+// in, and the register budget is set for those.
+constexpr int trace_waves(uint32_t d) { return d > 16 ? 3 : 6; }
 template <bool COUNT, bool MULTI, uint32_t D, bool LEAF1>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves(D)))) void k_extend(DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 }
 
 template <bool COUNT, bool MULTI, uint32_t D, bool ALPHA, bool LEAF1>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves(D)))) void k_shadow(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q)
 {
     __shared__ uint32_t lds[(D + COLD_ROWS) * TRACE_BLOCK];
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 // end of the shadow work leaves no part of the chip idle. The shadow rays add to the radiance
 // of queue q's entries, which the extension part does not touch.
 template <bool MULTI, uint32_t D, bool ALPHA, bool LEAF1>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_both(
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves(D)))) void k_trace_both(
     DeviceScene s0, const DeviceScene* scenes, Wavefront wf, int q, const ViewDev* views, const JobDev* j0,
     const JobDev* j1, int carry)
 {
